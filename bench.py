@@ -1,0 +1,241 @@
+"""Benchmark: Tet4 Neo-Hookean f_int + K_T global assembly on one MI355X.
+
+Workload: jittered, node-permuted 100^3-hex Kuhn box (6.0M Tet4, 1.03M nodes,
+3.09M DOF, 136.9M CSR nonzeros), per-element lognormal mu_e with
+lambda_e = 5 mu_e, smooth seeded displacement with max |grad u| = 0.1
+(psierra_amd.workloads.jittered_box_tet4, seed 1).  One step = one
+psx_assemble call (f_int and every K_T value into the fixed CSR pattern) with
+u, mu, lambda already resident in HBM.  ``value`` is elements/s of the FP32
+element arithmetic / FP64 accumulation mode; the plain FP64 mode is timed
+beside it.  Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+METRIC = "Tet4 NeoHookean f_int+K_T assembly elements/s & %HBM roofline vs plain FP64"
+HBM_PEAK_GBPS = 8000.0
+
+
+def algorithmic_bytes(E, N, npe, nnz, coord_bytes, param_bytes):
+    """Compulsory bytes of one assembly: read X and u once, connectivity
+    (int32), two per-element parameters; write K values and f in FP64."""
+    n = 3 * N
+    return 2 * n * coord_bytes + E * npe * 4 + E * 2 * param_bytes + 8 * nnz + 8 * n
+
+
+def cpu_baseline(wl, sample_elems, seed=0):
+    """Time the oracle (the FP64 restatement of the reference) on a sample.
+
+    faithful: per-element Python loop, element evaluated twice (force loop,
+    tangent loop), np.add.at and coo.tocsr -- the reference's own structure
+    (src/psierra/domain/solid.py:252,302; src/psierra/utility/assembly.py).
+    vectorised: batched numpy over the same sample.  Single thread each.
+    """
+    from oracle import assembly as A
+
+    rng = np.random.default_rng(seed)
+    idx = np.sort(rng.choice(wl.mesh.n_elements, sample_elems, replace=False))
+    conn = wl.mesh.elements[idx]
+    t0 = time.perf_counter()
+    A.assemble_loop(wl.mesh.nodes, conn, wl.u, wl.mu[idx], wl.lam[idx])
+    t_loop = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    A.assemble(wl.mesh.nodes, conn, wl.u, wl.mu[idx], wl.lam[idx])
+    t_vec = time.perf_counter() - t0
+    return sample_elems / t_loop, sample_elems / t_vec, t_loop, t_vec
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=100, help="hexes per box side (100 -> 6.0M tets)")
+    ap.add_argument("--cpu-sample", type=int, default=20000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--tile", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(
+            f"bench.py measures one GPU only: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
+            file=sys.stderr,
+        )
+        sys.exit(2)
+
+    from psierra_amd import workloads as W
+
+    t0 = time.perf_counter()
+    wl = W.jittered_box_tet4(n=args.n, seed=1)
+    t_build = time.perf_counter() - t0
+    print(f"[bench] workload {wl.name}: E={wl.mesh.n_elements} N={wl.mesh.n_nodes} ({t_build:.1f}s)",
+          file=sys.stderr, flush=True)
+
+    cpu = None
+    if not args.no_cpu:
+        loop_eps, vec_eps, t_loop, t_vec = cpu_baseline(wl, args.cpu_sample)
+        cpu = {
+            "value": loop_eps,
+            "unit": "elements/s",
+            "cores": 1,
+            "kind": "port",
+            "sample": (
+                f"{args.cpu_sample} seeded elements of the same mesh; FP64 oracle per-element loop "
+                f"(element evaluated twice, add.at, coo.tocsr) took {t_loop:.2f}s; "
+                f"vectorised FP64 numpy on the same sample: {vec_eps:.4g} elements/s ({t_vec:.2f}s)"
+            ),
+            "vectorised_value": vec_eps,
+            "cpu_model": platform.processor() or platform.machine(),
+            "numpy": np.__version__,
+        }
+        print(f"[bench] cpu baseline loop {loop_eps:.4g} el/s, vectorised {vec_eps:.4g} el/s",
+              file=sys.stderr, flush=True)
+
+    import torch
+
+    from psierra_amd.engine import AssemblyPlan
+
+    torch.cuda.init()
+    t0 = time.perf_counter()
+    plan = AssemblyPlan(wl.mesh.nodes, wl.mesh.elements, tile=args.tile)
+    t_plan = time.perf_counter() - t0
+    dev = plan.device
+    u = torch.as_tensor(wl.u.reshape(-1), device=dev)
+    mu = torch.as_tensor(wl.mu, device=dev)
+    lam = torch.as_tensor(wl.lam, device=dev)
+    f, k = plan.alloc_outputs(True)
+    E, N, nnz = plan.n_elements, plan.n_nodes, plan.nnz
+    print(f"[bench] plan {t_plan:.1f}s nnz={nnz} tiles={plan.info['n_tiles']} tile={plan.info['tile']}",
+          file=sys.stderr, flush=True)
+
+    def timed(precision, steps, warmup):
+        for _ in range(warmup):
+            plan.assemble(u, mu, lam, f=f, kval=k, precision=precision, check_status=False)
+        plan.raise_on_status(u)
+        torch.cuda.synchronize()
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+        t0 = time.perf_counter()
+        for a, b in evs:
+            a.record()
+            plan.assemble(u, mu, lam, f=f, kval=k, precision=precision, check_status=False)
+            b.record()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        plan.raise_on_status(u)
+        per = [a.elapsed_time(b) for a, b in evs]
+        return wall / steps * 1e3, float(np.mean(per)), float(np.median(per))
+
+    ms64_wall, ms64_ev, _ = timed("fp64", args.steps, args.warmup)
+    f64 = f.clone()
+    k64 = k.clone()
+    print(f"[bench] fp64 plain {ms64_ev:.3f} ms/assembly", file=sys.stderr, flush=True)
+    ms32_wall, ms32_ev, ms32_med = timed("fp32", args.steps, args.warmup)
+    print(f"[bench] fp32/fp64-acc {ms32_ev:.3f} ms/assembly", file=sys.stderr, flush=True)
+
+    def rel(a, b):
+        return float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
+
+    errors = {"fp32_vs_fp64_gpu": {"K": rel(k, k64), "f": rel(f, f64)}}
+    if not args.no_check:
+        from oracle import assembly as A
+
+        indptr, indices = plan.pattern()
+        rng = np.random.default_rng(2024)
+        sample = rng.choice(N, 20, replace=False)
+        f_ref, blocks = A.node_rows(wl.mesh.nodes, wl.mesh.elements, wl.u, wl.mu, wl.lam, sample)
+        k32h, f32h, k64h, f64h = k.cpu().numpy(), f.cpu().numpy(), k64.cpu().numpy(), f64.cpu().numpy()
+        num = {"K32": 0.0, "K64": 0.0, "f32": 0.0, "f64": 0.0}
+        den = {"K": 0.0, "f": 0.0}
+        for i, a in enumerate(sample):
+            a = int(a)
+            cols = indices[indptr[3 * a] : indptr[3 * a + 1]]
+            ref = np.zeros((3, cols.size))
+            for s, b in enumerate(cols[::3] // 3):
+                ref[:, 3 * s : 3 * s + 3] = blocks[a][int(b)]
+            g32 = np.stack([k32h[indptr[3 * a + r] : indptr[3 * a + r + 1]] for r in range(3)])
+            g64 = np.stack([k64h[indptr[3 * a + r] : indptr[3 * a + r + 1]] for r in range(3)])
+            num["K32"] += float(((g32 - ref) ** 2).sum())
+            num["K64"] += float(((g64 - ref) ** 2).sum())
+            den["K"] += float((ref**2).sum())
+            num["f32"] += float(((f32h[3 * a : 3 * a + 3] - f_ref[i]) ** 2).sum())
+            num["f64"] += float(((f64h[3 * a : 3 * a + 3] - f_ref[i]) ** 2).sum())
+            den["f"] += float((f_ref[i] ** 2).sum())
+        errors["vs_fp64_oracle_20_node_rows"] = {
+            "fp32": {"K": (num["K32"] / den["K"]) ** 0.5, "f": (num["f32"] / den["f"]) ** 0.5},
+            "fp64": {"K": (num["K64"] / den["K"]) ** 0.5, "f": (num["f64"] / den["f"]) ** 0.5},
+        }
+
+    # copy probe: achievable HBM bandwidth in this process
+    src = torch.empty(1 << 28, dtype=torch.float64, device=dev)  # 2 GiB
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    a_ev, b_ev = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a_ev.record()
+    for _ in range(5):
+        dst.copy_(src)
+    b_ev.record()
+    torch.cuda.synchronize()
+    copy_gbps = 5 * 2 * src.numel() * 8 / (a_ev.elapsed_time(b_ev) * 1e-3) / 1e9
+    del src, dst
+
+    b_alg32 = algorithmic_bytes(E, N, 4, nnz, coord_bytes=4, param_bytes=4)
+    b_alg64 = algorithmic_bytes(E, N, 4, nnz, coord_bytes=8, param_bytes=8)
+    achieved = b_alg32 / (ms32_ev * 1e-3) / 1e9
+    value = E / (ms32_wall * 1e-3)
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "elements/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms32_wall,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded jittered Kuhn box, permuted node ids, lognormal mu_e)",
+        "config": {
+            "workload": wl.name,
+            "elements": E,
+            "nodes": N,
+            "dofs": 3 * N,
+            "nnz": nnz,
+            "mode": "fp32 element arithmetic, fp64 scatter/f accumulation",
+            "fp64_plain_ms": ms64_ev,
+            "fp64_plain_elements_per_s": E / (ms64_wall * 1e-3),
+            "speedup_fp32_vs_fp64_plain": ms64_ev / ms32_ev,
+            "event_ms_mean": ms32_ev,
+            "event_ms_median": ms32_med,
+            "setup_s": {"workload": t_build, "plan": t_plan},
+            "tile": plan.info["tile"],
+            "errors": errors,
+            "copy_probe_GBps": copy_gbps,
+            "B_alg_bytes": {"fp32_mode": b_alg32, "fp64_mode": b_alg64},
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBPS,
+            "traffic": None,
+            "kernel": "psx_assemble (permute_u + tet4_assemble_kernel<float,256,true>)",
+            "fp64_plain_frac": b_alg64 / (ms64_ev * 1e-3) / 1e9 / HBM_PEAK_GBPS,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,139 @@
+/*
+ * psierra_amd -- MI355X (gfx950) element-evaluation and global-assembly engine
+ * for pSierra's Total Lagrangian Tet4/Tet10 compressible Neo-Hookean path.
+ *
+ * C ABI: plain pointers and sizes only.  Device pointers are HIP device
+ * memory (e.g. torch tensors' data_ptr()); host pointers are ordinary memory.
+ * Every device entry point is stream-ordered on the hipStream_t passed as
+ * `stream` (NULL = the default stream), performs no allocation, no
+ * host<->device copy and no synchronisation, so it can be captured in a
+ * hipGraph.
+ *
+ * Return codes: 0 success; > 0 invalid argument or unsupported input
+ * (psx_last_error() has the message); < 0 a negated hipError_t.
+ *
+ * Element inversion (det F <= 0, pSierra raises
+ * "Element inversion detected: det(F) = ... <= 0.") is reported through a
+ * device int32 status[2] = {flag, lowest offending element index}; the host
+ * wrapper turns it into the same ValueError.
+ *
+ * Numbering conventions are pSierra's: DOF = 3 * node + component
+ * (src/psierra/utility/indexing.py:35-59); the global matrix is the CSR that
+ * scipy's coo_matrix(...).tocsr() builds from the element triplets
+ * (src/psierra/utility/assembly.py:37-76): rows sorted, columns sorted,
+ * duplicates summed, explicit zeros kept.
+ */
+#ifndef PSIERRA_AMD_H
+#define PSIERRA_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct psx_plan psx_plan;
+
+enum {
+  PSX_OK = 0,
+  PSX_EINVAL = 1,       /* bad argument (null pointer, size, range) */
+  PSX_EUNSUPPORTED = 2, /* input outside what the kernels support */
+  PSX_EVOLUME = 3,      /* element with det([1 X]) <= 0 (or det J <= 0) */
+  PSX_ENOTREADY = 4     /* e.g. no GPU visible */
+};
+
+/* Element arithmetic precision.  PSX_FP64: everything in double.
+ * PSX_FP32: coordinate and displacement differences in double, then element
+ * arithmetic in float; every accumulation (global scatter, f) in double. */
+enum { PSX_FP64 = 0, PSX_FP32 = 1 };
+
+/* Info slots filled by psx_plan_info. */
+enum {
+  PSX_INFO_ELEMENTS = 0,
+  PSX_INFO_NODES = 1,
+  PSX_INFO_NPE = 2,
+  PSX_INFO_NNZ = 3,
+  PSX_INFO_TILES = 4,
+  PSX_INFO_TILE = 5,
+  PSX_INFO_MAX_DEGREE = 6,
+  PSX_INFO_DEVICE_BYTES = 7,
+  PSX_INFO_INCIDENCES = 8,
+  PSX_INFO_COUNT = 9
+};
+
+int32_t psx_abi_version(void);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* psx_last_error(void);
+
+/* ---- Structural pattern (host only, no GPU needed) -----------------------
+ * Replaces compute_element_dofs (src/psierra/utility/indexing.py:35-59),
+ * create_coo_sparsity_indices (src/psierra/utility/assembly.py:6-34) and the
+ * index half of coo_matrix(...).tocsr() (src/psierra/utility/assembly.py:75-76).
+ * conn: n_elements x npe node ids (row-major int64).  nnz < 2^31 required. */
+int32_t psx_pattern_nnz(const int64_t* conn, int64_t n_elements, int32_t npe,
+                        int64_t n_nodes, int64_t* nnz);
+int32_t psx_pattern_build(const int64_t* conn, int64_t n_elements, int32_t npe,
+                          int64_t n_nodes, int32_t* indptr /* 3N+1 */,
+                          int32_t* indices /* nnz */);
+
+/* ---- Assembly plan --------------------------------------------------------
+ * Built once per mesh, like SolidDomain.__init__'s precomputation
+ * (src/psierra/domain/solid.py:29-69).  Host inputs; the plan uploads the
+ * pattern, a node->element incidence map with slot ranks, a locality order
+ * of the nodes and the reference coordinates to the current HIP device.
+ * Rejects elements with det([1 X]) <= 0 (Tet4, src/psierra/element/
+ * tet4_base.py:56-61) or det J <= 0 at a quadrature point (Tet10,
+ * src/psierra/element/tet10_total_lagrangian.py:104-108) with PSX_EVOLUME.
+ * tile: incidences per work-group (0 = default; 64, 128 or 256). */
+int32_t psx_plan_create(const int64_t* conn, int64_t n_elements, int32_t npe,
+                        const double* X /* N x 3 */, int64_t n_nodes,
+                        int32_t tile, psx_plan** out);
+int32_t psx_plan_destroy(psx_plan* plan);
+int32_t psx_plan_info(const psx_plan* plan, int64_t* info, int32_t n_info);
+/* Host copies of the CSR pattern (indptr 3N+1, indices nnz). */
+int32_t psx_plan_pattern(const psx_plan* plan, int32_t* indptr, int32_t* indices);
+/* Change the work-group tile of an existing plan (re-tiles on the host). */
+int32_t psx_plan_set_tile(psx_plan* plan, int32_t tile);
+
+/* ---- Global assembly (device) ----------------------------------------------
+ * Replaces SolidDomain.assemble_internal_forces
+ * (src/psierra/domain/solid.py:230-274, with assemble_vector,
+ * src/psierra/utility/assembly.py:79-108) and
+ * SolidDomain.assemble_tangent_stiffness_matrix
+ * (src/psierra/domain/solid.py:280-325, with assemble_sparse_matrix,
+ * src/psierra/utility/assembly.py:37-76), evaluated in one pass.
+ * u: N x 3 double (device), mu/lam: per-element Lame parameters (device,
+ * n_elements each; the material is NeoHookean with bulk modulus
+ * K = lam + 2 mu / 3, src/psierra/material/parameters.py:135).
+ * f: 3N doubles (device, overwritten); kval: nnz doubles (device,
+ * overwritten) in the CSR order of psx_plan_pattern, or NULL for f only.
+ * status: device int32[2], reset by the call. */
+int32_t psx_assemble(psx_plan* plan, const double* u, const double* mu,
+                     const double* lam, double* f, double* kval,
+                     int32_t precision, void* stream, int32_t* status);
+
+/* ---- Element batch (device) -------------------------------------------------
+ * Replaces Tet4TotalLagrangian.compute_nonlinear_force_and_tangent_stiffness
+ * (src/psierra/element/tet4_total_lagrangian.py:77-139) and the Tet10 one
+ * (src/psierra/element/tet10_total_lagrangian.py:63-136) for n independent
+ * elements.  X, u: n x npe x 3; mu, lam: n; k: n x 3npe x 3npe (or NULL);
+ * f: n x 3npe.  status as above (volume failures set flag 2). */
+int32_t psx_element_batch(int32_t npe, int64_t n, const double* X, const double* u,
+                          const double* mu, const double* lam, double* k, double* f,
+                          int32_t precision, void* stream, int32_t* status);
+
+/* ---- Material batch (device) ------------------------------------------------
+ * Replaces NeoHookean.compute_stress_and_tangent
+ * (src/psierra/material/neohookean.py:55-122) for n deformation gradients.
+ * F: n x 3 x 3; mu, bulk: n; S: n x 3 x 3; C: n x 6 x 6 (Voigt order
+ * xx, yy, zz, xy, yz, zx, src/psierra/utility/tensor.py:5-12). */
+int32_t psx_nh_material_batch(int64_t n, const double* F, const double* mu,
+                              const double* bulk, double* S, double* C,
+                              int32_t precision, void* stream, int32_t* status);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PSIERRA_AMD_H */

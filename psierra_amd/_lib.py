@@ -1,0 +1,111 @@
+"""ctypes binding of libpsierra_amd.so (the C ABI declared in include/psierra_amd.h).
+
+The library is built in-tree by ``python -m psierra_amd.build``.  Loading
+fails loudly if it is missing: the engine has no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+from ctypes import POINTER, c_char_p, c_double, c_int32, c_int64, c_void_p
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libpsierra_amd.so"
+
+PSX_OK = 0
+PSX_EINVAL = 1
+PSX_EUNSUPPORTED = 2
+PSX_EVOLUME = 3
+PSX_ENOTREADY = 4
+PSX_FP64 = 0
+PSX_FP32 = 1
+
+INFO_KEYS = (
+    "n_elements",
+    "n_nodes",
+    "npe",
+    "nnz",
+    "n_tiles",
+    "tile",
+    "max_degree",
+    "device_bytes",
+    "n_incidences",
+)
+
+_i64p = POINTER(c_int64)
+_i32p = POINTER(c_int32)
+_f64p = POINTER(c_double)
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "psx_abi_version": (c_int32, []),
+    "psx_last_error": (c_char_p, []),
+    "psx_pattern_nnz": (c_int32, [c_void_p, c_int64, c_int32, c_int64, _i64p]),
+    "psx_pattern_build": (c_int32, [c_void_p, c_int64, c_int32, c_int64, c_void_p, c_void_p]),
+    "psx_plan_create": (c_int32, [c_void_p, c_int64, c_int32, c_void_p, c_int64, c_int32, POINTER(c_void_p)]),
+    "psx_plan_destroy": (c_int32, [c_void_p]),
+    "psx_plan_info": (c_int32, [c_void_p, _i64p, c_int32]),
+    "psx_plan_pattern": (c_int32, [c_void_p, c_void_p, c_void_p]),
+    "psx_plan_set_tile": (c_int32, [c_void_p, c_int32]),
+    "psx_assemble": (
+        c_int32,
+        [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_void_p],
+    ),
+    "psx_element_batch": (
+        c_int32,
+        [c_int32, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_void_p],
+    ),
+    "psx_nh_material_batch": (
+        c_int32,
+        [c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_void_p],
+    ),
+}
+
+_lib = None
+
+
+class PsxError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"psierra_amd error {code}: {message}")
+        self.code = code
+        self.message = message
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(
+                f"{LIB_PATH} is not built; run `python -m psierra_amd.build` (the engine has no CPU fallback)"
+            )
+        L = ctypes.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def last_error() -> str:
+    msg = lib().psx_last_error()
+    return msg.decode() if msg else ""
+
+
+def check(rc: int) -> None:
+    if rc == PSX_OK:
+        return
+    msg = last_error()
+    if rc == PSX_EVOLUME:
+        raise ValueError(msg)
+    if rc == PSX_EINVAL:
+        raise ValueError(msg)
+    raise PsxError(rc, msg)
+
+
+def ptr(x) -> c_void_p:
+    """Raw pointer of a numpy array or torch tensor (or None)."""
+    if x is None:
+        return c_void_p(0)
+    if hasattr(x, "data_ptr"):
+        return c_void_p(x.data_ptr())
+    return c_void_p(x.ctypes.data)

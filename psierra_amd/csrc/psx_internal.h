@@ -1,0 +1,89 @@
+// Internal declarations shared by the host plan builder and the HIP kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/psierra_amd.h"
+
+namespace psx {
+
+// Thread-local error message behind psx_last_error().
+void set_error(const std::string& msg);
+void clear_error();
+int32_t fail(int32_t code, const std::string& msg);
+
+#define PSX_HIP_TRY(call)                                                          \
+  do {                                                                             \
+    hipError_t psx_e_ = (call);                                                    \
+    if (psx_e_ != hipSuccess)                                                      \
+      return ::psx::fail(-static_cast<int32_t>(psx_e_),                            \
+                         std::string(#call) + ": " + hipGetErrorString(psx_e_));   \
+  } while (0)
+
+// Incidence packing: element id in the low 28 bits, local node in the top 4.
+constexpr int kIncElemBits = 28;
+constexpr uint32_t kIncElemMask = (1u << kIncElemBits) - 1u;
+
+// Host-side structural data (all int32 unless noted), built once per mesh.
+struct HostPattern {
+  int64_t E = 0, N = 0;
+  int npe = 0;
+  std::vector<int32_t> adj_ptr;  // N+1: node adjacency (user ids, sorted)
+  std::vector<int32_t> adj;      // node neighbours
+  std::vector<int32_t> n2e_ptr;  // N+1: incident elements per node
+  std::vector<int32_t> n2e;      // e * npe + p, sorted by e
+  int64_t nnz() const { return 9 * static_cast<int64_t>(adj_ptr.empty() ? 0 : adj_ptr.back()); }
+};
+
+int32_t build_host_pattern(const int64_t* conn, int64_t E, int npe, int64_t N, HostPattern& out);
+void fill_csr(const HostPattern& hp, int32_t* indptr, int32_t* indices);
+
+// Device view of a plan, passed by value to kernels.
+struct DevPlan {
+  int32_t E, N, npe, n_tiles, tile;
+  int32_t conn_stride;            // ints per element in conn (4 or 12)
+  const int32_t* conn;            // internal node ids, AoS [E][conn_stride]
+  const double4* Xi;              // reference coordinates, internal order
+  const int32_t* perm;            // internal -> user node id
+  const int32_t* inc_ptr;         // N+1, internal order
+  const uint32_t* inc_elem;       // packed (element | local << 28)
+  const uint32_t* inc_rank;       // slot ranks, rank_words per incidence
+  int32_t rank_words;             // 1 (Tet4) or 3 (Tet10)
+  const int32_t* kofs;            // N: first CSR value of the node's 3 rows
+  const int32_t* deg;             // N: neighbours of the node
+  const int32_t* uoff;            // N+1: prefix of deg in internal order
+  const int32_t* tile_node;       // n_tiles+1
+  double4* ui;                    // scratch: displacement in internal order
+};
+
+}  // namespace psx
+
+struct psx_plan {
+  int device = 0;
+  psx::HostPattern hp;
+  std::vector<int32_t> perm_h;     // internal -> user
+  std::vector<int32_t> inc_ptr_h;  // internal order
+  std::vector<int32_t> deg_h;      // internal order
+  int32_t max_deg = 0;
+  int64_t n_inc = 0;
+  int64_t device_bytes = 0;
+  psx::DevPlan d{};
+  std::vector<void*> allocations;
+};
+
+namespace psx {
+// Kernel launchers (defined in the .hip files).
+int32_t launch_tet4_assemble(const DevPlan& p, const double* u, const double* mu, const double* lam,
+                             double* f, double* kval, int precision, hipStream_t s, int32_t* status);
+int32_t launch_tet10_assemble(const DevPlan& p, const double* u, const double* mu, const double* lam,
+                              double* f, double* kval, int precision, hipStream_t s, int32_t* status);
+int32_t launch_element_batch(int npe, int64_t n, const double* X, const double* u, const double* mu,
+                             const double* lam, double* k, double* f, int precision, hipStream_t s,
+                             int32_t* status);
+int32_t launch_material_batch(int64_t n, const double* F, const double* mu, const double* bulk,
+                              double* S, double* C, int precision, hipStream_t s, int32_t* status);
+}  // namespace psx

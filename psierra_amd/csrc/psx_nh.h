@@ -1,0 +1,183 @@
+// Device arithmetic for the compressible Neo-Hookean Total Lagrangian path.
+//
+// Constitutive law (src/psierra/material/neohookean.py:13-15, 55-122):
+//   Psi = mu/2 (J^-2/3 I1 - 3) + K/2 (ln J)^2,  K = lambda + 2 mu / 3,
+//   S   = f (I - I1/3 C^-1) + K ln J C^-1,      f = mu J^-2/3,
+//   C4  = a C^-1 (x) C^-1 + b I_{C^-1} + c (C^-1 (x) I + I (x) C^-1),
+//   a = K + 2/9 f I1,  b = -2 K ln J + 2/3 f I1,  c = -2/3 f,
+//   I_{C^-1}_ijkl = (C^-1_ik C^-1_jl + C^-1_il C^-1_jk) / 2.
+//
+// The element tangent B^T C B V + G^T (I3 x S) G V
+// (src/psierra/element/tet4_total_lagrangian.py:127-131) is evaluated in
+// closed form per node pair (a, b).  With h = grad_X N, g = F^-T h (the
+// spatial gradient) and q = F h, the contraction of C4 with F h_a and F h_b
+// collapses, because F C^-1 F^T = I, to
+//   K_ab = V [ a g_a g_b^T + b/2 ((g_a . g_b) I + g_b g_a^T)
+//            + c (g_a q_b^T + q_a g_b^T) + (h_a . S h_b) I ],
+// and f_a = V F S h_a (= B^T S_voigt V, tet4_total_lagrangian.py:136-137).
+//
+// Small-strain accuracy: everything is formed from A = grad u, never from
+// F - I: J - 1 = tr A + ((tr A)^2 - tr(A^2))/2 + det A, ln J = log1p(J - 1),
+// E = (A + A^T + A^T A)/2 and S = C^-1 (2 f dev E + K ln J I) (which equals
+// the form above because I - I1/3 C^-1 = 2 C^-1 dev E).  This keeps FP32
+// element arithmetic within a few ulps of the stress itself even when the
+// strain is small.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace psx {
+
+template <typename R>
+__device__ __forceinline__ R d_log1p(R x);
+template <>
+__device__ __forceinline__ float d_log1p<float>(float x) { return log1pf(x); }
+template <>
+__device__ __forceinline__ double d_log1p<double>(double x) { return log1p(x); }
+
+template <typename R>
+__device__ __forceinline__ R d_exp(R x);
+template <>
+__device__ __forceinline__ float d_exp<float>(float x) { return expf(x); }
+template <>
+__device__ __forceinline__ double d_exp<double>(double x) { return exp(x); }
+
+template <typename R>
+__device__ __forceinline__ R det3(const R M[3][3]) {
+  return M[0][0] * (M[1][1] * M[2][2] - M[1][2] * M[2][1]) -
+         M[0][1] * (M[1][0] * M[2][2] - M[1][2] * M[2][0]) +
+         M[0][2] * (M[1][0] * M[2][1] - M[1][1] * M[2][0]);
+}
+
+// adj(M): M^-1 = adj(M) / det(M).
+template <typename R>
+__device__ __forceinline__ void adj3(const R M[3][3], R o[3][3]) {
+  o[0][0] = M[1][1] * M[2][2] - M[1][2] * M[2][1];
+  o[0][1] = M[0][2] * M[2][1] - M[0][1] * M[2][2];
+  o[0][2] = M[0][1] * M[1][2] - M[0][2] * M[1][1];
+  o[1][0] = M[1][2] * M[2][0] - M[1][0] * M[2][2];
+  o[1][1] = M[0][0] * M[2][2] - M[0][2] * M[2][0];
+  o[1][2] = M[0][2] * M[1][0] - M[0][0] * M[1][2];
+  o[2][0] = M[1][0] * M[2][1] - M[1][1] * M[2][0];
+  o[2][1] = M[0][1] * M[2][0] - M[0][0] * M[2][1];
+  o[2][2] = M[0][0] * M[1][1] - M[0][1] * M[1][0];
+}
+
+// Material state at one integration point.
+template <typename R>
+struct NHPoint {
+  R F[3][3];   // deformation gradient I + A
+  R Gm[3][3];  // F^-T
+  R Ci[3][3];  // C^-1
+  R S[3][3];   // 2nd Piola-Kirchhoff stress (symmetric)
+  R al, be, ga;
+  R J, lnJ;
+};
+
+// A = grad_X u (A[i][j] = du_i/dX_j); mu, K = shear and bulk modulus.
+template <typename R>
+__device__ __forceinline__ void nh_point(const R A[3][3], R mu, R K, NHPoint<R>& o) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) o.F[i][j] = A[i][j] + (i == j ? R(1) : R(0));
+  const R trA = A[0][0] + A[1][1] + A[2][2];
+  R trA2 = R(0);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) trA2 += A[i][j] * A[j][i];
+  const R Jm1 = trA + R(0.5) * (trA * trA - trA2) + det3(A);
+  o.J = R(1) + Jm1;
+  o.lnJ = d_log1p(Jm1);
+  R adj[3][3];
+  adj3(o.F, adj);
+  const R iJ = R(1) / o.J;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) o.Gm[i][j] = adj[j][i] * iJ;  // F^-T = adj(F)^T / J
+  // C^-1 = F^-1 F^-T = Gm^T Gm
+#pragma unroll
+  for (int I = 0; I < 3; ++I)
+#pragma unroll
+    for (int L = I; L < 3; ++L) {
+      R s = R(0);
+#pragma unroll
+      for (int i = 0; i < 3; ++i) s += o.Gm[i][I] * o.Gm[i][L];
+      o.Ci[I][L] = s;
+      o.Ci[L][I] = s;
+    }
+  // Green-Lagrange strain from A only
+  R E[3][3];
+#pragma unroll
+  for (int I = 0; I < 3; ++I)
+#pragma unroll
+    for (int L = I; L < 3; ++L) {
+      R s = A[I][L] + A[L][I];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) s += A[k][I] * A[k][L];
+      E[I][L] = R(0.5) * s;
+      E[L][I] = E[I][L];
+    }
+  const R trE = E[0][0] + E[1][1] + E[2][2];
+  const R fac = mu * d_exp(R(-2.0 / 3.0) * o.lnJ);
+  const R I1 = R(3) + R(2) * trE;
+  // Sbar = 2 f dev(E) + K lnJ I ;  S = sym(C^-1 Sbar)
+  R Sb[3][3];
+#pragma unroll
+  for (int I = 0; I < 3; ++I)
+#pragma unroll
+    for (int L = 0; L < 3; ++L)
+      Sb[I][L] = R(2) * fac * (E[I][L] - (I == L ? trE * R(1.0 / 3.0) : R(0))) +
+                 (I == L ? K * o.lnJ : R(0));
+  R M[3][3];
+#pragma unroll
+  for (int I = 0; I < 3; ++I)
+#pragma unroll
+    for (int L = 0; L < 3; ++L) {
+      R s = R(0);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) s += o.Ci[I][k] * Sb[k][L];
+      M[I][L] = s;
+    }
+#pragma unroll
+  for (int I = 0; I < 3; ++I)
+#pragma unroll
+    for (int L = 0; L < 3; ++L) o.S[I][L] = R(0.5) * (M[I][L] + M[L][I]);
+  o.al = K + R(2.0 / 9.0) * fac * I1;
+  o.be = R(-2) * K * o.lnJ + R(2.0 / 3.0) * fac * I1;
+  o.ga = R(-2.0 / 3.0) * fac;
+}
+
+template <typename R>
+__device__ __forceinline__ void mat3vec(const R M[3][3], const R v[3], R o[3]) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i) o[i] = M[i][0] * v[0] + M[i][1] * v[1] + M[i][2] * v[2];
+}
+
+// One 3x3 tangent block K_ab (row-major, scaled by the weight w = V or
+// w_q det J) and nothing else.  ga_, qa_ belong to the row node; gb, qb to
+// the column node; Sha = S h_a; hb = h_b.
+template <typename R>
+__device__ __forceinline__ void nh_block(const NHPoint<R>& m, R w, const R ga_[3], const R qa_[3],
+                                         const R Sha[3], const R gb[3], const R qb[3],
+                                         const R hb[3], R out[9]) {
+  R P[3], Rv[3], T[3];
+  const R hb2 = R(0.5) * m.be * w;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    P[i] = w * (m.al * ga_[i] + m.ga * qa_[i]);
+    Rv[i] = hb2 * ga_[i];
+    T[i] = w * m.ga * ga_[i];
+  }
+  const R d = hb2 * (ga_[0] * gb[0] + ga_[1] * gb[1] + ga_[2] * gb[2]) +
+              w * (Sha[0] * hb[0] + Sha[1] * hb[1] + Sha[2] * hb[2]);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      out[3 * i + k] = P[i] * gb[k] + gb[i] * Rv[k] + T[i] * qb[k] + (i == k ? d : R(0));
+}
+
+}  // namespace psx

@@ -1,0 +1,528 @@
+// Host side of the engine: error state, structural pattern, plan construction.
+//
+// The structural pattern is the CSR that pSierra obtains from
+// create_coo_sparsity_indices + coo_matrix(...).tocsr()
+// (src/psierra/utility/assembly.py:6-34, 75-76).  It is built here from the
+// node adjacency instead of sorting E * nde^2 triplets: node a's three rows
+// hold, for every node b sharing an element with a (sorted by id), the three
+// columns 3b, 3b+1, 3b+2.  Hence
+//   indptr[3a + r] = 9 * adj_ptr[a] + 3 * r * deg(a),
+//   value(3a + r, 3 b_s + c) = kofs(a) + 3 r deg(a) + 3 s + c,
+// with b_s the s-th neighbour of a ("slot rank" s).
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <thread>
+
+#include "psx_internal.h"
+
+namespace psx {
+
+namespace {
+thread_local std::string g_error;
+
+unsigned worker_count() {
+  unsigned n = std::thread::hardware_concurrency();
+  if (n == 0) n = 1;
+  return std::min(n, 16u);
+}
+
+template <typename Fn>
+void parallel_for(int64_t n, Fn fn) {
+  unsigned T = worker_count();
+  if (n < 4096 || T == 1) {
+    fn(int64_t(0), n);
+    return;
+  }
+  std::vector<std::thread> th;
+  int64_t chunk = (n + T - 1) / T;
+  for (unsigned t = 0; t < T; ++t) {
+    int64_t b = t * chunk, e = std::min<int64_t>(n, b + chunk);
+    if (b >= e) break;
+    th.emplace_back([=] { fn(b, e); });
+  }
+  for (auto& x : th) x.join();
+}
+
+std::string fmt_e(double v) {
+  char buf[64];
+  std::snprintf(buf, sizeof buf, "%.4e", v);
+  return buf;
+}
+}  // namespace
+
+void set_error(const std::string& msg) { g_error = msg; }
+void clear_error() { g_error.clear(); }
+int32_t fail(int32_t code, const std::string& msg) {
+  g_error = msg;
+  return code;
+}
+
+int32_t build_host_pattern(const int64_t* conn, int64_t E, int npe, int64_t N, HostPattern& hp) {
+  if (!conn && E > 0) return fail(PSX_EINVAL, "conn is null");
+  if (npe != 4 && npe != 10) return fail(PSX_EINVAL, "npe must be 4 or 10");
+  if (E < 0 || N <= 0) return fail(PSX_EINVAL, "need n_elements >= 0 and n_nodes > 0");
+  if (E >= (int64_t(1) << kIncElemBits))
+    return fail(PSX_EUNSUPPORTED, "more than 2^28 elements");
+  if (N * 3 >= (int64_t(1) << 31) || E * npe >= (int64_t(1) << 31))
+    return fail(PSX_EUNSUPPORTED, "mesh too large for 32-bit indices");
+  for (int64_t i = 0; i < E * npe; ++i)
+    if (conn[i] < 0 || conn[i] >= N)
+      return fail(PSX_EINVAL, "connectivity entry " + std::to_string(i) + " = " +
+                                  std::to_string(conn[i]) + " outside [0, n_nodes)");
+  hp.E = E;
+  hp.N = N;
+  hp.npe = npe;
+  // node -> (element, local) incidence, in element order
+  hp.n2e_ptr.assign(N + 1, 0);
+  for (int64_t i = 0; i < E * npe; ++i) hp.n2e_ptr[conn[i] + 1]++;
+  for (int64_t a = 0; a < N; ++a) hp.n2e_ptr[a + 1] += hp.n2e_ptr[a];
+  hp.n2e.resize(E * npe);
+  {
+    std::vector<int32_t> fillp(hp.n2e_ptr.begin(), hp.n2e_ptr.end() - 1);
+    for (int64_t e = 0; e < E; ++e)
+      for (int p = 0; p < npe; ++p) hp.n2e[fillp[conn[e * npe + p]]++] = int32_t(e * npe + p);
+  }
+  // adjacency: sorted unique nodes of the incident elements
+  std::vector<int32_t> deg(N);
+  auto collect = [&](int64_t a, std::vector<int32_t>& buf) {
+    buf.clear();
+    for (int32_t k = hp.n2e_ptr[a]; k < hp.n2e_ptr[a + 1]; ++k) {
+      int64_t e = hp.n2e[k] / npe;
+      for (int q = 0; q < npe; ++q) buf.push_back(int32_t(conn[e * npe + q]));
+    }
+    std::sort(buf.begin(), buf.end());
+    buf.erase(std::unique(buf.begin(), buf.end()), buf.end());
+  };
+  parallel_for(N, [&](int64_t b, int64_t e) {
+    std::vector<int32_t> buf;
+    for (int64_t a = b; a < e; ++a) {
+      collect(a, buf);
+      deg[a] = int32_t(buf.size());
+    }
+  });
+  hp.adj_ptr.assign(N + 1, 0);
+  for (int64_t a = 0; a < N; ++a) hp.adj_ptr[a + 1] = hp.adj_ptr[a] + deg[a];
+  if (int64_t(hp.adj_ptr[N]) * 9 >= (int64_t(1) << 31))
+    return fail(PSX_EUNSUPPORTED, "nnz >= 2^31 is not supported (int32 CSR)");
+  hp.adj.resize(hp.adj_ptr[N]);
+  parallel_for(N, [&](int64_t b, int64_t e) {
+    std::vector<int32_t> buf;
+    for (int64_t a = b; a < e; ++a) {
+      collect(a, buf);
+      std::copy(buf.begin(), buf.end(), hp.adj.begin() + hp.adj_ptr[a]);
+    }
+  });
+  return PSX_OK;
+}
+
+void fill_csr(const HostPattern& hp, int32_t* indptr, int32_t* indices) {
+  const int64_t N = hp.N;
+  parallel_for(N, [&](int64_t b, int64_t e) {
+    for (int64_t a = b; a < e; ++a) {
+      const int32_t d = hp.adj_ptr[a + 1] - hp.adj_ptr[a];
+      const int64_t base = 9 * int64_t(hp.adj_ptr[a]);
+      for (int r = 0; r < 3; ++r) {
+        indptr[3 * a + r] = int32_t(base + 3 * r * d);
+        int32_t* out = indices + base + 3 * r * d;
+        for (int s = 0; s < d; ++s)
+          for (int c = 0; c < 3; ++c) out[3 * s + c] = 3 * hp.adj[hp.adj_ptr[a] + s] + c;
+      }
+    }
+  });
+  indptr[3 * N] = int32_t(hp.nnz());
+}
+
+namespace {
+
+// Morton (Z-order) key of a point quantised to 21 bits per axis.
+uint64_t spread3(uint64_t v) {
+  v &= 0x1fffff;
+  v = (v | v << 32) & 0x1f00000000ffffULL;
+  v = (v | v << 16) & 0x1f0000ff0000ffULL;
+  v = (v | v << 8) & 0x100f00f00f00f00fULL;
+  v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
+  v = (v | v << 2) & 0x1249249249249249ULL;
+  return v;
+}
+
+std::vector<int32_t> locality_order(const double* X, int64_t N) {
+  double lo[3] = {X[0], X[1], X[2]}, hi[3] = {X[0], X[1], X[2]};
+  for (int64_t i = 0; i < N; ++i)
+    for (int c = 0; c < 3; ++c) {
+      lo[c] = std::min(lo[c], X[3 * i + c]);
+      hi[c] = std::max(hi[c], X[3 * i + c]);
+    }
+  double ext = std::max({hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2], 1e-300});
+  std::vector<std::pair<uint64_t, int32_t>> key(N);
+  parallel_for(N, [&](int64_t b, int64_t e) {
+    for (int64_t i = b; i < e; ++i) {
+      uint64_t k = 0;
+      for (int c = 0; c < 3; ++c) {
+        double t = (X[3 * i + c] - lo[c]) / ext;
+        uint64_t q = uint64_t(std::min(std::max(t, 0.0), 1.0) * double((1 << 21) - 1));
+        k |= spread3(q) << c;
+      }
+      key[i] = {k, int32_t(i)};
+    }
+  });
+  std::sort(key.begin(), key.end());
+  std::vector<int32_t> perm(N);
+  for (int64_t i = 0; i < N; ++i) perm[i] = key[i].second;
+  return perm;
+}
+
+double det3(const double a[3][3]) {
+  return a[0][0] * (a[1][1] * a[2][2] - a[1][2] * a[2][1]) -
+         a[0][1] * (a[1][0] * a[2][2] - a[1][2] * a[2][0]) +
+         a[0][2] * (a[1][0] * a[2][1] - a[1][1] * a[2][0]);
+}
+
+// Tet10 reference shape-function gradients (src/psierra/element/tet10_base.py:117-179).
+void tet10_dN(double xi, double eta, double zeta, double dN[10][3]) {
+  double L1 = xi, L2 = eta, L3 = zeta, L0 = 1.0 - L1 - L2 - L3;
+  std::memset(dN, 0, sizeof(double) * 30);
+  for (int c = 0; c < 3; ++c) dN[0][c] = -(4.0 * L0 - 1.0);
+  dN[1][0] = 4.0 * L1 - 1.0;
+  dN[2][1] = 4.0 * L2 - 1.0;
+  dN[3][2] = 4.0 * L3 - 1.0;
+  dN[4][0] = 4.0 * (L0 - L1); dN[4][1] = -4.0 * L1; dN[4][2] = -4.0 * L1;
+  dN[5][0] = 4.0 * L2; dN[5][1] = 4.0 * L1;
+  dN[6][0] = -4.0 * L2; dN[6][1] = 4.0 * (L0 - L2); dN[6][2] = -4.0 * L2;
+  dN[7][0] = -4.0 * L3; dN[7][1] = -4.0 * L3; dN[7][2] = 4.0 * (L0 - L3);
+  dN[8][0] = 4.0 * L3; dN[8][2] = 4.0 * L1;
+  dN[9][1] = 4.0 * L3; dN[9][2] = 4.0 * L2;
+}
+
+int32_t check_volumes(const int64_t* conn, int64_t E, int npe, const double* X) {
+  if (npe == 4) {
+    for (int64_t e = 0; e < E; ++e) {
+      const int64_t* c = conn + 4 * e;
+      double a[3][3];
+      for (int b = 0; b < 3; ++b)
+        for (int r = 0; r < 3; ++r) a[b][r] = X[3 * c[b + 1] + r] - X[3 * c[0] + r];
+      double d = det3(a);
+      if (!(d > 0.0))
+        return fail(PSX_EVOLUME, "Negative or zero element volume detected (det(A) = " + fmt_e(d) +
+                                     ") at element " + std::to_string(e) +
+                                     ". Check element vertex ordering.");
+    }
+    return PSX_OK;
+  }
+  const double s5 = std::sqrt(5.0);
+  const double qa = (5.0 - s5) / 20.0, qb = (5.0 + 3.0 * s5) / 20.0;
+  const double qp[4][3] = {{qa, qa, qa}, {qb, qa, qa}, {qa, qb, qa}, {qa, qa, qb}};
+  double dN[4][10][3];
+  for (int q = 0; q < 4; ++q) tet10_dN(qp[q][0], qp[q][1], qp[q][2], dN[q]);
+  for (int64_t e = 0; e < E; ++e) {
+    const int64_t* c = conn + 10 * e;
+    for (int q = 0; q < 4; ++q) {
+      double J[3][3] = {};
+      for (int a = 0; a < 10; ++a)
+        for (int i = 0; i < 3; ++i)
+          for (int j = 0; j < 3; ++j) J[i][j] += X[3 * c[a] + i] * dN[q][a][j];
+      double d = det3(J);
+      if (!(d > 0.0))
+        return fail(PSX_EVOLUME, "Negative or zero Jacobian determinant: det(J) = " + fmt_e(d) +
+                                     " at element " + std::to_string(e) + ".");
+    }
+  }
+  return PSX_OK;
+}
+
+template <typename T>
+int32_t upload(psx_plan* plan, const T* host, size_t count, T** dev) {
+  size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+  void* p = nullptr;
+  PSX_HIP_TRY(hipMalloc(&p, bytes));
+  plan->allocations.push_back(p);
+  plan->device_bytes += int64_t(bytes);
+  if (count) PSX_HIP_TRY(hipMemcpy(p, host, count * sizeof(T), hipMemcpyHostToDevice));
+  *dev = static_cast<T*>(p);
+  return PSX_OK;
+}
+
+int32_t make_tiles(psx_plan* plan, int32_t tile) {
+  if (tile == 0) tile = plan->hp.npe == 4 ? 256 : 64;
+  if (tile != 64 && tile != 128 && tile != 256)
+    return fail(PSX_EINVAL, "tile must be 64, 128 or 256");
+  const int64_t N = plan->hp.N;
+  // Greedy packing of consecutive nodes: at most `tile` incidences and at
+  // most `tile` nodes per tile (nodes without elements count as nodes).
+  std::vector<int32_t> tiles{0};
+  int64_t start_inc = 0, start_node = 0;
+  for (int64_t i = 0; i < N; ++i) {
+    int64_t cnt = plan->inc_ptr_h[i + 1] - plan->inc_ptr_h[i];
+    if (cnt > tile)
+      return fail(PSX_EUNSUPPORTED, "node with " + std::to_string(cnt) +
+                                        " incident elements exceeds the tile of " +
+                                        std::to_string(tile));
+    if (plan->inc_ptr_h[i + 1] - start_inc > tile || i + 1 - start_node > tile) {
+      tiles.push_back(int32_t(i));
+      start_inc = plan->inc_ptr_h[i];
+      start_node = i;
+    }
+  }
+  tiles.push_back(int32_t(N));
+  if (tiles.size() >= 2 && tiles[tiles.size() - 2] == tiles.back()) tiles.pop_back();
+  int32_t* dtiles = nullptr;
+  int32_t rc = upload(plan, tiles.data(), tiles.size(), &dtiles);
+  if (rc) return rc;
+  plan->d.tile_node = dtiles;
+  plan->d.n_tiles = int32_t(tiles.size() - 1);
+  plan->d.tile = tile;
+  return PSX_OK;
+}
+
+}  // namespace
+
+int32_t create_plan(const int64_t* conn, int64_t E, int npe, const double* X, int64_t N, int32_t tile,
+                    psx_plan** out) {
+  if (!out) return fail(PSX_EINVAL, "out is null");
+  *out = nullptr;
+  if (!X) return fail(PSX_EINVAL, "X is null");
+  if (E <= 0) return fail(PSX_EINVAL, "plan needs at least one element");
+  auto* plan = new psx_plan();
+  auto bail = [&](int32_t rc) {
+    for (void* p : plan->allocations) (void)hipFree(p);
+    delete plan;
+    return rc;
+  };
+  int32_t rc = build_host_pattern(conn, E, npe, N, plan->hp);
+  if (rc) return bail(rc);
+  rc = check_volumes(conn, E, npe, X);
+  if (rc) return bail(rc);
+  if (hipGetDevice(&plan->device) != hipSuccess)
+    return bail(fail(PSX_ENOTREADY, "no HIP device available"));
+
+  const HostPattern& hp = plan->hp;
+  // locality order of the nodes (internal id i -> user id perm[i])
+  plan->perm_h = locality_order(X, N);
+  std::vector<int32_t> inv(N);
+  for (int64_t i = 0; i < N; ++i) inv[plan->perm_h[i]] = int32_t(i);
+
+  // internal connectivity (AoS, 4 ints for Tet4, 12 for Tet10)
+  const int cs = npe == 4 ? 4 : 12;
+  std::vector<int32_t> conn_i(size_t(E) * cs, 0);
+  parallel_for(E, [&](int64_t b, int64_t e) {
+    for (int64_t k = b; k < e; ++k)
+      for (int p = 0; p < npe; ++p) conn_i[k * cs + p] = inv[conn[k * npe + p]];
+  });
+  std::vector<double4> Xi(N);
+  for (int64_t i = 0; i < N; ++i) {
+    int64_t a = plan->perm_h[i];
+    Xi[i] = make_double4(X[3 * a], X[3 * a + 1], X[3 * a + 2], 0.0);
+  }
+
+  // incidences in internal node order, with slot ranks
+  const int rw = npe == 4 ? 1 : 3;
+  plan->inc_ptr_h.assign(N + 1, 0);
+  plan->deg_h.assign(N, 0);
+  std::vector<int32_t> kofs(N), uoff(N + 1, 0);
+  for (int64_t i = 0; i < N; ++i) {
+    int64_t a = plan->perm_h[i];
+    plan->inc_ptr_h[i + 1] = plan->inc_ptr_h[i] + (hp.n2e_ptr[a + 1] - hp.n2e_ptr[a]);
+    int32_t d = hp.adj_ptr[a + 1] - hp.adj_ptr[a];
+    plan->deg_h[i] = d;
+    plan->max_deg = std::max(plan->max_deg, d);
+    kofs[i] = 9 * hp.adj_ptr[a];
+    uoff[i + 1] = uoff[i] + d;
+  }
+  if (plan->max_deg > 255)
+    return bail(fail(PSX_EUNSUPPORTED, "node degree " + std::to_string(plan->max_deg) +
+                                           " above 255 is not supported"));
+  plan->n_inc = plan->inc_ptr_h[N];
+  std::vector<uint32_t> inc_elem(plan->n_inc), inc_rank(plan->n_inc * rw, 0);
+  parallel_for(N, [&](int64_t b, int64_t e) {
+    for (int64_t i = b; i < e; ++i) {
+      int64_t a = plan->perm_h[i];
+      const int32_t* nb = hp.adj.data() + hp.adj_ptr[a];
+      const int32_t d = hp.adj_ptr[a + 1] - hp.adj_ptr[a];
+      int64_t k = plan->inc_ptr_h[i];
+      for (int32_t t = hp.n2e_ptr[a]; t < hp.n2e_ptr[a + 1]; ++t, ++k) {
+        int64_t el = hp.n2e[t] / npe;
+        int p = hp.n2e[t] % npe;
+        inc_elem[k] = uint32_t(el) | (uint32_t(p) << kIncElemBits);
+        auto rank_of = [&](int64_t node) -> uint32_t {
+          return uint32_t(std::lower_bound(nb, nb + d, int32_t(node)) - nb);
+        };
+        if (npe == 4) {
+          // Tet4 incidences are evaluated with the row node first, then the
+          // other three in increasing local order (see tet4 kernel).
+          uint32_t r = rank_of(conn[el * 4 + p]);
+          int slot = 1;
+          for (int q = 0; q < 4; ++q) {
+            if (q == p) continue;
+            r |= rank_of(conn[el * 4 + q]) << (8 * slot);
+            ++slot;
+          }
+          inc_rank[k] = r;
+        } else {
+          for (int q = 0; q < 10; ++q)
+            inc_rank[k * rw + q / 4] |= rank_of(conn[el * 10 + q]) << (8 * (q % 4));
+        }
+      }
+    }
+  });
+
+  psx::DevPlan& d = plan->d;
+  d.E = int32_t(E);
+  d.N = int32_t(N);
+  d.npe = npe;
+  d.conn_stride = cs;
+  d.rank_words = rw;
+  int32_t *dconn, *dperm, *dincp, *dkofs, *ddeg, *duoff;
+  uint32_t *dince, *dincr;
+  double4* dX;
+  if ((rc = upload(plan, conn_i.data(), conn_i.size(), &dconn)) ||
+      (rc = upload(plan, Xi.data(), Xi.size(), &dX)) ||
+      (rc = upload(plan, plan->perm_h.data(), plan->perm_h.size(), &dperm)) ||
+      (rc = upload(plan, plan->inc_ptr_h.data(), plan->inc_ptr_h.size(), &dincp)) ||
+      (rc = upload(plan, inc_elem.data(), inc_elem.size(), &dince)) ||
+      (rc = upload(plan, inc_rank.data(), inc_rank.size(), &dincr)) ||
+      (rc = upload(plan, kofs.data(), kofs.size(), &dkofs)) ||
+      (rc = upload(plan, plan->deg_h.data(), plan->deg_h.size(), &ddeg)) ||
+      (rc = upload(plan, uoff.data(), uoff.size(), &duoff)))
+    return bail(rc);
+  d.conn = dconn;
+  d.Xi = dX;
+  d.perm = dperm;
+  d.inc_ptr = dincp;
+  d.inc_elem = dince;
+  d.inc_rank = dincr;
+  d.kofs = dkofs;
+  d.deg = ddeg;
+  d.uoff = duoff;
+  void* ui = nullptr;
+  if (hipMalloc(&ui, size_t(N) * sizeof(double4)) != hipSuccess)
+    return bail(fail(PSX_EINVAL, "hipMalloc of the displacement scratch failed"));
+  plan->allocations.push_back(ui);
+  plan->device_bytes += int64_t(N) * int64_t(sizeof(double4));
+  d.ui = static_cast<double4*>(ui);
+  if ((rc = make_tiles(plan, tile))) return bail(rc);
+  *out = plan;
+  return PSX_OK;
+}
+
+int32_t retile(psx_plan* plan, int32_t tile) {
+  // The old tile array stays allocated until the plan is destroyed.
+  return make_tiles(plan, tile);
+}
+
+}  // namespace psx
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int32_t psx_abi_version(void) { return 1; }
+
+const char* psx_last_error(void) { return psx::g_error.c_str(); }
+
+int32_t psx_pattern_nnz(const int64_t* conn, int64_t n_elements, int32_t npe, int64_t n_nodes,
+                        int64_t* nnz) {
+  psx::clear_error();
+  if (!nnz) return psx::fail(PSX_EINVAL, "nnz is null");
+  psx::HostPattern hp;
+  int32_t rc = psx::build_host_pattern(conn, n_elements, npe, n_nodes, hp);
+  if (rc) return rc;
+  *nnz = hp.nnz();
+  return PSX_OK;
+}
+
+int32_t psx_pattern_build(const int64_t* conn, int64_t n_elements, int32_t npe, int64_t n_nodes,
+                          int32_t* indptr, int32_t* indices) {
+  psx::clear_error();
+  if (!indptr || !indices) return psx::fail(PSX_EINVAL, "indptr/indices is null");
+  psx::HostPattern hp;
+  int32_t rc = psx::build_host_pattern(conn, n_elements, npe, n_nodes, hp);
+  if (rc) return rc;
+  psx::fill_csr(hp, indptr, indices);
+  return PSX_OK;
+}
+
+int32_t psx_plan_create(const int64_t* conn, int64_t n_elements, int32_t npe, const double* X,
+                        int64_t n_nodes, int32_t tile, psx_plan** out) {
+  psx::clear_error();
+  return psx::create_plan(conn, n_elements, npe, X, n_nodes, tile, out);
+}
+
+int32_t psx_plan_destroy(psx_plan* plan) {
+  psx::clear_error();
+  if (!plan) return PSX_OK;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  (void)hipSetDevice(plan->device);
+  for (void* p : plan->allocations) (void)hipFree(p);
+  (void)hipSetDevice(dev);
+  delete plan;
+  return PSX_OK;
+}
+
+int32_t psx_plan_info(const psx_plan* plan, int64_t* info, int32_t n_info) {
+  psx::clear_error();
+  if (!plan || !info) return psx::fail(PSX_EINVAL, "plan/info is null");
+  int64_t v[PSX_INFO_COUNT] = {plan->hp.E,     plan->hp.N,      plan->hp.npe,
+                               plan->hp.nnz(), plan->d.n_tiles, plan->d.tile,
+                               plan->max_deg,  plan->device_bytes, plan->n_inc};
+  for (int32_t i = 0; i < n_info && i < PSX_INFO_COUNT; ++i) info[i] = v[i];
+  return PSX_OK;
+}
+
+int32_t psx_plan_pattern(const psx_plan* plan, int32_t* indptr, int32_t* indices) {
+  psx::clear_error();
+  if (!plan || !indptr || !indices) return psx::fail(PSX_EINVAL, "null argument");
+  psx::fill_csr(plan->hp, indptr, indices);
+  return PSX_OK;
+}
+
+int32_t psx_plan_set_tile(psx_plan* plan, int32_t tile) {
+  psx::clear_error();
+  if (!plan) return psx::fail(PSX_EINVAL, "plan is null");
+  return psx::retile(plan, tile);
+}
+
+int32_t psx_assemble(psx_plan* plan, const double* u, const double* mu, const double* lam, double* f,
+                     double* kval, int32_t precision, void* stream, int32_t* status) {
+  psx::clear_error();
+  if (!plan || !u || !mu || !lam || !f || !status)
+    return psx::fail(PSX_EINVAL, "null argument to psx_assemble");
+  if (precision != PSX_FP64 && precision != PSX_FP32)
+    return psx::fail(PSX_EINVAL, "precision must be PSX_FP64 or PSX_FP32");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (plan->hp.npe == 4)
+    return psx::launch_tet4_assemble(plan->d, u, mu, lam, f, kval, precision, s, status);
+  return psx::launch_tet10_assemble(plan->d, u, mu, lam, f, kval, precision, s, status);
+}
+
+int32_t psx_element_batch(int32_t npe, int64_t n, const double* X, const double* u, const double* mu,
+                          const double* lam, double* k, double* f, int32_t precision, void* stream,
+                          int32_t* status) {
+  psx::clear_error();
+  if (npe != 4 && npe != 10) return psx::fail(PSX_EINVAL, "npe must be 4 or 10");
+  if (n < 0 || n >= (int64_t(1) << 31)) return psx::fail(PSX_EINVAL, "bad element count");
+  if (n == 0) return PSX_OK;
+  if (!X || !u || !mu || !lam || !f || !status) return psx::fail(PSX_EINVAL, "null argument");
+  if (precision != PSX_FP64 && precision != PSX_FP32)
+    return psx::fail(PSX_EINVAL, "precision must be PSX_FP64 or PSX_FP32");
+  return psx::launch_element_batch(npe, n, X, u, mu, lam, k, f, precision,
+                                   static_cast<hipStream_t>(stream), status);
+}
+
+int32_t psx_nh_material_batch(int64_t n, const double* F, const double* mu, const double* bulk,
+                              double* S, double* C, int32_t precision, void* stream,
+                              int32_t* status) {
+  psx::clear_error();
+  if (n < 0 || n >= (int64_t(1) << 31)) return psx::fail(PSX_EINVAL, "bad count");
+  if (n == 0) return PSX_OK;
+  if (!F || !mu || !bulk || !S || !C || !status) return psx::fail(PSX_EINVAL, "null argument");
+  if (precision != PSX_FP64 && precision != PSX_FP32)
+    return psx::fail(PSX_EINVAL, "precision must be PSX_FP64 or PSX_FP32");
+  return psx::launch_material_batch(n, F, mu, bulk, S, C, precision, static_cast<hipStream_t>(stream),
+                                    status);
+}
+
+}  // extern "C"

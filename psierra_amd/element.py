@@ -1,0 +1,137 @@
+"""GPU Total Lagrangian Tet4/Tet10 elements with pSierra's Element interface.
+
+Drop-ins for ``Tet4TotalLagrangian`` (src/psierra/element/tet4_total_lagrangian.py)
+and ``Tet10TotalLagrangian`` (src/psierra/element/tet10_total_lagrangian.py).
+
+pSierra's ``SolidDomain`` hands the whole assembly to an element that has
+``assemble_internal_forces(mesh, displacement)`` and
+``assemble_tangent_stiffness_matrix(mesh, displacement)``
+(src/psierra/domain/solid.py:238-241, 288-291), the hook ``Tet4Fbar`` uses
+(src/psierra/element/tet4_fbar.py:217-459).  These classes implement both
+hooks on the GPU with a plan cached per mesh (keyed on the mesh object, as
+Tet4Fbar keys its geometry cache, tet4_fbar.py:54-57), so an unmodified
+``SolidDomain`` and every analysis above it run the HIP kernels.
+
+The per-element ``compute_nonlinear_force_and_tangent_stiffness`` runs the
+device element-batch kernel on one element (used by code such as the ECSW
+model that calls elements one at a time).
+"""
+from __future__ import annotations
+
+import numpy as np
+import scipy.sparse as spsp
+
+from . import engine
+from .material import lame_arrays
+
+
+class _GpuTotalLagrangian:
+    supports_element_indexing: bool = True
+    _npe = 4
+
+    def __init__(self, material, precision: str = "fp64", tile: int = 0) -> None:
+        engine.precision_code(precision)
+        self._material = material
+        self.precision = precision
+        self.tile = tile
+        self._cache = {}
+
+    # -- Element interface (src/psierra/element/base.py:28-50) --------------
+    @property
+    def material(self):
+        return self._material
+
+    @material.setter
+    def material(self, mat) -> None:
+        self._material = mat
+        for st in self._cache.values():
+            st["params_key"] = None
+
+    @property
+    def nodes_per_element(self) -> int:
+        return self._npe
+
+    @property
+    def dofs_per_node(self) -> int:
+        return 3
+
+    @property
+    def dofs_per_element(self) -> int:
+        return 3 * self._npe
+
+    # -- plan / parameter cache ---------------------------------------------
+    def _state(self, mesh):
+        key = id(mesh)
+        st = self._cache.get(key)
+        if st is None or st["mesh"] is not mesh:
+            elements = np.asarray(mesh.elements, dtype=np.int64)
+            if elements.shape[1] != self._npe:
+                raise ValueError(f"{type(self).__name__} needs {self._npe}-node elements")
+            st = {"mesh": mesh, "plan": engine.AssemblyPlan(mesh.nodes, elements, self.tile), "params_key": None}
+            self._cache = {key: st}
+        pkey = (id(self._material), getattr(self._material, "version", 0))
+        if st["params_key"] != pkey:
+            import torch
+
+            mu, lam = lame_arrays(self._material, st["plan"].n_elements)
+            dev = st["plan"].device
+            st["mu"] = torch.as_tensor(mu, dtype=torch.float64, device=dev)
+            st["lam"] = torch.as_tensor(lam, dtype=torch.float64, device=dev)
+            st["params_key"] = pkey
+        return st
+
+    def plan(self, mesh) -> engine.AssemblyPlan:
+        return self._state(mesh)["plan"]
+
+    def assemble_device(self, mesh, u, want_k: bool = True, f=None, kval=None, check_status: bool = True):
+        """f_int and K values as device tensors (u may be a device tensor)."""
+        st = self._state(mesh)
+        return st["plan"].assemble(
+            u, st["mu"], st["lam"], f=f, kval=kval, precision=self.precision, want_k=want_k,
+            check_status=check_status,
+        )
+
+    # -- SolidDomain hooks (src/psierra/domain/solid.py:238-241, 288-291) ---
+    def assemble_internal_forces(self, mesh, displacement) -> np.ndarray:
+        f, _ = self.assemble_device(mesh, np.asarray(displacement, dtype=np.float64).reshape(-1), want_k=False)
+        return f.cpu().numpy()
+
+    def assemble_tangent_stiffness_matrix(self, mesh, displacement) -> spsp.csr_matrix:
+        plan = self._state(mesh)["plan"]
+        _, k = self.assemble_device(mesh, np.asarray(displacement, dtype=np.float64).reshape(-1), want_k=True)
+        indptr, indices = plan.pattern()
+        n = plan.n_dofs
+        return spsp.csr_matrix((k.cpu().numpy(), indices, indptr), shape=(n, n))
+
+    # -- per-element API ------------------------------------------------------
+    def _element_params(self, elem_index):
+        mat = self._material
+        if getattr(mat, "supports_element_indexing", False) and hasattr(mat, "mu"):
+            e = 0 if elem_index is None else int(elem_index)
+            return float(mat.mu[e]), float(mat.lam[e])
+        mu, lam = lame_arrays(mat, 1)
+        return float(mu[0]), float(lam[0])
+
+    def compute_nonlinear_force_and_tangent_stiffness(self, node_coords, u_nodes, elem_index=None, **kw):
+        quadrature_order = kw.pop("quadrature_order", None)
+        if kw:
+            raise TypeError(f"unexpected arguments {sorted(kw)}")
+        if quadrature_order not in (None, 4):
+            raise ValueError("the GPU Tet10 element implements the 4-point rule only")
+        mu, lam = self._element_params(elem_index)
+        X = np.asarray(node_coords, dtype=np.float64).reshape(1, self._npe, 3)
+        u = np.asarray(u_nodes, dtype=np.float64).reshape(1, self._npe, 3)
+        k, f = engine.element_batch(X, u, [mu], [lam], want_k=True, precision=self.precision)
+        return k[0].cpu().numpy(), f[0].cpu().numpy()
+
+
+class GpuTet4TotalLagrangian(_GpuTotalLagrangian):
+    _npe = 4
+
+
+class GpuTet10TotalLagrangian(_GpuTotalLagrangian):
+    _npe = 10
+
+    @property
+    def default_quadrature_order(self) -> int:
+        return 4

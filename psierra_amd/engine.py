@@ -1,0 +1,264 @@
+"""Host-side engine objects: the assembly plan and stream-ordered launches.
+
+``AssemblyPlan`` owns a ``psx_plan`` (pattern, incidence map, node order,
+reference coordinates on the device) for one mesh.  ``assemble`` evaluates
+f_int and the K_T values into caller-visible torch tensors on the current
+HIP stream.  No work falls back to the CPU: if the library or a GPU is
+missing the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib, ptr
+
+PRECISIONS = {"fp64": _lib.PSX_FP64, "fp32": _lib.PSX_FP32}
+
+INVERSION_MSG = "Element inversion detected: det(F) = {:.4e} <= 0."
+
+
+def _torch():
+    import torch
+
+    if not torch.cuda.is_available():
+        raise RuntimeError("psierra_amd needs a HIP GPU (torch.cuda.is_available() is False)")
+    return torch
+
+
+def current_stream_handle():
+    torch = _torch()
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def precision_code(precision: str) -> int:
+    try:
+        return PRECISIONS[precision]
+    except KeyError:
+        raise ValueError(f"precision must be one of {sorted(PRECISIONS)}, got {precision!r}") from None
+
+
+def pattern_host(elements: np.ndarray, n_nodes: int) -> tuple[np.ndarray, np.ndarray]:
+    """Structural CSR (indptr, indices) of the global matrix, built natively on the host."""
+    conn = np.ascontiguousarray(elements, dtype=np.int64)
+    E, npe = conn.shape
+    nnz = ctypes.c_int64(0)
+    check(lib().psx_pattern_nnz(ptr(conn), E, npe, n_nodes, ctypes.byref(nnz)))
+    indptr = np.empty(3 * n_nodes + 1, dtype=np.int32)
+    indices = np.empty(nnz.value, dtype=np.int32)
+    check(lib().psx_pattern_build(ptr(conn), E, npe, n_nodes, ptr(indptr), ptr(indices)))
+    return indptr, indices
+
+
+def _det3(M):
+    return float(np.linalg.det(M))
+
+
+def inversion_error(nodes: np.ndarray, elements: np.ndarray, u_host_rows, e: int) -> ValueError:
+    """The reference's exception for element e (det F of its first failing point)."""
+    conn = elements[e]
+    X = nodes[conn]
+    u = u_host_rows(conn)
+    if conn.size == 4:
+        D = (X[1:] - X[0]).T
+        H = np.vstack([-np.linalg.inv(D).sum(axis=0), np.linalg.inv(D)])
+        return ValueError(INVERSION_MSG.format(_det3((X + u).T @ H)))
+    from .workloads import _tet10_dN, tet10_quadrature_points
+
+    for qp in tet10_quadrature_points():
+        dN = _tet10_dN(*qp)
+        H = dN @ np.linalg.inv(X.T @ dN)
+        d = _det3((X + u).T @ H)
+        if d <= 0.0:
+            return ValueError(INVERSION_MSG.format(d))
+    return ValueError(INVERSION_MSG.format(float("nan")))
+
+
+class AssemblyPlan:
+    """Device plan for one mesh (pattern + incidence map + node order)."""
+
+    def __init__(self, nodes: np.ndarray, elements: np.ndarray, tile: int = 0):
+        self.nodes = np.ascontiguousarray(nodes, dtype=np.float64)
+        self.elements = np.ascontiguousarray(elements, dtype=np.int64)
+        if self.nodes.ndim != 2 or self.nodes.shape[1] != 3:
+            raise ValueError("nodes must be (N, 3)")
+        torch = _torch()
+        self.device = torch.device("cuda", torch.cuda.current_device())
+        E, npe = self.elements.shape
+        handle = ctypes.c_void_p()
+        check(
+            lib().psx_plan_create(
+                ptr(self.elements), E, npe, ptr(self.nodes), self.nodes.shape[0], tile, ctypes.byref(handle)
+            )
+        )
+        self._h = handle
+        self._indptr = None
+        self._indices = None
+        self.status = torch.zeros(2, dtype=torch.int32, device=self.device)
+        self.refresh_info()
+
+    def refresh_info(self):
+        buf = (ctypes.c_int64 * len(_lib.INFO_KEYS))()
+        check(lib().psx_plan_info(self._h, buf, len(_lib.INFO_KEYS)))
+        self.info = dict(zip(_lib.INFO_KEYS, (int(v) for v in buf)))
+        self.n_elements = self.info["n_elements"]
+        self.n_nodes = self.info["n_nodes"]
+        self.npe = self.info["npe"]
+        self.nnz = self.info["nnz"]
+        self.n_dofs = 3 * self.n_nodes
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                lib().psx_plan_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    def set_tile(self, tile: int) -> None:
+        check(lib().psx_plan_set_tile(self._h, tile))
+        self.refresh_info()
+
+    def pattern(self) -> tuple[np.ndarray, np.ndarray]:
+        if self._indptr is None:
+            indptr = np.empty(self.n_dofs + 1, dtype=np.int32)
+            indices = np.empty(self.nnz, dtype=np.int32)
+            check(lib().psx_plan_pattern(self._h, ptr(indptr), ptr(indices)))
+            self._indptr, self._indices = indptr, indices
+        return self._indptr, self._indices
+
+    def alloc_outputs(self, want_k: bool = True):
+        torch = _torch()
+        f = torch.empty(self.n_dofs, dtype=torch.float64, device=self.device)
+        k = torch.empty(self.nnz, dtype=torch.float64, device=self.device) if want_k else None
+        return f, k
+
+    def assemble(self, u, mu, lam, f=None, kval=None, precision="fp64", want_k=True, check_status=True):
+        """f_int (3N) and K_T values (nnz, CSR order of ``pattern()``) on the device.
+
+        u: (N,3) or (3N,) float64 device tensor; mu, lam: (E,) float64 device
+        tensors.  Launches on the current stream.  With ``check_status`` the
+        call synchronises and raises the reference's ValueError on inversion.
+        """
+        torch = _torch()
+        u = self._device_f64(u, self.n_dofs, "u")
+        mu = self._device_f64(mu, self.n_elements, "mu")
+        lam = self._device_f64(lam, self.n_elements, "lam")
+        if f is None or (want_k and kval is None):
+            f0, k0 = self.alloc_outputs(want_k)
+            f = f if f is not None else f0
+            kval = kval if kval is not None else k0
+        self._check_out(f, self.n_dofs, "f")
+        if want_k:
+            self._check_out(kval, self.nnz, "kval")
+        else:
+            kval = None
+        check(
+            lib().psx_assemble(
+                self._h,
+                ptr(u),
+                ptr(mu),
+                ptr(lam),
+                ptr(f),
+                ptr(kval),
+                precision_code(precision),
+                current_stream_handle(),
+                ptr(self.status),
+            )
+        )
+        if check_status:
+            self.raise_on_status(u)
+        return f, kval
+
+    def raise_on_status(self, u):
+        st = self.status.cpu().numpy()
+        if st[0] != 0:
+            uh = u.reshape(-1, 3)
+            raise inversion_error(
+                self.nodes, self.elements, lambda conn: uh[conn].cpu().numpy(), int(st[1])
+            )
+
+    def _device_f64(self, x, n, name):
+        torch = _torch()
+        if not isinstance(x, torch.Tensor):
+            x = torch.as_tensor(np.asarray(x, dtype=np.float64), device=self.device)
+        if x.device != self.device or x.dtype != torch.float64:
+            x = x.to(device=self.device, dtype=torch.float64)
+        x = x.contiguous()
+        if x.numel() != n:
+            raise ValueError(f"{name} has {x.numel()} entries, expected {n}")
+        return x
+
+    def _check_out(self, t, n, name):
+        torch = _torch()
+        if not isinstance(t, torch.Tensor) or t.dtype != torch.float64 or not t.is_contiguous():
+            raise ValueError(f"{name} must be a contiguous float64 device tensor")
+        if t.device != self.device or t.numel() != n:
+            raise ValueError(f"{name} must have {n} entries on {self.device}")
+
+
+def element_batch(X, u, mu, lam, want_k=True, precision="fp64"):
+    """Per-element k (n, nde, nde) and f (n, nde) on the device (no assembly)."""
+    torch = _torch()
+    dev = torch.device("cuda", torch.cuda.current_device())
+    X = torch.as_tensor(X, dtype=torch.float64, device=dev).contiguous()
+    u = torch.as_tensor(u, dtype=torch.float64, device=dev).contiguous()
+    mu = torch.as_tensor(mu, dtype=torch.float64, device=dev).contiguous().reshape(-1)
+    lam = torch.as_tensor(lam, dtype=torch.float64, device=dev).contiguous().reshape(-1)
+    n, npe, _ = X.shape
+    if u.shape != X.shape or mu.numel() != n or lam.numel() != n:
+        raise ValueError("element_batch: inconsistent shapes")
+    nd = 3 * npe
+    k = torch.empty((n, nd, nd), dtype=torch.float64, device=dev) if want_k else None
+    f = torch.empty((n, nd), dtype=torch.float64, device=dev)
+    status = torch.zeros(2, dtype=torch.int32, device=dev)
+    check(
+        lib().psx_element_batch(
+            npe, n, ptr(X), ptr(u), ptr(mu), ptr(lam), ptr(k), ptr(f), precision_code(precision),
+            current_stream_handle(), ptr(status),
+        )
+    )
+    st = status.cpu().numpy()
+    if st[0] == 2:
+        e = int(st[1])
+        Xe = X[e].cpu().numpy()
+        if npe == 4:
+            A = np.ones((4, 4))
+            A[:, 1:] = Xe
+            raise ValueError(
+                f"Negative or zero element volume detected (det(A) = {np.linalg.det(A)}). "
+                "Check element vertex ordering."
+            )
+        raise ValueError("Negative or zero Jacobian determinant detected in element batch.")
+    if st[0] == 1:
+        e = int(st[1])
+        Xe = X[e].cpu().numpy()
+        ue = u[e].cpu().numpy()
+        raise inversion_error(Xe, np.arange(npe)[None, :], lambda conn: ue[conn], 0)
+    return k, f
+
+
+def material_batch(F, mu, bulk, precision="fp64"):
+    """Neo-Hookean (C_voigt (n,6,6), S (n,3,3)) for deformation gradients F (n,3,3)."""
+    torch = _torch()
+    dev = torch.device("cuda", torch.cuda.current_device())
+    F = torch.as_tensor(F, dtype=torch.float64, device=dev).contiguous()
+    n = F.shape[0]
+    mu = torch.as_tensor(mu, dtype=torch.float64, device=dev).reshape(-1).expand(n).contiguous()
+    bulk = torch.as_tensor(bulk, dtype=torch.float64, device=dev).reshape(-1).expand(n).contiguous()
+    S = torch.empty((n, 3, 3), dtype=torch.float64, device=dev)
+    C = torch.empty((n, 6, 6), dtype=torch.float64, device=dev)
+    status = torch.zeros(2, dtype=torch.int32, device=dev)
+    check(
+        lib().psx_nh_material_batch(
+            n, ptr(F), ptr(mu), ptr(bulk), ptr(S), ptr(C), precision_code(precision),
+            current_stream_handle(), ptr(status),
+        )
+    )
+    st = status.cpu().numpy()
+    if st[0] != 0:
+        raise ValueError(INVERSION_MSG.format(float(np.linalg.det(F[int(st[1])].cpu().numpy()))))
+    return C, S

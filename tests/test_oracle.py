@@ -1,0 +1,341 @@
+"""Pins the FP64 oracle before it is trusted as the checker.
+
+Each check fixes the oracle independently of its own formula: the reference's
+own invariant tests (tests/test_element_tet4_total_lagrangian.py,
+tests/test_element_tet10_total_lagrangian.py, tests/test_material.py:91-166,
+tests/test_element_tet4_linear.py:31-44 in the reference), finite differences
+of the closed-form strain energy, closed-form pattern counts, and planted
+mistakes that the checks must catch.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import assembly as A
+from oracle import fem
+from tests.conftest import rotation_30deg_111
+
+# ---------------------------------------------------------------- material
+
+
+def test_nh_zero_stress_and_hookean_tangent_at_identity(soft_lame):
+    lam, mu = soft_lame
+    C, S = fem.neohookean_stress_and_tangent(np.eye(3), mu, fem.bulk_from_lame(lam, mu))
+    assert np.allclose(S, 0.0, atol=1e-14)
+    assert np.allclose(C, fem.hookean_C(lam, mu), rtol=1e-5)
+
+
+def _fd_tangent_polar(material, F0, eps=1e-7):
+    """C by FD of S w.r.t. E through F = R U(E) (reference tests/test_material.py:106-160)."""
+    C_an, S0 = material(F0)
+    S0v = fem.tensor_to_voigt_stress(S0)
+    Cn = np.zeros((6, 6))
+    E0 = 0.5 * (F0.T @ F0 - np.eye(3))
+    w0, V0 = np.linalg.eigh(F0.T @ F0)
+    R = F0 @ np.linalg.inv(V0 @ np.diag(np.sqrt(w0)) @ V0.T)
+    for j in range(6):
+        r, c = fem.VOIGT_MAP_3D[j]
+        dE = np.zeros((3, 3))
+        if r == c:
+            dE[r, c] = eps
+        else:
+            dE[r, c] = dE[c, r] = 0.5 * eps
+        w, V = np.linalg.eigh(2.0 * (E0 + dE) + np.eye(3))
+        Fp = R @ (V @ np.diag(np.sqrt(w)) @ V.T)
+        Cn[:, j] = (fem.tensor_to_voigt_stress(material(Fp)[1]) - S0v) / eps
+    return C_an, Cn
+
+
+F_MODERATE = np.array([[1.08, 0.03, 0.01], [0.02, 0.96, 0.04], [0.01, 0.02, 1.04]])
+
+
+def test_nh_tangent_matches_fd_of_stress(soft_lame):
+    lam, mu = soft_lame
+    C_an, Cn = _fd_tangent_polar(fem.nh_material(mu, lam), F_MODERATE)
+    assert np.linalg.norm(C_an - Cn) / np.linalg.norm(C_an) < 1e-4
+
+
+def test_nh_stress_is_energy_derivative(soft_lame):
+    """S = 2 dPsi/dC, checked by central differences of the closed-form energy."""
+    lam, mu = soft_lame
+    K = fem.bulk_from_lame(lam, mu)
+    F = F_MODERATE
+    _, S = fem.neohookean_stress_and_tangent(F, mu, K)
+    # dPsi/dF = F S ; FD over the 9 entries of F
+    h = 1e-6
+    P = np.zeros((3, 3))
+    for i in range(3):
+        for j in range(3):
+            Fp, Fm = F.copy(), F.copy()
+            Fp[i, j] += h
+            Fm[i, j] -= h
+            P[i, j] = (fem.neohookean_energy(Fp, mu, K) - fem.neohookean_energy(Fm, mu, K)) / (2 * h)
+    assert np.allclose(F @ S, P, rtol=1e-6, atol=1e-6 * np.abs(P).max())
+
+
+def test_nh_inversion_raises(soft_lame):
+    lam, mu = soft_lame
+    with pytest.raises(ValueError, match="Element inversion detected"):
+        fem.neohookean_stress_and_tangent(np.diag([1.0, 1.0, -0.5]), mu, fem.bulk_from_lame(lam, mu))
+
+
+def test_planted_tangent_mistake_is_caught(soft_lame):
+    """Dropping the C^-1 (x) I terms from C4 must fail the FD check above."""
+    lam, mu = soft_lame
+    K = fem.bulk_from_lame(lam, mu)
+
+    def broken(F):
+        C, S = fem.neohookean_stress_and_tangent(F, mu, K)
+        J = np.linalg.det(F)
+        Ci = np.linalg.inv(F.T @ F)
+        fac = mu * J ** (-2.0 / 3.0)
+        for a, (i, j) in fem.VOIGT_MAP_3D.items():
+            for b, (k, l) in fem.VOIGT_MAP_3D.items():
+                C[a, b] += (2.0 / 3.0) * fac * (Ci[i, j] * (k == l) + (i == j) * Ci[k, l])
+        return C, S
+
+    C_an, Cn = _fd_tangent_polar(broken, F_MODERATE)
+    assert np.linalg.norm(C_an - Cn) / np.linalg.norm(C_an) > 1e-3
+
+
+# ---------------------------------------------------------------- Tet4
+
+
+def test_tet4_volume_and_gradients(regular_tet_coords):
+    V, H = fem.tet4_volume_and_gradients(regular_tet_coords)
+    assert np.isclose(V, 1.0 / 6.0, atol=1e-14)
+    assert np.allclose(H.sum(axis=0), 0.0, atol=1e-14)
+    assert np.allclose(H[1:], np.eye(3), atol=1e-14)
+
+
+def test_tet4_inverted_volume_raises(inverted_tet_coords):
+    with pytest.raises(ValueError, match="volume detected"):
+        fem.tet4_volume_and_gradients(inverted_tet_coords)
+
+
+def _hookean(lam, mu):
+    return lambda F: fem.hookean_stress_and_tangent(F, lam, mu)
+
+
+def test_tet4_translation_invariance(regular_tet_coords, steel_lame):
+    lam, mu = steel_lame
+    u = np.tile([12.5, -34.2, 5.8], (4, 1))
+    _, f = fem.tet4_force_and_tangent(regular_tet_coords, u, _hookean(lam, mu))
+    assert np.allclose(f, 0.0, atol=1e-10)
+
+
+def test_tet4_rotation_invariance(regular_tet_coords, soft_lame):
+    lam, mu = soft_lame
+    X = regular_tet_coords
+    u = X @ rotation_30deg_111().T - X
+    _, f = fem.tet4_force_and_tangent(X, u, fem.nh_material(mu, lam))
+    assert np.allclose(f, 0.0, atol=1e-10)
+
+
+def test_tet4_small_strain_matches_linear(regular_tet_coords, steel_lame):
+    lam, mu = steel_lame
+    X = regular_tet_coords
+    V, H = fem.tet4_volume_and_gradients(X)
+    BL = fem.linear_B(H)
+    K_lin = BL.T @ fem.hookean_C(lam, mu) @ BL * V
+    rng = np.random.RandomState(42)
+    u = 1e-5 * rng.randn(4, 3)
+    _, f = fem.tet4_force_and_tangent(X, u, _hookean(lam, mu))
+    ref = K_lin @ u.ravel()
+    assert np.linalg.norm(f - ref) / np.linalg.norm(ref) < 1e-4
+
+
+def _fd_stiffness(elem_fn, X, u, material, h=1e-7):
+    n = u.size
+    K = np.zeros((n, n))
+    for d in range(n):
+        up, um = u.copy(), u.copy()
+        up.flat[d] += h
+        um.flat[d] -= h
+        K[:, d] = (elem_fn(X, up, material)[1] - elem_fn(X, um, material)[1]) / (2 * h)
+    return K
+
+
+def test_tet4_tangent_matches_fd(regular_tet_coords, soft_lame):
+    lam, mu = soft_lame
+    rng = np.random.RandomState(123)
+    u = 0.02 * rng.randn(4, 3)
+    mat = fem.nh_material(mu, lam)
+    k, _ = fem.tet4_force_and_tangent(regular_tet_coords, u, mat)
+    Kfd = _fd_stiffness(fem.tet4_force_and_tangent, regular_tet_coords, u, mat)
+    assert np.linalg.norm(k - Kfd) / np.linalg.norm(k) < 1e-5
+    assert np.allclose(k, k.T, atol=1e-8)
+
+
+@pytest.mark.parametrize("npe", [4, 10])
+def test_force_is_energy_gradient(npe, regular_tet_coords, regular_tet10_coords, soft_lame):
+    """f = dW/du by central differences of W = sum_q w_q detJ Psi(F_q)."""
+    lam, mu = soft_lame
+    X = regular_tet_coords if npe == 4 else regular_tet10_coords
+    rng = np.random.default_rng(7)
+    u = 0.03 * rng.standard_normal((npe, 3))
+    fn = fem.tet4_force_and_tangent if npe == 4 else fem.tet10_force_and_tangent
+    _, f = fn(X, u, fem.nh_material(mu, lam))
+    h = 1e-6
+    g = np.zeros(3 * npe)
+    for d in range(3 * npe):
+        up, um = u.copy(), u.copy()
+        up.flat[d] += h
+        um.flat[d] -= h
+        g[d] = (fem.element_energy(X, up, mu, lam) - fem.element_energy(X, um, mu, lam)) / (2 * h)
+    assert np.linalg.norm(f - g) / np.linalg.norm(g) < 1e-7
+
+
+def test_planted_force_mistake_is_caught(regular_tet_coords, soft_lame):
+    """A wrong Voigt shear pairing (zx <-> yz in S_voigt) must fail the energy check."""
+    lam, mu = soft_lame
+    X = regular_tet_coords
+    u = 0.03 * np.random.default_rng(7).standard_normal((4, 3))
+    V, H = fem.tet4_volume_and_gradients(X)
+    F = (X + u).T @ H
+    _, S = fem.neohookean_stress_and_tangent(F, mu, fem.bulk_from_lame(lam, mu))
+    Sv = fem.tensor_to_voigt_stress(S)[[0, 1, 2, 3, 5, 4]]
+    f_bad = fem.nonlinear_B(H, u).T @ Sv * V
+    h = 1e-6
+    g = np.zeros(12)
+    for d in range(12):
+        up, um = u.copy(), u.copy()
+        up.flat[d] += h
+        um.flat[d] -= h
+        g[d] = (fem.element_energy(X, up, mu, lam) - fem.element_energy(X, um, mu, lam)) / (2 * h)
+    assert np.linalg.norm(f_bad - g) / np.linalg.norm(g) > 1e-3
+
+
+# ---------------------------------------------------------------- Tet10
+
+
+def test_tet10_translation(regular_tet10_coords, steel_lame):
+    lam, mu = steel_lame
+    X = regular_tet10_coords
+    u = np.tile([15.2, -42.8, 8.4], (10, 1))
+    _, f = fem.tet10_force_and_tangent(X, u, _hookean(lam, mu))
+    qp, qw = fem.tetrahedral_quadrature_4()
+    Ke = np.zeros((30, 30))
+    for q in range(4):
+        dN = fem.tet10_shape_gradients_ref(*qp[q])
+        J = X.T @ dN
+        BL = fem.linear_B(dN @ np.linalg.inv(J))
+        Ke += qw[q] * BL.T @ fem.hookean_C(lam, mu) @ BL * np.linalg.det(J)
+    assert np.linalg.norm(f) / (np.linalg.norm(Ke) * np.linalg.norm(u)) < 1e-12
+
+
+def test_tet10_rotation(regular_tet10_coords, soft_lame):
+    lam, mu = soft_lame
+    X = regular_tet10_coords
+    u = (rotation_30deg_111() @ X.T).T - X
+    _, f = fem.tet10_force_and_tangent(X, u, fem.nh_material(mu, lam))
+    assert np.allclose(f, 0.0, atol=1e-7)
+
+
+def test_tet10_patch_test(regular_tet10_coords):
+    X = regular_tet10_coords
+    eps0 = np.array([[0.002, 0.001, 0.0005], [0.001, -0.0015, 0.0008], [0.0005, 0.0008, 0.001]])
+    expected = np.array([eps0[0, 0], eps0[1, 1], eps0[2, 2], 2 * eps0[0, 1], 2 * eps0[1, 2], 2 * eps0[2, 0]])
+    u = (eps0 @ X.T).T
+    qp, _ = fem.tetrahedral_quadrature_4()
+    for q in range(4):
+        dN = fem.tet10_shape_gradients_ref(*qp[q])
+        H = dN @ np.linalg.inv(X.T @ dN)
+        assert np.allclose(fem.linear_B(H) @ u.ravel(), expected, atol=1e-14)
+
+
+def test_tet10_tangent_matches_fd(regular_tet10_coords, soft_lame):
+    lam, mu = soft_lame
+    u = 0.01 * np.random.RandomState(42).randn(10, 3)
+    mat = fem.nh_material(mu, lam)
+    k, _ = fem.tet10_force_and_tangent(regular_tet10_coords, u, mat)
+    Kfd = _fd_stiffness(fem.tet10_force_and_tangent, regular_tet10_coords, u, mat)
+    assert np.linalg.norm(k - Kfd) / np.linalg.norm(k) < 1e-5
+    assert np.allclose(k, k.T, atol=1e-8)
+
+
+# ---------------------------------------------------------------- batched == loop
+
+
+@pytest.mark.parametrize("npe", [4, 10])
+def test_batch_equals_per_element(npe, soft_lame):
+    lam, mu = soft_lame
+    from psierra_amd.mesh import box_tet4, tet4_to_tet10
+
+    m = box_tet4(3, 2, 2, 0.3, 0.2, 0.2)
+    nodes, conn = m.nodes, m.elements
+    if npe == 10:
+        nodes, conn = tet4_to_tet10(nodes, conn)
+    rng = np.random.default_rng(3)
+    u = 0.002 * rng.standard_normal(nodes.shape)
+    E = conn.shape[0]
+    mus = mu * np.exp(0.3 * rng.standard_normal(E))
+    lams = 5 * mus
+    kb, fb = fem.element_batch(nodes[conn], u[conn], mus, lams)
+    fn = fem.tet4_force_and_tangent if npe == 4 else fem.tet10_force_and_tangent
+    for e in range(E):
+        k, f = fn(nodes[conn[e]], u[conn[e]], fem.nh_material(mus[e], lams[e]))
+        assert np.allclose(kb[e], k, rtol=1e-12, atol=1e-12 * np.abs(k).max())
+        assert np.allclose(fb[e], f, rtol=1e-10, atol=1e-12 * np.abs(f).max())
+
+
+# ---------------------------------------------------------------- assembly
+
+
+def test_pattern_counts_cantilever():
+    """Structural nnz of the 20x4x4 Kuhn box is 55,053 (25 face-layers closed form)."""
+    from psierra_amd.mesh import box_tet4
+
+    m = box_tet4(20, 4, 4, 1.0, 0.2, 0.2)
+    indptr, indices = A.structural_pattern(m.elements, m.n_nodes)
+    assert (m.n_nodes, m.n_elements, indptr[-1]) == (525, 1920, 55053)
+    # every row of a node carries 3 x (node degree) columns, sorted
+    assert np.all(np.diff(indptr) % 3 == 0)
+    for r in range(0, 3 * m.n_nodes, 97):
+        cols = indices[indptr[r] : indptr[r + 1]]
+        assert np.all(np.diff(cols) > 0)
+
+
+def _kuhn_nnz(nx, ny, nz):
+    """9 x number of ordered node pairs (incl. self) sharing a Kuhn tet.
+
+    Each node connects to itself and, through the edges of the Kuhn split, to
+    the grid neighbours along +-x, +-y, +-z, +-(x+y), +-(y+z), +-(x+z) and
+    +-(x+y+z) that exist: 14 directions.
+    """
+    dirs = [(1, 0, 0), (0, 1, 0), (0, 0, 1), (1, 1, 0), (0, 1, 1), (1, 0, 1), (1, 1, 1)]
+    count = (nx + 1) * (ny + 1) * (nz + 1)
+    for d in dirs:
+        count += 2 * (nx + 1 - d[0]) * (ny + 1 - d[1]) * (nz + 1 - d[2])
+    return 9 * count
+
+
+@pytest.mark.parametrize("dims", [(20, 4, 4), (5, 3, 2), (7, 7, 7)])
+def test_pattern_matches_closed_form(dims):
+    from psierra_amd.mesh import box_tet4
+
+    m = box_tet4(*dims)
+    indptr, _ = A.structural_pattern(m.elements, m.n_nodes)
+    assert indptr[-1] == _kuhn_nnz(*dims)
+
+
+def test_assemble_vector_matches_dense_sum():
+    rng = np.random.default_rng(0)
+    dofs = rng.integers(0, 30, size=(50, 12))
+    vals = rng.standard_normal((50, 12))
+    dense = np.zeros(30)
+    for e in range(50):
+        for i in range(12):
+            dense[dofs[e, i]] += vals[e, i]
+    assert np.allclose(A.assemble_vector(vals, dofs, 30), dense, rtol=1e-14, atol=1e-14)
+
+
+def test_coo_to_csr_keeps_explicit_zeros_and_sum_drops_them():
+    """scipy semantics the reference relies on (assembly.py:75-76; nonlinear_static.py:354-355)."""
+    rows = np.array([0, 0, 1, 1])
+    cols = np.array([0, 1, 0, 1])
+    K = A.assemble_sparse_matrix(np.array([1.0, 0.0, 0.0, 2.0]), rows, cols, 2)
+    assert K.nnz == 4
+    C = A.free_dof_csc(K, np.array([0, 1]))
+    assert C.nnz == 2
