@@ -113,6 +113,61 @@ int32_t psx_assemble(psx_plan* plan, const double* u, const double* mu,
                      const double* lam, double* f, double* kval,
                      int32_t precision, void* stream, int32_t* status);
 
+/* ---- Adjoint of the internal force (device, Tet4) ------------------------
+ * No counterpart in the reference, which differentiates only by finite
+ * differences: MaterialParameterIdentifier.identify calls
+ * scipy.optimize.least_squares with its default 2-point Jacobian
+ * (src/psierra/inverse/parameter_identification.py:370-379).
+ * For a cotangent w (3N, device) computes g_u = K_T w (3N), and, if g_mu and
+ * g_lam are non-NULL, d(w^T f_int)/d mu_e and d/d lambda_e (n_elements each).
+ * Matrix-free; the same node-row scatter as psx_assemble (deterministic). */
+int32_t psx_vjp(psx_plan* plan, const double* u, const double* mu, const double* lam,
+                const double* w, double* g_u, double* g_mu, double* g_lam,
+                int32_t precision, void* stream, int32_t* status);
+
+/* ---- Free-DOF solver matrix and residual (device) -------------------------
+ * Replaces K_total = K_tan + K_contact; K_total[free][:, free].tocsc()
+ * (src/psierra/analysis/nonlinear_static.py:354-355) and
+ * r_free = f_step_free - f_int[free], ||r|| (nonlinear_static.py:322-323).
+ * psx_csc_create (host inputs): free dofs strictly increasing; builds the
+ * structural CSC of K[free][:, free] for a plan.  psx_csc_extract: from the
+ * plan's CSR values kval, writes colptr (n_free + 1), rowind and vals
+ * (capacity nnz_struct each) with row indices ascending; with drop_zeros,
+ * entries whose value is exactly 0.0 are dropped, as scipy's CSR '+' does.
+ * The number of stored entries is colptr[n_free].  psx_free_residual: r_free
+ * (n_free) = f_ext_free - f_int[free] (f_ext_free may be NULL = 0) and
+ * sumsq[0] = sum r^2, accumulated in FP64 in a fixed order. */
+typedef struct psx_csc psx_csc;
+int32_t psx_csc_create(const psx_plan* plan, const int64_t* free_dofs, int64_t n_free,
+                       psx_csc** out);
+int32_t psx_csc_destroy(psx_csc* csc);
+int32_t psx_csc_info(const psx_csc* csc, int64_t* n_free, int64_t* nnz_struct);
+int32_t psx_csc_extract(psx_csc* csc, const double* kval, int32_t drop_zeros,
+                        int32_t* colptr, int32_t* rowind, double* vals, void* stream);
+int32_t psx_free_residual(psx_csc* csc, const double* f_int, const double* f_ext_free,
+                          double* r_free, double* sumsq, void* stream);
+
+/* ---- Batched ECSW hyper-reduced evaluation (device, Tet4, rank 64) -------
+ * Replaces HyperReducedECSWModel.evaluate_forces_and_stiffness
+ * (src/psierra/hyperreduction/model.py:254-318) for many independent
+ * instances in one persistent launch; Phi_e is cached at creation as the
+ * model does (model.py:185-213).  Host inputs to psx_ecsw_create, concatenated
+ * over instances with rid_offsets (n_instances + 1): X_e (n x 4 x 3), phi_e
+ * (n x 12 x 64, rows of the basis for the element's 12 DOFs), weights (> 0),
+ * mu, lam (n each).  psx_ecsw_evaluate: u_r (n_instances x 64, device) ->
+ * f_r (n_instances x 64) and K_r (n_instances x 64 x 64, symmetrised
+ * 0.5 (K + K^T) as model.py:317), FP64 outputs of FP32 element arithmetic
+ * and FP32 matrix-core accumulation.
+ * Info slots: n_instances, n_rid_elements, n_work_items, grid, device_bytes. */
+typedef struct psx_ecsw psx_ecsw;
+int32_t psx_ecsw_create(int32_t n_instances, const int64_t* rid_offsets, const double* X_e,
+                        const double* phi_e, const double* weights, const double* mu,
+                        const double* lam, int32_t rank, psx_ecsw** out);
+int32_t psx_ecsw_destroy(psx_ecsw* ecsw);
+int32_t psx_ecsw_info(const psx_ecsw* ecsw, int64_t* info, int32_t n_info);
+int32_t psx_ecsw_evaluate(psx_ecsw* ecsw, const double* u_r, double* f_r, double* K_r,
+                          void* stream, int32_t* status);
+
 /* ---- Element batch (device) -------------------------------------------------
  * Replaces Tet4TotalLagrangian.compute_nonlinear_force_and_tangent_stiffness
  * (src/psierra/element/tet4_total_lagrangian.py:77-139) and the Tet10 one

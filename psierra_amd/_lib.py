@@ -50,6 +50,22 @@ SIGNATURES = {
         c_int32,
         [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_void_p],
     ),
+    "psx_vjp": (
+        c_int32,
+        [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_void_p],
+    ),
+    "psx_csc_create": (c_int32, [c_void_p, c_void_p, c_int64, POINTER(c_void_p)]),
+    "psx_csc_destroy": (c_int32, [c_void_p]),
+    "psx_csc_info": (c_int32, [c_void_p, _i64p, _i64p]),
+    "psx_csc_extract": (c_int32, [c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "psx_free_residual": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "psx_ecsw_create": (
+        c_int32,
+        [c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int32, POINTER(c_void_p)],
+    ),
+    "psx_ecsw_destroy": (c_int32, [c_void_p]),
+    "psx_ecsw_info": (c_int32, [c_void_p, _i64p, c_int32]),
+    "psx_ecsw_evaluate": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "psx_element_batch": (
         c_int32,
         [c_int32, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_void_p],

@@ -1,0 +1,323 @@
+// Batched ECSW hyper-reduced force and tangent on FP32 MFMA.
+//
+// For each instance (a mesh, a reduced basis V of rank 64 and a reduced
+// integration domain RID with weights w_e) and reduced coordinates u_r:
+//   f_r = sum_{e in RID} w_e Phi_e^T f_e(Phi_e u_r)
+//   K_r = sym( sum_{e in RID} w_e Phi_e^T k_e(Phi_e u_r) Phi_e )
+// as HyperReducedECSWModel.evaluate_forces_and_stiffness does per element
+// (src/psierra/hyperreduction/model.py:254-318, Phi_e cached at
+// construction as in model.py:185-213).
+//
+// The tangent projection is one GEMM per instance: with A = [Phi_e] and
+// B = [w_e k_e Phi_e] stacked over the RID (12 |RID| x 64 each),
+// K_r = A^T B.  Every work item (an instance and a chunk of its RID) runs in
+// a persistent launch; per batch of EB elements a work-group stages Phi_e in
+// LDS, evaluates u_e = Phi_e u_r, the element k_e and f_e (FP32), forms
+// w k_e Phi_e, and its four waves accumulate the four 32x32 quadrants of
+// A^T B with v_mfma_f32_32x32x2f32 (FP32 in, FP32 accumulate).  Each work
+// item writes its 64x64 partial; a second kernel adds the partials of an
+// instance in a fixed order, symmetrises (model.py:317) and writes FP64.
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "psx_internal.h"
+#include "psx_tet4_elem.h"
+
+namespace psx {
+
+constexpr int kRank = 64;
+constexpr int kEB = 8;        // elements staged per batch
+constexpr int kChunk = 128;   // RID elements per work item
+constexpr int kPhi = 12 * kRank;
+
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+struct DevEcsw {
+  int32_t n_inst, n_elem, n_items;
+  const float4* geom;        // [n_elem][4]: h1, h2, h3 (9), V, w, mu, K, pad
+  const float* phi;          // [n_elem][12][64]
+  const int32_t* item_inst;  // [n_items]
+  const int32_t* item_e0;    // [n_items]
+  const int32_t* item_cnt;   // [n_items]
+  const int32_t* inst_item0; // [n_inst + 1]
+  float* partK;              // [n_items][64][64]
+  float* partF;              // [n_items][64]
+};
+
+__global__ __launch_bounds__(256) void ecsw_project_kernel(DevEcsw d, const double* __restrict__ ur_all,
+                                                           int32_t* status) {
+  __shared__ float s_phi[kEB * kPhi];
+  __shared__ float s_B[kEB * kPhi];
+  __shared__ float s_k[kEB * 144];
+  __shared__ float s_f[kEB * 12];
+  __shared__ float s_ue[kEB * 12];
+  __shared__ float s_ur[kRank];
+  const int t = threadIdx.x;
+  const int lane = t & 63;
+  const int wv = t >> 6;
+  const int qI = wv >> 1, qJ = wv & 1;
+  for (int item = blockIdx.x; item < d.n_items; item += gridDim.x) {
+    const int inst = d.item_inst[item];
+    const int e0 = d.item_e0[item];
+    const int cnt = d.item_cnt[item];
+    __syncthreads();
+    if (t < kRank) s_ur[t] = float(ur_all[int64_t(inst) * kRank + t]);
+    floatx16 acc;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
+    float fr = 0.0f;
+    for (int b0 = 0; b0 < cnt; b0 += kEB) {
+      const int nb = min(kEB, cnt - b0);
+      __syncthreads();
+      const float4* src = reinterpret_cast<const float4*>(d.phi + int64_t(e0 + b0) * kPhi);
+      for (int i = t; i < nb * (kPhi / 4); i += 256) reinterpret_cast<float4*>(s_phi)[i] = src[i];
+      __syncthreads();
+      if (t < nb * 12) {
+        const float* row = s_phi + (t / 12) * kPhi + (t % 12) * kRank;
+        float s = 0.0f;
+#pragma unroll 16
+        for (int j = 0; j < kRank; ++j) s += row[j] * s_ur[j];
+        s_ue[t] = s;
+      }
+      __syncthreads();
+      if (t < nb) {
+        const int e = e0 + b0 + t;
+        const float4 g0 = d.geom[4 * int64_t(e)], g1 = d.geom[4 * int64_t(e) + 1];
+        const float4 g2 = d.geom[4 * int64_t(e) + 2], g3 = d.geom[4 * int64_t(e) + 3];
+        float h[4][3] = {{0, 0, 0}, {g0.x, g0.y, g0.z}, {g0.w, g1.x, g1.y}, {g1.z, g1.w, g2.x}};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) h[0][c] = -(h[1][c] + h[2][c] + h[3][c]);
+        const float V = g2.y, w = g2.z, mu = g2.w, K = g3.x;
+        const float* ue = s_ue + 12 * t;
+        float A[3][3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int j = 0; j < 3; ++j)
+            A[i][j] = (ue[3 + i] - ue[i]) * h[1][j] + (ue[6 + i] - ue[i]) * h[2][j] +
+                      (ue[9 + i] - ue[i]) * h[3][j];
+        const float J = tet4_full<float>(h, V, A, mu, K, w, s_k + 144 * t, s_f + 12 * t);
+        if (J <= 0.0f) {
+          atomicMin(&status[1], e);
+          status[0] = 1;
+        }
+      }
+      __syncthreads();
+      // B_e = (w k_e) Phi_e
+      for (int o = t; o < nb * kPhi; o += 256) {
+        const int e = o / kPhi, r = (o / kRank) % 12, j = o % kRank;
+        const float* kr = s_k + 144 * e + 12 * r;
+        const float* ph = s_phi + e * kPhi + j;
+        float s = 0.0f;
+#pragma unroll
+        for (int c = 0; c < 12; ++c) s += kr[c] * ph[c * kRank];
+        s_B[o] = s;
+      }
+      if (t < kRank) {
+        for (int e = 0; e < nb; ++e)
+#pragma unroll
+          for (int r = 0; r < 12; ++r) fr += s_f[12 * e + r] * s_phi[e * kPhi + r * kRank + t];
+      }
+      __syncthreads();
+      // A^T B on the matrix cores: quadrant (qI, qJ), K-steps of 2 rows.
+      for (int e = 0; e < nb; ++e) {
+#pragma unroll
+        for (int kp = 0; kp < 6; ++kp) {
+          const int row = 2 * kp + (lane >> 5);
+          const float a = s_phi[e * kPhi + row * kRank + 32 * qI + (lane & 31)];
+          const float b = s_B[e * kPhi + row * kRank + 32 * qJ + (lane & 31)];
+          acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
+        }
+      }
+    }
+    // 32x32 accumulator layout: register r of lane l holds
+    // row 8 (r / 4) + 4 (l / 32) + r % 4, column l % 32.
+    float* out = d.partK + int64_t(item) * kRank * kRank;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int i = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+      out[(32 * qI + i) * kRank + 32 * qJ + (lane & 31)] = acc[r];
+    }
+    if (t < kRank) d.partF[int64_t(item) * kRank + t] = fr;
+  }
+}
+
+__global__ __launch_bounds__(256) void ecsw_reduce_kernel(DevEcsw d, double* __restrict__ f_r,
+                                                          double* __restrict__ K_r) {
+  const int inst = blockIdx.x;
+  const int i0 = d.inst_item0[inst], i1 = d.inst_item0[inst + 1];
+  for (int o = threadIdx.x; o < kRank * kRank; o += 256) {
+    const int i = o / kRank, j = o % kRank;
+    double a = 0.0, b = 0.0;
+    for (int it = i0; it < i1; ++it) {
+      a += double(d.partK[int64_t(it) * kRank * kRank + i * kRank + j]);
+      b += double(d.partK[int64_t(it) * kRank * kRank + j * kRank + i]);
+    }
+    K_r[int64_t(inst) * kRank * kRank + o] = 0.5 * (a + b);
+  }
+  if (threadIdx.x < kRank) {
+    double s = 0.0;
+    for (int it = i0; it < i1; ++it) s += double(d.partF[int64_t(it) * kRank + threadIdx.x]);
+    f_r[int64_t(inst) * kRank + threadIdx.x] = s;
+  }
+}
+
+}  // namespace psx
+
+struct psx_ecsw {
+  int device = 0;
+  psx::DevEcsw d{};
+  int64_t device_bytes = 0;
+  int grid = 0;
+  std::vector<void*> allocations;
+};
+
+namespace psx {
+namespace {
+
+int32_t ecsw_create(int32_t n_inst, const int64_t* rid_offsets, const double* Xe, const double* phi,
+                    const double* weights, const double* mu, const double* lam, int32_t rank,
+                    psx_ecsw** out) {
+  if (!out) return fail(PSX_EINVAL, "out is null");
+  *out = nullptr;
+  if (rank != kRank) return fail(PSX_EUNSUPPORTED, "the MFMA projection is built for rank 64");
+  if (n_inst <= 0 || !rid_offsets || !Xe || !phi || !weights || !mu || !lam)
+    return fail(PSX_EINVAL, "null or empty ECSW input");
+  if (rid_offsets[0] != 0) return fail(PSX_EINVAL, "rid_offsets[0] must be 0");
+  for (int i = 0; i < n_inst; ++i)
+    if (rid_offsets[i + 1] < rid_offsets[i]) return fail(PSX_EINVAL, "rid_offsets must be non-decreasing");
+  const int64_t n_elem = rid_offsets[n_inst];
+  if (n_elem <= 0 || n_elem >= (int64_t(1) << 31) / kPhi * 4)
+    return fail(PSX_EINVAL, "bad RID size");
+  std::vector<float4> geom(size_t(n_elem) * 4);
+  for (int64_t e = 0; e < n_elem; ++e) {
+    const double* X = Xe + 12 * e;
+    double D[3][3];
+    for (int b = 0; b < 3; ++b)
+      for (int r = 0; r < 3; ++r) D[r][b] = X[3 * (b + 1) + r] - X[r];
+    double adj[3][3];
+    adj[0][0] = D[1][1] * D[2][2] - D[1][2] * D[2][1];
+    adj[0][1] = D[0][2] * D[2][1] - D[0][1] * D[2][2];
+    adj[0][2] = D[0][1] * D[1][2] - D[0][2] * D[1][1];
+    adj[1][0] = D[1][2] * D[2][0] - D[1][0] * D[2][2];
+    adj[1][1] = D[0][0] * D[2][2] - D[0][2] * D[2][0];
+    adj[1][2] = D[0][2] * D[1][0] - D[0][0] * D[1][2];
+    adj[2][0] = D[1][0] * D[2][1] - D[1][1] * D[2][0];
+    adj[2][1] = D[0][1] * D[2][0] - D[0][0] * D[2][1];
+    adj[2][2] = D[0][0] * D[1][1] - D[0][1] * D[1][0];
+    const double det = D[0][0] * adj[0][0] + D[0][1] * adj[1][0] + D[0][2] * adj[2][0];
+    if (!(det > 0.0))
+      return fail(PSX_EVOLUME, "Negative or zero element volume detected in RID element " +
+                                   std::to_string(e) + ". Check element vertex ordering.");
+    if (!(weights[e] > 0.0)) return fail(PSX_EINVAL, "All ECSW element weights must be strictly positive (w_e > 0).");
+    float h[9];
+    for (int b = 0; b < 3; ++b)
+      for (int c = 0; c < 3; ++c) h[3 * b + c] = float(adj[b][c] / det);
+    const double K = lam[e] + (2.0 / 3.0) * mu[e];
+    geom[4 * e + 0] = make_float4(h[0], h[1], h[2], h[3]);
+    geom[4 * e + 1] = make_float4(h[4], h[5], h[6], h[7]);
+    geom[4 * e + 2] = make_float4(h[8], float(det / 6.0), float(weights[e]), float(mu[e]));
+    geom[4 * e + 3] = make_float4(float(K), 0.f, 0.f, 0.f);
+  }
+  std::vector<float> phif(size_t(n_elem) * kPhi);
+  for (size_t i = 0; i < phif.size(); ++i) phif[i] = float(phi[i]);
+  std::vector<int32_t> item_inst, item_e0, item_cnt, inst_item0{0};
+  for (int i = 0; i < n_inst; ++i) {
+    for (int64_t e = rid_offsets[i]; e < rid_offsets[i + 1]; e += kChunk) {
+      item_inst.push_back(i);
+      item_e0.push_back(int32_t(e));
+      item_cnt.push_back(int32_t(std::min<int64_t>(kChunk, rid_offsets[i + 1] - e)));
+    }
+    inst_item0.push_back(int32_t(item_inst.size()));
+  }
+  auto* h = new psx_ecsw();
+  auto bail = [&](int32_t rc) {
+    for (void* p : h->allocations) (void)hipFree(p);
+    delete h;
+    return rc;
+  };
+  (void)hipGetDevice(&h->device);
+  auto up = [&](const void* src, size_t bytes, void** dst) -> int32_t {
+    PSX_HIP_TRY(hipMalloc(dst, std::max<size_t>(bytes, 16)));
+    h->allocations.push_back(*dst);
+    h->device_bytes += int64_t(bytes);
+    if (src && bytes) PSX_HIP_TRY(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+    return PSX_OK;
+  };
+  const size_t n_items = item_inst.size();
+  void *dg, *dp, *di, *de, *dc, *d0, *dk, *df;
+  int32_t rc;
+  if ((rc = up(geom.data(), geom.size() * sizeof(float4), &dg)) ||
+      (rc = up(phif.data(), phif.size() * 4, &dp)) || (rc = up(item_inst.data(), n_items * 4, &di)) ||
+      (rc = up(item_e0.data(), n_items * 4, &de)) || (rc = up(item_cnt.data(), n_items * 4, &dc)) ||
+      (rc = up(inst_item0.data(), inst_item0.size() * 4, &d0)) ||
+      (rc = up(nullptr, n_items * kRank * kRank * 4, &dk)) || (rc = up(nullptr, n_items * kRank * 4, &df)))
+    return bail(rc);
+  h->d.n_inst = n_inst;
+  h->d.n_elem = int32_t(n_elem);
+  h->d.n_items = int32_t(n_items);
+  h->d.geom = static_cast<const float4*>(dg);
+  h->d.phi = static_cast<const float*>(dp);
+  h->d.item_inst = static_cast<const int32_t*>(di);
+  h->d.item_e0 = static_cast<const int32_t*>(de);
+  h->d.item_cnt = static_cast<const int32_t*>(dc);
+  h->d.inst_item0 = static_cast<const int32_t*>(d0);
+  h->d.partK = static_cast<float*>(dk);
+  h->d.partF = static_cast<float*>(df);
+  int cus = 256;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, h->device) == hipSuccess && prop.multiProcessorCount > 0)
+    cus = prop.multiProcessorCount;
+  h->grid = int(std::min<size_t>(n_items, size_t(cus) * 2));
+  *out = h;
+  return PSX_OK;
+}
+
+}  // namespace
+
+int32_t launch_status_reset(hipStream_t s, int32_t* status);
+
+}  // namespace psx
+
+extern "C" {
+
+int32_t psx_ecsw_create(int32_t n_instances, const int64_t* rid_offsets, const double* X_e,
+                        const double* phi_e, const double* weights, const double* mu, const double* lam,
+                        int32_t rank, psx_ecsw** out) {
+  psx::clear_error();
+  return psx::ecsw_create(n_instances, rid_offsets, X_e, phi_e, weights, mu, lam, rank, out);
+}
+
+int32_t psx_ecsw_destroy(psx_ecsw* h) {
+  psx::clear_error();
+  if (!h) return PSX_OK;
+  for (void* p : h->allocations) (void)hipFree(p);
+  delete h;
+  return PSX_OK;
+}
+
+int32_t psx_ecsw_info(const psx_ecsw* h, int64_t* info, int32_t n_info) {
+  psx::clear_error();
+  if (!h || !info) return psx::fail(PSX_EINVAL, "null argument");
+  const int64_t v[5] = {h->d.n_inst, h->d.n_elem, h->d.n_items, h->grid, h->device_bytes};
+  for (int i = 0; i < n_info && i < 5; ++i) info[i] = v[i];
+  return PSX_OK;
+}
+
+int32_t psx_ecsw_evaluate(psx_ecsw* h, const double* u_r, double* f_r, double* K_r, void* stream,
+                          int32_t* status) {
+  psx::clear_error();
+  if (!h || !u_r || !f_r || !K_r || !status) return psx::fail(PSX_EINVAL, "null argument");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  int32_t rc = psx::launch_status_reset(s, status);
+  if (rc) return rc;
+  hipLaunchKernelGGL(psx::ecsw_project_kernel, dim3(h->grid), dim3(256), 0, s, h->d, u_r, status);
+  hipLaunchKernelGGL(psx::ecsw_reduce_kernel, dim3(h->d.n_inst), dim3(256), 0, s, h->d, f_r, K_r);
+  PSX_HIP_TRY(hipGetLastError());
+  return PSX_OK;
+}
+
+}  // extern "C"

@@ -58,9 +58,28 @@ struct DevPlan {
   const int32_t* uoff;            // N+1: prefix of deg in internal order
   const int32_t* tile_node;       // n_tiles+1
   double4* ui;                    // scratch: displacement in internal order
+  double4* wi;                    // scratch: adjoint cotangent in internal order
+};
+
+// Structural CSC of K[free][:, free] (device view).
+struct DevCsc {
+  int32_t n_free;
+  const int32_t* colptr_s;  // n_free + 1
+  const int32_t* row_s;     // free-numbered rows, ascending per column
+  const int32_t* src;       // CSR value position of each entry
+  const int32_t* free;      // free dof ids (ascending)
+  int32_t* scratch;         // per-block counts / offsets
+  double* partial;          // per-block residual partial sums
 };
 
 }  // namespace psx
+
+struct psx_csc {
+  int device = 0;
+  int64_t n_free = 0, nnz_struct = 0;
+  psx::DevCsc d{};
+  std::vector<void*> allocations;
+};
 
 struct psx_plan {
   int device = 0;
@@ -81,6 +100,13 @@ int32_t launch_tet4_assemble(const DevPlan& p, const double* u, const double* mu
                              double* f, double* kval, int precision, hipStream_t s, int32_t* status);
 int32_t launch_tet10_assemble(const DevPlan& p, const double* u, const double* mu, const double* lam,
                               double* f, double* kval, int precision, hipStream_t s, int32_t* status);
+int32_t launch_tet4_vjp(const DevPlan& p, const double* u, const double* mu, const double* lam,
+                        const double* w, double* gu, double* gmu, double* glam, int precision,
+                        hipStream_t s, int32_t* status);
+int32_t launch_csc_extract(const DevCsc& c, const double* kval, int drop, int* colptr, int* rowind,
+                           double* vals, hipStream_t s);
+int32_t launch_free_residual(const DevCsc& c, const double* f_int, const double* f_ext_free, double* r,
+                             double* sumsq, hipStream_t s);
 int32_t launch_element_batch(int npe, int64_t n, const double* X, const double* u, const double* mu,
                              const double* lam, double* k, double* f, int precision, hipStream_t s,
                              int32_t* status);

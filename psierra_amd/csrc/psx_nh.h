@@ -72,6 +72,9 @@ struct NHPoint {
   R S[3][3];   // 2nd Piola-Kirchhoff stress (symmetric)
   R al, be, ga;
   R J, lnJ;
+  R jm23;      // J^-2/3
+  R E[3][3];   // Green-Lagrange strain
+  R trE;
 };
 
 // A = grad_X u (A[i][j] = du_i/dX_j); mu, K = shear and bulk modulus.
@@ -121,7 +124,13 @@ __device__ __forceinline__ void nh_point(const R A[3][3], R mu, R K, NHPoint<R>&
       E[L][I] = E[I][L];
     }
   const R trE = E[0][0] + E[1][1] + E[2][2];
-  const R fac = mu * d_exp(R(-2.0 / 3.0) * o.lnJ);
+  o.jm23 = d_exp(R(-2.0 / 3.0) * o.lnJ);
+  const R fac = mu * o.jm23;
+#pragma unroll
+  for (int I = 0; I < 3; ++I)
+#pragma unroll
+    for (int L = 0; L < 3; ++L) o.E[I][L] = E[I][L];
+  o.trE = trE;
   const R I1 = R(3) + R(2) * trE;
   // Sbar = 2 f dev(E) + K lnJ I ;  S = sym(C^-1 Sbar)
   R Sb[3][3];
@@ -178,6 +187,28 @@ __device__ __forceinline__ void nh_block(const NHPoint<R>& m, R w, const R ga_[3
 #pragma unroll
     for (int k = 0; k < 3; ++k)
       out[3 * i + k] = P[i] * gb[k] + gb[i] * Rv[k] + T[i] * qb[k] + (i == k ? d : R(0));
+}
+
+// dS/dmu at fixed lambda (K = lambda + 2 mu / 3):
+//   C^-1 (2 J^-2/3 dev E + 2/3 ln J I);   dS/dlambda = ln J C^-1.
+template <typename R>
+__device__ __forceinline__ void nh_dS_dmu(const NHPoint<R>& m, R o[3][3]) {
+  R B[3][3];
+#pragma unroll
+  for (int I = 0; I < 3; ++I)
+#pragma unroll
+    for (int L = 0; L < 3; ++L)
+      B[I][L] = R(2) * m.jm23 * (m.E[I][L] - (I == L ? m.trE * R(1.0 / 3.0) : R(0))) +
+                (I == L ? R(2.0 / 3.0) * m.lnJ : R(0));
+  R M[3][3];
+#pragma unroll
+  for (int I = 0; I < 3; ++I)
+#pragma unroll
+    for (int L = 0; L < 3; ++L) M[I][L] = m.Ci[I][0] * B[0][L] + m.Ci[I][1] * B[1][L] + m.Ci[I][2] * B[2][L];
+#pragma unroll
+  for (int I = 0; I < 3; ++I)
+#pragma unroll
+    for (int L = 0; L < 3; ++L) o[I][L] = R(0.5) * (M[I][L] + M[L][I]);
 }
 
 }  // namespace psx

@@ -401,8 +401,84 @@ int32_t create_plan(const int64_t* conn, int64_t E, int npe, const double* X, in
   plan->allocations.push_back(ui);
   plan->device_bytes += int64_t(N) * int64_t(sizeof(double4));
   d.ui = static_cast<double4*>(ui);
+  void* wi = nullptr;
+  if (hipMalloc(&wi, size_t(N) * sizeof(double4)) != hipSuccess)
+    return bail(fail(PSX_EINVAL, "hipMalloc of the cotangent scratch failed"));
+  plan->allocations.push_back(wi);
+  plan->device_bytes += int64_t(N) * int64_t(sizeof(double4));
+  d.wi = static_cast<double4*>(wi);
   if ((rc = make_tiles(plan, tile))) return bail(rc);
   *out = plan;
+  return PSX_OK;
+}
+
+int32_t create_csc(const psx_plan* plan, const int64_t* free_dofs, int64_t n_free, psx_csc** out) {
+  if (!out) return fail(PSX_EINVAL, "out is null");
+  *out = nullptr;
+  const HostPattern& hp = plan->hp;
+  const int64_t n = 3 * hp.N;
+  if (n_free < 0 || n_free > n || (n_free > 0 && !free_dofs))
+    return fail(PSX_EINVAL, "bad free dof list");
+  std::vector<int32_t> newidx(n, -1), freev(n_free);
+  for (int64_t k = 0; k < n_free; ++k) {
+    const int64_t j = free_dofs[k];
+    if (j < 0 || j >= n) return fail(PSX_EINVAL, "free dof " + std::to_string(j) + " out of range");
+    if (k > 0 && j <= free_dofs[k - 1])
+      return fail(PSX_EINVAL, "free dofs must be strictly increasing");
+    newidx[j] = int32_t(k);
+    freev[k] = int32_t(j);
+  }
+  auto deg = [&](int64_t a) { return hp.adj_ptr[a + 1] - hp.adj_ptr[a]; };
+  auto rank = [&](int64_t a, int64_t b) {
+    const int32_t* nb = hp.adj.data() + hp.adj_ptr[a];
+    return int32_t(std::lower_bound(nb, nb + deg(a), int32_t(b)) - nb);
+  };
+  std::vector<int32_t> colptr(n_free + 1, 0), rows, src;
+  rows.reserve(size_t(n_free) * 48);
+  src.reserve(size_t(n_free) * 48);
+  for (int64_t k = 0; k < n_free; ++k) {
+    const int64_t j = freev[k], a = j / 3, cj = j % 3;
+    for (int32_t t = hp.adj_ptr[a]; t < hp.adj_ptr[a + 1]; ++t) {
+      const int64_t b = hp.adj[t];
+      for (int c = 0; c < 3; ++c) {
+        const int64_t i = 3 * b + c;
+        if (newidx[i] < 0) continue;
+        rows.push_back(newidx[i]);
+        src.push_back(int32_t(9 * int64_t(hp.adj_ptr[b]) + 3 * c * deg(b) + 3 * rank(b, a) + cj));
+      }
+    }
+    colptr[k + 1] = int32_t(rows.size());
+  }
+  auto* csc = new psx_csc();
+  auto bail = [&](int32_t rc) {
+    for (void* p : csc->allocations) (void)hipFree(p);
+    delete csc;
+    return rc;
+  };
+  (void)hipGetDevice(&csc->device);
+  csc->n_free = n_free;
+  csc->nnz_struct = int64_t(rows.size());
+  auto up = [&](const void* h, size_t bytes, void** d) -> int32_t {
+    PSX_HIP_TRY(hipMalloc(d, std::max<size_t>(bytes, 16)));
+    csc->allocations.push_back(*d);
+    if (bytes && h) PSX_HIP_TRY(hipMemcpy(*d, h, bytes, hipMemcpyHostToDevice));
+    return PSX_OK;
+  };
+  void *dcol, *drow, *dsrc, *dfree, *dscr, *dpart;
+  const size_t nb = size_t((n_free + 255) / 256) + 1;
+  int32_t rc;
+  if ((rc = up(colptr.data(), colptr.size() * 4, &dcol)) || (rc = up(rows.data(), rows.size() * 4, &drow)) ||
+      (rc = up(src.data(), src.size() * 4, &dsrc)) || (rc = up(freev.data(), freev.size() * 4, &dfree)) ||
+      (rc = up(nullptr, nb * 4, &dscr)) || (rc = up(nullptr, nb * 8, &dpart)))
+    return bail(rc);
+  csc->d.n_free = int32_t(n_free);
+  csc->d.colptr_s = static_cast<int32_t*>(dcol);
+  csc->d.row_s = static_cast<int32_t*>(drow);
+  csc->d.src = static_cast<int32_t*>(dsrc);
+  csc->d.free = static_cast<int32_t*>(dfree);
+  csc->d.scratch = static_cast<int32_t*>(dscr);
+  csc->d.partial = static_cast<double*>(dpart);
+  *out = csc;
   return PSX_OK;
 }
 
@@ -496,6 +572,63 @@ int32_t psx_assemble(psx_plan* plan, const double* u, const double* mu, const do
   if (plan->hp.npe == 4)
     return psx::launch_tet4_assemble(plan->d, u, mu, lam, f, kval, precision, s, status);
   return psx::launch_tet10_assemble(plan->d, u, mu, lam, f, kval, precision, s, status);
+}
+
+int32_t psx_vjp(psx_plan* plan, const double* u, const double* mu, const double* lam, const double* w,
+                double* g_u, double* g_mu, double* g_lam, int32_t precision, void* stream,
+                int32_t* status) {
+  psx::clear_error();
+  if (!plan || !u || !mu || !lam || !w || !g_u || !status)
+    return psx::fail(PSX_EINVAL, "null argument to psx_vjp");
+  if ((g_mu == nullptr) != (g_lam == nullptr))
+    return psx::fail(PSX_EINVAL, "g_mu and g_lam must both be given or both be null");
+  if (precision != PSX_FP64 && precision != PSX_FP32)
+    return psx::fail(PSX_EINVAL, "precision must be PSX_FP64 or PSX_FP32");
+  if (plan->hp.npe != 4) return psx::fail(PSX_EUNSUPPORTED, "psx_vjp supports Tet4 plans");
+  return psx::launch_tet4_vjp(plan->d, u, mu, lam, w, g_u, g_mu, g_lam, precision,
+                              static_cast<hipStream_t>(stream), status);
+}
+
+int32_t psx_csc_create(const psx_plan* plan, const int64_t* free_dofs, int64_t n_free, psx_csc** out) {
+  psx::clear_error();
+  if (!plan) return psx::fail(PSX_EINVAL, "plan is null");
+  return psx::create_csc(plan, free_dofs, n_free, out);
+}
+
+int32_t psx_csc_destroy(psx_csc* csc) {
+  psx::clear_error();
+  if (!csc) return PSX_OK;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  (void)hipSetDevice(csc->device);
+  for (void* p : csc->allocations) (void)hipFree(p);
+  (void)hipSetDevice(dev);
+  delete csc;
+  return PSX_OK;
+}
+
+int32_t psx_csc_info(const psx_csc* csc, int64_t* n_free, int64_t* nnz_struct) {
+  psx::clear_error();
+  if (!csc || !n_free || !nnz_struct) return psx::fail(PSX_EINVAL, "null argument");
+  *n_free = csc->n_free;
+  *nnz_struct = csc->nnz_struct;
+  return PSX_OK;
+}
+
+int32_t psx_csc_extract(psx_csc* csc, const double* kval, int32_t drop_zeros, int32_t* colptr,
+                        int32_t* rowind, double* vals, void* stream) {
+  psx::clear_error();
+  if (!csc || !kval || !colptr || !rowind || !vals) return psx::fail(PSX_EINVAL, "null argument");
+  return psx::launch_csc_extract(csc->d, kval, drop_zeros ? 1 : 0, colptr, rowind, vals,
+                                 static_cast<hipStream_t>(stream));
+}
+
+int32_t psx_free_residual(psx_csc* csc, const double* f_int, const double* f_ext_free, double* r_free,
+                          double* sumsq, void* stream) {
+  psx::clear_error();
+  if (!csc || !f_int || !r_free || !sumsq) return psx::fail(PSX_EINVAL, "null argument");
+  return psx::launch_free_residual(csc->d, f_int, f_ext_free, r_free, sumsq,
+                                   static_cast<hipStream_t>(stream));
 }
 
 int32_t psx_element_batch(int32_t npe, int64_t n, const double* X, const double* u, const double* mu,

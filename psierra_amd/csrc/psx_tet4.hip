@@ -18,6 +18,7 @@
 
 #include "psx_internal.h"
 #include "psx_nh.h"
+#include "psx_tet4_elem.h"
 
 namespace psx {
 
@@ -45,61 +46,6 @@ __global__ void permute_u_kernel(DevPlan p, const double* __restrict__ u, int32_
 __device__ __forceinline__ void flag_inversion(int32_t* status, int e) {
   atomicMin(&status[1], e);
   status[0] = 1;
-}
-
-// Element seen from its local node 0 (callers rotate the node order so the
-// row node comes first).  Writes the four 3x3 blocks K_{0b} (b = 0..3, row-
-// major, 36 values) and f_0 (3 values).  Returns J = det F.
-template <typename R, bool WANT_K>
-__device__ __forceinline__ R tet4_row(const double4 X[4], const double4 U[4], R mu, R K, R* out) {
-  R D[3][3], dU[3][3];
-#pragma unroll
-  for (int b = 0; b < 3; ++b) {
-    D[0][b] = R(X[b + 1].x - X[0].x);
-    D[1][b] = R(X[b + 1].y - X[0].y);
-    D[2][b] = R(X[b + 1].z - X[0].z);
-    dU[0][b] = R(U[b + 1].x - U[0].x);
-    dU[1][b] = R(U[b + 1].y - U[0].y);
-    dU[2][b] = R(U[b + 1].z - U[0].z);
-  }
-  R adj[3][3];
-  adj3(D, adj);
-  const R det = D[0][0] * adj[0][0] + D[0][1] * adj[1][0] + D[0][2] * adj[2][0];
-  const R V = fabs(det) * R(1.0 / 6.0);
-  const R idet = R(1) / det;
-  R h[4][3];
-#pragma unroll
-  for (int b = 0; b < 3; ++b)
-#pragma unroll
-    for (int c = 0; c < 3; ++c) h[b + 1][c] = adj[b][c] * idet;
-#pragma unroll
-  for (int c = 0; c < 3; ++c) h[0][c] = -(h[1][c] + h[2][c] + h[3][c]);
-  // A = grad u = sum_b (u_b - u_0) (x) h_b
-  R A[3][3];
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
-#pragma unroll
-    for (int j = 0; j < 3; ++j) A[i][j] = dU[i][0] * h[1][j] + dU[i][1] * h[2][j] + dU[i][2] * h[3][j];
-  NHPoint<R> m;
-  nh_point(A, mu, K, m);
-  R Sh0[3];
-  mat3vec(m.S, h[0], Sh0);
-  // f_0 = V F S h_0
-  R FSh[3];
-  mat3vec(m.F, Sh0, FSh);
-#pragma unroll
-  for (int i = 0; i < 3; ++i) out[(WANT_K ? 36 : 0) + i] = V * FSh[i];
-  if (WANT_K) {
-    R g[4][3], q[4][3];
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      mat3vec(m.Gm, h[b], g[b]);
-      mat3vec(m.F, h[b], q[b]);
-    }
-#pragma unroll
-    for (int b = 0; b < 4; ++b) nh_block(m, V, g[0], q[0], Sh0, g[b], q[b], h[b], out + 9 * b);
-  }
-  return m.J;
 }
 
 template <typename R, int TILE, bool WANT_K>

@@ -1,0 +1,183 @@
+// Tet4 element routines shared by the assembly and adjoint kernels.
+#pragma once
+
+#include "psx_nh.h"
+
+namespace psx {
+
+// Geometry and displacement gradient of a Tet4 from its four nodes
+// (reference coordinates and displacements as double4, differences taken in
+// FP64 before rounding to R).  h[b] = grad_X N_b, V = |det D| / 6 with D the
+// edge matrix (det([1 X]) of src/psierra/element/tet4_base.py:51-55 up to
+// the vertex order), A = grad_X u.
+template <typename R>
+__device__ __forceinline__ void tet4_geometry(const double4 X[4], const double4 U[4], R h[4][3], R& V,
+                                              R A[3][3]) {
+  R D[3][3], dU[3][3];
+#pragma unroll
+  for (int b = 0; b < 3; ++b) {
+    D[0][b] = R(X[b + 1].x - X[0].x);
+    D[1][b] = R(X[b + 1].y - X[0].y);
+    D[2][b] = R(X[b + 1].z - X[0].z);
+    dU[0][b] = R(U[b + 1].x - U[0].x);
+    dU[1][b] = R(U[b + 1].y - U[0].y);
+    dU[2][b] = R(U[b + 1].z - U[0].z);
+  }
+  R adj[3][3];
+  adj3(D, adj);
+  const R det = D[0][0] * adj[0][0] + D[0][1] * adj[1][0] + D[0][2] * adj[2][0];
+  V = fabs(det) * R(1.0 / 6.0);
+  const R idet = R(1) / det;
+#pragma unroll
+  for (int b = 0; b < 3; ++b)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) h[b + 1][c] = adj[b][c] * idet;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) h[0][c] = -(h[1][c] + h[2][c] + h[3][c]);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) A[i][j] = dU[i][0] * h[1][j] + dU[i][1] * h[2][j] + dU[i][2] * h[3][j];
+}
+
+// Element seen from its local node 0 (callers rotate the node order so the
+// row node comes first).  Writes the four 3x3 blocks K_{0b} (b = 0..3, row-
+// major, 36 values) and f_0 (3 values).  Returns J = det F.
+template <typename R, bool WANT_K>
+__device__ __forceinline__ R tet4_row(const double4 X[4], const double4 U[4], R mu, R K, R* out) {
+  R h[4][3], V, A[3][3];
+  tet4_geometry(X, U, h, V, A);
+  NHPoint<R> m;
+  nh_point(A, mu, K, m);
+  R Sh0[3];
+  mat3vec(m.S, h[0], Sh0);
+  // f_0 = V F S h_0
+  R FSh[3];
+  mat3vec(m.F, Sh0, FSh);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) out[(WANT_K ? 36 : 0) + i] = V * FSh[i];
+  if (WANT_K) {
+    R g[4][3], q[4][3];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      mat3vec(m.Gm, h[b], g[b]);
+      mat3vec(m.F, h[b], q[b]);
+    }
+#pragma unroll
+    for (int b = 0; b < 4; ++b) nh_block(m, V, g[0], q[0], Sh0, g[b], q[b], h[b], out + 9 * b);
+  }
+  return m.J;
+}
+
+// Adjoint of f_int for one incidence: the row-0 part of K_e w,
+//   (K_e w)_0 = V [ a tr(L) g_0 + b/2 (L + L^T) g_0 + c ((F : grad w) g_0
+//               + tr(L) q_0) + grad w S h_0 ],
+// with grad w = sum_b w_b (x) h_b and L = grad w F^-1 (the spatial gradient
+// of w); the closed form of sum_b K_0b w_b with the blocks above.  Also the
+// element's parameter sensitivities of w^T f_e (f is linear in mu, lambda):
+//   d/dmu = V dS/dmu : (F^T grad w),  d/dlambda = V ln J C^-1 : (F^T grad w).
+template <typename R>
+__device__ __forceinline__ R tet4_vjp_row(const double4 X[4], const double4 U[4], const double4 W[4],
+                                          R mu, R K, R out[3], R& gmu, R& glam) {
+  R h[4][3], V, A[3][3];
+  tet4_geometry(X, U, h, V, A);
+  NHPoint<R> m;
+  nh_point(A, mu, K, m);
+  R dW[3][3];
+#pragma unroll
+  for (int b = 0; b < 3; ++b) {
+    dW[0][b] = R(W[b + 1].x - W[0].x);
+    dW[1][b] = R(W[b + 1].y - W[0].y);
+    dW[2][b] = R(W[b + 1].z - W[0].z);
+  }
+  R Gw[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) Gw[i][j] = dW[i][0] * h[1][j] + dW[i][1] * h[2][j] + dW[i][2] * h[3][j];
+  // L = Gw F^-1 ; F^-1[j][k] = Gm[k][j]
+  R L[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) L[i][k] = Gw[i][0] * m.Gm[k][0] + Gw[i][1] * m.Gm[k][1] + Gw[i][2] * m.Gm[k][2];
+  const R trL = L[0][0] + L[1][1] + L[2][2];
+  R FdW = R(0);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) FdW += m.F[i][j] * Gw[i][j];
+  R g0[3], q0[3], Sh0[3], GSh[3];
+  mat3vec(m.Gm, h[0], g0);
+  mat3vec(m.F, h[0], q0);
+  mat3vec(m.S, h[0], Sh0);
+  mat3vec(Gw, Sh0, GSh);
+  const R c1 = m.al * trL + m.ga * FdW;
+  const R c2 = m.ga * trL;
+  const R hb = R(0.5) * m.be;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const R Lg = (L[i][0] + L[0][i]) * g0[0] + (L[i][1] + L[1][i]) * g0[1] + (L[i][2] + L[2][i]) * g0[2];
+    out[i] = V * (c1 * g0[i] + hb * Lg + c2 * q0[i] + GSh[i]);
+  }
+  // M = F^T grad w
+  R M[3][3];
+#pragma unroll
+  for (int I = 0; I < 3; ++I)
+#pragma unroll
+    for (int J = 0; J < 3; ++J) M[I][J] = m.F[0][I] * Gw[0][J] + m.F[1][I] * Gw[1][J] + m.F[2][I] * Gw[2][J];
+  R Smu[3][3];
+  nh_dS_dmu(m, Smu);
+  R smu = R(0), slam = R(0);
+#pragma unroll
+  for (int I = 0; I < 3; ++I)
+#pragma unroll
+    for (int J = 0; J < 3; ++J) {
+      smu += Smu[I][J] * M[I][J];
+      slam += m.Ci[I][J] * M[I][J];
+    }
+  gmu = V * smu;
+  glam = V * m.lnJ * slam;
+  return m.J;
+}
+
+
+// Whole element from precomputed geometry: k (12x12 row-major, scaled by w)
+// and f (12, scaled by w).  h[b] = grad_X N_b, V = volume, A = grad u.
+// The material point is evaluated once; K_ba = K_ab^T fills the lower half.
+template <typename R>
+__device__ __forceinline__ R tet4_full(const R h[4][3], R V, const R A[3][3], R mu, R K, R w, R* k,
+                                       R* f) {
+  NHPoint<R> m;
+  nh_point(A, mu, K, m);
+  R g[4][3], q[4][3], Sh[4][3];
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    mat3vec(m.Gm, h[b], g[b]);
+    mat3vec(m.F, h[b], q[b]);
+    mat3vec(m.S, h[b], Sh[b]);
+  }
+  const R Vw = V * w;
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    R FSh[3];
+    mat3vec(m.F, Sh[a], FSh);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) f[3 * a + i] = Vw * FSh[i];
+#pragma unroll
+    for (int b = a; b < 4; ++b) {
+      R blk[9];
+      nh_block(m, Vw, g[a], q[a], Sh[a], g[b], q[b], h[b], blk);
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          k[12 * (3 * a + i) + 3 * b + c] = blk[3 * i + c];
+          if (b != a) k[12 * (3 * b + c) + 3 * a + i] = blk[3 * i + c];
+        }
+    }
+  }
+  return m.J;
+}
+
+}  // namespace psx

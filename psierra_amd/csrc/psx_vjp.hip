@@ -1,0 +1,128 @@
+// Adjoint (vector-Jacobian product) of the Tet4 internal force.
+//
+// For a cotangent w (one value per DOF) it returns, with f = f_int(u; mu, lambda):
+//   g_u      = d(w^T f)/du      = K_T w   (K_T symmetric),
+//   g_mu[e]  = d(w^T f)/d mu_e  = V_e dS/dmu : (F^T grad w),
+//   g_lam[e] = d(w^T f)/d lam_e = V_e ln J C^-1 : (F^T grad w).
+// pSierra has no analytic gradient: its parameter identification uses a
+// 2-point finite-difference Jacobian of a global parameter through
+// scipy.optimize.least_squares (src/psierra/inverse/parameter_identification.py:370-379);
+// by the chain rule that derivative is sum_e g_mu[e] (or g_lam).
+//
+// Matrix-free and owner-computes like the assembly: a work-group takes a tile
+// of nodes, lane t evaluates incidence t's row of K_e w (3 values) into LDS,
+// then node lanes sum in FP64 in incidence order.  The per-element outputs
+// are written by the single incidence with local node 0.  Deterministic,
+// no atomics.
+#include <climits>
+
+#include "psx_internal.h"
+#include "psx_tet4_elem.h"
+
+namespace psx {
+
+__global__ void permute_w_kernel(DevPlan p, const double* __restrict__ w) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < p.N) {
+    const int64_t a = p.perm[i];
+    p.wi[i] = make_double4(w[3 * a], w[3 * a + 1], w[3 * a + 2], 0.0);
+  }
+}
+
+template <typename R, int TILE>
+__global__ __launch_bounds__(TILE) void tet4_vjp_kernel(DevPlan p, const double* __restrict__ mu,
+                                                        const double* __restrict__ lam,
+                                                        double* __restrict__ gu,
+                                                        double* __restrict__ gmu,
+                                                        double* __restrict__ glam, int32_t* status) {
+  __shared__ R contrib[3 * TILE];
+  __shared__ int32_t s_incp[TILE + 1];
+  const int per = gridDim.x >> 3;
+  const int tile = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  if (tile >= p.n_tiles) return;
+  const int n0 = p.tile_node[tile], n1 = p.tile_node[tile + 1];
+  const int nn = n1 - n0;
+  const int k0 = p.inc_ptr[n0];
+  const int nk = p.inc_ptr[n1] - k0;
+  const int t = threadIdx.x;
+  for (int i = t; i <= nn; i += TILE) s_incp[i] = p.inc_ptr[n0 + i] - k0;
+  if (t < nk) {
+    const uint32_t pe = p.inc_elem[k0 + t];
+    const int e = int(pe & kIncElemMask);
+    const int lp = int(pe >> kIncElemBits);
+    const int4 c = *reinterpret_cast<const int4*>(p.conn + 4 * int64_t(e));
+    const int ids[4] = {lp == 0 ? c.x : lp == 1 ? c.y : lp == 2 ? c.z : c.w, lp == 0 ? c.y : c.x,
+                        lp <= 1 ? c.z : c.y, lp <= 2 ? c.w : c.z};
+    double4 X[4], U[4], W[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      X[b] = p.Xi[ids[b]];
+      U[b] = p.ui[ids[b]];
+      W[b] = p.wi[ids[b]];
+    }
+    const double m = mu[e], l = lam[e];
+    R out[3], dmu, dlam;
+    const R J = tet4_vjp_row<R>(X, U, W, R(m), R(l + (2.0 / 3.0) * m), out, dmu, dlam);
+    if (J <= R(0)) {
+      atomicMin(&status[1], e);
+      status[0] = 1;
+    }
+#pragma unroll
+    for (int v = 0; v < 3; ++v) contrib[v * TILE + t] = out[v];
+    if (lp == 0 && gmu) {
+      gmu[e] = double(dmu);
+      glam[e] = double(dlam);
+    }
+  }
+  __syncthreads();
+  for (int j = t; j < nn; j += TILE) {
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int kk = s_incp[j]; kk < s_incp[j + 1]; ++kk)
+#pragma unroll
+      for (int v = 0; v < 3; ++v) acc[v] += double(contrib[v * TILE + kk]);
+    const int64_t a = p.perm[n0 + j];
+#pragma unroll
+    for (int v = 0; v < 3; ++v) gu[3 * a + v] = acc[v];
+  }
+}
+
+int32_t launch_permute_u(const DevPlan& p, const double* u, hipStream_t s, int32_t* status);
+
+template <typename R, int TILE>
+static void launch_vjp_t(const DevPlan& p, const double* mu, const double* lam, double* gu, double* gmu,
+                         double* glam, hipStream_t s, int32_t* status) {
+  const int blocks = 8 * ((p.n_tiles + 7) / 8);
+  hipLaunchKernelGGL((tet4_vjp_kernel<R, TILE>), dim3(blocks), dim3(TILE), 0, s, p, mu, lam, gu, gmu,
+                     glam, status);
+}
+
+int32_t launch_tet4_vjp(const DevPlan& p, const double* u, const double* mu, const double* lam,
+                        const double* w, double* gu, double* gmu, double* glam, int precision,
+                        hipStream_t s, int32_t* status) {
+  int32_t rc = launch_permute_u(p, u, s, status);
+  if (rc) return rc;
+  hipLaunchKernelGGL(permute_w_kernel, dim3((p.N + 255) / 256), dim3(256), 0, s, p, w);
+  PSX_HIP_TRY(hipGetLastError());
+  if (p.n_tiles <= 0) return PSX_OK;
+  const bool fp32 = precision == PSX_FP32;
+  switch (p.tile) {
+    case 64:
+      fp32 ? launch_vjp_t<float, 64>(p, mu, lam, gu, gmu, glam, s, status)
+           : launch_vjp_t<double, 64>(p, mu, lam, gu, gmu, glam, s, status);
+      break;
+    case 128:
+      fp32 ? launch_vjp_t<float, 128>(p, mu, lam, gu, gmu, glam, s, status)
+           : launch_vjp_t<double, 128>(p, mu, lam, gu, gmu, glam, s, status);
+      break;
+    case 256:
+      fp32 ? launch_vjp_t<float, 256>(p, mu, lam, gu, gmu, glam, s, status)
+           : launch_vjp_t<double, 256>(p, mu, lam, gu, gmu, glam, s, status);
+      break;
+    default:
+      return fail(PSX_EINVAL, "unsupported tile");
+  }
+  PSX_HIP_TRY(hipGetLastError());
+  return PSX_OK;
+}
+
+}  // namespace psx

@@ -173,6 +173,28 @@ class AssemblyPlan:
             self.raise_on_status(u)
         return f, kval
 
+    def vjp(self, u, mu, lam, w, precision="fp64", want_params=True, check_status=True):
+        """(K_T w, d(w^T f)/dmu, d(w^T f)/dlambda) as device tensors (Tet4 plans)."""
+        torch = _torch()
+        u = self._device_f64(u, self.n_dofs, "u")
+        w = self._device_f64(w, self.n_dofs, "w")
+        mu = self._device_f64(mu, self.n_elements, "mu")
+        lam = self._device_f64(lam, self.n_elements, "lam")
+        gu = torch.empty(self.n_dofs, dtype=torch.float64, device=self.device)
+        gmu = glam = None
+        if want_params:
+            gmu = torch.empty(self.n_elements, dtype=torch.float64, device=self.device)
+            glam = torch.empty(self.n_elements, dtype=torch.float64, device=self.device)
+        check(
+            lib().psx_vjp(
+                self._h, ptr(u), ptr(mu), ptr(lam), ptr(w), ptr(gu), ptr(gmu), ptr(glam),
+                precision_code(precision), current_stream_handle(), ptr(self.status),
+            )
+        )
+        if check_status:
+            self.raise_on_status(u)
+        return gu, gmu, glam
+
     def raise_on_status(self, u):
         st = self.status.cpu().numpy()
         if st[0] != 0:

@@ -1,0 +1,76 @@
+"""hipGraph capture of the assembly launches and a tile autotuner.
+
+``GraphedAssembly`` captures one ``psx_assemble`` call (displacement
+permutation + assembly kernel, no allocation, no memset, no host sync) into
+a HIP graph through torch's graph API; ``replay()`` re-runs it on whatever
+the captured input buffers hold, which is what a Newton loop needs: copy the
+new iterate into ``u`` and replay.
+
+``autotune_tile`` times the work-group tile sizes the kernels are built for
+on the caller's own plan and inputs and keeps the fastest.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .engine import AssemblyPlan, _torch
+
+
+class GraphedAssembly:
+    def __init__(self, plan: AssemblyPlan, mu, lam, precision: str = "fp32", want_k: bool = True, warmup: int = 2):
+        torch = _torch()
+        self.plan = plan
+        dev = plan.device
+        self.u = torch.zeros(plan.n_dofs, dtype=torch.float64, device=dev)
+        self.mu = torch.as_tensor(mu, dtype=torch.float64, device=dev).contiguous()
+        self.lam = torch.as_tensor(lam, dtype=torch.float64, device=dev).contiguous()
+        self.f, self.k = plan.alloc_outputs(want_k)
+        self.precision = precision
+        self.want_k = want_k
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            for _ in range(warmup):
+                self._launch()
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self._launch()
+        torch.cuda.synchronize()
+
+    def _launch(self):
+        self.plan.assemble(self.u, self.mu, self.lam, f=self.f, kval=self.k, precision=self.precision,
+                           want_k=self.want_k, check_status=False)
+
+    def replay(self, u=None, check_status: bool = False):
+        if u is not None:
+            self.u.copy_(self.plan._device_f64(u, self.plan.n_dofs, "u"))
+        self.graph.replay()
+        if check_status:
+            self.plan.raise_on_status(self.u)
+        return self.f, self.k
+
+
+def autotune_tile(plan: AssemblyPlan, u, mu, lam, precision: str = "fp32", reps: int = 5, tiles=None):
+    """Pick the fastest work-group tile for this plan; returns {tile: ms}."""
+    torch = _torch()
+    if tiles is None:
+        tiles = (64, 128, 256) if plan.npe == 4 else (64, 128)
+    f, k = plan.alloc_outputs(True)
+    timings = {}
+    for tile in tiles:
+        plan.set_tile(tile)
+        plan.assemble(u, mu, lam, f=f, kval=k, precision=precision)
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(reps):
+            plan.assemble(u, mu, lam, f=f, kval=k, precision=precision, check_status=False)
+        b.record()
+        torch.cuda.synchronize()
+        timings[tile] = a.elapsed_time(b) / reps
+    best = min(timings, key=timings.get)
+    plan.set_tile(best)
+    plan.raise_on_status(plan._device_f64(u, plan.n_dofs, "u"))
+    return best, timings
