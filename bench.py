@@ -53,6 +53,115 @@ def cpu_baseline(wl, sample_elems, seed=0):
     return sample_elems / t_loop, sample_elems / t_vec, t_loop, t_vec
 
 
+def _event_time(fn, steps, warmup):
+    import torch
+
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(steps):
+        fn()
+    b.record()
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / steps
+
+
+def bench_tet10(args):
+    """Tet10 40^3 box (384k elements, 531k nodes): f_int + K_T assembly."""
+    import torch
+
+    from psierra_amd import workloads as W
+    from psierra_amd.engine import AssemblyPlan
+
+    wl = W.jittered_box_tet10(n=args.n if args.n != 100 else 40, seed=2)
+    plan = AssemblyPlan(wl.mesh.nodes, wl.mesh.elements)
+    dev = plan.device
+    u = torch.as_tensor(wl.u.reshape(-1), device=dev)
+    mu = torch.as_tensor(wl.mu, device=dev)
+    lam = torch.as_tensor(wl.lam, device=dev)
+    f, k = plan.alloc_outputs(True)
+    res = {}
+    for prec in ("fp64", "fp32"):
+        ms = _event_time(lambda: plan.assemble(u, mu, lam, f=f, kval=k, precision=prec, check_status=False),
+                         args.steps, args.warmup)
+        plan.raise_on_status(u)
+        res[prec] = ms
+    E, N, nnz = plan.n_elements, plan.n_nodes, plan.nnz
+    b_alg = algorithmic_bytes(E, N, 10, nnz, coord_bytes=4, param_bytes=4)
+    ach = b_alg / (res["fp32"] * 1e-3) / 1e9
+    return {
+        "metric": "Tet10 NeoHookean f_int+K_T assembly elements/s",
+        "value": E / (res["fp32"] * 1e-3), "unit": "elements/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": res["fp32"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": wl.name, "elements": E, "nodes": N, "nnz": nnz, "fp64_plain_ms": res["fp64"],
+                   "speedup_fp32_vs_fp64_plain": res["fp64"] / res["fp32"], "tile": plan.info["tile"]},
+        "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": ach / HBM_PEAK_GBPS, "traffic": None},
+        "cpu_baseline": None,
+    }
+
+
+def bench_vjp(args):
+    """Adjoint on the 48^3 Tet4 box (663k tets): g_u = K_T w and d/d(mu_e, lambda_e)."""
+    import torch
+
+    from psierra_amd import workloads as W
+    from psierra_amd.engine import AssemblyPlan
+
+    wl = W.vjp_box_tet4(n=48, seed=3)
+    plan = AssemblyPlan(wl.mesh.nodes, wl.mesh.elements)
+    dev = plan.device
+    t = [torch.as_tensor(x, device=dev) for x in (wl.u.reshape(-1), wl.mu, wl.lam, wl.extra["w"])]
+    res = {}
+    for prec in ("fp64", "fp32"):
+        res[prec] = _event_time(lambda: plan.vjp(*t, precision=prec, check_status=False), args.steps, args.warmup)
+        plan.raise_on_status(t[0])
+    E = plan.n_elements
+    return {
+        "metric": "Tet4 NeoHookean adjoint (K_T w, dmu, dlambda) elements/s",
+        "value": E / (res["fp32"] * 1e-3), "unit": "elements/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": res["fp32"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": wl.name, "elements": E, "fp64_ms": res["fp64"]},
+        "roofline": None, "cpu_baseline": None,
+    }
+
+
+def bench_ecsw(args):
+    """128 ECSW instances (8..48 hexes per axis, rank 64, 2% RID) in one launch."""
+    import torch
+
+    from psierra_amd import workloads as W
+    from psierra_amd.ecsw import BatchedECSW
+
+    t0 = time.perf_counter()
+    insts = W.ecsw_instances(n_instances=128, seed=4)
+    t_build = time.perf_counter() - t0
+    b = BatchedECSW(insts)
+    ur = torch.as_tensor(np.stack([i["u_r"] for i in insts]), device=b.device)
+    f_r, K_r = b.evaluate(ur)
+    ms = _event_time(lambda: b.evaluate(ur, f_r, K_r, check_status=False), args.steps, args.warmup)
+    n_rid = b.info["n_rid_elements"]
+    r = 64
+    flop_mfma = 2.0 * r * r * 12 * n_rid
+    flop_total = flop_mfma + n_rid * (2.0 * 12 * 12 * r + 2 * 2.0 * 12 * r)
+    tf = flop_mfma / (ms * 1e-3) / 1e12
+    return {
+        "metric": "ECSW batched reduced tangent sum_e w_e V_e^T K_e V_e, RID elements/s",
+        "value": n_rid / (ms * 1e-3), "unit": "elements/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": "ecsw_128_instances", "rid_elements": n_rid, "work_items": b.info["n_work_items"],
+                   "total_gflop": flop_total / 1e9, "setup_s": t_build},
+        "roofline": {"bound": "mfma", "achieved": tf, "peak": 157.3, "unit": "TFLOP/s", "frac": tf / 157.3,
+                     "traffic": None},
+        "cpu_baseline": None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -63,6 +172,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--tile", type=int, default=0)
+    ap.add_argument("--workload", choices=("tet4", "tet10", "vjp", "ecsw"), default="tet4")
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -70,6 +180,10 @@ def main():
             file=sys.stderr,
         )
         sys.exit(2)
+    if args.workload != "tet4":
+        fn = {"tet10": bench_tet10, "vjp": bench_vjp, "ecsw": bench_ecsw}[args.workload]
+        print(json.dumps(fn(args)), flush=True)
+        return
 
     from psierra_amd import workloads as W
 

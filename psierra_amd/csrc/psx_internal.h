@@ -27,6 +27,10 @@ int32_t fail(int32_t code, const std::string& msg);
 // Incidence packing: element id in the low 28 bits, local node in the top 4.
 constexpr int kIncElemBits = 28;
 constexpr uint32_t kIncElemMask = (1u << kIncElemBits) - 1u;
+// A tile holds at most kTileUnitsPerIncidence * TILE (node, slot) units, and
+// every node at most kMaxNodeIncidences incidences (bit masks are 64-bit).
+constexpr int kTileUnitsPerIncidence = 2;
+constexpr int kMaxNodeIncidences = 64;
 
 // Host-side structural data (all int32 unless noted), built once per mesh.
 struct HostPattern {

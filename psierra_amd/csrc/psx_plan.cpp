@@ -251,19 +251,27 @@ int32_t make_tiles(psx_plan* plan, int32_t tile) {
   const int64_t N = plan->hp.N;
   // Greedy packing of consecutive nodes: at most `tile` incidences and at
   // most `tile` nodes per tile (nodes without elements count as nodes).
+  // Greedy packing of consecutive nodes: at most `tile` incidences, `tile`
+  // nodes and kTileUnitsPerIncidence * tile (node, slot) units per tile.
   std::vector<int32_t> tiles{0};
-  int64_t start_inc = 0, start_node = 0;
+  int64_t start_inc = 0, start_node = 0, units = 0;
+  const int64_t umax = int64_t(kTileUnitsPerIncidence) * tile;
   for (int64_t i = 0; i < N; ++i) {
     int64_t cnt = plan->inc_ptr_h[i + 1] - plan->inc_ptr_h[i];
-    if (cnt > tile)
+    if (cnt > std::min<int64_t>(tile, kMaxNodeIncidences))
       return fail(PSX_EUNSUPPORTED, "node with " + std::to_string(cnt) +
-                                        " incident elements exceeds the tile of " +
-                                        std::to_string(tile));
-    if (plan->inc_ptr_h[i + 1] - start_inc > tile || i + 1 - start_node > tile) {
+                                        " incident elements exceeds the supported " +
+                                        std::to_string(std::min<int64_t>(tile, kMaxNodeIncidences)));
+    if (plan->deg_h[i] > umax)
+      return fail(PSX_EUNSUPPORTED, "node degree exceeds the tile's unit capacity");
+    if (plan->inc_ptr_h[i + 1] - start_inc > tile || i + 1 - start_node > tile ||
+        units + plan->deg_h[i] > umax) {
       tiles.push_back(int32_t(i));
       start_inc = plan->inc_ptr_h[i];
       start_node = i;
+      units = 0;
     }
+    units += plan->deg_h[i];
   }
   tiles.push_back(int32_t(N));
   if (tiles.size() >= 2 && tiles[tiles.size() - 2] == tiles.back()) tiles.pop_back();

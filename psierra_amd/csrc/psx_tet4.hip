@@ -55,10 +55,12 @@ __global__ __launch_bounds__(TILE) void tet4_assemble_kernel(DevPlan p, const do
                                                              double* __restrict__ kval,
                                                              int32_t* status) {
   constexpr int NV = WANT_K ? 39 : 3;
+  constexpr int UMAX = kTileUnitsPerIncidence * TILE;
   __shared__ R contrib[NV * TILE];
   __shared__ uint32_t rk[TILE];
   __shared__ int32_t s_incp[TILE + 1];
   __shared__ int32_t s_uoff[TILE + 1];
+  __shared__ unsigned long long s_mask[WANT_K ? UMAX : 1];
 
   const int tile = xcd_tile();
   if (tile >= p.n_tiles) return;
@@ -71,11 +73,13 @@ __global__ __launch_bounds__(TILE) void tet4_assemble_kernel(DevPlan p, const do
     s_incp[i] = p.inc_ptr[n0 + i] - k0;
     s_uoff[i] = p.uoff[n0 + i] - p.uoff[n0];
   }
+  if (WANT_K)
+    for (int i = t; i < UMAX; i += TILE) s_mask[i] = 0ull;
+  __syncthreads();
   if (t < nk) {
     const uint32_t pe = p.inc_elem[k0 + t];
     const int e = int(pe & kIncElemMask);
     const int lp = int(pe >> kIncElemBits);
-    if (WANT_K) rk[t] = p.inc_rank[k0 + t];
     const int4 c = *reinterpret_cast<const int4*>(p.conn + 4 * int64_t(e));
     const int i0 = lp == 0 ? c.x : lp == 1 ? c.y : lp == 2 ? c.z : c.w;
     const int i1 = lp == 0 ? c.y : c.x;
@@ -91,6 +95,20 @@ __global__ __launch_bounds__(TILE) void tet4_assemble_kernel(DevPlan p, const do
     U[2] = p.ui[i2];
     U[3] = p.ui[i3];
     const double m = mu[e], l = lam[e];
+    if (WANT_K) {
+      // Mark this incidence in the masks of the three off-diagonal slots it
+      // feeds (bit = position within its node's incidence run).
+      const uint32_t r = p.inc_rank[k0 + t];
+      rk[t] = r;
+      int lo = 0, hi = nn;
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (s_incp[mid] <= t) lo = mid; else hi = mid;
+      }
+      const unsigned long long bit = 1ull << (t - s_incp[lo]);
+#pragma unroll
+      for (int q = 1; q < 4; ++q) atomicOr(&s_mask[s_uoff[lo] + ((r >> (8 * q)) & 0xffu)], bit);
+    }
     R out[NV];
     const R J = tet4_row<R, WANT_K>(X, U, R(m), R(l + (2.0 / 3.0) * m), out);
     if (J <= R(0)) flag_inversion(status, e);
@@ -99,54 +117,58 @@ __global__ __launch_bounds__(TILE) void tet4_assemble_kernel(DevPlan p, const do
   }
   __syncthreads();
   if (WANT_K) {
+    // (i) lanes (node, value): the diagonal block (value 0..8) and f (9..11)
+    //     are plain sums over the node's contiguous incidence run.
+    // (ii) lanes (off-diagonal slot): sum the incidences flagged in its mask,
+    //     in increasing incidence order.
     const int units = s_uoff[nn];
-    for (int w = t; w < units; w += TILE) {
-      // node j of the tile owning unit w: s_uoff[j] <= w < s_uoff[j+1]
-      int lo = 0, hi = nn;
-      while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (s_uoff[mid] <= w) lo = mid; else hi = mid;
-      }
-      const int j = lo;
-      const uint32_t s = uint32_t(w - s_uoff[j]);
-      const int node = n0 + j;
-      double acc[9];
-#pragma unroll
-      for (int c = 0; c < 9; ++c) acc[c] = 0.0;
-      double fa[3] = {0.0, 0.0, 0.0};
-      const int kb = s_incp[j], ke = s_incp[j + 1];
-      for (int kk = kb; kk < ke; ++kk) {
-        const uint32_t r = rk[kk];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          if (((r >> (8 * q)) & 0xffu) == s) {
-#pragma unroll
-            for (int c = 0; c < 9; ++c) acc[c] += double(contrib[(9 * q + c) * TILE + kk]);
+    const int nd = nn * 12;
+    for (int w = t; w < nd + units; w += TILE) {
+      if (w < nd) {
+        const int j = w / 12, v = w - 12 * (w / 12);
+        const int kb = s_incp[j], ke = s_incp[j + 1];
+        const int src = v < 9 ? v : 27 + v;
+        double acc = 0.0;
+        for (int kk = kb; kk < ke; ++kk) acc += double(contrib[src * TILE + kk]);
+        const int node = n0 + j;
+        if (v < 9) {
+          if (ke > kb) {
+            const int d = p.deg[node];
+            const int self = int(rk[kb] & 0xffu);
+            kval[p.kofs[node] + (v / 3) * 3 * d + 3 * self + (v % 3)] = acc;
           }
+        } else {
+          f[3 * int64_t(p.perm[node]) + (v - 9)] = acc;
         }
-        if (s == 0) {
-#pragma unroll
-          for (int c = 0; c < 3; ++c) fa[c] += double(contrib[(36 + c) * TILE + kk]);
+      } else {
+        const int u = w - nd;
+        int lo = 0, hi = nn;
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (s_uoff[mid] <= u) lo = mid; else hi = mid;
         }
-      }
-      const int d = p.deg[node];
-      double* kb_out = kval + p.kofs[node] + 3 * int(s);
+        const uint32_t s = uint32_t(u - s_uoff[lo]);
+        const int kb = s_incp[lo];
+        if ((rk[kb] & 0xffu) == s) continue;  // diagonal slot: done in (i)
+        double acc[9];
 #pragma unroll
-      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 9; ++c) acc[c] = 0.0;
+        unsigned long long msk = s_mask[u];
+        while (msk) {
+          const int kk = kb + __ffsll(static_cast<long long>(msk)) - 1;
+          msk &= msk - 1ull;
+          const uint32_t r = rk[kk];
+          const int q = ((r >> 8) & 0xffu) == s ? 1 : ((r >> 16) & 0xffu) == s ? 2 : 3;
 #pragma unroll
-        for (int c = 0; c < 3; ++c) kb_out[r * 3 * d + c] = acc[3 * r + c];
-      if (s == 0) {
-        const int64_t a = p.perm[node];
+          for (int c = 0; c < 9; ++c) acc[c] += double(contrib[(9 * q + c) * TILE + kk]);
+        }
+        const int node = n0 + lo;
+        const int d = p.deg[node];
+        double* kb_out = kval + p.kofs[node] + 3 * int(s);
 #pragma unroll
-        for (int c = 0; c < 3; ++c) f[3 * a + c] = fa[c];
-      }
-    }
-    // nodes used by no element have no CSR slot; their f entries are zero
-    for (int j = t; j < nn; j += TILE) {
-      if (s_uoff[j + 1] == s_uoff[j]) {
-        const int64_t a = p.perm[n0 + j];
+        for (int r = 0; r < 3; ++r)
 #pragma unroll
-        for (int c = 0; c < 3; ++c) f[3 * a + c] = 0.0;
+          for (int c = 0; c < 3; ++c) kb_out[r * 3 * d + c] = acc[3 * r + c];
       }
     }
   } else {

@@ -54,7 +54,16 @@ def test_free_csc_residual_and_newton_step(precision):
     assert rel_err(r.cpu().numpy(), r_ref) < (1e-11 if precision == "fp64" else 1e-5)
     assert abs(rn - np.linalg.norm(r_ref)) <= (1e-11 if precision == "fp64" else 1e-5) * np.linalg.norm(r_ref)
     du = DirectSolver().solve(Kff, r.cpu().numpy())
-    assert rel_err(du, du_ref) < (1e-10 if precision == "fp64" else 1e-4)
+    if precision == "fp64":
+        assert rel_err(du, du_ref) < 1e-10
+    else:
+        # FP32 element errors (~1e-7 of K) are amplified by cond(K_ff) in the
+        # forward error; the backward error against the oracle system is the
+        # precision-independent check.
+        back = np.linalg.norm(Kff_ref @ du - r_ref) / (
+            abs(Kff_ref).max() * np.linalg.norm(du) + np.linalg.norm(r_ref))
+        assert back < 1e-6
+        assert rel_err(du, du_ref) < 1e-3
 
 
 def test_csc_drops_exact_zeros_like_scipy():

@@ -125,7 +125,7 @@ def test_global_parameter_gradient_matches_least_squares_fd(vjp_case):
         f, _ = plan.assemble(t["u"], mu, lam, want_k=False)
         return np.array([float(torch.dot(f, t["w"]))])
 
-    J = approx_derivative(wf, np.array([1.0, 1.0]), method="2-point")
+    J = np.atleast_2d(approx_derivative(wf, np.array([1.0, 1.0]), method="2-point"))
     ref_mu = float((gmu * t["mu"]).sum())
     ref_lam = float((glam * t["lam"]).sum())
     assert abs(J[0, 0] - ref_mu) <= 1e-4 * abs(J[0, 0])
