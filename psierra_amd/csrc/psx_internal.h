@@ -27,9 +27,10 @@ int32_t fail(int32_t code, const std::string& msg);
 // Incidence packing: element id in the low 28 bits, local node in the top 4.
 constexpr int kIncElemBits = 28;
 constexpr uint32_t kIncElemMask = (1u << kIncElemBits) - 1u;
-// A tile holds at most kTileUnitsPerIncidence * TILE (node, slot) units, and
-// every node at most kMaxNodeIncidences incidences (bit masks are 64-bit).
-constexpr int kTileUnitsPerIncidence = 2;
+// A tile of TILE lanes holds at most TILE incidences, and 4 * nodes +
+// (node, slot) units <= TILE (one reduction lane each); every node has at
+// most kMaxNodeIncidences incidences (slot bit masks are 64-bit).
+constexpr int kReduceLanesPerNode = 4;
 constexpr int kMaxNodeIncidences = 64;
 
 // Host-side structural data (all int32 unless noted), built once per mesh.
@@ -52,6 +53,7 @@ struct DevPlan {
   int32_t conn_stride;            // ints per element in conn (4 or 12)
   const int32_t* conn;            // internal node ids, AoS [E][conn_stride]
   const double4* Xi;              // reference coordinates, internal order
+  const float4* geo;              // Tet4: per element h1, h2, h3, V (FP32), 3 float4
   const int32_t* perm;            // internal -> user node id
   const int32_t* inc_ptr;         // N+1, internal order
   const uint32_t* inc_elem;       // packed (element | local << 28)

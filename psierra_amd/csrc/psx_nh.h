@@ -38,9 +38,18 @@ __device__ __forceinline__ double d_log1p<double>(double x) { return log1p(x); }
 template <typename R>
 __device__ __forceinline__ R d_exp(R x);
 template <>
-__device__ __forceinline__ float d_exp<float>(float x) { return expf(x); }
+__device__ __forceinline__ float d_exp<float>(float x) { return __expf(x); }
 template <>
 __device__ __forceinline__ double d_exp<double>(double x) { return exp(x); }
+
+// Reciprocal: IEEE division in FP64; the hardware v_rcp_f32 (1 ulp) in FP32,
+// where the element arithmetic is only required to ~1e-7 relative anyway.
+template <typename R>
+__device__ __forceinline__ R d_rcp(R x);
+template <>
+__device__ __forceinline__ float d_rcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+template <>
+__device__ __forceinline__ double d_rcp<double>(double x) { return 1.0 / x; }
 
 template <typename R>
 __device__ __forceinline__ R det3(const R M[3][3]) {
@@ -95,7 +104,7 @@ __device__ __forceinline__ void nh_point(const R A[3][3], R mu, R K, NHPoint<R>&
   o.lnJ = d_log1p(Jm1);
   R adj[3][3];
   adj3(o.F, adj);
-  const R iJ = R(1) / o.J;
+  const R iJ = d_rcp(o.J);
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -140,20 +149,16 @@ __device__ __forceinline__ void nh_point(const R A[3][3], R mu, R K, NHPoint<R>&
     for (int L = 0; L < 3; ++L)
       Sb[I][L] = R(2) * fac * (E[I][L] - (I == L ? trE * R(1.0 / 3.0) : R(0))) +
                  (I == L ? K * o.lnJ : R(0));
-  R M[3][3];
+  // C^-1 and Sbar commute, so C^-1 Sbar is symmetric: form its upper
+  // triangle and mirror it.
 #pragma unroll
   for (int I = 0; I < 3; ++I)
 #pragma unroll
-    for (int L = 0; L < 3; ++L) {
-      R s = R(0);
-#pragma unroll
-      for (int k = 0; k < 3; ++k) s += o.Ci[I][k] * Sb[k][L];
-      M[I][L] = s;
+    for (int L = I; L < 3; ++L) {
+      const R v = o.Ci[I][0] * Sb[0][L] + o.Ci[I][1] * Sb[1][L] + o.Ci[I][2] * Sb[2][L];
+      o.S[I][L] = v;
+      o.S[L][I] = v;
     }
-#pragma unroll
-  for (int I = 0; I < 3; ++I)
-#pragma unroll
-    for (int L = 0; L < 3; ++L) o.S[I][L] = R(0.5) * (M[I][L] + M[L][I]);
   o.al = K + R(2.0 / 9.0) * fac * I1;
   o.be = R(-2) * K * o.lnJ + R(2.0 / 3.0) * fac * I1;
   o.ga = R(-2.0 / 3.0) * fac;

@@ -251,27 +251,26 @@ int32_t make_tiles(psx_plan* plan, int32_t tile) {
   const int64_t N = plan->hp.N;
   // Greedy packing of consecutive nodes: at most `tile` incidences and at
   // most `tile` nodes per tile (nodes without elements count as nodes).
-  // Greedy packing of consecutive nodes: at most `tile` incidences, `tile`
-  // nodes and kTileUnitsPerIncidence * tile (node, slot) units per tile.
+  // Greedy packing of consecutive nodes: at most `tile` incidences and
+  // kReduceLanesPerNode * nodes + (node, slot) units <= tile per tile.
   std::vector<int32_t> tiles{0};
-  int64_t start_inc = 0, start_node = 0, units = 0;
-  const int64_t umax = int64_t(kTileUnitsPerIncidence) * tile;
+  int64_t start_inc = 0, lanes = 0;
   for (int64_t i = 0; i < N; ++i) {
     int64_t cnt = plan->inc_ptr_h[i + 1] - plan->inc_ptr_h[i];
     if (cnt > std::min<int64_t>(tile, kMaxNodeIncidences))
       return fail(PSX_EUNSUPPORTED, "node with " + std::to_string(cnt) +
                                         " incident elements exceeds the supported " +
                                         std::to_string(std::min<int64_t>(tile, kMaxNodeIncidences)));
-    if (plan->deg_h[i] > umax)
-      return fail(PSX_EUNSUPPORTED, "node degree exceeds the tile's unit capacity");
-    if (plan->inc_ptr_h[i + 1] - start_inc > tile || i + 1 - start_node > tile ||
-        units + plan->deg_h[i] > umax) {
+    // Tet4 reduces with one lane per (node, part) and per (node, slot);
+    // the Tet10 kernel loops its reduction lanes, so only nodes count.
+    const int64_t need = plan->hp.npe == 4 ? kReduceLanesPerNode + plan->deg_h[i] : 1;
+    if (need > tile) return fail(PSX_EUNSUPPORTED, "node degree exceeds the tile's reduction lanes");
+    if (plan->inc_ptr_h[i + 1] - start_inc > tile || lanes + need > tile) {
       tiles.push_back(int32_t(i));
       start_inc = plan->inc_ptr_h[i];
-      start_node = i;
-      units = 0;
+      lanes = 0;
     }
-    units += plan->deg_h[i];
+    lanes += need;
   }
   tiles.push_back(int32_t(N));
   if (tiles.size() >= 2 && tiles[tiles.size() - 2] == tiles.back()) tiles.pop_back();
@@ -318,6 +317,30 @@ int32_t create_plan(const int64_t* conn, int64_t E, int npe, const double* X, in
     for (int64_t k = b; k < e; ++k)
       for (int p = 0; p < npe; ++p) conn_i[k * cs + p] = inv[conn[k * npe + p]];
   });
+  // Tet4: per-element shape-function gradients and volume, computed in FP64
+  // and stored rounded to FP32 for the FP32 element path:
+  // g0 = (h1, h2.x), g1 = (h2.y, h2.z, h3.x, h3.y), g2 = (h3.z, V, 0, 0).
+  std::vector<float4> geo(npe == 4 ? size_t(E) * 3 : 0);
+  if (npe == 4) {
+    parallel_for(E, [&](int64_t b, int64_t e) {
+      for (int64_t k = b; k < e; ++k) {
+        const int64_t* c = conn + 4 * k;
+        double D[3][3];
+        for (int q = 0; q < 3; ++q)
+          for (int r = 0; r < 3; ++r) D[r][q] = X[3 * c[q + 1] + r] - X[3 * c[0] + r];
+        const double a00 = D[1][1] * D[2][2] - D[1][2] * D[2][1], a01 = D[0][2] * D[2][1] - D[0][1] * D[2][2],
+                     a02 = D[0][1] * D[1][2] - D[0][2] * D[1][1], a10 = D[1][2] * D[2][0] - D[1][0] * D[2][2],
+                     a11 = D[0][0] * D[2][2] - D[0][2] * D[2][0], a12 = D[0][2] * D[1][0] - D[0][0] * D[1][2],
+                     a20 = D[1][0] * D[2][1] - D[1][1] * D[2][0], a21 = D[0][1] * D[2][0] - D[0][0] * D[2][1],
+                     a22 = D[0][0] * D[1][1] - D[0][1] * D[1][0];
+        const double det = D[0][0] * a00 + D[0][1] * a10 + D[0][2] * a20;
+        const double id = 1.0 / det;
+        geo[3 * k + 0] = make_float4(float(a00 * id), float(a01 * id), float(a02 * id), float(a10 * id));
+        geo[3 * k + 1] = make_float4(float(a11 * id), float(a12 * id), float(a20 * id), float(a21 * id));
+        geo[3 * k + 2] = make_float4(float(a22 * id), float(det / 6.0), 0.f, 0.f);
+      }
+    });
+  }
   std::vector<double4> Xi(N);
   for (int64_t i = 0; i < N; ++i) {
     int64_t a = plan->perm_h[i];
@@ -384,6 +407,9 @@ int32_t create_plan(const int64_t* conn, int64_t E, int npe, const double* X, in
   int32_t *dconn, *dperm, *dincp, *dkofs, *ddeg, *duoff;
   uint32_t *dince, *dincr;
   double4* dX;
+  float4* dgeo = nullptr;
+  if (npe == 4 && (rc = upload(plan, geo.data(), geo.size(), &dgeo))) return bail(rc);
+  d.geo = dgeo;
   if ((rc = upload(plan, conn_i.data(), conn_i.size(), &dconn)) ||
       (rc = upload(plan, Xi.data(), Xi.size(), &dX)) ||
       (rc = upload(plan, plan->perm_h.data(), plan->perm_h.size(), &dperm)) ||

@@ -40,13 +40,12 @@ __device__ __forceinline__ void tet4_geometry(const double4 X[4], const double4 
     for (int j = 0; j < 3; ++j) A[i][j] = dU[i][0] * h[1][j] + dU[i][1] * h[2][j] + dU[i][2] * h[3][j];
 }
 
-// Element seen from its local node 0 (callers rotate the node order so the
-// row node comes first).  Writes the four 3x3 blocks K_{0b} (b = 0..3, row-
-// major, 36 values) and f_0 (3 values).  Returns J = det F.
+// Row 0 of an element, given the shape-function gradients in the row's node
+// order (h[0] = row node), the volume V and grad u: the four 3x3 blocks
+// K_{0b} (b = 0..3, row-major, 36 values) and f_0 (3 values), or f_0 only.
+// Returns J = det F.
 template <typename R, bool WANT_K>
-__device__ __forceinline__ R tet4_row(const double4 X[4], const double4 U[4], R mu, R K, R* out) {
-  R h[4][3], V, A[3][3];
-  tet4_geometry(X, U, h, V, A);
+__device__ __forceinline__ R tet4_row_core(const R h[4][3], R V, const R A[3][3], R mu, R K, R* out) {
   NHPoint<R> m;
   nh_point(A, mu, K, m);
   R Sh0[3];
@@ -59,14 +58,86 @@ __device__ __forceinline__ R tet4_row(const double4 X[4], const double4 U[4], R 
   if (WANT_K) {
     R g[4][3], q[4][3];
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
+    for (int b = 1; b < 4; ++b) {
       mat3vec(m.Gm, h[b], g[b]);
       mat3vec(m.F, h[b], q[b]);
     }
 #pragma unroll
-    for (int b = 0; b < 4; ++b) nh_block(m, V, g[0], q[0], Sh0, g[b], q[b], h[b], out + 9 * b);
+    for (int c = 0; c < 3; ++c) {
+      g[0][c] = -(g[1][c] + g[2][c] + g[3][c]);
+      q[0][c] = -(q[1][c] + q[2][c] + q[3][c]);
+    }
+    // K_0b = P g_b^T + g_b Rv^T + T q_b^T + (g_b . Rv + h_b . W) I
+    R P[3], Rv[3], T[3], W[3];
+    const R va = V * m.al, vb = R(0.5) * V * m.be, vg = V * m.ga;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      P[i] = va * g[0][i] + vg * q[0][i];
+      Rv[i] = vb * g[0][i];
+      T[i] = vg * g[0][i];
+      W[i] = V * Sh0[i];
+    }
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const R sb = g[b][0] * Rv[0] + g[b][1] * Rv[1] + g[b][2] * Rv[2] + h[b][0] * W[0] + h[b][1] * W[1] +
+                   h[b][2] * W[2];
+      R* o = out + 9 * b;
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int k = (b == 0 ? i : 0); k < 3; ++k) {
+          const R v = P[i] * g[b][k] + g[b][i] * Rv[k] + T[i] * q[b][k] + (i == k ? sb : R(0));
+          o[3 * i + k] = v;
+          if (b == 0 && k != i) o[3 * k + i] = v;  // K_00 is symmetric
+        }
+    }
   }
   return m.J;
+}
+
+// Element seen from its local node 0 (callers rotate the node order so the
+// row node comes first), geometry from the reference coordinates.
+template <typename R, bool WANT_K>
+__device__ __forceinline__ R tet4_row(const double4 X[4], const double4 U[4], R mu, R K, R* out) {
+  R h[4][3], V, A[3][3];
+  tet4_geometry(X, U, h, V, A);
+  return tet4_row_core<R, WANT_K>(h, V, A, mu, K, out);
+}
+
+// Same row from precomputed per-element geometry in the element's own node
+// order (g0 = (h1, h2.x), g1 = (h2.yz, h3.xy), g2 = (h3.z, V)), the
+// displacements U in the element's node order and the row node's local
+// index lp; the gradients are rotated to the row order [lp, others
+// ascending] after grad u is formed.
+template <bool WANT_K>
+__device__ __forceinline__ float tet4_row_geo(const float4 g0, const float4 g1, const float4 g2,
+                                              const double4 U[4], int lp, float mu, float K, float* out) {
+  float h[4][3] = {{0.f, 0.f, 0.f}, {g0.x, g0.y, g0.z}, {g0.w, g1.x, g1.y}, {g1.z, g1.w, g2.x}};
+#pragma unroll
+  for (int c = 0; c < 3; ++c) h[0][c] = -(h[1][c] + h[2][c] + h[3][c]);
+  float A[3][3];
+  {
+    float dU[3][3];
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      dU[0][b] = float(U[b + 1].x - U[0].x);
+      dU[1][b] = float(U[b + 1].y - U[0].y);
+      dU[2][b] = float(U[b + 1].z - U[0].z);
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) A[i][j] = dU[i][0] * h[1][j] + dU[i][1] * h[2][j] + dU[i][2] * h[3][j];
+  }
+  float hr[4][3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    hr[0][c] = lp == 0 ? h[0][c] : lp == 1 ? h[1][c] : lp == 2 ? h[2][c] : h[3][c];
+    hr[1][c] = lp == 0 ? h[1][c] : h[0][c];
+    hr[2][c] = lp <= 1 ? h[2][c] : h[1][c];
+    hr[3][c] = lp <= 2 ? h[3][c] : h[2][c];
+  }
+  return tet4_row_core<float, WANT_K>(hr, g2.y, A, mu, K, out);
 }
 
 // Adjoint of f_int for one incidence: the row-0 part of K_e w,

@@ -70,7 +70,9 @@ def test_force_only_matches_fused(jittered, precision):
     banner(f"tet4 f-only {precision}")
     _, f_full, _ = gpu_assemble(wl, precision)
     _, f_only, _ = gpu_assemble(wl, precision, want_k=False)
-    assert rel_err(f_only, f_full) < 1e-13
+    # FP64: same arithmetic; FP32: the two kernels may contract FMAs
+    # differently, so they agree to FP32 rounding, not bit for bit.
+    assert rel_err(f_only, f_full) < (1e-13 if precision == "fp64" else 1e-6)
 
 
 def test_inversion_raises_reference_message():
