@@ -63,6 +63,8 @@ struct DevPlan {
   const int32_t* deg;             // N: neighbours of the node
   const int32_t* uoff;            // N+1: prefix of deg in internal order
   const int32_t* tile_node;       // n_tiles+1
+  const int4* tile_info;          // n_tiles: first node, nodes, first incidence, incidences
+  const int4* node_rec;           // Tet4, N+1: first incidence, first unit, kofs, self slot rank
   double4* ui;                    // scratch: displacement in internal order
   double4* wi;                    // scratch: adjoint cotangent in internal order
 };

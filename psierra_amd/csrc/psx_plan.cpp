@@ -274,10 +274,19 @@ int32_t make_tiles(psx_plan* plan, int32_t tile) {
   }
   tiles.push_back(int32_t(N));
   if (tiles.size() >= 2 && tiles[tiles.size() - 2] == tiles.back()) tiles.pop_back();
+  // per tile {first node, nodes, first incidence, incidences}
+  std::vector<int4> info(tiles.size() - 1);
+  for (size_t t = 0; t + 1 < tiles.size(); ++t) {
+    const int32_t k0 = plan->inc_ptr_h[tiles[t]];
+    info[t] = make_int4(tiles[t], tiles[t + 1] - tiles[t], k0, plan->inc_ptr_h[tiles[t + 1]] - k0);
+  }
   int32_t* dtiles = nullptr;
+  int4* dinfo = nullptr;
   int32_t rc = upload(plan, tiles.data(), tiles.size(), &dtiles);
   if (rc) return rc;
+  if ((rc = upload(plan, info.data(), info.size(), &dinfo))) return rc;
   plan->d.tile_node = dtiles;
+  plan->d.tile_info = dinfo;
   plan->d.n_tiles = int32_t(tiles.size() - 1);
   plan->d.tile = tile;
   return PSX_OK;
@@ -381,8 +390,10 @@ int32_t create_plan(const int64_t* conn, int64_t E, int npe, const double* X, in
         };
         if (npe == 4) {
           // Tet4 incidences are evaluated with the row node first, then the
-          // other three in increasing local order (see tet4 kernel).
-          uint32_t r = rank_of(conn[el * 4 + p]);
+          // other three in increasing local order (see tet4 kernel).  Byte 0
+          // is the incidence's position in its node's run (< 64); bytes 1..3
+          // are the slot ranks of the other three nodes in a's row.
+          uint32_t r = uint32_t(k - plan->inc_ptr_h[i]);
           int slot = 1;
           for (int q = 0; q < 4; ++q) {
             if (q == p) continue;
@@ -398,6 +409,23 @@ int32_t create_plan(const int64_t* conn, int64_t E, int npe, const double* X, in
     }
   });
 
+  // Tet4 node records: {first incidence, first unit, first CSR value, slot
+  // rank of the node in its own row}; record N closes the last node.
+  std::vector<int4> node_rec;
+  if (npe == 4) {
+    node_rec.resize(N + 1);
+    parallel_for(N, [&](int64_t b, int64_t e) {
+      for (int64_t i = b; i < e; ++i) {
+        const int64_t a = plan->perm_h[i];
+        const int32_t* nb = hp.adj.data() + hp.adj_ptr[a];
+        const int32_t d = hp.adj_ptr[a + 1] - hp.adj_ptr[a];
+        const int32_t self = int32_t(std::lower_bound(nb, nb + d, int32_t(a)) - nb);
+        node_rec[i] = make_int4(plan->inc_ptr_h[i], uoff[i], kofs[i], self);
+      }
+    });
+    node_rec[N] = make_int4(plan->inc_ptr_h[N], uoff[N], 0, 0);
+  }
+
   psx::DevPlan& d = plan->d;
   d.E = int32_t(E);
   d.N = int32_t(N);
@@ -408,8 +436,12 @@ int32_t create_plan(const int64_t* conn, int64_t E, int npe, const double* X, in
   uint32_t *dince, *dincr;
   double4* dX;
   float4* dgeo = nullptr;
-  if (npe == 4 && (rc = upload(plan, geo.data(), geo.size(), &dgeo))) return bail(rc);
+  int4* dnrec = nullptr;
+  if (npe == 4 && ((rc = upload(plan, geo.data(), geo.size(), &dgeo)) ||
+                   (rc = upload(plan, node_rec.data(), node_rec.size(), &dnrec))))
+    return bail(rc);
   d.geo = dgeo;
+  d.node_rec = dnrec;
   if ((rc = upload(plan, conn_i.data(), conn_i.size(), &dconn)) ||
       (rc = upload(plan, Xi.data(), Xi.size(), &dX)) ||
       (rc = upload(plan, plan->perm_h.data(), plan->perm_h.size(), &dperm)) ||

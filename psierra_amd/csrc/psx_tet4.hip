@@ -2,16 +2,25 @@
 //
 // Global assembly is owner-computes over node rows.  The plan orders nodes
 // along a Morton curve (locality) and packs consecutive nodes into tiles of
-// at most TILE element incidences.  One work-group handles one tile:
+// at most TILE element incidences and at most TILE reduction lanes
+// (4 per node + one per (node, slot) unit).  One work-group handles one tile:
 //   phase 1: lane t takes incidence t (element e seen from one of its nodes
-//            a): it gathers the four nodes' X and u, evaluates the element in
-//            registers (FP64 or FP32) and leaves the 4 tangent blocks of row
-//            a (36 values) and f_a (3 values) in LDS;
-//   phase 2: lane (a, s) sums, in FP64 and in incidence order, the blocks
-//            whose column node has slot rank s in a's CSR row, and writes the
-//            3x3 block straight into the CSR values; f_a likewise.
-// Every CSR value and every f entry is written exactly once by one lane, in
-// a fixed order: the scatter is deterministic and uses no atomics.
+//            a): it gathers the element's nodal displacements, evaluates the
+//            element in registers (FP64, or FP32 from precomputed gradients)
+//            and leaves row a's contributions in LDS -- the diagonal block
+//            K_aa (6 values, symmetric), f_a (3) and the three off-diagonal
+//            blocks K_ab (27);
+//   phase 2: every reduction lane sums, in FP64 and in increasing incidence
+//            order, one list of 9-value contributions: four lanes per node
+//            take a quarter of the node's incidence run each (K_aa and f_a,
+//            combined across the four lanes in a fixed order), one lane per
+//            off-diagonal CSR block walks the incidences that feed it.  The
+//            lists have similar lengths, so the lanes of a wave stay in step.
+//   phase 3: the tile's CSR values, staged in LDS, are written node block by
+//            node block with consecutive lanes on consecutive values.
+// Every CSR value and every f entry is written exactly once by one lane, and
+// every sum runs in an order fixed by the mesh alone (not by the tile size):
+// the scatter is deterministic and uses no floating-point atomics.
 // This replaces SolidDomain's element loops + np.add.at + coo.tocsr()
 // (src/psierra/domain/solid.py:230-325, src/psierra/utility/assembly.py:37-108).
 #include <climits>
@@ -33,6 +42,13 @@ __device__ __forceinline__ int xcd_tile() {
   return (b & 7) * per + (b >> 3);
 }
 
+// A work-group-uniform record read through the scalar unit (constant
+// address space), so it needs no vector load and no lane broadcast.
+__device__ __forceinline__ int4 load_uniform(const int4* p, int i) {
+  const auto* q = (const __attribute__((address_space(4))) int*)(p + i);
+  return make_int4(q[0], q[1], q[2], q[3]);
+}
+
 __global__ void permute_u_kernel(DevPlan p, const double* __restrict__ u, int32_t* status) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i == 0) {
@@ -50,211 +66,29 @@ __device__ __forceinline__ void flag_inversion(int32_t* status, int e) {
   status[0] = 1;
 }
 
-template <typename R, int TILE, bool WANT_K>
-__global__ __launch_bounds__(TILE) void tet4_assemble_kernel(DevPlan p, const double* __restrict__ mu,
-                                                             const double* __restrict__ lam,
-                                                             double* __restrict__ f,
-                                                             double* __restrict__ kval,
-                                                             int32_t* status) {
-  constexpr int NV = WANT_K ? 39 : 3;
-  // contributions in LDS; reused as the FP64 staging buffer of the tile's
-  // CSR values (9 * units <= 9 * TILE doubles fit in 39 * TILE floats).
-  __shared__ R contrib[NV * TILE];
-  __shared__ uint32_t rk[TILE];
-  __shared__ int32_t s_incp[TILE + 1];
-  __shared__ int32_t s_uoff[TILE + 1];
-  __shared__ int32_t s_kofs[WANT_K ? TILE : 1];
-  __shared__ unsigned long long s_mask[WANT_K ? TILE : 1];
-
-  const int tile = xcd_tile();
-  if (tile >= p.n_tiles) return;
-  const int n0 = p.tile_node[tile], n1 = p.tile_node[tile + 1];
-  const int nn = n1 - n0;
-  const int k0 = p.inc_ptr[n0];
-  const int nk = p.inc_ptr[n1] - k0;
-  const int t = threadIdx.x;
-  for (int i = t; i <= nn; i += TILE) {
-    s_incp[i] = p.inc_ptr[n0 + i] - k0;
-    s_uoff[i] = p.uoff[n0 + i] - p.uoff[n0];
-    if (WANT_K && i < nn) s_kofs[i] = p.kofs[n0 + i];
-  }
-  if (WANT_K) s_mask[t] = 0ull;
-  __syncthreads();
-  if (t < nk) {
-    const uint32_t pe = p.inc_elem[k0 + t];
-    const int e = int(pe & kIncElemMask);
-    const int lp = int(pe >> kIncElemBits);
-    const int4 c = *reinterpret_cast<const int4*>(p.conn + 4 * int64_t(e));
-    const double m = mu[e], l = lam[e];
-    R out[NV];
-    R J;
-    if constexpr (std::is_same<R, float>::value) {
-      // FP32: precomputed gradients, displacements in the element's order
-      const float4 g0 = p.geo[3 * int64_t(e)], g1 = p.geo[3 * int64_t(e) + 1], g2 = p.geo[3 * int64_t(e) + 2];
-      double4 U[4];
-      U[0] = p.ui[c.x];
-      U[1] = p.ui[c.y];
-      U[2] = p.ui[c.z];
-      U[3] = p.ui[c.w];
-      J = tet4_row_geo<WANT_K>(g0, g1, g2, U, lp, float(m), float(l + (2.0 / 3.0) * m), out);
-    } else {
-      const int i0 = lp == 0 ? c.x : lp == 1 ? c.y : lp == 2 ? c.z : c.w;
-      const int i1 = lp == 0 ? c.y : c.x;
-      const int i2 = lp <= 1 ? c.z : c.y;
-      const int i3 = lp <= 2 ? c.w : c.z;
-      double4 X[4], U[4];
-      X[0] = p.Xi[i0];
-      X[1] = p.Xi[i1];
-      X[2] = p.Xi[i2];
-      X[3] = p.Xi[i3];
-      U[0] = p.ui[i0];
-      U[1] = p.ui[i1];
-      U[2] = p.ui[i2];
-      U[3] = p.ui[i3];
-      J = tet4_row<R, WANT_K>(X, U, R(m), R(l + (2.0 / 3.0) * m), out);
-    }
-    if (WANT_K) {
-      // Mark this incidence in the masks of the three off-diagonal slots it
-      // feeds (bit = position within its node's incidence run).
-      const uint32_t r = p.inc_rank[k0 + t];
-      rk[t] = r;
-      int lo = 0, hi = nn;
-      while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (s_incp[mid] <= t) lo = mid; else hi = mid;
-      }
-      const unsigned long long bit = 1ull << (t - s_incp[lo]);
-#pragma unroll
-      for (int q = 1; q < 4; ++q) atomicOr(&s_mask[s_uoff[lo] + ((r >> (8 * q)) & 0xffu)], bit);
-    }
-    if (J <= R(0)) flag_inversion(status, e);
-#pragma unroll
-    for (int v = 0; v < NV; ++v) contrib[v * TILE + t] = out[v];
-  }
-  __syncthreads();
-  if (WANT_K) {
-    // Lanes [0, 4 nn): (node, part) -- parts 0..2 sum row `part` of the
-    // diagonal block over the node's contiguous incidence run, part 3 sums
-    // f.  Lanes [4 nn, 4 nn + units): one (node, slot) unit each; an
-    // off-diagonal slot sums the incidences flagged in its mask in
-    // increasing order.  Every sum is FP64 in a fixed order.
-    const int units = s_uoff[nn];
-    const int ni = 4 * nn;
-    double acc[9];
-#pragma unroll
-    for (int c = 0; c < 9; ++c) acc[c] = 0.0;
-    int dst = -1, nrow = 0, rstride = 0;
-    if (t < ni) {
-      const int j = t >> 2, part = t & 3;
-      const int kb = s_incp[j], ke = s_incp[j + 1];
-      const int src = part < 3 ? 3 * part : 36;
-      for (int kk = kb; kk < ke; ++kk)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) acc[c] += double(contrib[(src + c) * TILE + kk]);
-      if (part == 3) {
-        const int64_t a = p.perm[n0 + j];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) f[3 * a + c] = acc[c];
-      } else if (ke > kb) {
-        const int d = s_uoff[j + 1] - s_uoff[j];
-        dst = 9 * s_uoff[j] + part * 3 * d + 3 * int(rk[kb] & 0xffu);
-        nrow = 1;
-      }
-    } else if (t < ni + units) {
-      const int u = t - ni;
-      int lo = 0, hi = nn;
-      while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (s_uoff[mid] <= u) lo = mid; else hi = mid;
-      }
-      const uint32_t s = uint32_t(u - s_uoff[lo]);
-      const int kb = s_incp[lo];
-      if ((rk[kb] & 0xffu) != s) {
-        unsigned long long msk = s_mask[u];
-        while (msk) {
-          const int kk = kb + __ffsll(static_cast<long long>(msk)) - 1;
-          msk &= msk - 1ull;
-          const uint32_t r = rk[kk];
-          const int q = ((r >> 8) & 0xffu) == s ? 1 : ((r >> 16) & 0xffu) == s ? 2 : 3;
-#pragma unroll
-          for (int c = 0; c < 9; ++c) acc[c] += double(contrib[(9 * q + c) * TILE + kk]);
-        }
-        const int d = s_uoff[lo + 1] - s_uoff[lo];
-        dst = 9 * s_uoff[lo] + 3 * int(s);
-        rstride = 3 * d;
-        nrow = 3;
-      }
-    }
-    __syncthreads();  // every read of contrib is done: reuse it for staging
-    double* stage = reinterpret_cast<double*>(contrib);
-    if (nrow == 1) {
-#pragma unroll
-      for (int c = 0; c < 3; ++c) stage[dst + c] = acc[c];
-    } else if (nrow == 3) {
-#pragma unroll
-      for (int r = 0; r < 3; ++r)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) stage[dst + r * rstride + c] = acc[3 * r + c];
-    }
-    __syncthreads();
-    // Copy-out: the tile's values, node block by node block, with
-    // consecutive lanes on consecutive CSR positions.
-    for (int w = t; w < 9 * units; w += TILE) {
-      int lo = 0, hi = nn;
-      while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (9 * s_uoff[mid] <= w) lo = mid; else hi = mid;
-      }
-      kval[s_kofs[lo] + (w - 9 * s_uoff[lo])] = stage[w];
-    }
-  } else {
-    for (int j = t; j < nn; j += TILE) {
-      double fa[3] = {0.0, 0.0, 0.0};
-      for (int kk = s_incp[j]; kk < s_incp[j + 1]; ++kk)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) fa[c] += double(contrib[c * TILE + kk]);
-      const int64_t a = p.perm[n0 + j];
-#pragma unroll
-      for (int c = 0; c < 3; ++c) f[3 * a + c] = fa[c];
-    }
-  }
-}
-
-// Inputs of one incidence, gathered ahead of use (software pipelining).
+// Inputs of one incidence.
 template <typename R>
 struct IncInputs {
   uint32_t pe = 0, rank = 0;
-  int4 c{};
   double m = 0.0, l = 0.0;
   float4 g0{}, g1{}, g2{};   // FP32 path: precomputed gradients
-  double4 U[4];
-  double4 X[4];               // FP64 path: reference coordinates
+  double4 U[4];              // FP32: element order; FP64: row node first
+  double4 X[4];              // FP64 path: reference coordinates
 };
 
 template <typename R>
-__device__ __forceinline__ void load_stage_a(const DevPlan& p, int k, IncInputs<R>& in) {
+__device__ __forceinline__ void load_incidence(const DevPlan& p, const double* mu, const double* lam, int k,
+                                               IncInputs<R>& in) {
   in.pe = p.inc_elem[k];
   in.rank = p.inc_rank[k];
-}
-
-template <typename R>
-__device__ __forceinline__ void load_stage_b(const DevPlan& p, const double* mu, const double* lam,
-                                             IncInputs<R>& in) {
   const int64_t e = int64_t(in.pe & kIncElemMask);
-  in.c = *reinterpret_cast<const int4*>(p.conn + 4 * e);
+  const int4 c = *reinterpret_cast<const int4*>(p.conn + 4 * e);
   in.m = mu[e];
   in.l = lam[e];
   if constexpr (std::is_same<R, float>::value) {
     in.g0 = p.geo[3 * e];
     in.g1 = p.geo[3 * e + 1];
     in.g2 = p.geo[3 * e + 2];
-  }
-}
-
-template <typename R>
-__device__ __forceinline__ void load_stage_c(const DevPlan& p, IncInputs<R>& in) {
-  const int4 c = in.c;
-  if constexpr (std::is_same<R, float>::value) {
     in.U[0] = p.ui[c.x];
     in.U[1] = p.ui[c.y];
     in.U[2] = p.ui[c.z];
@@ -276,261 +110,262 @@ __device__ __forceinline__ void load_stage_c(const DevPlan& p, IncInputs<R>& in)
   }
 }
 
-// Node-table entries of one tile for one lane (lane j holds node j's
-// incidence range, unit range and first CSR value).
-struct TileRow {
-  int32_t inc0 = 0, inc1 = 0, u0 = 0, u1 = 0, kofs = 0;
-};
+// Row of one incidence: out[0..35] = K_{a b} for b = a, then the other three
+// nodes (row-major 3x3 each), out[36..38] = f_a.  Returns det F.
+template <typename R, bool WANT_K>
+__device__ __forceinline__ R incidence_row(const IncInputs<R>& in, R* out) {
+  const int lp = int(in.pe >> kIncElemBits);
+  if constexpr (std::is_same<R, float>::value)
+    return tet4_row_geo<WANT_K>(in.g0, in.g1, in.g2, in.U, lp, float(in.m), float(in.l + (2.0 / 3.0) * in.m),
+                                out);
+  else
+    return tet4_row<R, WANT_K>(in.X, in.U, R(in.m), R(in.l + (2.0 / 3.0) * in.m), out);
+}
 
-// Persistent f_int + K_T assembly.  Work-group j of XCD x walks the XCD's
-// contiguous tile range with stride (work-groups per XCD).  The gathers are
-// software-pipelined three tiles deep: at the top of the iteration for tile
-// T_i the work-group issues, all at once and independent of each other, the
-// nodal displacements of T_{i+1} (its connectivity arrived one iteration
-// earlier), the connectivity / parameters / geometry of T_{i+2} (its
-// incidence records arrived one iteration earlier), the incidence records of
-// T_{i+3} and the node tables of T_{i+1}; phase 1 of T_i then runs on data
-// already in registers while those loads are in flight.
-//
-// LDS per tile: contributions (39 values per incidence, row stride TILE + 1
-// so the four lanes of a node read different banks), the incidence ->
-// node and unit -> node maps, and for every (node, slot) unit three 64-bit
-// masks, one per block position q = 1..3, of the incidences that feed it.
-// The reduction lanes then issue their LDS loads without dependent lookups;
-// the tile's CSR values are staged in LDS and written node block by node
-// block, one wave per node, 64 consecutive doubles per store.
+// f_int only (line-search trials): one lane per incidence, f_a summed in FP64
+// over the node's incidence run.
 template <typename R, int TILE>
-__global__ __launch_bounds__(TILE) void tet4_assemble_persistent(DevPlan p, const double* __restrict__ mu,
-                                                                 const double* __restrict__ lam,
-                                                                 double* __restrict__ f,
-                                                                 double* __restrict__ kval,
-                                                                 int32_t* status) {
-  constexpr int NV = 39;
-  constexpr int LD = TILE + 1;
-  __shared__ R contrib[NV * LD];
-  __shared__ uint8_t s_inode[TILE];
-  __shared__ uint8_t s_unode[TILE];
-  __shared__ int32_t s_incp[TILE / 4 + 1];
-  __shared__ int32_t s_uoff[TILE / 4 + 1];
-  __shared__ int32_t s_kofs[TILE / 4];
-  __shared__ uint8_t s_self[TILE / 4];
-  __shared__ unsigned long long s_mask[3 * TILE];
-
+__global__ __launch_bounds__(TILE) void tet4_force_kernel(DevPlan p, const double* __restrict__ mu,
+                                                          const double* __restrict__ lam,
+                                                          double* __restrict__ f, int32_t* status) {
+  __shared__ R contrib[3 * TILE];
+  __shared__ int32_t s_incp[TILE + 1];
+  const int tile = xcd_tile();
+  if (tile >= p.n_tiles) return;
+  const int4 ti = load_uniform(p.tile_info, tile);
+  const int n0 = ti.x, nn = ti.y, k0 = ti.z, nk = ti.w;
   const int t = threadIdx.x;
-  const int lane = t & 63, wave = t >> 6;
-  const int x = blockIdx.x & 7;
-  const int nwg = gridDim.x >> 3;
-  const int per = (p.n_tiles + 7) >> 3;
-  const int tile_end = min(p.n_tiles, (x + 1) * per);
-  const int tile0 = x * per + (blockIdx.x >> 3);
-  if (tile0 >= tile_end) return;
-
-  auto tile_range = [&](int tl, int& n0, int& nn, int& k0, int& nk) {
-    if (tl < tile_end) {
-      n0 = p.tile_node[tl];
-      nn = p.tile_node[tl + 1] - n0;
-      k0 = p.inc_ptr[n0];
-      nk = p.inc_ptr[n0 + nn] - k0;
-    } else {
-      n0 = nn = k0 = nk = 0;
-    }
-  };
-  auto load_rows = [&](int n0, int nn, TileRow& row) {
-    if (t < nn) {
-      row.inc0 = p.inc_ptr[n0 + t];
-      row.inc1 = p.inc_ptr[n0 + t + 1];
-      row.u0 = p.uoff[n0 + t];
-      row.u1 = p.uoff[n0 + t + 1];
-      row.kofs = p.kofs[n0 + t];
-    }
-  };
-
-  IncInputs<R> cur, nx1, nx2, nx3;
-  TileRow row_cur, row_nx;
-  int n0c, nnc, k0c, nkc, n01, nn1, k01, nk1, n02, nn2, k02, nk2;
-  tile_range(tile0, n0c, nnc, k0c, nkc);
-  tile_range(tile0 + nwg, n01, nn1, k01, nk1);
-  tile_range(tile0 + 2 * nwg, n02, nn2, k02, nk2);
-  load_rows(n0c, nnc, row_cur);
-  if (t < nkc) {
-    load_stage_a(p, k0c + t, cur);
-    load_stage_b(p, mu, lam, cur);
-    load_stage_c(p, cur);
+  IncInputs<R> in;
+  if (t < nk) load_incidence<R>(p, mu, lam, k0 + t, in);
+  for (int i = t; i <= nn; i += TILE) s_incp[i] = p.inc_ptr[n0 + i] - k0;
+  if (t < nk) {
+    R out[3];
+    const R J = incidence_row<R, false>(in, out);
+    if (J <= R(0)) flag_inversion(status, int(in.pe & kIncElemMask));
+#pragma unroll
+    for (int v = 0; v < 3; ++v) contrib[v * TILE + t] = out[v];
   }
-  if (t < nk1) {
-    load_stage_a(p, k01 + t, nx1);
-    load_stage_b(p, mu, lam, nx1);
-  }
-  if (t < nk2) load_stage_a(p, k02 + t, nx2);
-
-  for (int tile = tile0; tile < tile_end; tile += nwg) {
-    int n03, nn3, k03, nk3;
-    tile_range(tile + 3 * nwg, n03, nn3, k03, nk3);
-    const int n0 = n0c, nn = nnc, k0 = k0c, nk = nkc;
-    // node tables and the incidence -> node / unit -> node maps
-    const int uoff0 = p.uoff[n0];
-    if (t < nn) {
-      s_incp[t] = row_cur.inc0 - k0;
-      s_uoff[t] = row_cur.u0 - uoff0;
-      s_kofs[t] = row_cur.kofs;
-      if (t == nn - 1) {
-        s_incp[nn] = row_cur.inc1 - k0;
-        s_uoff[nn] = row_cur.u1 - uoff0;
-      }
-      for (int k = row_cur.inc0 - k0; k < row_cur.inc1 - k0; ++k) s_inode[k] = uint8_t(t);
-      for (int u = row_cur.u0 - uoff0; u < row_cur.u1 - uoff0; ++u) s_unode[u] = uint8_t(t);
-    }
+  __syncthreads();
+  for (int j = t; j < nn; j += TILE) {
+    double fa[3] = {0.0, 0.0, 0.0};
+    for (int kk = s_incp[j]; kk < s_incp[j + 1]; ++kk)
 #pragma unroll
-    for (int q = 0; q < 3; ++q) s_mask[q * TILE + t] = 0ull;
-    // pipeline: issue every next-tile load before this tile's compute
-    if (t < nk1) load_stage_c(p, nx1);
-    if (t < nk2) load_stage_b(p, mu, lam, nx2);
-    if (t < nk3) load_stage_a(p, k03 + t, nx3);
-    load_rows(n01, nn1, row_nx);
-    __syncthreads();
-    // ---- phase 1: incidence t of this tile
-    if (t < nk) {
-      const int e = int(cur.pe & kIncElemMask);
-      const int lp = int(cur.pe >> kIncElemBits);
-      R out[NV];
-      R J;
-      if constexpr (std::is_same<R, float>::value)
-        J = tet4_row_geo<true>(cur.g0, cur.g1, cur.g2, cur.U, lp, float(cur.m),
-                               float(cur.l + (2.0 / 3.0) * cur.m), out);
-      else
-        J = tet4_row<R, true>(cur.X, cur.U, R(cur.m), R(cur.l + (2.0 / 3.0) * cur.m), out);
-      if (J <= R(0)) flag_inversion(status, e);
-      const uint32_t r = cur.rank;
-      const int j = s_inode[t];
-      const int kb = s_incp[j];
-      if (t == kb) s_self[j] = uint8_t(r & 0xffu);
-      const unsigned long long bit = 1ull << (t - kb);
-      const int ubase = s_uoff[j];
+      for (int c = 0; c < 3; ++c) fa[c] += double(contrib[c * TILE + kk]);
+    const int64_t a = p.perm[n0 + j];
 #pragma unroll
-      for (int q = 1; q < 4; ++q) atomicOr(&s_mask[(q - 1) * TILE + ubase + ((r >> (8 * q)) & 0xffu)], bit);
-#pragma unroll
-      for (int v = 0; v < NV; ++v) contrib[v * LD + t] = out[v];
-    }
-    __syncthreads();
-    // ---- phase 2: FP64 sums in a fixed order
-    const int units = s_uoff[nn];
-    const int ni = 4 * nn;
-    double acc[9];
-#pragma unroll
-    for (int c = 0; c < 9; ++c) acc[c] = 0.0;
-    int dst = -1, nrow = 0, rstride = 0;
-    if (t < ni) {
-      // (node, part): part 0..2 = row of the diagonal block, 3 = f
-      const int j = t >> 2, part = t & 3;
-      const int kb = s_incp[j], ke = s_incp[j + 1];
-      const R* src = contrib + (part < 3 ? 3 * part : 36) * LD;
-      int kk = kb;
-      for (; kk + 4 <= ke; kk += 4) {
-        R v[4][3];
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-          for (int c = 0; c < 3; ++c) v[i][c] = src[c * LD + kk + i];
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-          for (int c = 0; c < 3; ++c) acc[c] += double(v[i][c]);
-      }
-      for (; kk < ke; ++kk)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) acc[c] += double(src[c * LD + kk]);
-      if (part == 3) {
-        const int64_t a = p.perm[n0 + j];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) f[3 * a + c] = acc[c];
-      } else if (ke > kb) {
-        const int d = s_uoff[j + 1] - s_uoff[j];
-        dst = 9 * s_uoff[j] + part * 3 * d + 3 * int(s_self[j]);
-        nrow = 1;
-      }
-    } else if (t < ni + units) {
-      const int u = t - ni;
-      const int j = s_unode[u];
-      const uint32_t s = uint32_t(u - s_uoff[j]);
-      if (s != s_self[j]) {
-        const int kb = s_incp[j];
-#pragma unroll
-        for (int q = 1; q < 4; ++q) {
-          unsigned long long msk = s_mask[(q - 1) * TILE + u];
-          const R* src = contrib + 9 * q * LD + kb;
-          while (msk) {
-            const int a = __ffsll(static_cast<long long>(msk)) - 1;
-            msk &= msk - 1ull;
-            const int b = msk ? __ffsll(static_cast<long long>(msk)) - 1 : a;
-            const bool two = msk != 0ull;
-            msk &= msk - 1ull;
-            R va[9], vb[9];
-#pragma unroll
-            for (int c = 0; c < 9; ++c) {
-              va[c] = src[c * LD + a];
-              vb[c] = src[c * LD + b];
-            }
-#pragma unroll
-            for (int c = 0; c < 9; ++c) acc[c] += double(va[c]);
-            if (two)
-#pragma unroll
-              for (int c = 0; c < 9; ++c) acc[c] += double(vb[c]);
-          }
-        }
-        const int d = s_uoff[j + 1] - s_uoff[j];
-        dst = 9 * s_uoff[j] + 3 * int(s);
-        rstride = 3 * d;
-        nrow = 3;
-      }
-    }
-    __syncthreads();
-    double* stage = reinterpret_cast<double*>(contrib);
-    if (nrow == 1) {
-#pragma unroll
-      for (int c = 0; c < 3; ++c) stage[dst + c] = acc[c];
-    } else if (nrow == 3) {
-#pragma unroll
-      for (int r = 0; r < 3; ++r)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) stage[dst + r * rstride + c] = acc[3 * r + c];
-    }
-    __syncthreads();
-    // copy-out: one wave per node block, consecutive lanes on consecutive values
-    for (int j = wave; j < nn; j += TILE / 64) {
-      const int base = 9 * s_uoff[j], len = 9 * (s_uoff[j + 1] - s_uoff[j]);
-      double* dstp = kval + s_kofs[j];
-      for (int i = lane; i < len; i += 64) dstp[i] = stage[base + i];
-    }
-    // rotate the pipeline
-    cur = nx1;
-    nx1 = nx2;
-    nx2 = nx3;
-    row_cur = row_nx;
-    n0c = n01; nnc = nn1; k0c = k01; nkc = nk1;
-    n01 = n02; nn1 = nn2; k01 = k02; nk1 = nk2;
-    n02 = n03; nn2 = nn3; k02 = k03; nk2 = nk3;
-    __syncthreads();
+    for (int c = 0; c < 3; ++c) f[3 * a + c] = fa[c];
   }
 }
 
+// Position in incidence_row's output of LDS value v: 0..5 the upper
+// triangle of K_aa, 6..8 f_a, 9..35 the three off-diagonal blocks.
+__host__ __device__ constexpr int lds_source(int v) {
+  return v < 3 ? v : v == 3 ? 4 : v == 4 ? 5 : v == 5 ? 8 : v < 9 ? 30 + v : v;
+}
+
+// Lane exchange inside groups of four lanes (DPP quad permutation; 0xB1
+// swaps neighbours, 0x4E swaps pairs), for a double.
+template <int CTRL>
+__device__ __forceinline__ double quad_dpp(double x) {
+  const unsigned long long v = static_cast<unsigned long long>(__double_as_longlong(x));
+  const unsigned lo = unsigned(__builtin_amdgcn_update_dpp(0, int(unsigned(v)), CTRL, 0xF, 0xF, false));
+  const unsigned hi = unsigned(__builtin_amdgcn_update_dpp(0, int(unsigned(v >> 32)), CTRL, 0xF, 0xF, false));
+  return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
+}
+
+// f_int + K_T, one tile of node rows per work-group (see the file comment).
+//
+// LDS: contributions (36 values per incidence, structure of arrays), per
+// off-diagonal unit three 64-bit masks (one per block position q = 1..3) of
+// the incidences that feed it, and the node table.  Incidence t finds its
+// node through the run offset kept in byte 0 of its rank word; a unit lane
+// finds its node by counting unit-run starts (one bit per node with units).
 template <typename R, int TILE>
-static int persistent_grid(const DevPlan& p) {
-  static int per_cu = 0;
-  static int cus = 0;
-  if (per_cu == 0) {
-    int dev = 0, n = 0;
-    (void)hipGetDevice(&dev);
-    hipDeviceProp_t prop;
-    cus = hipGetDeviceProperties(&prop, dev) == hipSuccess ? prop.multiProcessorCount : 256;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, tet4_assemble_persistent<R, TILE>, TILE, 0) !=
-            hipSuccess ||
-        n <= 0)
-      n = 1;
-    per_cu = n;
+__global__ __launch_bounds__(TILE) void tet4_assemble_rows(DevPlan p, const double* __restrict__ mu,
+                                                           const double* __restrict__ lam,
+                                                           double* __restrict__ f,
+                                                           double* __restrict__ kval, int32_t* status) {
+  constexpr int NV = 36;
+  constexpr int NN = TILE / 4;   // 4 lanes per node
+  constexpr int NW = TILE / 64;
+  __shared__ R contrib[NV * TILE];
+  __shared__ unsigned long long s_mask[3 * TILE];
+  __shared__ unsigned long long s_ustart[NW];
+  __shared__ int32_t s_incp[NN + 1];
+  __shared__ int32_t s_uoff[NN + 1];
+  __shared__ int32_t s_kofs[NN];
+  __shared__ uint8_t s_self[NN];
+  __shared__ uint8_t s_nz[NN];
+  __shared__ uint8_t s_first[TILE];
+
+  const int tile = xcd_tile();
+  if (tile >= p.n_tiles) return;
+  const int4 ti = load_uniform(p.tile_info, tile);
+  const int n0 = ti.x, nn = ti.y, k0 = ti.z, nk = ti.w;
+  const int t = threadIdx.x;
+  const int lane = t & 63, wave = t >> 6;
+
+  IncInputs<R> in;
+  if (t < nk) load_incidence<R>(p, mu, lam, k0 + t, in);
+  // node table
+  int ub = 0, ue = 0;
+  if (t < nn) {
+    const int4 r0 = p.node_rec[n0 + t], r1 = p.node_rec[n0 + t + 1];
+    const int kb = r0.x - k0, ke = r1.x - k0;
+    s_incp[t] = kb;
+    s_uoff[t] = r0.y;
+    s_kofs[t] = r0.z;
+    s_self[t] = uint8_t(r0.w);
+    if (t == nn - 1) {
+      s_incp[nn] = ke;
+      s_uoff[nn] = r1.y;
+    }
+    if (ke > kb) s_first[kb] = uint8_t(t);
+    ub = r0.y;
+    ue = r1.y;
   }
-  const int per_xcd_tiles = (p.n_tiles + 7) / 8;
-  const int wg_per_xcd = std::max(1, std::min(per_xcd_tiles, per_cu * cus / 8));
-  return 8 * wg_per_xcd;
+  if (wave == 0) {
+    // nodes with units, in order (node lanes all lie in wave 0: nn <= 64)
+    const bool has = t < nn && ue > ub;
+    const unsigned long long bal = __ballot(has);
+    if (has) s_nz[__popcll(bal & ((1ull << lane) - 1ull))] = uint8_t(t);
+  }
+  if (t < NW) s_ustart[t] = 0ull;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) s_mask[q * TILE + t] = 0ull;
+  __syncthreads();
+
+  // ---- phase 1: incidence t
+  const int u0 = s_uoff[0];
+  if (t < nn && ue > ub) atomicOr(&s_ustart[(ub - u0) >> 6], 1ull << ((ub - u0) & 63));
+  if (t < nk) {
+    R out[39];
+    const R J = incidence_row<R, true>(in, out);
+    if (J <= R(0)) flag_inversion(status, int(in.pe & kIncElemMask));
+    const uint32_t r = in.rank;
+    const int off = int(r & 0xffu);
+    const int j = s_first[t - off];
+    const int ubase = s_uoff[j] - u0;
+    const unsigned long long bit = 1ull << off;
+#pragma unroll
+    for (int q = 1; q < 4; ++q) atomicOr(&s_mask[(q - 1) * TILE + ubase + int((r >> (8 * q)) & 0xffu)], bit);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) contrib[v * TILE + t] = out[lds_source(v)];
+  }
+  __syncthreads();
+
+  // ---- phase 2: FP64 sums of 9-value contributions in incidence order
+  const int ni = 4 * nn;
+  const int units = s_uoff[nn] - u0;
+  int j = 0, kb = 0, slot = 0;
+  bool diag = false, offd = false;
+  unsigned long long m1 = 0ull, m2 = 0ull, m3 = 0ull;
+  int64_t arow = 0;
+  if (t < ni) {
+    // (node, quarter of its run): K_aa upper triangle and f_a
+    j = t >> 2;
+    const int qq = t & 3;
+    kb = s_incp[j];
+    const int L = s_incp[j + 1] - kb;
+    const int b = (qq * L) >> 2, e = ((qq + 1) * L) >> 2;
+    m1 = ((1ull << (e - b)) - 1ull) << b;   // e - b <= 16
+    diag = true;
+    if (qq == 3) arow = p.perm[n0 + j];
+  } else if (t < ni + units) {
+    // (node, off-diagonal slot)
+    const int u = t - ni;
+    int c = 0;
+    const int w = u >> 6;
+#pragma unroll
+    for (int i = 0; i < NW; ++i) {
+      const unsigned long long m = s_ustart[i];
+      if (i < w)
+        c += __popcll(m);
+      else if (i == w)
+        c += __popcll(m & ((2ull << (u & 63)) - 1ull));
+    }
+    j = s_nz[c - 1];
+    kb = s_incp[j];
+    slot = u - (s_uoff[j] - u0);
+    if (slot != int(s_self[j])) {
+      offd = true;
+      m1 = s_mask[u];
+      m2 = s_mask[TILE + u];
+      m3 = s_mask[2 * TILE + u];
+    }
+  }
+  double acc[9];
+#pragma unroll
+  for (int c = 0; c < 9; ++c) acc[c] = 0.0;
+  {
+    const int q1 = diag ? 0 : 1;
+    const R* base = contrib + kb;
+    unsigned long long all = m1 | m2 | m3;
+    while (all) {
+      const int a = __ffsll(static_cast<long long>(all)) - 1;
+      all &= all - 1ull;
+      const bool two = all != 0ull;
+      const int b = two ? __ffsll(static_cast<long long>(all)) - 1 : a;
+      all &= all - 1ull;
+      const int qa = ((m1 >> a) & 1ull) ? q1 : ((m2 >> a) & 1ull) ? 2 : 3;
+      const int qb = ((m1 >> b) & 1ull) ? q1 : ((m2 >> b) & 1ull) ? 2 : 3;
+      const R* sa = base + 9 * qa * TILE + a;
+      const R* sb = base + 9 * qb * TILE + b;
+      R va[9], vb[9];
+#pragma unroll
+      for (int c = 0; c < 9; ++c) {
+        va[c] = sa[c * TILE];
+        vb[c] = sb[c * TILE];
+      }
+#pragma unroll
+      for (int c = 0; c < 9; ++c) acc[c] += double(va[c]);
+      if (two)
+#pragma unroll
+        for (int c = 0; c < 9; ++c) acc[c] += double(vb[c]);
+    }
+  }
+  if (diag) {
+    // quarters 0+1 and 2+3, then the two halves: every lane of the quad
+    // ends with the same sum, in an order fixed by the run length
+#pragma unroll
+    for (int c = 0; c < 9; ++c) acc[c] += quad_dpp<0xB1>(acc[c]);
+#pragma unroll
+    for (int c = 0; c < 9; ++c) acc[c] += quad_dpp<0x4E>(acc[c]);
+  }
+  __syncthreads();   // every read of contrib is done: reuse it for staging
+  double* stage = reinterpret_cast<double*>(contrib);
+  if (diag) {
+    const int qq = t & 3;
+    const int d = s_uoff[j + 1] - s_uoff[j];
+    if (qq == 3) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) f[3 * arow + c] = acc[6 + c];
+    } else if (d > 0) {
+      // row qq of K_aa from its upper triangle (xx xy xz yy yz zz)
+      const double e0 = qq == 0 ? acc[0] : qq == 1 ? acc[1] : acc[2];
+      const double e1 = qq == 0 ? acc[1] : qq == 1 ? acc[3] : acc[4];
+      const double e2 = qq == 0 ? acc[2] : qq == 1 ? acc[4] : acc[5];
+      double* dst = stage + 9 * (s_uoff[j] - u0) + qq * 3 * d + 3 * int(s_self[j]);
+      dst[0] = e0;
+      dst[1] = e1;
+      dst[2] = e2;
+    }
+  } else if (offd) {
+    const int d = s_uoff[j + 1] - s_uoff[j];
+    double* dst = stage + 9 * (s_uoff[j] - u0) + 3 * slot;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) dst[r * 3 * d + c] = acc[3 * r + c];
+  }
+  __syncthreads();
+
+  // ---- phase 3: copy-out, one wave per node block
+  for (int jj = wave; jj < nn; jj += NW) {
+    const int base = 9 * (s_uoff[jj] - u0), len = 9 * (s_uoff[jj + 1] - s_uoff[jj]);
+    double* dstp = kval + s_kofs[jj];
+    for (int i = lane; i < len; i += 64) dstp[i] = stage[base + i];
+  }
 }
 
 template <typename R, int TILE>
@@ -538,11 +373,10 @@ static void launch_t4(const DevPlan& p, const double* mu, const double* lam, dou
                       hipStream_t s, int32_t* status) {
   const int blocks = 8 * ((p.n_tiles + 7) / 8);
   if (kval)
-    hipLaunchKernelGGL((tet4_assemble_persistent<R, TILE>), dim3(persistent_grid<R, TILE>(p)), dim3(TILE),
-                       0, s, p, mu, lam, f, kval, status);
+    hipLaunchKernelGGL((tet4_assemble_rows<R, TILE>), dim3(blocks), dim3(TILE), 0, s, p, mu, lam, f, kval,
+                       status);
   else
-    hipLaunchKernelGGL((tet4_assemble_kernel<R, TILE, false>), dim3(blocks), dim3(TILE), 0, s, p, mu,
-                       lam, f, kval, status);
+    hipLaunchKernelGGL((tet4_force_kernel<R, TILE>), dim3(blocks), dim3(TILE), 0, s, p, mu, lam, f, status);
 }
 
 int32_t launch_permute_u(const DevPlan& p, const double* u, hipStream_t s, int32_t* status) {
