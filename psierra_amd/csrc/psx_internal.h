@@ -32,6 +32,8 @@ constexpr uint32_t kIncElemMask = (1u << kIncElemBits) - 1u;
 // most kMaxNodeIncidences incidences (slot bit masks are 64-bit).
 constexpr int kReduceLanesPerNode = 4;
 constexpr int kMaxNodeIncidences = 64;
+// Tet10 tiles: 4 * nodes + units <= kTet10LanePasses * TILE.
+constexpr int kTet10LanePasses = 4;
 
 // Host-side structural data (all int32 unless noted), built once per mesh.
 struct HostPattern {
@@ -64,7 +66,7 @@ struct DevPlan {
   const int32_t* uoff;            // N+1: prefix of deg in internal order
   const int32_t* tile_node;       // n_tiles+1
   const int4* tile_info;          // n_tiles: first node, nodes, first incidence, incidences
-  const int4* node_rec;           // Tet4, N+1: first incidence, first unit, kofs, self slot rank
+  const int4* node_rec;           // N+1: first incidence, first unit, kofs, self slot rank
   double4* ui;                    // scratch: displacement in internal order
   double4* wi;                    // scratch: adjoint cotangent in internal order
 };

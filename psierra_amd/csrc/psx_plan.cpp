@@ -250,22 +250,25 @@ int32_t make_tiles(psx_plan* plan, int32_t tile) {
     return fail(PSX_EINVAL, "tile must be 64, 128 or 256");
   const int64_t N = plan->hp.N;
   // Greedy packing of consecutive nodes: at most `tile` incidences and at
-  // most `tile` nodes per tile (nodes without elements count as nodes).
-  // Greedy packing of consecutive nodes: at most `tile` incidences and
-  // kReduceLanesPerNode * nodes + (node, slot) units <= tile per tile.
+  // most `cap` reduction lanes (kReduceLanesPerNode per node plus one per
+  // (node, slot) unit) per tile.
+  // Tet10 tiles also hold at most 64 nodes (their node lanes are one wave).
   std::vector<int32_t> tiles{0};
   int64_t start_inc = 0, lanes = 0;
+  const int64_t max_nodes = plan->hp.npe == 4 ? int64_t(tile) : 64;
   for (int64_t i = 0; i < N; ++i) {
     int64_t cnt = plan->inc_ptr_h[i + 1] - plan->inc_ptr_h[i];
     if (cnt > std::min<int64_t>(tile, kMaxNodeIncidences))
       return fail(PSX_EUNSUPPORTED, "node with " + std::to_string(cnt) +
                                         " incident elements exceeds the supported " +
                                         std::to_string(std::min<int64_t>(tile, kMaxNodeIncidences)));
-    // Tet4 reduces with one lane per (node, part) and per (node, slot);
-    // the Tet10 kernel loops its reduction lanes, so only nodes count.
-    const int64_t need = plan->hp.npe == 4 ? kReduceLanesPerNode + plan->deg_h[i] : 1;
-    if (need > tile) return fail(PSX_EUNSUPPORTED, "node degree exceeds the tile's reduction lanes");
-    if (plan->inc_ptr_h[i + 1] - start_inc > tile || lanes + need > tile) {
+    // Reduction lanes: four per node and one per (node, slot) unit.  Tet4
+    // has one pass of TILE lanes; the Tet10 kernel loops its lanes over at
+    // most kTet10LanePasses passes.
+    const int64_t need = kReduceLanesPerNode + plan->deg_h[i];
+    const int64_t cap = plan->hp.npe == 4 ? tile : int64_t(kTet10LanePasses) * tile;
+    if (need > cap) return fail(PSX_EUNSUPPORTED, "node degree exceeds the tile's reduction lanes");
+    if (plan->inc_ptr_h[i + 1] - start_inc > tile || lanes + need > cap || i - tiles.back() >= max_nodes) {
       tiles.push_back(int32_t(i));
       start_inc = plan->inc_ptr_h[i];
       lanes = 0;
@@ -402,17 +405,20 @@ int32_t create_plan(const int64_t* conn, int64_t E, int npe, const double* X, in
           }
           inc_rank[k] = r;
         } else {
+          // bytes 0..9: slot ranks of the element's nodes in a's row (byte p
+          // is a's own); byte 10: the incidence's position in a's run
           for (int q = 0; q < 10; ++q)
             inc_rank[k * rw + q / 4] |= rank_of(conn[el * 10 + q]) << (8 * (q % 4));
+          inc_rank[k * rw + 2] |= uint32_t(k - plan->inc_ptr_h[i]) << 16;
         }
       }
     }
   });
 
-  // Tet4 node records: {first incidence, first unit, first CSR value, slot
-  // rank of the node in its own row}; record N closes the last node.
+  // Node records: {first incidence, first unit, first CSR value, slot rank
+  // of the node in its own row}; record N closes the last node.
   std::vector<int4> node_rec;
-  if (npe == 4) {
+  {
     node_rec.resize(N + 1);
     parallel_for(N, [&](int64_t b, int64_t e) {
       for (int64_t i = b; i < e; ++i) {
@@ -437,8 +443,8 @@ int32_t create_plan(const int64_t* conn, int64_t E, int npe, const double* X, in
   double4* dX;
   float4* dgeo = nullptr;
   int4* dnrec = nullptr;
-  if (npe == 4 && ((rc = upload(plan, geo.data(), geo.size(), &dgeo)) ||
-                   (rc = upload(plan, node_rec.data(), node_rec.size(), &dnrec))))
+  if ((npe == 4 && (rc = upload(plan, geo.data(), geo.size(), &dgeo))) ||
+      (rc = upload(plan, node_rec.data(), node_rec.size(), &dnrec)))
     return bail(rc);
   d.geo = dgeo;
   d.node_rec = dnrec;

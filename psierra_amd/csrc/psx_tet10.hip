@@ -11,6 +11,7 @@
 // src/psierra/element/quadrature.py:36-50.
 #include <climits>
 
+#include "psx_device.h"
 #include "psx_internal.h"
 #include "psx_nh.h"
 
@@ -40,12 +41,6 @@ __host__ __device__ constexpr double tet10_dn(int q, int a, int j) {
     case 8: return j == 0 ? 4.0 * L3 : j == 2 ? 4.0 * L1 : 0.0;
     default: return j == 1 ? 4.0 * L3 : j == 2 ? 4.0 * L2 : 0.0;
   }
-}
-
-__device__ __forceinline__ int xcd_tile10() {
-  const int per = gridDim.x >> 3;
-  const int b = blockIdx.x;
-  return (b & 7) * per + (b >> 3);
 }
 
 // Row p of a Tet10 element.  dX, dU: node coordinates/displacements minus
@@ -130,127 +125,255 @@ __device__ __forceinline__ R tet10_row(const R dX[10][3], const R dU[10][3], R m
   return Jmin;
 }
 
-template <typename R, int TILE, bool WANT_K>
-__global__ __launch_bounds__(TILE) void tet10_assemble_kernel(DevPlan p, const double* __restrict__ mu,
-                                                              const double* __restrict__ lam,
-                                                              double* __restrict__ f,
-                                                              double* __restrict__ kval,
-                                                              int32_t* status) {
-  constexpr int NV = 93;
-  __shared__ R contrib[NV * TILE];
-  __shared__ uint32_t rk[3 * TILE];
-  __shared__ int32_t s_incp[TILE + 1];
-  __shared__ int32_t s_uoff[TILE + 1];
-
-  const int tile = xcd_tile10();
-  if (tile >= p.n_tiles) return;
-  const int n0 = p.tile_node[tile], n1 = p.tile_node[tile + 1];
-  const int nn = n1 - n0;
-  const int k0 = p.inc_ptr[n0];
-  const int nk = p.inc_ptr[n1] - k0;
-  const int t = threadIdx.x;
-  for (int i = t; i <= nn; i += TILE) {
-    s_incp[i] = p.inc_ptr[n0 + i] - k0;
-    s_uoff[i] = p.uoff[n0 + i] - p.uoff[n0];
+// Gathered inputs of one Tet10 incidence: node coordinates and displacements
+// relative to node 0 (differences in FP64, then rounded to R).
+template <typename R>
+__device__ __forceinline__ void tet10_gather(const DevPlan& p, int e, R dX[10][3], R dU[10][3]) {
+  const int32_t* c = p.conn + 12 * int64_t(e);
+  const int4 c0 = *reinterpret_cast<const int4*>(c);
+  const int4 c1 = *reinterpret_cast<const int4*>(c + 4);
+  const int2 c2 = *reinterpret_cast<const int2*>(c + 8);
+  const int ids[10] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w, c2.x, c2.y};
+  const double4 X0 = p.Xi[ids[0]];
+  const double4 U0 = p.ui[ids[0]];
+#pragma unroll
+  for (int a = 0; a < 10; ++a) {
+    if (a == 0) {
+      dX[0][0] = dX[0][1] = dX[0][2] = R(0);
+      dU[0][0] = dU[0][1] = dU[0][2] = R(0);
+    } else {
+      const double4 Xa = p.Xi[ids[a]];
+      const double4 Ua = p.ui[ids[a]];
+      dX[a][0] = R(Xa.x - X0.x);
+      dX[a][1] = R(Xa.y - X0.y);
+      dX[a][2] = R(Xa.z - X0.z);
+      dU[a][0] = R(Ua.x - U0.x);
+      dU[a][1] = R(Ua.y - U0.y);
+      dU[a][2] = R(Ua.z - U0.z);
+    }
   }
+}
+
+// f_int only: one lane per incidence, f_a summed in FP64 over the run.
+template <typename R, int TILE>
+__global__ __launch_bounds__(TILE) void tet10_force_kernel(DevPlan p, const double* __restrict__ mu,
+                                                           const double* __restrict__ lam,
+                                                           double* __restrict__ f, int32_t* status) {
+  __shared__ R contrib[3 * TILE];
+  __shared__ int32_t s_incp[TILE + 1];
+  const int tile = xcd_tile();
+  if (tile >= p.n_tiles) return;
+  const int4 ti = load_uniform(p.tile_info, tile);
+  const int n0 = ti.x, nn = ti.y, k0 = ti.z, nk = ti.w;
+  const int t = threadIdx.x;
+  for (int i = t; i <= nn; i += TILE) s_incp[i] = p.node_rec[n0 + i].x - k0;
   if (t < nk) {
     const uint32_t pe = p.inc_elem[k0 + t];
     const int e = int(pe & kIncElemMask);
     const int lp = int(pe >> kIncElemBits);
-#pragma unroll
-    for (int w = 0; w < 3; ++w) rk[w * TILE + t] = p.inc_rank[3 * int64_t(k0 + t) + w];
-    const int32_t* c = p.conn + 12 * int64_t(e);
-    const int4 c0 = *reinterpret_cast<const int4*>(c);
-    const int4 c1 = *reinterpret_cast<const int4*>(c + 4);
-    const int2 c2 = *reinterpret_cast<const int2*>(c + 8);
-    const int ids[10] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w, c2.x, c2.y};
-    const double4 X0 = p.Xi[ids[0]];
-    const double4 U0 = p.ui[ids[0]];
     R dX[10][3], dU[10][3];
-#pragma unroll
-    for (int a = 0; a < 10; ++a) {
-      if (a == 0) {
-        dX[0][0] = dX[0][1] = dX[0][2] = R(0);
-        dU[0][0] = dU[0][1] = dU[0][2] = R(0);
-      } else {
-        const double4 Xa = p.Xi[ids[a]];
-        const double4 Ua = p.ui[ids[a]];
-        dX[a][0] = R(Xa.x - X0.x);
-        dX[a][1] = R(Xa.y - X0.y);
-        dX[a][2] = R(Xa.z - X0.z);
-        dU[a][0] = R(Ua.x - U0.x);
-        dU[a][1] = R(Ua.y - U0.y);
-        dU[a][2] = R(Ua.z - U0.z);
-      }
-    }
-#pragma unroll
-    for (int v = 0; v < NV; ++v) contrib[v * TILE + t] = R(0);
+    tet10_gather<R>(p, e, dX, dU);
     const double m = mu[e], l = lam[e];
-    const R J = tet10_row<R, WANT_K>(dX, dU, R(m), R(l + (2.0 / 3.0) * m), lp,
-                                     [&](int v, R val) { contrib[v * TILE + t] += val; });
+    R fa[3] = {R(0), R(0), R(0)};
+    const R J = tet10_row<R, false>(dX, dU, R(m), R(l + (2.0 / 3.0) * m), lp,
+                                    [&](int v, R val) { fa[v - 90] += val; });
     if (J <= R(0)) {
       atomicMin(&status[1], e);
       status[0] = 1;
     }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) contrib[c * TILE + t] = fa[c];
   }
   __syncthreads();
-  if (WANT_K) {
-    const int units = s_uoff[nn];
-    for (int w = t; w < units; w += TILE) {
-      int lo = 0, hi = nn;
-      while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (s_uoff[mid] <= w) lo = mid; else hi = mid;
+  for (int j = t; j < nn; j += TILE) {
+    double fs[3] = {0.0, 0.0, 0.0};
+    for (int kk = s_incp[j]; kk < s_incp[j + 1]; ++kk)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) fs[c] += double(contrib[c * TILE + kk]);
+    const int64_t a = p.perm[n0 + j];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) f[3 * a + c] = fs[c];
+  }
+}
+
+// Element position q (0..9) of the node whose slot rank in the row is s,
+// from the three rank words of an incidence (bytes 0..9).
+__device__ __forceinline__ int tet10_rank_pos(uint32_t w0, uint32_t w1, uint32_t w2, uint32_t s) {
+  const uint32_t s4 = s * 0x01010101u;
+  // lowest zero byte of x: flagged exactly (higher flags may be spurious)
+  auto zero_bytes = [](uint32_t x) { return (x - 0x01010101u) & ~x & 0x80808080u; };
+  const uint32_t z0 = zero_bytes(w0 ^ s4);
+  const uint32_t z1 = zero_bytes(w1 ^ s4);
+  const uint32_t z2 = zero_bytes(w2 ^ s4) & 0x8080u;
+  return z0 ? (__ffs(int(z0)) - 1) >> 3 : z1 ? 4 + ((__ffs(int(z1)) - 1) >> 3) : 8 + ((__ffs(int(z2)) - 1) >> 3);
+}
+
+// f_int + K_T over one tile of node rows (same scheme as the Tet4 kernel):
+// phase 1, lane t = incidence t: the row's 10 blocks and f summed over the
+// 4 quadrature points in registers, then 93 values to LDS, and the
+// incidence's bit set in the feed mask of each of the 10 units it feeds;
+// phase 2, reduction lanes -- four per node (quarters of the node's run:
+// K_aa and f_a, combined across the quad in a fixed order) and one per
+// off-diagonal (node, slot) unit -- sum their feeds in FP64 in increasing
+// incidence order, passing over the lanes up to kTet10LanePasses times, and
+// write each 3x3 block once, three consecutive values per row.  A tile has
+// at most 64 nodes (the plan's rule), so the node lanes are one wave.
+template <typename R, int TILE>
+__global__ __launch_bounds__(TILE) void tet10_assemble_rows(DevPlan p, const double* __restrict__ mu,
+                                                            const double* __restrict__ lam,
+                                                            double* __restrict__ f,
+                                                            double* __restrict__ kval, int32_t* status) {
+  constexpr int NV = 93;
+  constexpr int UMAX = kTet10LanePasses * TILE;   // reduction lanes per tile
+  constexpr int NN = 64;
+  constexpr int NW = UMAX / 64;
+  __shared__ R contrib[NV * TILE];
+  __shared__ uint32_t rk[3 * TILE];
+  __shared__ unsigned long long s_mask[UMAX];
+  __shared__ unsigned long long s_ustart[NW];
+  __shared__ int32_t s_incp[NN + 1];
+  __shared__ int32_t s_uoff[NN + 1];
+  __shared__ int32_t s_kofs[NN];
+  __shared__ uint8_t s_self[NN];
+  __shared__ uint8_t s_nz[NN];
+  __shared__ uint8_t s_first[TILE];
+
+  const int tile = xcd_tile();
+  if (tile >= p.n_tiles) return;
+  const int4 ti = load_uniform(p.tile_info, tile);
+  const int n0 = ti.x, nn = ti.y, k0 = ti.z, nk = ti.w;
+  const int t = threadIdx.x;
+  const int lane = t & 63;
+
+  int ub = 0, ue = 0;
+  if (t < nn) {
+    const int4 r0 = p.node_rec[n0 + t], r1 = p.node_rec[n0 + t + 1];
+    const int kb = r0.x - k0, ke = r1.x - k0;
+    s_incp[t] = kb;
+    s_uoff[t] = r0.y;
+    s_kofs[t] = r0.z;
+    s_self[t] = uint8_t(r0.w);
+    if (t == nn - 1) {
+      s_incp[nn] = ke;
+      s_uoff[nn] = r1.y;
+    }
+    if (ke > kb) s_first[kb] = uint8_t(t);
+    ub = r0.y;
+    ue = r1.y;
+  }
+  if (t < 64) {
+    const bool has = t < nn && ue > ub;
+    const unsigned long long bal = __ballot(has);
+    if (has) s_nz[__popcll(bal & ((1ull << lane) - 1ull))] = uint8_t(t);
+  }
+  for (int i = t; i < NW; i += TILE) s_ustart[i] = 0ull;
+  for (int i = t; i < UMAX; i += TILE) s_mask[i] = 0ull;
+  __syncthreads();
+
+  // ---- phase 1
+  const int u0 = s_uoff[0];
+  if (t < nn && ue > ub) atomicOr(&s_ustart[(ub - u0) >> 6], 1ull << ((ub - u0) & 63));
+  if (t < nk) {
+    const uint32_t pe = p.inc_elem[k0 + t];
+    const int e = int(pe & kIncElemMask);
+    const int lp = int(pe >> kIncElemBits);
+    uint32_t w[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) w[i] = p.inc_rank[3 * int64_t(k0 + t) + i];
+    R dX[10][3], dU[10][3];
+    tet10_gather<R>(p, e, dX, dU);
+    const double m = mu[e], l = lam[e];
+    // the four quadrature points accumulate in LDS (keeps 93 accumulators
+    // out of registers)
+#pragma unroll
+    for (int v = 0; v < NV; ++v) contrib[v * TILE + t] = R(0);
+    const R J = tet10_row<R, true>(dX, dU, R(m), R(l + (2.0 / 3.0) * m), lp,
+                                   [&](int v, R val) { contrib[v * TILE + t] += val; });
+    if (J <= R(0)) {
+      atomicMin(&status[1], e);
+      status[0] = 1;
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) rk[i * TILE + t] = w[i];
+    const int off = int((w[2] >> 16) & 0xffu);
+    const int j = s_first[t - off];
+    const int ubase = s_uoff[j] - u0;
+    const unsigned long long bit = 1ull << off;
+#pragma unroll
+    for (int q = 0; q < 10; ++q)
+      if (q != lp) atomicOr(&s_mask[ubase + int((w[q >> 2] >> (8 * (q & 3))) & 0xffu)], bit);
+  }
+  __syncthreads();
+
+  // ---- phase 2
+  const int ni = 4 * nn;
+  const int lanes = ni + s_uoff[nn] - u0;
+  for (int base = 0; base < lanes; base += TILE) {
+    const int w = base + t;
+    int j = 0, kb = 0, slot = 0;
+    bool diag = false, offd = false;
+    unsigned long long msk = 0ull;
+    if (w < ni) {
+      j = w >> 2;
+      const int qq = w & 3;
+      kb = s_incp[j];
+      const int L = s_incp[j + 1] - kb;
+      msk = bit_range((qq * L) >> 2, ((qq + 1) * L) >> 2);   // <= 16 bits
+      diag = true;
+    } else if (w < lanes) {
+      const int u = w - ni;
+      j = unit_owner<NW>(s_ustart, s_nz, u);
+      kb = s_incp[j];
+      slot = u - (s_uoff[j] - u0);
+      if (slot != int(s_self[j])) {
+        offd = true;
+        msk = s_mask[u];
       }
-      const int j = lo;
-      const uint32_t s = uint32_t(w - s_uoff[j]);
-      const int node = n0 + j;
-      double acc[9];
+    }
+    const uint32_t s = diag ? uint32_t(s_self[j]) : uint32_t(slot);
+    double acc[12];
 #pragma unroll
-      for (int c = 0; c < 9; ++c) acc[c] = 0.0;
-      double fa[3] = {0.0, 0.0, 0.0};
-      for (int kk = s_incp[j]; kk < s_incp[j + 1]; ++kk) {
+    for (int c = 0; c < 12; ++c) acc[c] = 0.0;
+    while (msk) {
+      const int a = __ffsll(static_cast<long long>(msk)) - 1;
+      msk &= msk - 1ull;
+      const int k = kb + a;
+      const int q = tet10_rank_pos(rk[k], rk[TILE + k], rk[2 * TILE + k], s);
+      const R* src = contrib + 9 * q * TILE + k;
+      R v[9];
 #pragma unroll
-        for (int q = 0; q < 10; ++q) {
-          const uint32_t r = (rk[(q >> 2) * TILE + kk] >> (8 * (q & 3))) & 0xffu;
-          if (r == s) {
+      for (int c = 0; c < 9; ++c) v[c] = src[c * TILE];
 #pragma unroll
-            for (int c = 0; c < 9; ++c) acc[c] += double(contrib[(9 * q + c) * TILE + kk]);
-          }
-        }
-        if (s == 0) {
+      for (int c = 0; c < 9; ++c) acc[c] += double(v[c]);
+      if (diag) {
 #pragma unroll
-          for (int c = 0; c < 3; ++c) fa[c] += double(contrib[(90 + c) * TILE + kk]);
-        }
+        for (int c = 0; c < 3; ++c) acc[9 + c] += double(contrib[(90 + c) * TILE + k]);
       }
-      const int d = p.deg[node];
-      double* kb_out = kval + p.kofs[node] + 3 * int(s);
+    }
+    const int d = s_uoff[j + 1] - s_uoff[j];
+    if (diag) {
+#pragma unroll
+      for (int c = 0; c < 12; ++c) acc[c] = quad_sum(acc[c]);
+      const int qq = w & 3;
+      if (qq == 3) {
+        const int64_t a = p.perm[n0 + j];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) f[3 * a + c] = acc[9 + c];
+      } else if (d > 0) {
+        double* dst = kval + s_kofs[j] + qq * 3 * d + 3 * int(s);
+        const double e0 = qq == 0 ? acc[0] : qq == 1 ? acc[3] : acc[6];
+        const double e1 = qq == 0 ? acc[1] : qq == 1 ? acc[4] : acc[7];
+        const double e2 = qq == 0 ? acc[2] : qq == 1 ? acc[5] : acc[8];
+        dst[0] = e0;
+        dst[1] = e1;
+        dst[2] = e2;
+      }
+    } else if (offd) {
+      double* dst = kval + s_kofs[j] + 3 * slot;
 #pragma unroll
       for (int r = 0; r < 3; ++r)
 #pragma unroll
-        for (int c = 0; c < 3; ++c) kb_out[r * 3 * d + c] = acc[3 * r + c];
-      if (s == 0) {
-        const int64_t a = p.perm[node];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) f[3 * a + c] = fa[c];
-      }
-    }
-    for (int j = t; j < nn; j += TILE) {
-      if (s_uoff[j + 1] == s_uoff[j]) {
-        const int64_t a = p.perm[n0 + j];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) f[3 * a + c] = 0.0;
-      }
-    }
-  } else {
-    for (int j = t; j < nn; j += TILE) {
-      double fa[3] = {0.0, 0.0, 0.0};
-      for (int kk = s_incp[j]; kk < s_incp[j + 1]; ++kk)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) fa[c] += double(contrib[(90 + c) * TILE + kk]);
-      const int64_t a = p.perm[n0 + j];
-#pragma unroll
-      for (int c = 0; c < 3; ++c) f[3 * a + c] = fa[c];
+        for (int c = 0; c < 3; ++c) dst[r * 3 * d + c] = acc[3 * r + c];
     }
   }
 }
@@ -262,11 +385,10 @@ static void launch_t10(const DevPlan& p, const double* mu, const double* lam, do
                        hipStream_t s, int32_t* status) {
   const int blocks = 8 * ((p.n_tiles + 7) / 8);
   if (kval)
-    hipLaunchKernelGGL((tet10_assemble_kernel<R, TILE, true>), dim3(blocks), dim3(TILE), 0, s, p, mu,
-                       lam, f, kval, status);
+    hipLaunchKernelGGL((tet10_assemble_rows<R, TILE>), dim3(blocks), dim3(TILE), 0, s, p, mu, lam, f, kval,
+                       status);
   else
-    hipLaunchKernelGGL((tet10_assemble_kernel<R, TILE, false>), dim3(blocks), dim3(TILE), 0, s, p, mu,
-                       lam, f, kval, status);
+    hipLaunchKernelGGL((tet10_force_kernel<R, TILE>), dim3(blocks), dim3(TILE), 0, s, p, mu, lam, f, status);
 }
 
 int32_t launch_tet10_assemble(const DevPlan& p, const double* u, const double* mu, const double* lam,

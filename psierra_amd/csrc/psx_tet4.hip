@@ -27,27 +27,12 @@
 #include <algorithm>
 #include <type_traits>
 
+#include "psx_device.h"
 #include "psx_internal.h"
 #include "psx_nh.h"
 #include "psx_tet4_elem.h"
 
 namespace psx {
-
-// Work-group -> tile mapping that gives each of the 8 XCDs a contiguous run
-// of tiles (consecutive work-groups are dispatched round-robin over XCDs),
-// so neighbouring tiles share an L2.
-__device__ __forceinline__ int xcd_tile() {
-  const int per = gridDim.x >> 3;
-  const int b = blockIdx.x;
-  return (b & 7) * per + (b >> 3);
-}
-
-// A work-group-uniform record read through the scalar unit (constant
-// address space), so it needs no vector load and no lane broadcast.
-__device__ __forceinline__ int4 load_uniform(const int4* p, int i) {
-  const auto* q = (const __attribute__((address_space(4))) int*)(p + i);
-  return make_int4(q[0], q[1], q[2], q[3]);
-}
 
 __global__ void permute_u_kernel(DevPlan p, const double* __restrict__ u, int32_t* status) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -163,16 +148,6 @@ __host__ __device__ constexpr int lds_source(int v) {
   return v < 3 ? v : v == 3 ? 4 : v == 4 ? 5 : v == 5 ? 8 : v < 9 ? 30 + v : v;
 }
 
-// Lane exchange inside groups of four lanes (DPP quad permutation; 0xB1
-// swaps neighbours, 0x4E swaps pairs), for a double.
-template <int CTRL>
-__device__ __forceinline__ double quad_dpp(double x) {
-  const unsigned long long v = static_cast<unsigned long long>(__double_as_longlong(x));
-  const unsigned lo = unsigned(__builtin_amdgcn_update_dpp(0, int(unsigned(v)), CTRL, 0xF, 0xF, false));
-  const unsigned hi = unsigned(__builtin_amdgcn_update_dpp(0, int(unsigned(v >> 32)), CTRL, 0xF, 0xF, false));
-  return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
-}
-
 // f_int + K_T, one tile of node rows per work-group (see the file comment).
 //
 // LDS: contributions (36 values per incidence, structure of arrays), per
@@ -268,23 +243,13 @@ __global__ __launch_bounds__(TILE) void tet4_assemble_rows(DevPlan p, const doub
     kb = s_incp[j];
     const int L = s_incp[j + 1] - kb;
     const int b = (qq * L) >> 2, e = ((qq + 1) * L) >> 2;
-    m1 = ((1ull << (e - b)) - 1ull) << b;   // e - b <= 16
+    m1 = bit_range(b, e);   // e - b <= 16
     diag = true;
     if (qq == 3) arow = p.perm[n0 + j];
   } else if (t < ni + units) {
     // (node, off-diagonal slot)
     const int u = t - ni;
-    int c = 0;
-    const int w = u >> 6;
-#pragma unroll
-    for (int i = 0; i < NW; ++i) {
-      const unsigned long long m = s_ustart[i];
-      if (i < w)
-        c += __popcll(m);
-      else if (i == w)
-        c += __popcll(m & ((2ull << (u & 63)) - 1ull));
-    }
-    j = s_nz[c - 1];
+    j = unit_owner<NW>(s_ustart, s_nz, u);
     kb = s_incp[j];
     slot = u - (s_uoff[j] - u0);
     if (slot != int(s_self[j])) {
@@ -328,9 +293,7 @@ __global__ __launch_bounds__(TILE) void tet4_assemble_rows(DevPlan p, const doub
     // quarters 0+1 and 2+3, then the two halves: every lane of the quad
     // ends with the same sum, in an order fixed by the run length
 #pragma unroll
-    for (int c = 0; c < 9; ++c) acc[c] += quad_dpp<0xB1>(acc[c]);
-#pragma unroll
-    for (int c = 0; c < 9; ++c) acc[c] += quad_dpp<0x4E>(acc[c]);
+    for (int c = 0; c < 9; ++c) acc[c] = quad_sum(acc[c]);
   }
   __syncthreads();   // every read of contrib is done: reuse it for staging
   double* stage = reinterpret_cast<double*>(contrib);
