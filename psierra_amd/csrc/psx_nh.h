@@ -28,10 +28,37 @@
 
 namespace psx {
 
+// log1p in FP32 (a few ulp for x > -1): with u = fl(1 + x) and the rounding
+// correction c (1 + x = u + c to first order, as in fdlibm's log1p),
+// log1p(x) = e ln 2 + log(m) + c / u, where u = m 2^e, m in [sqrt(1/2),
+// sqrt(2)), and log(m) = 2 atanh(s), s = (m - 1) / (m + 1), |s| <= 0.172,
+// by its odd series through s^11 (truncation < 2e-9 relative).  Inputs of
+// -1 or less (an inverted element) give a meaningless value; the caller
+// flags det F <= 0 separately.
+__device__ __forceinline__ float log1p_f32(float x) {
+  const float u = 1.0f + x;
+  const float c = u > 2.0f ? 1.0f - (u - x) : x - (u - 1.0f);
+  int e = __builtin_amdgcn_frexp_expf(u);
+  float m = __builtin_amdgcn_frexp_mantf(u);   // [0.5, 1)
+  if (m < 0.70710678f) {
+    m *= 2.0f;
+    e -= 1;
+  }
+  const float s = (m - 1.0f) * __builtin_amdgcn_rcpf(m + 1.0f);
+  const float z = s * s;
+  float r = 1.0f / 11.0f;
+  r = r * z + 1.0f / 9.0f;
+  r = r * z + 1.0f / 7.0f;
+  r = r * z + 1.0f / 5.0f;
+  r = r * z + 1.0f / 3.0f;
+  const float lm = 2.0f * s + (2.0f * s) * (z * r);
+  return float(e) * 0.69314718f + (lm + c * __builtin_amdgcn_rcpf(u));
+}
+
 template <typename R>
 __device__ __forceinline__ R d_log1p(R x);
 template <>
-__device__ __forceinline__ float d_log1p<float>(float x) { return log1pf(x); }
+__device__ __forceinline__ float d_log1p<float>(float x) { return log1p_f32(x); }
 template <>
 __device__ __forceinline__ double d_log1p<double>(double x) { return log1p(x); }
 
@@ -92,7 +119,7 @@ __device__ __forceinline__ void nh_point(const R A[3][3], R mu, R K, NHPoint<R>&
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll
-    for (int j = 0; j < 3; ++j) o.F[i][j] = A[i][j] + (i == j ? R(1) : R(0));
+    for (int j = 0; j < 3; ++j) o.F[i][j] = i == j ? A[i][j] + R(1) : A[i][j];
   const R trA = A[0][0] + A[1][1] + A[2][2];
   R trA2 = R(0);
 #pragma unroll
