@@ -245,7 +245,7 @@ int32_t upload(psx_plan* plan, const T* host, size_t count, T** dev) {
 }
 
 int32_t make_tiles(psx_plan* plan, int32_t tile) {
-  if (tile == 0) tile = plan->hp.npe == 4 ? 256 : 64;
+  if (tile == 0) tile = plan->hp.npe == 4 ? 128 : 64;
   if (tile != 64 && tile != 128 && tile != 256)
     return fail(PSX_EINVAL, "tile must be 64, 128 or 256");
   const int64_t N = plan->hp.N;

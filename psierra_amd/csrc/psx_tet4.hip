@@ -16,8 +16,9 @@
 //            combined across the four lanes in a fixed order), one lane per
 //            off-diagonal CSR block walks the incidences that feed it.  The
 //            lists have similar lengths, so the lanes of a wave stay in step.
-//   phase 3: the tile's CSR values, staged in LDS, are written node block by
-//            node block with consecutive lanes on consecutive values.
+//            Each lane then writes its 3x3 block (or f_a) into the CSR
+//            values; the lanes of one node are its consecutive slots, so a
+//            wave's stores of one row form contiguous runs.
 // Every CSR value and every f entry is written exactly once by one lane, and
 // every sum runs in an order fixed by the mesh alone (not by the tile size):
 // the scatter is deterministic and uses no floating-point atomics.
@@ -295,8 +296,8 @@ __global__ __launch_bounds__(TILE) void tet4_assemble_rows(DevPlan p, const doub
 #pragma unroll
     for (int c = 0; c < 9; ++c) acc[c] = quad_sum(acc[c]);
   }
-  __syncthreads();   // every read of contrib is done: reuse it for staging
-  double* stage = reinterpret_cast<double*>(contrib);
+  // ---- write-out: each 3x3 block (and f_a) once; a unit lane's rows are
+  // three consecutive values, so neighbouring slots give contiguous runs
   if (diag) {
     const int qq = t & 3;
     const int d = s_uoff[j + 1] - s_uoff[j];
@@ -308,26 +309,18 @@ __global__ __launch_bounds__(TILE) void tet4_assemble_rows(DevPlan p, const doub
       const double e0 = qq == 0 ? acc[0] : qq == 1 ? acc[1] : acc[2];
       const double e1 = qq == 0 ? acc[1] : qq == 1 ? acc[3] : acc[4];
       const double e2 = qq == 0 ? acc[2] : qq == 1 ? acc[4] : acc[5];
-      double* dst = stage + 9 * (s_uoff[j] - u0) + qq * 3 * d + 3 * int(s_self[j]);
+      double* dst = kval + s_kofs[j] + qq * 3 * d + 3 * int(s_self[j]);
       dst[0] = e0;
       dst[1] = e1;
       dst[2] = e2;
     }
   } else if (offd) {
     const int d = s_uoff[j + 1] - s_uoff[j];
-    double* dst = stage + 9 * (s_uoff[j] - u0) + 3 * slot;
+    double* dst = kval + s_kofs[j] + 3 * slot;
 #pragma unroll
     for (int r = 0; r < 3; ++r)
 #pragma unroll
       for (int c = 0; c < 3; ++c) dst[r * 3 * d + c] = acc[3 * r + c];
-  }
-  __syncthreads();
-
-  // ---- phase 3: copy-out, one wave per node block
-  for (int jj = wave; jj < nn; jj += NW) {
-    const int base = 9 * (s_uoff[jj] - u0), len = 9 * (s_uoff[jj + 1] - s_uoff[jj]);
-    double* dstp = kval + s_kofs[jj];
-    for (int i = lane; i < len; i += 64) dstp[i] = stage[base + i];
   }
 }
 
