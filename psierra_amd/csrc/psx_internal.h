@@ -27,12 +27,14 @@ int32_t fail(int32_t code, const std::string& msg);
 // Incidence packing: element id in the low 28 bits, local node in the top 4.
 constexpr int kIncElemBits = 28;
 constexpr uint32_t kIncElemMask = (1u << kIncElemBits) - 1u;
-// A tile of TILE lanes holds at most TILE incidences, and 4 * nodes +
-// (node, slot) units <= TILE (one reduction lane each); every node has at
-// most kMaxNodeIncidences incidences (slot bit masks are 64-bit).
-constexpr int kReduceLanesPerNode = 4;
+// A tile of TILE lanes holds at most TILE incidences and at most TILE
+// reduction lanes (Tet4; Tet10: kTet10LanePasses * TILE): kTet4DiagLanes
+// (Tet10: kTet10DiagLanes) per node for the diagonal block and f, plus one
+// per (node, slot) unit.  Every node has at most kMaxNodeIncidences
+// incidences (feed bit masks are 64-bit).
+constexpr int kTet4DiagLanes = 8;
+constexpr int kTet10DiagLanes = 4;
 constexpr int kMaxNodeIncidences = 64;
-// Tet10 tiles: 4 * nodes + units <= kTet10LanePasses * TILE.
 constexpr int kTet10LanePasses = 4;
 
 // Host-side structural data (all int32 unless noted), built once per mesh.

@@ -245,13 +245,13 @@ int32_t upload(psx_plan* plan, const T* host, size_t count, T** dev) {
 }
 
 int32_t make_tiles(psx_plan* plan, int32_t tile) {
-  if (tile == 0) tile = plan->hp.npe == 4 ? 128 : 64;
+  if (tile == 0) tile = plan->hp.npe == 4 ? 256 : 64;
   if (tile != 64 && tile != 128 && tile != 256)
     return fail(PSX_EINVAL, "tile must be 64, 128 or 256");
   const int64_t N = plan->hp.N;
   // Greedy packing of consecutive nodes: at most `tile` incidences and at
-  // most `cap` reduction lanes (kReduceLanesPerNode per node plus one per
-  // (node, slot) unit) per tile.
+  // most `cap` reduction lanes (the diagonal lanes of each node plus one
+  // per (node, slot) unit) per tile.
   // Tet10 tiles also hold at most 64 nodes (their node lanes are one wave).
   std::vector<int32_t> tiles{0};
   int64_t start_inc = 0, lanes = 0;
@@ -262,10 +262,10 @@ int32_t make_tiles(psx_plan* plan, int32_t tile) {
       return fail(PSX_EUNSUPPORTED, "node with " + std::to_string(cnt) +
                                         " incident elements exceeds the supported " +
                                         std::to_string(std::min<int64_t>(tile, kMaxNodeIncidences)));
-    // Reduction lanes: four per node and one per (node, slot) unit.  Tet4
-    // has one pass of TILE lanes; the Tet10 kernel loops its lanes over at
-    // most kTet10LanePasses passes.
-    const int64_t need = kReduceLanesPerNode + plan->deg_h[i];
+    // Reduction lanes: the node's diagonal lanes and one per (node, slot)
+    // unit.  Tet4 has one pass of TILE lanes; the Tet10 kernel loops its
+    // lanes over at most kTet10LanePasses passes.
+    const int64_t need = (plan->hp.npe == 4 ? kTet4DiagLanes : kTet10DiagLanes) + plan->deg_h[i];
     const int64_t cap = plan->hp.npe == 4 ? tile : int64_t(kTet10LanePasses) * tile;
     if (need > cap) return fail(PSX_EUNSUPPORTED, "node degree exceeds the tile's reduction lanes");
     if (plan->inc_ptr_h[i + 1] - start_inc > tile || lanes + need > cap || i - tiles.back() >= max_nodes) {

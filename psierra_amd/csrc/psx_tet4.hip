@@ -149,20 +149,32 @@ __host__ __device__ constexpr int lds_source(int v) {
   return v < 3 ? v : v == 3 ? 4 : v == 4 ? 5 : v == 5 ? 8 : v < 9 ? 30 + v : v;
 }
 
+// Sum of one value over the eight lanes of an aligned group (quads first,
+// then the two quads), the same bits in every lane of the group.
+__device__ __forceinline__ double oct_sum(double x, bool upper_quad) {
+  x = quad_sum(x);
+  const double from_up = quad_dpp<0x104>(x);   // row_shl:4, lane i <- lane i + 4
+  const double from_lo = quad_dpp<0x114>(x);   // row_shr:4, lane i <- lane i - 4
+  return upper_quad ? from_lo + x : x + from_up;
+}
+
 // f_int + K_T, one tile of node rows per work-group (see the file comment).
 //
-// LDS: contributions (36 values per incidence, structure of arrays), per
-// off-diagonal unit three 64-bit masks (one per block position q = 1..3) of
-// the incidences that feed it, and the node table.  Incidence t finds its
-// node through the run offset kept in byte 0 of its rank word; a unit lane
-// finds its node by counting unit-run starts (one bit per node with units).
+// LDS holds 18 contribution values per incidence at a time, in two passes:
+// pass A the upper triangle of K_aa, f_a and the first off-diagonal block,
+// pass B the other two off-diagonal blocks (kept in registers meanwhile).
+// Per off-diagonal unit three 64-bit masks (one per block position q =
+// 1..3) mark the incidences that feed it.  Incidence t finds its node
+// through the run offset kept in byte 0 of its rank word; a unit lane finds
+// its node by counting unit-run starts (one bit per node with units).
 template <typename R, int TILE>
 __global__ __launch_bounds__(TILE) void tet4_assemble_rows(DevPlan p, const double* __restrict__ mu,
                                                            const double* __restrict__ lam,
                                                            double* __restrict__ f,
                                                            double* __restrict__ kval, int32_t* status) {
-  constexpr int NV = 36;
-  constexpr int NN = TILE / 4;   // 4 lanes per node
+  constexpr int NV = 18;
+  constexpr int ND = kTet4DiagLanes;
+  constexpr int NN = TILE / ND;
   constexpr int NW = TILE / 64;
   __shared__ R contrib[NV * TILE];
   __shared__ unsigned long long s_mask[3 * TILE];
@@ -214,6 +226,7 @@ __global__ __launch_bounds__(TILE) void tet4_assemble_rows(DevPlan p, const doub
   // ---- phase 1: incidence t
   const int u0 = s_uoff[0];
   if (t < nn && ue > ub) atomicOr(&s_ustart[(ub - u0) >> 6], 1ull << ((ub - u0) & 63));
+  R keep[NV];   // pass B values: the blocks at q = 2 and 3
   if (t < nk) {
     R out[39];
     const R J = incidence_row<R, true>(in, out);
@@ -227,29 +240,28 @@ __global__ __launch_bounds__(TILE) void tet4_assemble_rows(DevPlan p, const doub
     for (int q = 1; q < 4; ++q) atomicOr(&s_mask[(q - 1) * TILE + ubase + int((r >> (8 * q)) & 0xffu)], bit);
 #pragma unroll
     for (int v = 0; v < NV; ++v) contrib[v * TILE + t] = out[lds_source(v)];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) keep[v] = out[18 + v];
   }
   __syncthreads();
 
   // ---- phase 2: FP64 sums of 9-value contributions in incidence order
-  const int ni = 4 * nn;
+  const int nd = ND * nn;
   const int units = s_uoff[nn] - u0;
   int j = 0, kb = 0, slot = 0;
   bool diag = false, offd = false;
   unsigned long long m1 = 0ull, m2 = 0ull, m3 = 0ull;
-  int64_t arow = 0;
-  if (t < ni) {
-    // (node, quarter of its run): K_aa upper triangle and f_a
-    j = t >> 2;
-    const int qq = t & 3;
+  if (t < nd) {
+    // (node, eighth of its run): K_aa upper triangle and f_a
+    j = t / ND;
+    const int part = t % ND;
     kb = s_incp[j];
     const int L = s_incp[j + 1] - kb;
-    const int b = (qq * L) >> 2, e = ((qq + 1) * L) >> 2;
-    m1 = bit_range(b, e);   // e - b <= 16
+    m1 = bit_range((part * L) / ND, ((part + 1) * L) / ND);   // <= 8 bits
     diag = true;
-    if (qq == 3) arow = p.perm[n0 + j];
-  } else if (t < ni + units) {
+  } else if (t < nd + units) {
     // (node, off-diagonal slot)
-    const int u = t - ni;
+    const int u = t - nd;
     j = unit_owner<NW>(s_ustart, s_nz, u);
     kb = s_incp[j];
     slot = u - (s_uoff[j] - u0);
@@ -263,20 +275,18 @@ __global__ __launch_bounds__(TILE) void tet4_assemble_rows(DevPlan p, const doub
   double acc[9];
 #pragma unroll
   for (int c = 0; c < 9; ++c) acc[c] = 0.0;
-  {
-    const int q1 = diag ? 0 : 1;
+  // Sum the 9 values at rows row0.. of the incidences in `msk`, two at a
+  // time, in increasing incidence order; rowb(a) picks the rows per bit.
+  auto walk = [&](unsigned long long msk, auto rowb) {
     const R* base = contrib + kb;
-    unsigned long long all = m1 | m2 | m3;
-    while (all) {
-      const int a = __ffsll(static_cast<long long>(all)) - 1;
-      all &= all - 1ull;
-      const bool two = all != 0ull;
-      const int b = two ? __ffsll(static_cast<long long>(all)) - 1 : a;
-      all &= all - 1ull;
-      const int qa = ((m1 >> a) & 1ull) ? q1 : ((m2 >> a) & 1ull) ? 2 : 3;
-      const int qb = ((m1 >> b) & 1ull) ? q1 : ((m2 >> b) & 1ull) ? 2 : 3;
-      const R* sa = base + 9 * qa * TILE + a;
-      const R* sb = base + 9 * qb * TILE + b;
+    while (msk) {
+      const int a = __ffsll(static_cast<long long>(msk)) - 1;
+      msk &= msk - 1ull;
+      const bool two = msk != 0ull;
+      const int b = two ? __ffsll(static_cast<long long>(msk)) - 1 : a;
+      msk &= msk - 1ull;
+      const R* sa = base + rowb(a) * TILE + a;
+      const R* sb = base + rowb(b) * TILE + b;
       R va[9], vb[9];
 #pragma unroll
       for (int c = 0; c < 9; ++c) {
@@ -289,32 +299,45 @@ __global__ __launch_bounds__(TILE) void tet4_assemble_rows(DevPlan p, const doub
 #pragma unroll
         for (int c = 0; c < 9; ++c) acc[c] += double(vb[c]);
     }
+  };
+  // pass A: K_aa and f_a (rows 0..8), blocks at q = 1 (rows 9..17)
+  {
+    const int row = diag ? 0 : 9;
+    walk(m1, [&](int) { return row; });
   }
   if (diag) {
-    // quarters 0+1 and 2+3, then the two halves: every lane of the quad
-    // ends with the same sum, in an order fixed by the run length
+    // the eight partial sums of the node, combined in a fixed order
+    const bool upper = (t & 4) != 0;
 #pragma unroll
-    for (int c = 0; c < 9; ++c) acc[c] = quad_sum(acc[c]);
-  }
-  // ---- write-out: each 3x3 block (and f_a) once; a unit lane's rows are
-  // three consecutive values, so neighbouring slots give contiguous runs
-  if (diag) {
-    const int qq = t & 3;
+    for (int c = 0; c < 9; ++c) acc[c] = oct_sum(acc[c], upper);
+    const int part = t % ND;
     const int d = s_uoff[j + 1] - s_uoff[j];
-    if (qq == 3) {
+    if (part == 3) {
+      const int64_t a = p.perm[n0 + j];
 #pragma unroll
-      for (int c = 0; c < 3; ++c) f[3 * arow + c] = acc[6 + c];
-    } else if (d > 0) {
-      // row qq of K_aa from its upper triangle (xx xy xz yy yz zz)
-      const double e0 = qq == 0 ? acc[0] : qq == 1 ? acc[1] : acc[2];
-      const double e1 = qq == 0 ? acc[1] : qq == 1 ? acc[3] : acc[4];
-      const double e2 = qq == 0 ? acc[2] : qq == 1 ? acc[4] : acc[5];
-      double* dst = kval + s_kofs[j] + qq * 3 * d + 3 * int(s_self[j]);
+      for (int c = 0; c < 3; ++c) f[3 * a + c] = acc[6 + c];
+    } else if (part < 3 && d > 0) {
+      // row `part` of K_aa from its upper triangle (xx xy xz yy yz zz)
+      const double e0 = part == 0 ? acc[0] : part == 1 ? acc[1] : acc[2];
+      const double e1 = part == 0 ? acc[1] : part == 1 ? acc[3] : acc[4];
+      const double e2 = part == 0 ? acc[2] : part == 1 ? acc[4] : acc[5];
+      double* dst = kval + s_kofs[j] + part * 3 * d + 3 * int(s_self[j]);
       dst[0] = e0;
       dst[1] = e1;
       dst[2] = e2;
     }
-  } else if (offd) {
+  }
+  __syncthreads();   // pass A reads are done
+  if (t < nk) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) contrib[v * TILE + t] = keep[v];
+  }
+  __syncthreads();
+  // pass B: blocks at q = 2 (rows 0..8) and q = 3 (rows 9..17)
+  if (offd) {
+    walk(m2 | m3, [&](int a) { return ((m2 >> a) & 1ull) ? 0 : 9; });
+    // each 3x3 block once; a unit lane's rows are three consecutive values,
+    // so neighbouring slots give contiguous runs
     const int d = s_uoff[j + 1] - s_uoff[j];
     double* dst = kval + s_kofs[j] + 3 * slot;
 #pragma unroll
