@@ -23,11 +23,12 @@ __device__ __forceinline__ int4 load_uniform(const int4* p, int i) {
 
 // Lane exchange inside groups of four lanes (DPP quad permutation; 0xB1
 // swaps neighbours, 0x4E swaps pairs), for a double.
+// Lanes whose source falls outside the row read 0 (bound_ctrl).
 template <int CTRL>
 __device__ __forceinline__ double quad_dpp(double x) {
   const unsigned long long v = static_cast<unsigned long long>(__double_as_longlong(x));
-  const unsigned lo = unsigned(__builtin_amdgcn_update_dpp(0, int(unsigned(v)), CTRL, 0xF, 0xF, false));
-  const unsigned hi = unsigned(__builtin_amdgcn_update_dpp(0, int(unsigned(v >> 32)), CTRL, 0xF, 0xF, false));
+  const unsigned lo = unsigned(__builtin_amdgcn_mov_dpp(int(unsigned(v)), CTRL, 0xF, 0xF, true));
+  const unsigned hi = unsigned(__builtin_amdgcn_mov_dpp(int(unsigned(v >> 32)), CTRL, 0xF, 0xF, true));
   return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
 }
 
@@ -42,15 +43,13 @@ __device__ __forceinline__ double quad_sum(double x) {
 // units, nz[] lists those nodes in order.
 template <int NW>
 __device__ __forceinline__ int unit_owner(const unsigned long long* starts, const uint8_t* nz, int u) {
-  int c = 0;
   const int w = u >> 6;
+  const unsigned long long below = (2ull << (u & 63)) - 1ull;   // bits 0..u%64
+  int c = 0;
 #pragma unroll
   for (int i = 0; i < NW; ++i) {
     const unsigned long long m = starts[i];
-    if (i < w)
-      c += __popcll(m);
-    else if (i == w)
-      c += __popcll(m & ((2ull << (u & 63)) - 1ull));
+    c += __popcll(i < w ? m : i == w ? (m & below) : 0ull);
   }
   return nz[c - 1];
 }

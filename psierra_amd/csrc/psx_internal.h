@@ -57,7 +57,7 @@ struct DevPlan {
   int32_t conn_stride;            // ints per element in conn (4 or 12)
   const int32_t* conn;            // internal node ids, AoS [E][conn_stride]
   const double4* Xi;              // reference coordinates, internal order
-  const float4* geo;              // Tet4: per element h1, h2, h3, V (FP32), 3 float4
+  const float4* geo;              // Tet4: per element and local node (h_b, V), FP32, 4 float4
   const int32_t* perm;            // internal -> user node id
   const int32_t* inc_ptr;         // N+1, internal order
   const uint32_t* inc_elem;       // packed (element | local << 28)

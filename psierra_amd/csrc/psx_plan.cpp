@@ -330,9 +330,9 @@ int32_t create_plan(const int64_t* conn, int64_t E, int npe, const double* X, in
       for (int p = 0; p < npe; ++p) conn_i[k * cs + p] = inv[conn[k * npe + p]];
   });
   // Tet4: per-element shape-function gradients and volume, computed in FP64
-  // and stored rounded to FP32 for the FP32 element path:
-  // g0 = (h1, h2.x), g1 = (h2.y, h2.z, h3.x, h3.y), g2 = (h3.z, V, 0, 0).
-  std::vector<float4> geo(npe == 4 ? size_t(E) * 3 : 0);
+  // and stored rounded to FP32 for the FP32 element path: one float4 per
+  // local node, (h_b, V), so a row can load its four gradients in any order.
+  std::vector<float4> geo(npe == 4 ? size_t(E) * 4 : 0);
   if (npe == 4) {
     parallel_for(E, [&](int64_t b, int64_t e) {
       for (int64_t k = b; k < e; ++k) {
@@ -347,9 +347,13 @@ int32_t create_plan(const int64_t* conn, int64_t E, int npe, const double* X, in
                      a22 = D[0][0] * D[1][1] - D[0][1] * D[1][0];
         const double det = D[0][0] * a00 + D[0][1] * a10 + D[0][2] * a20;
         const double id = 1.0 / det;
-        geo[3 * k + 0] = make_float4(float(a00 * id), float(a01 * id), float(a02 * id), float(a10 * id));
-        geo[3 * k + 1] = make_float4(float(a11 * id), float(a12 * id), float(a20 * id), float(a21 * id));
-        geo[3 * k + 2] = make_float4(float(a22 * id), float(det / 6.0), 0.f, 0.f);
+        const double h[3][3] = {{a00 * id, a01 * id, a02 * id}, {a10 * id, a11 * id, a12 * id},
+                                {a20 * id, a21 * id, a22 * id}};
+        const float V = float(det / 6.0);
+        geo[4 * k] = make_float4(float(-(h[0][0] + h[1][0] + h[2][0])), float(-(h[0][1] + h[1][1] + h[2][1])),
+                                 float(-(h[0][2] + h[1][2] + h[2][2])), V);
+        for (int b = 0; b < 3; ++b)
+          geo[4 * k + 1 + b] = make_float4(float(h[b][0]), float(h[b][1]), float(h[b][2]), V);
       }
     });
   }

@@ -52,13 +52,14 @@ __device__ __forceinline__ void flag_inversion(int32_t* status, int e) {
   status[0] = 1;
 }
 
-// Inputs of one incidence.
+// Inputs of one incidence, in the row's node order: the row node first,
+// then the element's other three nodes in increasing local index.
 template <typename R>
 struct IncInputs {
   uint32_t pe = 0, rank = 0;
   double m = 0.0, l = 0.0;
-  float4 g0{}, g1{}, g2{};   // FP32 path: precomputed gradients
-  double4 U[4];              // FP32: element order; FP64: row node first
+  float4 G[4];               // FP32 path: precomputed (h_b, V)
+  double4 U[4];
   double4 X[4];              // FP64 path: reference coordinates
 };
 
@@ -71,39 +72,35 @@ __device__ __forceinline__ void load_incidence(const DevPlan& p, const double* m
   const int4 c = *reinterpret_cast<const int4*>(p.conn + 4 * e);
   in.m = mu[e];
   in.l = lam[e];
+  const int lp = int(in.pe >> kIncElemBits);
+  const int r1 = lp == 0 ? 1 : 0, r2 = lp <= 1 ? 2 : 1, r3 = lp <= 2 ? 3 : 2;
+  const int i0 = lp == 0 ? c.x : lp == 1 ? c.y : lp == 2 ? c.z : c.w;
+  const int i1 = lp == 0 ? c.y : c.x;
+  const int i2 = lp <= 1 ? c.z : c.y;
+  const int i3 = lp <= 2 ? c.w : c.z;
   if constexpr (std::is_same<R, float>::value) {
-    in.g0 = p.geo[3 * e];
-    in.g1 = p.geo[3 * e + 1];
-    in.g2 = p.geo[3 * e + 2];
-    in.U[0] = p.ui[c.x];
-    in.U[1] = p.ui[c.y];
-    in.U[2] = p.ui[c.z];
-    in.U[3] = p.ui[c.w];
+    in.G[0] = p.geo[4 * e + lp];
+    in.G[1] = p.geo[4 * e + r1];
+    in.G[2] = p.geo[4 * e + r2];
+    in.G[3] = p.geo[4 * e + r3];
   } else {
-    const int lp = int(in.pe >> kIncElemBits);
-    const int i0 = lp == 0 ? c.x : lp == 1 ? c.y : lp == 2 ? c.z : c.w;
-    const int i1 = lp == 0 ? c.y : c.x;
-    const int i2 = lp <= 1 ? c.z : c.y;
-    const int i3 = lp <= 2 ? c.w : c.z;
     in.X[0] = p.Xi[i0];
     in.X[1] = p.Xi[i1];
     in.X[2] = p.Xi[i2];
     in.X[3] = p.Xi[i3];
-    in.U[0] = p.ui[i0];
-    in.U[1] = p.ui[i1];
-    in.U[2] = p.ui[i2];
-    in.U[3] = p.ui[i3];
   }
+  in.U[0] = p.ui[i0];
+  in.U[1] = p.ui[i1];
+  in.U[2] = p.ui[i2];
+  in.U[3] = p.ui[i3];
 }
 
 // Row of one incidence: out[0..35] = K_{a b} for b = a, then the other three
 // nodes (row-major 3x3 each), out[36..38] = f_a.  Returns det F.
 template <typename R, bool WANT_K>
 __device__ __forceinline__ R incidence_row(const IncInputs<R>& in, R* out) {
-  const int lp = int(in.pe >> kIncElemBits);
   if constexpr (std::is_same<R, float>::value)
-    return tet4_row_geo<WANT_K>(in.g0, in.g1, in.g2, in.U, lp, float(in.m), float(in.l + (2.0 / 3.0) * in.m),
-                                out);
+    return tet4_row_grad<WANT_K>(in.G, in.U, float(in.m), float(in.l + (2.0 / 3.0) * in.m), out);
   else
     return tet4_row<R, WANT_K>(in.X, in.U, R(in.m), R(in.l + (2.0 / 3.0) * in.m), out);
 }
@@ -254,10 +251,10 @@ __global__ __launch_bounds__(TILE) void tet4_assemble_rows(DevPlan p, const doub
   if (t < nd) {
     // (node, eighth of its run): K_aa upper triangle and f_a
     j = t / ND;
-    const int part = t % ND;
+    const unsigned part = unsigned(t) % ND;
     kb = s_incp[j];
-    const int L = s_incp[j + 1] - kb;
-    m1 = bit_range((part * L) / ND, ((part + 1) * L) / ND);   // <= 8 bits
+    const unsigned L = unsigned(s_incp[j + 1] - kb);
+    m1 = bit_range(int((part * L) / ND), int(((part + 1) * L) / ND));   // <= 8 bits
     diag = true;
   } else if (t < nd + units) {
     // (node, off-diagonal slot)
@@ -310,7 +307,7 @@ __global__ __launch_bounds__(TILE) void tet4_assemble_rows(DevPlan p, const doub
     const bool upper = (t & 4) != 0;
 #pragma unroll
     for (int c = 0; c < 9; ++c) acc[c] = oct_sum(acc[c], upper);
-    const int part = t % ND;
+    const int part = int(unsigned(t) % ND);
     const int d = s_uoff[j + 1] - s_uoff[j];
     if (part == 3) {
       const int64_t a = p.perm[n0 + j];
@@ -345,6 +342,7 @@ __global__ __launch_bounds__(TILE) void tet4_assemble_rows(DevPlan p, const doub
 #pragma unroll
       for (int c = 0; c < 3; ++c) dst[r * 3 * d + c] = acc[3 * r + c];
   }
+
 }
 
 template <typename R, int TILE>

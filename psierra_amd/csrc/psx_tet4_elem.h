@@ -104,17 +104,19 @@ __device__ __forceinline__ R tet4_row(const double4 X[4], const double4 U[4], R 
   return tet4_row_core<R, WANT_K>(h, V, A, mu, K, out);
 }
 
-// Same row from precomputed per-element geometry in the element's own node
-// order (g0 = (h1, h2.x), g1 = (h2.yz, h3.xy), g2 = (h3.z, V)), the
-// displacements U in the element's node order and the row node's local
-// index lp; the gradients are rotated to the row order [lp, others
-// ascending] after grad u is formed.
+// Same row from precomputed shape-function gradients already in the row's
+// node order: G[b] = (h_b, V) (b = 0 the row node), displacements U in the
+// same order (differences taken in FP64 before rounding).
 template <bool WANT_K>
-__device__ __forceinline__ float tet4_row_geo(const float4 g0, const float4 g1, const float4 g2,
-                                              const double4 U[4], int lp, float mu, float K, float* out) {
-  float h[4][3] = {{0.f, 0.f, 0.f}, {g0.x, g0.y, g0.z}, {g0.w, g1.x, g1.y}, {g1.z, g1.w, g2.x}};
+__device__ __forceinline__ float tet4_row_grad(const float4 G[4], const double4 U[4], float mu, float K,
+                                               float* out) {
+  float h[4][3];
 #pragma unroll
-  for (int c = 0; c < 3; ++c) h[0][c] = -(h[1][c] + h[2][c] + h[3][c]);
+  for (int b = 0; b < 4; ++b) {
+    h[b][0] = G[b].x;
+    h[b][1] = G[b].y;
+    h[b][2] = G[b].z;
+  }
   float A[3][3];
   {
     float dU[3][3];
@@ -129,15 +131,7 @@ __device__ __forceinline__ float tet4_row_geo(const float4 g0, const float4 g1, 
 #pragma unroll
       for (int j = 0; j < 3; ++j) A[i][j] = dU[i][0] * h[1][j] + dU[i][1] * h[2][j] + dU[i][2] * h[3][j];
   }
-  float hr[4][3];
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    hr[0][c] = lp == 0 ? h[0][c] : lp == 1 ? h[1][c] : lp == 2 ? h[2][c] : h[3][c];
-    hr[1][c] = lp == 0 ? h[1][c] : h[0][c];
-    hr[2][c] = lp <= 1 ? h[2][c] : h[1][c];
-    hr[3][c] = lp <= 2 ? h[3][c] : h[2][c];
-  }
-  return tet4_row_core<float, WANT_K>(hr, g2.y, A, mu, K, out);
+  return tet4_row_core<float, WANT_K>(h, G[0].w, A, mu, K, out);
 }
 
 // Adjoint of f_int for one incidence: the row-0 part of K_e w,
