@@ -22,6 +22,7 @@ import numpy as np
 
 METRIC = "Tet4 NeoHookean f_int+K_T assembly elements/s & %HBM roofline vs plain FP64"
 HBM_PEAK_GBPS = 8000.0
+TRAFFIC_FILE = "tet4_assemble_rows_box100_traffic.txt"
 
 
 def algorithmic_bytes(E, N, npe, nnz, coord_bytes, param_bytes):
@@ -51,6 +52,28 @@ def cpu_baseline(wl, sample_elems, seed=0):
     A.assemble(wl.mesh.nodes, conn, wl.u, wl.mu[idx], wl.lam[idx])
     t_vec = time.perf_counter() - t0
     return sample_elems / t_loop, sample_elems / t_vec, t_loop, t_vec
+
+
+PROFILE_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles")
+
+
+def profiled_traffic(name):
+    """HBM bytes per dispatch recorded by the rocprofv3 FETCH_SIZE and
+    WRITE_SIZE passes in profiles/<name> (``key = value`` lines, already
+    scaled by the copy-kernel calibration); None when absent."""
+    path = os.path.join(PROFILE_DIR, name)
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    with open(path) as fh:
+        for line in fh:
+            if "=" in line and not line.lstrip().startswith("#"):
+                k, v = line.split("=", 1)
+                try:
+                    vals[k.strip()] = float(v.split()[0])
+                except (ValueError, IndexError):
+                    pass
+    return vals
 
 
 def _event_time(fn, steps, warmup):
@@ -168,7 +191,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=100, help="hexes per box side (100 -> 6.0M tets)")
-    ap.add_argument("--cpu-sample", type=int, default=20000)
+    ap.add_argument("--cpu-sample", type=int, default=80000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--tile", type=int, default=0)
@@ -301,6 +324,7 @@ def main():
     copy_gbps = 5 * 2 * src.numel() * 8 / (a_ev.elapsed_time(b_ev) * 1e-3) / 1e9
     del src, dst
 
+    traffic = profiled_traffic(TRAFFIC_FILE) if args.n == 100 else None
     b_alg32 = algorithmic_bytes(E, N, 4, nnz, coord_bytes=4, param_bytes=4)
     b_alg64 = algorithmic_bytes(E, N, 4, nnz, coord_bytes=8, param_bytes=8)
     achieved = b_alg32 / (ms32_ev * 1e-3) / 1e9
@@ -342,8 +366,9 @@ def main():
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBPS,
-            "traffic": None,
-            "kernel": "psx_assemble (permute_u + tet4_assemble_kernel<float,256,true>)",
+            "traffic": traffic.get("traffic_bytes_fp32") if traffic else None,
+            "traffic_source": TRAFFIC_FILE if traffic else None,
+            "kernel": f"psx_assemble = permute_u_kernel + tet4_assemble_rows<float,{plan.info['tile']}>",
             "fp64_plain_frac": b_alg64 / (ms64_ev * 1e-3) / 1e9 / HBM_PEAK_GBPS,
         },
         "cpu_baseline": cpu,
