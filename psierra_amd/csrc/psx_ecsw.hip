@@ -12,11 +12,13 @@
 // B = [w_e k_e Phi_e] stacked over the RID (12 |RID| x 64 each),
 // K_r = A^T B.  Every work item (an instance and a chunk of its RID) runs in
 // a persistent launch; per batch of EB elements a work-group stages Phi_e in
-// LDS, evaluates u_e = Phi_e u_r, the element k_e and f_e (FP32), forms
-// w k_e Phi_e, and its four waves accumulate the four 32x32 quadrants of
-// A^T B with v_mfma_f32_32x32x2f32 (FP32 in, FP32 accumulate).  Each work
-// item writes its 64x64 partial; a second kernel adds the partials of an
-// instance in a fixed order, symmetrises (model.py:317) and writes FP64.
+// LDS (the next batch's Phi_e is already in flight into registers),
+// evaluates u_e = Phi_e u_r, the element rows of k_e and f_e (FP32, one lane
+// per (element, row node)), forms w k_e Phi_e, and its four waves
+// accumulate the four 32x32 quadrants of A^T B with v_mfma_f32_32x32x2f32
+// (FP32 in, FP32 accumulate).  Each work item writes its 64x64 partial; a
+// second kernel adds the partials of an instance in a fixed order, 16x16
+// tile pairs per work-group, symmetrises (model.py:317) and writes FP64.
 #include <algorithm>
 #include <climits>
 #include <cmath>
@@ -37,7 +39,7 @@ typedef float floatx16 __attribute__((ext_vector_type(16)));
 
 struct DevEcsw {
   int32_t n_inst, n_elem, n_items;
-  const float4* geom;        // [n_elem][4]: h1, h2, h3 (9), V, w, mu, K, pad
+  const float4* geom;        // [n_elem][5]: (h_b, 0) for b = 0..3, then (V w, mu, K, 0)
   const float* phi;          // [n_elem][12][64]
   const int32_t* item_inst;  // [n_items]
   const int32_t* item_e0;    // [n_items]
@@ -49,6 +51,7 @@ struct DevEcsw {
 
 __global__ __launch_bounds__(256) void ecsw_project_kernel(DevEcsw d, const double* __restrict__ ur_all,
                                                            int32_t* status) {
+  constexpr int kPre = kEB * kPhi / 4 / 256;   // float4 of Phi per thread per batch
   __shared__ float s_phi[kEB * kPhi];
   __shared__ float s_B[kEB * kPhi];
   __shared__ float s_k[kEB * 144];
@@ -63,7 +66,14 @@ __global__ __launch_bounds__(256) void ecsw_project_kernel(DevEcsw d, const doub
     const int inst = d.item_inst[item];
     const int e0 = d.item_e0[item];
     const int cnt = d.item_cnt[item];
-    __syncthreads();
+    float4 pre[kPre];
+    {
+      const int n4 = min(kEB, cnt) * (kPhi / 4);
+      const float4* src = reinterpret_cast<const float4*>(d.phi + int64_t(e0) * kPhi);
+#pragma unroll
+      for (int i = 0; i < kPre; ++i) pre[i] = t + 256 * i < n4 ? src[t + 256 * i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    __syncthreads();   // the previous item's reads of s_ur are done
     if (t < kRank) s_ur[t] = float(ur_all[int64_t(inst) * kRank + t]);
     floatx16 acc;
 #pragma unroll
@@ -71,10 +81,19 @@ __global__ __launch_bounds__(256) void ecsw_project_kernel(DevEcsw d, const doub
     float fr = 0.0f;
     for (int b0 = 0; b0 < cnt; b0 += kEB) {
       const int nb = min(kEB, cnt - b0);
+      __syncthreads();   // the previous batch's MFMA reads of s_phi / s_B are done
+#pragma unroll
+      for (int i = 0; i < kPre; ++i)
+        if (t + 256 * i < nb * (kPhi / 4)) reinterpret_cast<float4*>(s_phi)[t + 256 * i] = pre[i];
       __syncthreads();
-      const float4* src = reinterpret_cast<const float4*>(d.phi + int64_t(e0 + b0) * kPhi);
-      for (int i = t; i < nb * (kPhi / 4); i += 256) reinterpret_cast<float4*>(s_phi)[i] = src[i];
-      __syncthreads();
+      if (b0 + kEB < cnt) {
+        // the next batch's Phi_e, in flight during this batch
+        const int n4 = min(kEB, cnt - b0 - kEB) * (kPhi / 4);
+        const float4* src = reinterpret_cast<const float4*>(d.phi + int64_t(e0 + b0 + kEB) * kPhi);
+#pragma unroll
+        for (int i = 0; i < kPre; ++i)
+          pre[i] = t + 256 * i < n4 ? src[t + 256 * i] : make_float4(0.f, 0.f, 0.f, 0.f);
+      }
       if (t < nb * 12) {
         const float* row = s_phi + (t / 12) * kPhi + (t % 12) * kRank;
         float s = 0.0f;
@@ -83,27 +102,44 @@ __global__ __launch_bounds__(256) void ecsw_project_kernel(DevEcsw d, const doub
         s_ue[t] = s;
       }
       __syncthreads();
-      if (t < nb) {
-        const int e = e0 + b0 + t;
-        const float4 g0 = d.geom[4 * int64_t(e)], g1 = d.geom[4 * int64_t(e) + 1];
-        const float4 g2 = d.geom[4 * int64_t(e) + 2], g3 = d.geom[4 * int64_t(e) + 3];
-        float h[4][3] = {{0, 0, 0}, {g0.x, g0.y, g0.z}, {g0.w, g1.x, g1.y}, {g1.z, g1.w, g2.x}};
+      if (t < nb * 4) {
+        // row a of element el: K_{a b} for the four b and f_a
+        const int el = t >> 2, a = t & 3;
+        const int e = e0 + b0 + el;
+        const int r1 = a == 0 ? 1 : 0, r2 = a <= 1 ? 2 : 1, r3 = a <= 2 ? 3 : 2;
+        const int rot[4] = {a, r1, r2, r3};
+        const float4 prm = d.geom[5 * int64_t(e) + 4];
+        float h[4][3], U[4][3];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) h[0][c] = -(h[1][c] + h[2][c] + h[3][c]);
-        const float V = g2.y, w = g2.z, mu = g2.w, K = g3.x;
-        const float* ue = s_ue + 12 * t;
+        for (int r = 0; r < 4; ++r) {
+          const float4 g = d.geom[5 * int64_t(e) + rot[r]];
+          h[r][0] = g.x;
+          h[r][1] = g.y;
+          h[r][2] = g.z;
+#pragma unroll
+          for (int c = 0; c < 3; ++c) U[r][c] = s_ue[12 * el + 3 * rot[r] + c];
+        }
         float A[3][3];
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
           for (int j = 0; j < 3; ++j)
-            A[i][j] = (ue[3 + i] - ue[i]) * h[1][j] + (ue[6 + i] - ue[i]) * h[2][j] +
-                      (ue[9 + i] - ue[i]) * h[3][j];
-        const float J = tet4_full<float>(h, V, A, mu, K, w, s_k + 144 * t, s_f + 12 * t);
-        if (J <= 0.0f) {
+            A[i][j] = (U[1][i] - U[0][i]) * h[1][j] + (U[2][i] - U[0][i]) * h[2][j] + (U[3][i] - U[0][i]) * h[3][j];
+        float out[39];
+        const float J = tet4_row_core<float, true>(h, prm.x, A, prm.y, prm.z, out);
+        if (J <= 0.0f && a == 0) {
           atomicMin(&status[1], e);
           status[0] = 1;
         }
+        float* krow = s_k + 144 * el + 12 * 3 * a;
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+          for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) krow[12 * i + 3 * rot[r] + c] = out[9 * r + 3 * i + c];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) s_f[12 * el + 3 * a + i] = out[36 + i];
       }
       __syncthreads();
       // B_e = (w k_e) Phi_e
@@ -127,9 +163,9 @@ __global__ __launch_bounds__(256) void ecsw_project_kernel(DevEcsw d, const doub
 #pragma unroll
         for (int kp = 0; kp < 6; ++kp) {
           const int row = 2 * kp + (lane >> 5);
-          const float a = s_phi[e * kPhi + row * kRank + 32 * qI + (lane & 31)];
-          const float b = s_B[e * kPhi + row * kRank + 32 * qJ + (lane & 31)];
-          acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
+          const float av = s_phi[e * kPhi + row * kRank + 32 * qI + (lane & 31)];
+          const float bv = s_B[e * kPhi + row * kRank + 32 * qJ + (lane & 31)];
+          acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, acc, 0, 0, 0);
         }
       }
     }
@@ -145,23 +181,41 @@ __global__ __launch_bounds__(256) void ecsw_project_kernel(DevEcsw d, const doub
   }
 }
 
+// K_r = 1/2 (S + S^T), S = sum of an instance's partials in item order, in
+// FP64.  One work-group per (instance, pair of 16x16 tiles (I, J), I <= J):
+// it sums tile (I, J) and tile (J, I) with coalesced row reads, transposes
+// the second through LDS and writes both tiles.  The (0, 0) group also sums
+// f_r.
 __global__ __launch_bounds__(256) void ecsw_reduce_kernel(DevEcsw d, double* __restrict__ f_r,
                                                           double* __restrict__ K_r) {
-  const int inst = blockIdx.x;
-  const int i0 = d.inst_item0[inst], i1 = d.inst_item0[inst + 1];
-  for (int o = threadIdx.x; o < kRank * kRank; o += 256) {
-    const int i = o / kRank, j = o % kRank;
-    double a = 0.0, b = 0.0;
-    for (int it = i0; it < i1; ++it) {
-      a += double(d.partK[int64_t(it) * kRank * kRank + i * kRank + j]);
-      b += double(d.partK[int64_t(it) * kRank * kRank + j * kRank + i]);
-    }
-    K_r[int64_t(inst) * kRank * kRank + o] = 0.5 * (a + b);
+  __shared__ double s_t[16][17];
+  const int inst = blockIdx.x / 10;
+  int pair = blockIdx.x % 10, I = 0;
+  while (pair >= 4 - I) {
+    pair -= 4 - I;
+    ++I;
   }
-  if (threadIdx.x < kRank) {
+  const int J = I + pair;
+  const int t = threadIdx.x;
+  const int i = t >> 4, j = t & 15;
+  const int i0 = d.inst_item0[inst], i1 = d.inst_item0[inst + 1];
+  double a = 0.0, b = 0.0;
+  for (int it = i0; it < i1; ++it) {
+    const float* P = d.partK + int64_t(it) * kRank * kRank;
+    a += double(P[(16 * I + i) * kRank + 16 * J + j]);
+    b += double(P[(16 * J + i) * kRank + 16 * I + j]);
+  }
+  s_t[i][j] = b;   // S[16 J + i][16 I + j]
+  __syncthreads();
+  // element (16 I + i, 16 J + j): S_ij from a, S_ji = s_t[j][i]
+  const double v = 0.5 * (a + s_t[j][i]);
+  double* K = K_r + int64_t(inst) * kRank * kRank;
+  K[(16 * I + i) * kRank + 16 * J + j] = v;
+  K[(16 * J + j) * kRank + 16 * I + i] = v;
+  if (I == 0 && J == 0 && t < kRank) {
     double s = 0.0;
-    for (int it = i0; it < i1; ++it) s += double(d.partF[int64_t(it) * kRank + threadIdx.x]);
-    f_r[int64_t(inst) * kRank + threadIdx.x] = s;
+    for (int it = i0; it < i1; ++it) s += double(d.partF[int64_t(it) * kRank + t]);
+    f_r[int64_t(inst) * kRank + t] = s;
   }
 }
 
@@ -192,7 +246,7 @@ int32_t ecsw_create(int32_t n_inst, const int64_t* rid_offsets, const double* Xe
   const int64_t n_elem = rid_offsets[n_inst];
   if (n_elem <= 0 || n_elem >= (int64_t(1) << 31) / kPhi * 4)
     return fail(PSX_EINVAL, "bad RID size");
-  std::vector<float4> geom(size_t(n_elem) * 4);
+  std::vector<float4> geom(size_t(n_elem) * 5);
   for (int64_t e = 0; e < n_elem; ++e) {
     const double* X = Xe + 12 * e;
     double D[3][3];
@@ -213,14 +267,13 @@ int32_t ecsw_create(int32_t n_inst, const int64_t* rid_offsets, const double* Xe
       return fail(PSX_EVOLUME, "Negative or zero element volume detected in RID element " +
                                    std::to_string(e) + ". Check element vertex ordering.");
     if (!(weights[e] > 0.0)) return fail(PSX_EINVAL, "All ECSW element weights must be strictly positive (w_e > 0).");
-    float h[9];
+    double h[4][3];
     for (int b = 0; b < 3; ++b)
-      for (int c = 0; c < 3; ++c) h[3 * b + c] = float(adj[b][c] / det);
+      for (int c = 0; c < 3; ++c) h[b + 1][c] = adj[b][c] / det;
+    for (int c = 0; c < 3; ++c) h[0][c] = -(h[1][c] + h[2][c] + h[3][c]);
     const double K = lam[e] + (2.0 / 3.0) * mu[e];
-    geom[4 * e + 0] = make_float4(h[0], h[1], h[2], h[3]);
-    geom[4 * e + 1] = make_float4(h[4], h[5], h[6], h[7]);
-    geom[4 * e + 2] = make_float4(h[8], float(det / 6.0), float(weights[e]), float(mu[e]));
-    geom[4 * e + 3] = make_float4(float(K), 0.f, 0.f, 0.f);
+    for (int b = 0; b < 4; ++b) geom[5 * e + b] = make_float4(float(h[b][0]), float(h[b][1]), float(h[b][2]), 0.f);
+    geom[5 * e + 4] = make_float4(float(det / 6.0 * weights[e]), float(mu[e]), float(K), 0.f);
   }
   std::vector<float> phif(size_t(n_elem) * kPhi);
   for (size_t i = 0; i < phif.size(); ++i) phif[i] = float(phi[i]);
@@ -315,7 +368,7 @@ int32_t psx_ecsw_evaluate(psx_ecsw* h, const double* u_r, double* f_r, double* K
   int32_t rc = psx::launch_status_reset(s, status);
   if (rc) return rc;
   hipLaunchKernelGGL(psx::ecsw_project_kernel, dim3(h->grid), dim3(256), 0, s, h->d, u_r, status);
-  hipLaunchKernelGGL(psx::ecsw_reduce_kernel, dim3(h->d.n_inst), dim3(256), 0, s, h->d, f_r, K_r);
+  hipLaunchKernelGGL(psx::ecsw_reduce_kernel, dim3(10 * h->d.n_inst), dim3(256), 0, s, h->d, f_r, K_r);
   PSX_HIP_TRY(hipGetLastError());
   return PSX_OK;
 }

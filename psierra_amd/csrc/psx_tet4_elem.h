@@ -207,42 +207,4 @@ __device__ __forceinline__ R tet4_vjp_row(const double4 X[4], const double4 U[4]
 }
 
 
-// Whole element from precomputed geometry: k (12x12 row-major, scaled by w)
-// and f (12, scaled by w).  h[b] = grad_X N_b, V = volume, A = grad u.
-// The material point is evaluated once; K_ba = K_ab^T fills the lower half.
-template <typename R>
-__device__ __forceinline__ R tet4_full(const R h[4][3], R V, const R A[3][3], R mu, R K, R w, R* k,
-                                       R* f) {
-  NHPoint<R> m;
-  nh_point(A, mu, K, m);
-  R g[4][3], q[4][3], Sh[4][3];
-#pragma unroll
-  for (int b = 0; b < 4; ++b) {
-    mat3vec(m.Gm, h[b], g[b]);
-    mat3vec(m.F, h[b], q[b]);
-    mat3vec(m.S, h[b], Sh[b]);
-  }
-  const R Vw = V * w;
-#pragma unroll
-  for (int a = 0; a < 4; ++a) {
-    R FSh[3];
-    mat3vec(m.F, Sh[a], FSh);
-#pragma unroll
-    for (int i = 0; i < 3; ++i) f[3 * a + i] = Vw * FSh[i];
-#pragma unroll
-    for (int b = a; b < 4; ++b) {
-      R blk[9];
-      nh_block(m, Vw, g[a], q[a], Sh[a], g[b], q[b], h[b], blk);
-#pragma unroll
-      for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          k[12 * (3 * a + i) + 3 * b + c] = blk[3 * i + c];
-          if (b != a) k[12 * (3 * b + c) + 3 * a + i] = blk[3 * i + c];
-        }
-    }
-  }
-  return m.J;
-}
-
 }  // namespace psx
