@@ -1,0 +1,125 @@
+"""Edge cases of the Tet4 assembly on the GPU, against the FP64 oracle.
+
+* nodes that no element references (empty CSR rows, f = 0 exactly) and a
+  one-element mesh;
+* a node shared by 64 elements, the most incidences a node may have (its
+  feed masks are 64-bit), and the plan's rejection of a 65th;
+* zero displacement: f = 0 bit-exactly and K equal to the small-strain
+  stiffness of the oracle (S = 0, C = Hookean);
+* every supported tile size gives the same bits on a mesh whose node runs
+  straddle tile boundaries in many ways.
+Tolerances as in test_gpu_tet4.py: FP64 1e-12 (K) / 1e-11 (f), FP32 1e-5.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import assembly as A
+from psierra_amd.mesh import TetMesh
+from psierra_amd.workloads import Workload
+from tests.gpu_helpers import banner, gpu_assemble, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _check(wl, precision):
+    f_ref, K_ref = A.assemble(wl.mesh.nodes, wl.mesh.elements, wl.u, wl.mu, wl.lam)
+    plan, f, k = gpu_assemble(wl, precision)
+    indptr, indices = plan.pattern()
+    assert np.array_equal(indptr, K_ref.indptr) and np.array_equal(indices, K_ref.indices)
+    tk, tf = (1e-12, 1e-11) if precision == "fp64" else (1e-5, 1e-5)
+    assert rel_err(k, K_ref.data) < tk
+    assert rel_err(f, f_ref) < tf
+    return plan, f, k, f_ref
+
+
+def _fan(n_elem: int, seed: int) -> Workload:
+    """n_elem tetrahedra sharing node 0 and nothing else, all positively oriented."""
+    rng = np.random.default_rng(seed)
+    nodes = [np.zeros(3)]
+    conn = []
+    for e in range(n_elem):
+        th = 2 * np.pi * e / n_elem
+        d = np.array([np.cos(th), np.sin(th), 0.1 * rng.standard_normal()])
+        base = 1.0 + 0.05 * rng.standard_normal()
+        p1 = base * d
+        p2 = p1 + 0.3 * np.array([-np.sin(th), np.cos(th), 0.0])
+        p3 = p1 + np.array([0.0, 0.0, 0.4])
+        pts = [p1, p2, p3]
+        ids = list(range(len(nodes), len(nodes) + 3))
+        nodes.extend(pts)
+        X = np.array([nodes[0], *pts])
+        if np.linalg.det((X[1:] - X[0]).T) < 0:
+            ids[1], ids[2] = ids[2], ids[1]
+        conn.append([0, *ids])
+    nodes = np.array(nodes)
+    conn = np.array(conn, dtype=np.int64)
+    u = 0.01 * rng.standard_normal(nodes.shape)
+    mu = 1e3 * np.exp(0.3 * rng.standard_normal(n_elem))
+    return Workload(f"fan_{n_elem}", TetMesh(nodes, conn), u, mu, 5.0 * mu)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_isolated_nodes_and_single_element(precision):
+    nodes = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1], [5, 5, 5], [6, 5, 5], [7, 7, 7.0]])
+    # one positively oriented element on nodes 0..3; nodes 4..6 are unreferenced
+    conn = np.array([[0, 1, 2, 3]], dtype=np.int64)
+    rng = np.random.default_rng(7)
+    u = 0.02 * rng.standard_normal(nodes.shape)
+    wl = Workload("one_element", TetMesh(nodes, conn), u, np.array([1e3]), np.array([5e3]))
+    banner(f"edge: isolated nodes + single element {precision}")
+    plan, f, _, _ = _check(wl, precision)
+    assert np.all(f[12:] == 0.0)
+    indptr, _ = plan.pattern()
+    assert np.all(np.diff(indptr)[12:] == 0)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_node_with_64_incidences(precision):
+    wl = _fan(64, seed=8)
+    banner(f"edge: fan of 64 elements {precision}")
+    plan, _, _, _ = _check(wl, precision)
+    assert plan.info["tile"] == 256
+
+
+def test_node_with_65_incidences_is_rejected():
+    from psierra_amd.engine import AssemblyPlan
+
+    wl = _fan(65, seed=9)
+    banner("edge: fan of 65 elements (rejected)")
+    with pytest.raises(Exception, match="incident elements exceeds the supported 64"):
+        AssemblyPlan(wl.mesh.nodes, wl.mesh.elements)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_zero_displacement(precision):
+    from psierra_amd import workloads as W
+
+    wl = W.jittered_box_tet4(n=6, seed=12)
+    wl.u = np.zeros_like(wl.u)
+    banner(f"edge: zero displacement {precision}")
+    # The oracle forms F = (X + u)^T H, so its f is rounding noise (~1e-14
+    # of the stiffness scale) rather than 0; the engine forms grad u directly
+    # and returns exact zeros.  K is compared as usual.
+    f_ref, K_ref = A.assemble(wl.mesh.nodes, wl.mesh.elements, wl.u, wl.mu, wl.lam)
+    plan, f, k = gpu_assemble(wl, precision)
+    assert np.all(f == 0.0)
+    assert np.abs(f_ref).max() < 1e-9 * np.abs(K_ref.data).max()
+    assert rel_err(k, K_ref.data) < (1e-12 if precision == "fp64" else 1e-5)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_all_tiles_bit_identical(precision):
+    from psierra_amd import workloads as W
+
+    wl = W.jittered_box_tet4(n=9, seed=13)
+    banner(f"edge: tiles 64/128/256 {precision}")
+    ref = None
+    for tile in (64, 128, 256):
+        plan, f, k = gpu_assemble(wl, precision, tile=tile)
+        assert plan.info["tile"] == tile
+        if ref is None:
+            ref = (f, k)
+        else:
+            assert np.array_equal(f, ref[0]) and np.array_equal(k, ref[1])
