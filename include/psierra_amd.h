@@ -168,6 +168,24 @@ int32_t psx_ecsw_info(const psx_ecsw* ecsw, int64_t* info, int32_t n_info);
 int32_t psx_ecsw_evaluate(psx_ecsw* ecsw, const double* u_r, double* f_r, double* K_r,
                           void* stream, int32_t* status);
 
+/* ---- ECSW training matrix (device, Tet4, FP64) ----------------------------
+ * Replaces ECSWTrainingMatrix.build, single-domain path
+ * (src/psierra/hyperreduction/ecsw.py:171-365, loops at :319-365) for a
+ * Tet4 plan.  phi: basis (3N x rank, row-major, user DOF order, 1 <= rank
+ * <= 64); snapshots: (3N x n_snapshots, row-major); mu, lam: per element.
+ * Writes A (rows x n_elements, row-major) with rows = n_snapshots * rank
+ * (+ rank (rank + 1) / 2 when include_stiffness):
+ *   A[k rank + i][e] = (Phi_e^T f_e(u_k))_i,
+ *   A[n_snapshots rank + t][e] = (Phi_e^T K0_e Phi_e)[triu_t] (unscaled;
+ *   t in numpy.triu_indices order, K0_e the element tangent at u = 0).
+ * work: device scratch of rows x n_elements doubles.  The reference's
+ * balancing scale of the stiffness rows (ecsw.py:350-357) and b = A 1 are
+ * left to the caller.  status as psx_assemble (element inversion). */
+int32_t psx_ecsw_training(psx_plan* plan, const double* phi, int32_t rank, const double* snapshots,
+                          int32_t n_snapshots, const double* mu, const double* lam,
+                          int32_t include_stiffness, double* A, double* work, void* stream,
+                          int32_t* status);
+
 /* ---- Element batch (device) -------------------------------------------------
  * Replaces Tet4TotalLagrangian.compute_nonlinear_force_and_tangent_stiffness
  * (src/psierra/element/tet4_total_lagrangian.py:77-139) and the Tet10 one

@@ -1,0 +1,202 @@
+// ECSW training matrix on the GPU (FP64), for Tet4 Neo-Hookean meshes.
+//
+// ECSWTrainingMatrix.build, single-domain path
+// (src/psierra/hyperreduction/ecsw.py:171-365, loops at :319-365):
+//   A[k r + i][e] = (Phi_e^T f_e(u_k))_i            snapshot k, mode i < r,
+// and, on request, the upper triangle (numpy triu_indices order) of
+// Phi_e^T K0_e Phi_e, K0_e the element tangent at u = 0, in rows m r .. .
+// Phi_e = Phi[dofs of e, :] with Phi (3N x r, user DOF order, row-major) and
+// snapshots (3N x m, row-major) as the reference's arrays.  The stiffness
+// block is returned unscaled (the caller applies the reference's norm
+// balancing, :350-357, which needs global norms of both blocks).
+//
+// Layout of the work: one wave per element.  Lanes = snapshots evaluate the
+// element force in FP64 and leave it in LDS; lanes = modes then hold the
+// element's 12 basis rows (one coalesced 8-byte load per row) and form
+// Phi_e^T f for every snapshot, writing an element-major row of the work
+// array (consecutive lanes, consecutive values).  The stiffness block: four
+// lanes evaluate the rows of K0_e, the wave stages Phi_e in LDS, lane j forms
+// T = K0_e Phi_e[:, j] and the column j of Phi_e^T T above the diagonal.  A
+// tiled transpose through LDS turns the element-major work array into the
+// reference's (rows x E) layout.
+#include <algorithm>
+#include <climits>
+
+#include "psx_device.h"
+#include "psx_internal.h"
+#include "psx_nh.h"
+#include "psx_tet4_elem.h"
+
+namespace psx {
+
+constexpr int kTrainMaxRank = 64;
+constexpr int kTrainSnapPass = 64;
+
+__device__ __forceinline__ void train_flag(int32_t* status, int e) {
+  atomicMin(&status[1], e);
+  status[0] = 1;
+}
+
+__global__ __launch_bounds__(256) void ecsw_train_force_kernel(DevPlan p, const double* __restrict__ phi,
+                                                               int r, const double* __restrict__ snaps, int m,
+                                                               const double* __restrict__ mu,
+                                                               const double* __restrict__ lam, int rows,
+                                                               double* __restrict__ At, int32_t* status) {
+  __shared__ double s_f[4][kTrainSnapPass][13];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int64_t base = int64_t(blockIdx.x) * 4; base < p.E; base += int64_t(gridDim.x) * 4) {
+    const int64_t e = base + wave;
+    const bool valid = e < p.E;
+    int64_t un[4] = {0, 0, 0, 0};
+    double4 X[4];
+    double ph[12];
+    double mue = 0.0, Ke = 0.0;
+    if (valid) {
+      const int4 c = *reinterpret_cast<const int4*>(p.conn + 4 * e);
+      const int ci[4] = {c.x, c.y, c.z, c.w};
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        un[b] = p.perm[ci[b]];
+        X[b] = p.Xi[ci[b]];
+      }
+      if (lane < r)
+#pragma unroll
+        for (int d = 0; d < 12; ++d) ph[d] = phi[(3 * un[d / 3] + d % 3) * r + lane];
+      mue = mu[e];
+      Ke = lam[e] + (2.0 / 3.0) * mue;
+    }
+    for (int k0 = 0; k0 < m; k0 += kTrainSnapPass) {
+      const int nk = min(kTrainSnapPass, m - k0);
+      if (valid && lane < nk) {
+        const int64_t k = k0 + lane;
+        double4 U[4];
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+          U[b] = make_double4(snaps[(3 * un[b]) * m + k], snaps[(3 * un[b] + 1) * m + k],
+                              snaps[(3 * un[b] + 2) * m + k], 0.0);
+        double fe[12];
+        const double J = tet4_element_force<double>(X, U, mue, Ke, fe);
+        if (J <= 0.0) train_flag(status, int(e));
+#pragma unroll
+        for (int d = 0; d < 12; ++d) s_f[wave][lane][d] = fe[d];
+      }
+      __syncthreads();
+      if (valid && lane < r) {
+        double* dst = At + e * rows + int64_t(k0) * r + lane;
+        for (int kk = 0; kk < nk; ++kk) {
+          double cv = 0.0;
+#pragma unroll
+          for (int d = 0; d < 12; ++d) cv += ph[d] * s_f[wave][kk][d];
+          dst[int64_t(kk) * r] = cv;
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void ecsw_train_stiff_kernel(DevPlan p, const double* __restrict__ phi,
+                                                               int r, const double* __restrict__ mu,
+                                                               const double* __restrict__ lam, int rows,
+                                                               int row0, double* __restrict__ At,
+                                                               int32_t* status) {
+  __shared__ double s_phi[4][12][kTrainMaxRank];
+  __shared__ double s_K[4][12][13];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int64_t base = int64_t(blockIdx.x) * 4; base < p.E; base += int64_t(gridDim.x) * 4) {
+    const int64_t e = base + wave;
+    const bool valid = e < p.E;
+    if (valid) {
+      const int4 c = *reinterpret_cast<const int4*>(p.conn + 4 * e);
+      const int ci[4] = {c.x, c.y, c.z, c.w};
+      if (lane < r)
+#pragma unroll
+        for (int d = 0; d < 12; ++d) s_phi[wave][d][lane] = phi[(3 * int64_t(p.perm[ci[d / 3]]) + d % 3) * r + lane];
+      if (lane < 4) {
+        // row `a` of K0_e: blocks in the order [a, others ascending]
+        const int a = lane;
+        const int rot[4] = {a, a == 0 ? 1 : 0, a <= 1 ? 2 : 1, a <= 2 ? 3 : 2};
+        double4 Xr[4], Ur[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int cq = rot[q] == 0 ? ci[0] : rot[q] == 1 ? ci[1] : rot[q] == 2 ? ci[2] : ci[3];
+          Xr[q] = p.Xi[cq];
+          Ur[q] = make_double4(0.0, 0.0, 0.0, 0.0);
+        }
+        const double mue = mu[e];
+        double out[39];
+        const double J = tet4_row<double, true>(Xr, Ur, mue, lam[e] + (2.0 / 3.0) * mue, out);
+        if (J <= 0.0) train_flag(status, int(e));
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+          for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc) s_K[wave][3 * a + i][3 * rot[q] + cc] = out[9 * q + 3 * i + cc];
+      }
+    }
+    __syncthreads();
+    if (valid && lane < r) {
+      const int j = lane;
+      double T[12];
+#pragma unroll
+      for (int d = 0; d < 12; ++d) {
+        double s = 0.0;
+#pragma unroll
+        for (int cc = 0; cc < 12; ++cc) s += s_K[wave][d][cc] * s_phi[wave][cc][j];
+        T[d] = s;
+      }
+      double* dst = At + e * rows + row0;
+      for (int i = 0; i <= j; ++i) {
+        double s = 0.0;
+#pragma unroll
+        for (int d = 0; d < 12; ++d) s += s_phi[wave][d][i] * T[d];
+        dst[i * r - (i * (i - 1)) / 2 + (j - i)] = s;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// out (cols x n) = in (n x cols)^T, 32 x 32 tiles through LDS.
+__global__ __launch_bounds__(256) void transpose_f64_kernel(const double* __restrict__ in, int64_t n, int cols,
+                                                            double* __restrict__ out) {
+  __shared__ double tile[32][33];
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const int64_t r0 = int64_t(blockIdx.x) * 32;
+  const int c0 = blockIdx.y * 32;
+  for (int y = ty; y < 32; y += 8) {
+    const int64_t rr = r0 + y;
+    const int cc = c0 + tx;
+    if (rr < n && cc < cols) tile[y][tx] = in[rr * cols + cc];
+  }
+  __syncthreads();
+  for (int y = ty; y < 32; y += 8) {
+    const int cc = c0 + y;
+    const int64_t rr = r0 + tx;
+    if (rr < n && cc < cols) out[int64_t(cc) * n + rr] = tile[tx][y];
+  }
+}
+
+int32_t launch_status_reset(hipStream_t s, int32_t* status);
+
+int32_t launch_ecsw_training(const DevPlan& p, const double* phi, int r, const double* snaps, int m,
+                             const double* mu, const double* lam, int stiff, double* A, double* work,
+                             hipStream_t s, int32_t* status) {
+  const int64_t rows = int64_t(m) * r + (stiff ? int64_t(r) * (r + 1) / 2 : 0);
+  if (rows > INT_MAX / 2) return fail(PSX_EINVAL, "too many training rows");
+  int32_t rc = launch_status_reset(s, status);
+  if (rc) return rc;
+  const int64_t groups = std::min<int64_t>((int64_t(p.E) + 3) / 4, 1 << 16);
+  hipLaunchKernelGGL(ecsw_train_force_kernel, dim3(unsigned(groups)), dim3(256), 0, s, p, phi, r, snaps, m, mu,
+                     lam, int(rows), work, status);
+  if (stiff)
+    hipLaunchKernelGGL(ecsw_train_stiff_kernel, dim3(unsigned(groups)), dim3(256), 0, s, p, phi, r, mu, lam,
+                       int(rows), m * r, work, status);
+  hipLaunchKernelGGL(transpose_f64_kernel, dim3(unsigned((int64_t(p.E) + 31) / 32), unsigned((rows + 31) / 32)),
+                     dim3(256), 0, s, work, int64_t(p.E), int(rows), A);
+  PSX_HIP_TRY(hipGetLastError());
+  return PSX_OK;
+}
+
+}  // namespace psx

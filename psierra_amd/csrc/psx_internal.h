@@ -115,6 +115,9 @@ int32_t launch_tet10_assemble(const DevPlan& p, const double* u, const double* m
 int32_t launch_tet4_vjp(const DevPlan& p, const double* u, const double* mu, const double* lam,
                         const double* w, double* gu, double* gmu, double* glam, int precision,
                         hipStream_t s, int32_t* status);
+int32_t launch_ecsw_training(const DevPlan& p, const double* phi, int r, const double* snaps, int m,
+                             const double* mu, const double* lam, int stiff, double* A, double* work,
+                             hipStream_t s, int32_t* status);
 int32_t launch_csc_extract(const DevCsc& c, const double* kval, int drop, int* colptr, int* rowind,
                            double* vals, hipStream_t s);
 int32_t launch_free_residual(const DevCsc& c, const double* f_int, const double* f_ext_free, double* r,

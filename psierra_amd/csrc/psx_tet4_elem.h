@@ -134,6 +134,26 @@ __device__ __forceinline__ float tet4_row_grad(const float4 G[4], const double4 
   return tet4_row_core<float, WANT_K>(h, G[0].w, A, mu, K, out);
 }
 
+// Internal force of a whole element in its own node order: f[3 b + i] =
+// V (F S h_b)_i (the B^T S_voigt V of tet4_total_lagrangian.py:136-137).
+// Returns det F.
+template <typename R>
+__device__ __forceinline__ R tet4_element_force(const double4 X[4], const double4 U[4], R mu, R K, R f[12]) {
+  R h[4][3], V, A[3][3];
+  tet4_geometry(X, U, h, V, A);
+  NHPoint<R> m;
+  nh_point(A, mu, K, m);
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    R Sh[3], FSh[3];
+    mat3vec(m.S, h[b], Sh);
+    mat3vec(m.F, Sh, FSh);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) f[3 * b + i] = V * FSh[i];
+  }
+  return m.J;
+}
+
 // Adjoint of f_int for one incidence: the row-0 part of K_e w,
 //   (K_e w)_0 = V [ a tr(L) g_0 + b/2 (L + L^T) g_0 + c ((F : grad w) g_0
 //               + tr(L) q_0) + grad w S h_0 ],

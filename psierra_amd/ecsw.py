@@ -109,3 +109,92 @@ class HyperReducedECSWModel:
             raise ValueError(f"u_r size ({u.size}) must match reduced dimension ({self.r}).")
         f, K = self._batch.evaluate(u[None, :])
         return f[0].cpu().numpy(), K[0].cpu().numpy()
+
+
+def _snapshot_matrix(snapshots) -> np.ndarray:
+    """(n_dofs, m) snapshot matrix with the reference's argument checks
+    (src/psierra/hyperreduction/ecsw.py:208-225)."""
+    if isinstance(snapshots, np.ndarray):
+        if snapshots.ndim == 2:
+            return snapshots
+        if snapshots.ndim == 1:
+            return snapshots.reshape(-1, 1)
+        raise ValueError(f"Snapshot array must be 1D or 2D, got shape {snapshots.shape}")
+    if isinstance(snapshots, (list, tuple)):
+        if len(snapshots) == 0:
+            raise ValueError("Snapshots list cannot be empty.")
+        return np.column_stack(snapshots)
+    raise ValueError(f"Snapshots must be a 2D array or sequence of 1D vectors, got type {type(snapshots)}")
+
+
+class ECSWTrainingMatrix:
+    """Virtual-work training matrix for ECSW on the GPU.
+
+    ECSWTrainingMatrix.build (src/psierra/hyperreduction/ecsw.py:171-365),
+    single-domain Tet4 path: column e stacks Phi_e^T f_e(u_k) over the
+    snapshots k (``psx_ecsw_training``, FP64), optionally followed by the
+    upper triangle of Phi_e^T K0_e Phi_e scaled so that the RMS row norms of
+    the two blocks match (times ``stiffness_weight``), and b = A 1.  The
+    norm balancing and b are formed from the device array with torch
+    reductions.  ``free_dofs`` is accepted and unused, as in the reference.
+    """
+
+    @staticmethod
+    def build(domain, basis, snapshots, free_dofs=None, include_stiffness: bool = False,
+              stiffness_weight: float | None = None):
+        A, b = ECSWTrainingMatrix.build_device(domain, basis, snapshots, include_stiffness=include_stiffness,
+                                               stiffness_weight=stiffness_weight)
+        return A.cpu().numpy(), b.cpu().numpy()
+
+    @staticmethod
+    def build_device(domain, basis, snapshots, include_stiffness: bool = False,
+                     stiffness_weight: float | None = None):
+        torch = _torch()
+        snap = _snapshot_matrix(snapshots)
+        n_dofs, m = snap.shape
+        if n_dofs != domain.n_dofs:
+            raise ValueError(
+                f"Snapshot DOF count ({n_dofs}) does not match domain DOF count ({domain.n_dofs})."
+            )
+        phi = np.asarray(getattr(basis, "basis_matrix", basis), dtype=np.float64)
+        r = phi.shape[1]
+        element = domain.element
+        if not hasattr(element, "plan"):
+            raise TypeError("ECSWTrainingMatrix on the GPU needs a GPU element (GpuTet4TotalLagrangian)")
+        mesh = domain.mesh
+        plan = element.plan(mesh)
+        if plan.npe != 4:
+            raise NotImplementedError("the GPU training matrix is built for Tet4 meshes")
+        st = element._state(mesh)
+        dev = plan.device
+        E = plan.n_elements
+        phi_t = torch.as_tensor(np.ascontiguousarray(phi), device=dev)
+        snap_t = torch.as_tensor(np.ascontiguousarray(snap, dtype=np.float64), device=dev)
+        n_triu = r * (r + 1) // 2 if include_stiffness else 0
+        rows = m * r + n_triu
+        A = torch.empty((rows, E), dtype=torch.float64, device=dev)
+        work = torch.empty((E, rows), dtype=torch.float64, device=dev)
+        status = torch.zeros(2, dtype=torch.int32, device=dev)
+        check(lib().psx_ecsw_training(plan._h, ptr(phi_t), r, ptr(snap_t), m, ptr(st["mu"]), ptr(st["lam"]),
+                                      int(bool(include_stiffness)), ptr(A), ptr(work), current_stream_handle(),
+                                      ptr(status)))
+        del work
+        stv = status.cpu().numpy()
+        if stv[0] != 0:
+            # the reference's exception for the lowest inverted element, with
+            # det F of the first snapshot that inverts it
+            from .engine import INVERSION_MSG, _det3
+
+            conn = plan.elements[int(stv[1])]
+            X = plan.nodes[conn]
+            Dinv = np.linalg.inv((X[1:] - X[0]).T)
+            H = np.vstack([-Dinv.sum(axis=0), Dinv])
+            dets = [_det3((X + snap[:, k].reshape(-1, 3)[conn]).T @ H) for k in range(m)]
+            raise ValueError(INVERSION_MSG.format(next((d for d in dets if d <= 0.0), min(dets))))
+        if include_stiffness:
+            s_force = float(torch.linalg.vector_norm(A[: m * r])) / np.sqrt(max(m * r, 1))
+            s_stiff = float(torch.linalg.vector_norm(A[m * r:])) / np.sqrt(max(n_triu, 1))
+            if s_force > 1e-12 and s_stiff > 1e-12:
+                weight = float(stiffness_weight) if stiffness_weight is not None else 1.0
+                A[m * r:] *= (s_force / s_stiff) * weight
+        return A, A.sum(dim=1)
