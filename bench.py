@@ -185,6 +185,53 @@ def bench_ecsw(args):
     }
 
 
+def bench_ecsw_train(args):
+    """ECSW training matrix (FP64) on the 48^3 Tet4 box: rank 64, 8 snapshots,
+    with the linear-stiffness block (rows = 8*64 + 64*65/2 = 2592)."""
+    import torch
+
+    from psierra_amd import _lib
+    from psierra_amd import workloads as W
+    from psierra_amd.engine import AssemblyPlan, current_stream_handle
+
+    wl = W.vjp_box_tet4(n=48, seed=3)
+    plan = AssemblyPlan(wl.mesh.nodes, wl.mesh.elements)
+    dev = plan.device
+    r, m = 64, 8
+    rng = np.random.default_rng(6)
+    n = wl.mesh.n_dofs
+    phi = torch.as_tensor(np.linalg.qr(rng.standard_normal((n, r)))[0], device=dev).contiguous()
+    snaps = torch.as_tensor(1e-4 * rng.standard_normal((n, m)) + wl.u.reshape(-1, 1), device=dev).contiguous()
+    mu = torch.as_tensor(wl.mu, device=dev)
+    lam = torch.as_tensor(wl.lam, device=dev)
+    E = plan.n_elements
+    rows = m * r + r * (r + 1) // 2
+    A = torch.empty((rows, E), dtype=torch.float64, device=dev)
+    work = torch.empty((E, rows), dtype=torch.float64, device=dev)
+    status = torch.zeros(2, dtype=torch.int32, device=dev)
+
+    def call():
+        _lib.check(_lib.lib().psx_ecsw_training(plan._h, _lib.ptr(phi), r, _lib.ptr(snaps), m, _lib.ptr(mu),
+                                                _lib.ptr(lam), 1, _lib.ptr(A), _lib.ptr(work),
+                                                current_stream_handle(), _lib.ptr(status)))
+
+    ms = _event_time(call, args.steps, args.warmup)
+    assert int(status[0]) == 0
+    b_alg = 8.0 * rows * E + 8.0 * n * (r + m) + 16.0 * E + 4.0 * 4 * E + 8.0 * 3 * wl.mesh.n_nodes
+    ach = b_alg / (ms * 1e-3) / 1e9
+    return {
+        "metric": "ECSW training matrix (Phi_e^T f_e per snapshot + Phi_e^T K0_e Phi_e), element-snapshots/s",
+        "value": E * m / (ms * 1e-3), "unit": "element-snapshots/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp64", "data": "synthetic",
+        "config": {"workload": wl.name, "elements": E, "rank": r, "snapshots": m, "rows": rows,
+                   "A_bytes": 8 * rows * E},
+        "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": ach / HBM_PEAK_GBPS, "traffic": None},
+        "cpu_baseline": None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -195,7 +242,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--tile", type=int, default=0)
-    ap.add_argument("--workload", choices=("tet4", "tet10", "vjp", "ecsw"), default="tet4")
+    ap.add_argument("--workload", choices=("tet4", "tet10", "vjp", "ecsw", "ecsw_train"), default="tet4")
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -204,7 +251,8 @@ def main():
         )
         sys.exit(2)
     if args.workload != "tet4":
-        fn = {"tet10": bench_tet10, "vjp": bench_vjp, "ecsw": bench_ecsw}[args.workload]
+        fn = {"tet10": bench_tet10, "vjp": bench_vjp, "ecsw": bench_ecsw, "ecsw_train": bench_ecsw_train}[
+            args.workload]
         print(json.dumps(fn(args)), flush=True)
         return
 
