@@ -113,6 +113,16 @@ int32_t psx_assemble(psx_plan* plan, const double* u, const double* mu,
                      const double* lam, double* f, double* kval,
                      int32_t precision, void* stream, int32_t* status);
 
+/* ---- Consistent mass matrix (device, Tet4 and Tet10, FP64) ---------------
+ * Replaces SolidDomain.assemble_mass_matrix (src/psierra/domain/solid.py:
+ * 208-228) with Tet4Base.compute_mass_matrix
+ * (src/psierra/element/tet4_base.py:138-151) and Tet10Base.compute_mass_matrix
+ * (src/psierra/element/tet10_base.py:344-372): rho (n_elements, device) ->
+ * kval (nnz, device, overwritten) in the CSR order of psx_plan_pattern; the
+ * off-diagonal entries of every 3x3 block are explicit zeros.  V_e is the
+ * volume of the element's corner tetrahedron (straight-edged Tet10). */
+int32_t psx_assemble_mass(psx_plan* plan, const double* rho, double* kval, void* stream);
+
 /* ---- Adjoint of the internal force (device, Tet4) ------------------------
  * No counterpart in the reference, which differentiates only by finite
  * differences: MaterialParameterIdentifier.identify calls

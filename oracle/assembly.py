@@ -150,3 +150,87 @@ def node_rows(nodes, elements, u, mu, lam, sample_nodes):
                 else:
                     blocks[a][b] = blk.copy()
     return out_f, blocks
+
+
+# ---------------------------------------------------------------------------
+# Linear stiffness and consistent mass (src/psierra/domain/solid.py:172-228)
+# ---------------------------------------------------------------------------
+
+def _simplex_rule(n1d=6):
+    """Quadrature on the reference tetrahedron (volume 1/6) from a collapsed
+    (Duffy) tensor product of Gauss-Legendre rules; with n1d = 6 it integrates
+    polynomials of total degree <= 2 n1d - 3 exactly."""
+    x, w = np.polynomial.legendre.leggauss(n1d)
+    x = 0.5 * (x + 1.0)
+    w = 0.5 * w
+    pts, wts = [], []
+    for a, wa in zip(x, w):
+        for b, wb in zip(x, w):
+            for c, wc in zip(x, w):
+                # (a, b, c) in the unit cube -> tetrahedron
+                xi = a * (1 - b) * (1 - c)
+                eta = b * (1 - c)
+                zeta = c
+                jac = (1 - b) * (1 - c) ** 2
+                pts.append((xi, eta, zeta))
+                wts.append(wa * wb * wc * jac)
+    return np.array(pts), np.array(wts)
+
+
+def _shape_values(npe, pts):
+    xi, eta, zeta = pts[:, 0], pts[:, 1], pts[:, 2]
+    L = np.stack([1.0 - xi - eta - zeta, xi, eta, zeta], axis=1)
+    if npe == 4:
+        return L
+    N = np.empty((len(pts), 10))
+    for i in range(4):
+        N[:, i] = L[:, i] * (2 * L[:, i] - 1)
+    for k, (i, j) in enumerate(((0, 1), (1, 2), (2, 0), (0, 3), (1, 3), (2, 3))):
+        N[:, 4 + k] = 4 * L[:, i] * L[:, j]
+    return N
+
+
+def reference_mass_tensor(npe):
+    """(1/V) * integral of N_p N_q over the element by numerical quadrature
+    of the shape functions (Tet4: the linear ones of tet4_base.py; Tet10: the
+    quadratic VTK-ordered ones of tet10_base.py:117-179)."""
+    pts, wts = _simplex_rule()
+    N = _shape_values(npe, pts)
+    return 6.0 * np.einsum("q,qa,qb->ab", wts, N, N)
+
+
+def element_volumes(nodes, elements):
+    """Volume of each element's corner tetrahedron (straight edges)."""
+    X = nodes[elements[:, :4]]
+    return np.linalg.det(X[:, 1:] - X[:, :1]) / 6.0
+
+
+def mass_matrix(nodes, elements, rho):
+    """SolidDomain.assemble_mass_matrix (solid.py:208-228): element matrices
+    rho V_e c_ab I3 (tet4_base.py:138-151, tet10_base.py:344-372), summed
+    with coo.tocsr in the structural pattern (explicit zeros kept)."""
+    npe = elements.shape[1]
+    c = reference_mass_tensor(npe)
+    V = element_volumes(nodes, elements)
+    rho = np.broadcast_to(np.asarray(rho, dtype=np.float64), V.shape)
+    me = (rho * V)[:, None, None] * np.kron(c, np.eye(3))[None]
+    dofs = compute_element_dofs(elements)
+    rows, cols = create_coo_sparsity_indices(dofs)
+    return assemble_sparse_matrix(me, rows, cols, 3 * nodes.shape[0])
+
+
+def linear_stiffness(nodes, elements, mu, lam):
+    """SolidDomain.assemble_stiffness_matrix (solid.py:172-206) for Tet4:
+    k_e = B_L^T C B_L V with C Hooke's law from (lambda_e, mu_e)
+    (tet4_base.py:129-136, neohookean.py:33-60)."""
+    E = elements.shape[0]
+    mu = np.broadcast_to(np.asarray(mu, dtype=np.float64), (E,))
+    lam = np.broadcast_to(np.asarray(lam, dtype=np.float64), (E,))
+    ke = np.empty((E, 12, 12))
+    for e in range(E):
+        V, H = fem.tet4_volume_and_gradients(nodes[elements[e]])
+        BL = fem.linear_B(H)
+        ke[e] = BL.T @ fem.hookean_C(lam[e], mu[e]) @ BL * V
+    dofs = compute_element_dofs(elements)
+    rows, cols = create_coo_sparsity_indices(dofs)
+    return assemble_sparse_matrix(ke, rows, cols, 3 * nodes.shape[0])

@@ -100,6 +100,7 @@ struct psx_plan {
   std::vector<int32_t> inc_ptr_h;  // internal order
   std::vector<int32_t> deg_h;      // internal order
   int32_t max_deg = 0;
+  double* mass_coef = nullptr;     // device npe x npe reference mass tensor (lazy)
   int64_t n_inc = 0;
   int64_t device_bytes = 0;
   psx::DevPlan d{};
@@ -118,6 +119,8 @@ int32_t launch_tet4_vjp(const DevPlan& p, const double* u, const double* mu, con
 int32_t launch_ecsw_training(const DevPlan& p, const double* phi, int r, const double* snaps, int m,
                              const double* mu, const double* lam, int stiff, double* A, double* work,
                              hipStream_t s, int32_t* status);
+int32_t launch_mass(const DevPlan& p, const double* rho, const double* coef, double* kval, hipStream_t s,
+                    int64_t n_units);
 int32_t launch_csc_extract(const DevCsc& c, const double* kval, int drop, int* colptr, int* rowind,
                            double* vals, hipStream_t s);
 int32_t launch_free_residual(const DevCsc& c, const double* f_int, const double* f_ext_free, double* r,

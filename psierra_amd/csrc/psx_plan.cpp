@@ -10,6 +10,7 @@
 //   value(3a + r, 3 b_s + c) = kofs(a) + 3 r deg(a) + 3 s + c,
 // with b_s the s-th neighbour of a ("slot rank" s).
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -230,6 +231,60 @@ int32_t check_volumes(const int64_t* conn, int64_t E, int npe, const double* X) 
     }
   }
   return PSX_OK;
+}
+
+// Reference mass tensor c[p][q] = (1/V) * integral over the element of
+// N_p N_q, from the exact simplex integral of barycentric monomials,
+// (1/V) int L0^a L1^b L2^c L3^d = 6 a! b! c! d! / (a + b + c + d + 3)!.
+// Tet4: N_p = L_p.  Tet10 (VTK order): corners L_p (2 L_p - 1), mid-edge
+// nodes 4 L_i L_j on edges 01, 12, 20, 03, 13, 23.
+std::vector<double> reference_mass_tensor(int npe) {
+  using Mono = std::array<int, 4>;
+  using Poly = std::vector<std::pair<Mono, double>>;
+  auto L = [](int i, int pw) {
+    Mono m{0, 0, 0, 0};
+    m[i] = pw;
+    return m;
+  };
+  std::vector<Poly> N;
+  for (int i = 0; i < 4; ++i) {
+    if (npe == 4)
+      N.push_back({{L(i, 1), 1.0}});
+    else
+      N.push_back({{L(i, 2), 2.0}, {L(i, 1), -1.0}});
+  }
+  if (npe == 10) {
+    const int edges[6][2] = {{0, 1}, {1, 2}, {2, 0}, {0, 3}, {1, 3}, {2, 3}};
+    for (const auto& ed : edges) {
+      Mono m{0, 0, 0, 0};
+      m[ed[0]] = 1;
+      m[ed[1]] = 1;
+      N.push_back({{m, 4.0}});
+    }
+  }
+  auto fact = [](int n) {
+    double f = 1.0;
+    for (int i = 2; i <= n; ++i) f *= i;
+    return f;
+  };
+  std::vector<double> c(size_t(npe) * npe, 0.0);
+  for (int a = 0; a < npe; ++a)
+    for (int b = 0; b < npe; ++b) {
+      double sum = 0.0;
+      for (const auto& ta : N[a])
+        for (const auto& tb : N[b]) {
+          int tot = 0;
+          double num = 6.0;
+          for (int i = 0; i < 4; ++i) {
+            const int e = ta.first[i] + tb.first[i];
+            tot += e;
+            num *= fact(e);
+          }
+          sum += ta.second * tb.second * num / fact(tot + 3);
+        }
+      c[size_t(a) * npe + b] = sum;
+    }
+  return c;
 }
 
 template <typename T>
@@ -648,6 +703,20 @@ int32_t psx_assemble(psx_plan* plan, const double* u, const double* mu, const do
   if (plan->hp.npe == 4)
     return psx::launch_tet4_assemble(plan->d, u, mu, lam, f, kval, precision, s, status);
   return psx::launch_tet10_assemble(plan->d, u, mu, lam, f, kval, precision, s, status);
+}
+
+int32_t psx_assemble_mass(psx_plan* plan, const double* rho, double* kval, void* stream) {
+  psx::clear_error();
+  if (!plan || !rho || !kval) return psx::fail(PSX_EINVAL, "null argument to psx_assemble_mass");
+  if (!plan->mass_coef) {
+    const std::vector<double> c = psx::reference_mass_tensor(plan->hp.npe);
+    double* d = nullptr;
+    int32_t rc = psx::upload(plan, c.data(), c.size(), &d);
+    if (rc) return rc;
+    plan->mass_coef = d;
+  }
+  return psx::launch_mass(plan->d, rho, plan->mass_coef, kval, static_cast<hipStream_t>(stream),
+                          plan->hp.adj_ptr.back());
 }
 
 int32_t psx_vjp(psx_plan* plan, const double* u, const double* mu, const double* lam, const double* w,

@@ -65,3 +65,11 @@ class SolidDomain:
 
     def assemble_tangent_stiffness_matrix(self):
         return self.element.assemble_tangent_stiffness_matrix(self.mesh, self.displacement)
+
+    def assemble_stiffness_matrix(self):
+        """Linear stiffness from the reference geometry (solid.py:172-206)."""
+        return self.element.assemble_stiffness_matrix(self.mesh)
+
+    def assemble_mass_matrix(self):
+        """Consistent mass matrix (solid.py:208-228)."""
+        return self.element.assemble_mass_matrix(self.mesh)

@@ -103,7 +103,43 @@ class _GpuTotalLagrangian:
         n = plan.n_dofs
         return spsp.csr_matrix((k.cpu().numpy(), indices, indptr), shape=(n, n))
 
+    def _csr(self, mesh, kval) -> spsp.csr_matrix:
+        plan = self._state(mesh)["plan"]
+        indptr, indices = plan.pattern()
+        n = plan.n_dofs
+        return spsp.csr_matrix((kval.cpu().numpy(), indices, indptr), shape=(n, n))
+
+    # SolidDomain.assemble_stiffness_matrix delegates here when the element
+    # defines it (src/psierra/domain/solid.py:180-181): the linear stiffness
+    # B^T C B V with C Hooke's law at the reference state is the Neo-Hookean
+    # tangent at u = 0 (S = 0, C4 = Hookean), assembled in FP64.
+    def assemble_stiffness_matrix(self, mesh) -> spsp.csr_matrix:
+        plan = self._state(mesh)["plan"]
+        st = self._state(mesh)
+        import torch
+
+        u0 = torch.zeros(plan.n_dofs, dtype=torch.float64, device=plan.device)
+        _, k = plan.assemble(u0, st["mu"], st["lam"], precision="fp64", want_k=True)
+        return self._csr(mesh, k)
+
+    def assemble_mass_matrix(self, mesh) -> spsp.csr_matrix:
+        """Consistent mass on the GPU (SolidDomain.assemble_mass_matrix,
+        src/psierra/domain/solid.py:208-228) with rho = material.mass_density."""
+        plan = self._state(mesh)["plan"]
+        return self._csr(mesh, plan.assemble_mass(float(self._material.mass_density)))
+
     # -- per-element API ------------------------------------------------------
+    def compute_mass_matrix(self, volume: float) -> np.ndarray:
+        """Element consistent mass (tet4_base.py:138-151, tet10_base.py:344-372)."""
+        c = reference_mass_tensor(self._npe)
+        return float(self._material.mass_density) * float(volume) * np.kron(c, np.eye(3))
+
+    def compute_linear_stiffness_matrix(self, node_coords, elem_index=None) -> np.ndarray:
+        """Element B^T C B V at the reference state (tet4_base.py:129-136)."""
+        k, _ = self.compute_nonlinear_force_and_tangent_stiffness(node_coords, np.zeros((self._npe, 3)),
+                                                                   elem_index=elem_index)
+        return k
+
     def _element_params(self, elem_index):
         mat = self._material
         if getattr(mat, "supports_element_indexing", False) and hasattr(mat, "mu"):
@@ -123,6 +159,41 @@ class _GpuTotalLagrangian:
         u = np.asarray(u_nodes, dtype=np.float64).reshape(1, self._npe, 3)
         k, f = engine.element_batch(X, u, [mu], [lam], want_k=True, precision=self.precision)
         return k[0].cpu().numpy(), f[0].cpu().numpy()
+
+
+def reference_mass_tensor(npe: int) -> np.ndarray:
+    """c[p][q] = (1/V) * integral of N_p N_q over the element, exact: the
+    shape functions as barycentric polynomials (Tet4 L_p; Tet10 corners
+    L_p (2 L_p - 1), mid-edge 4 L_i L_j on edges 01, 12, 20, 03, 13, 23) and
+    (1/V) int L^alpha = 6 alpha! / (|alpha| + 3)!."""
+    from math import factorial
+
+    def lin(i, pw):
+        m = [0, 0, 0, 0]
+        m[i] = pw
+        return tuple(m)
+
+    if npe == 4:
+        N = [[(lin(i, 1), 1.0)] for i in range(4)]
+    else:
+        N = [[(lin(i, 2), 2.0), (lin(i, 1), -1.0)] for i in range(4)]
+        for i, j in ((0, 1), (1, 2), (2, 0), (0, 3), (1, 3), (2, 3)):
+            m = [0, 0, 0, 0]
+            m[i] = m[j] = 1
+            N.append([(tuple(m), 4.0)])
+    c = np.zeros((npe, npe))
+    for a in range(npe):
+        for b in range(npe):
+            tot = 0.0
+            for ma, ca in N[a]:
+                for mb, cb in N[b]:
+                    ex = [x + y for x, y in zip(ma, mb)]
+                    num = 6.0
+                    for x in ex:
+                        num *= factorial(x)
+                    tot += ca * cb * num / factorial(sum(ex) + 3)
+            c[a, b] = tot
+    return c
 
 
 class GpuTet4TotalLagrangian(_GpuTotalLagrangian):

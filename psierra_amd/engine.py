@@ -136,6 +136,18 @@ class AssemblyPlan:
         k = torch.empty(self.nnz, dtype=torch.float64, device=self.device) if want_k else None
         return f, k
 
+    def assemble_mass(self, rho, kval=None):
+        """Consistent mass matrix values (nnz, CSR order of ``pattern()``) on the
+        device, FP64; rho: per-element density (E,) or a scalar."""
+        torch = _torch()
+        if np.isscalar(rho) or (isinstance(rho, np.ndarray) and rho.ndim == 0):
+            rho = torch.full((self.n_elements,), float(rho), dtype=torch.float64, device=self.device)
+        rho = self._device_f64(rho, self.n_elements, "rho")
+        if kval is None:
+            kval = torch.empty(self.nnz, dtype=torch.float64, device=self.device)
+        check(lib().psx_assemble_mass(self._h, ptr(rho), ptr(kval), current_stream_handle()))
+        return kval
+
     def assemble(self, u, mu, lam, f=None, kval=None, precision="fp64", want_k=True, check_status=True):
         """f_int (3N) and K_T values (nnz, CSR order of ``pattern()``) on the device.
 
