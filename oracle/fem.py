@@ -92,6 +92,92 @@ def neohookean_energy(F, mu, K):
     return 0.5 * mu * (J ** (-2.0 / 3.0) * I1 - 3.0) + 0.5 * K * np.log(J) ** 2
 
 
+def _voigt_of(C4):
+    Cv = np.zeros((6, 6))
+    for a, (i, j) in VOIGT_MAP_3D.items():
+        for b, (k, l) in VOIGT_MAP_3D.items():
+            Cv[a, b] = C4[i, j, k, l]
+    return Cv
+
+
+def mooney_rivlin_stress_and_tangent(F, c10, c01, K):
+    """MooneyRivlin.compute_stress_and_tangent
+    (src/psierra/material/mooney_rivlin.py:69-152)."""
+    J = float(np.linalg.det(F))
+    if J <= 0.0:
+        raise ValueError(INVERSION_MSG.format(J))
+    C = F.T @ F
+    Ci = np.linalg.inv(C)
+    I1 = float(np.trace(C))
+    I2 = 0.5 * (I1 ** 2 - float(np.trace(C @ C)))
+    I3 = np.identity(3)
+    f1 = 2.0 * c10 * J ** (-2.0 / 3.0)
+    f2 = 2.0 * c01 * J ** (-4.0 / 3.0)
+    S = f1 * (I3 - I1 / 3.0 * Ci) + f2 * (I1 * I3 - C - 2.0 / 3.0 * I2 * Ci) + K * np.log(J) * Ci
+    outer = np.einsum("ij,kl->ijkl", Ci, Ci)
+    ICi = 0.5 * (np.einsum("ik,jl->ijkl", Ci, Ci) + np.einsum("il,jk->ijkl", Ci, Ci))
+    C4 = K * outer - 2.0 * K * np.log(J) * ICi
+    CiI = np.einsum("ij,kl->ijkl", Ci, I3) + np.einsum("ij,kl->ijkl", I3, Ci)
+    C4 += -2.0 / 3.0 * f1 * CiI + 2.0 / 9.0 * f1 * I1 * outer + 2.0 / 3.0 * f1 * I1 * ICi
+    Iout = np.einsum("ij,kl->ijkl", I3, I3)
+    Isym = 0.5 * (np.einsum("ik,jl->ijkl", I3, I3) + np.einsum("il,jk->ijkl", I3, I3))
+    B2 = I1 * I3 - C
+    CiB2 = np.einsum("ij,kl->ijkl", Ci, B2) + np.einsum("ij,kl->ijkl", B2, Ci)
+    C4 += (c01 * J ** (-4.0 / 3.0)) * (4.0 * (Iout - Isym) - 8.0 / 3.0 * CiB2 + 16.0 / 9.0 * I2 * outer
+                                       + 8.0 / 3.0 * I2 * ICi)
+    return _voigt_of(C4), S
+
+
+def mooney_rivlin_energy(F, c10, c01, K):
+    """Psi = c10 (I1bar - 3) + c01 (I2bar - 3) + K/2 (ln J)^2 (mooney_rivlin.py:15-19)."""
+    J = np.linalg.det(F)
+    C = F.T @ F
+    I1 = np.trace(C)
+    I2 = 0.5 * (I1 ** 2 - np.trace(C @ C))
+    return c10 * (J ** (-2.0 / 3.0) * I1 - 3.0) + c01 * (J ** (-4.0 / 3.0) * I2 - 3.0) + 0.5 * K * np.log(J) ** 2
+
+
+def yeoh_stress_and_tangent(F, c10, c20, c30, K):
+    """Yeoh.compute_stress_and_tangent (src/psierra/material/yeoh.py:69-148)."""
+    J = float(np.linalg.det(F))
+    if J <= 0.0:
+        raise ValueError(INVERSION_MSG.format(J))
+    C = F.T @ F
+    Ci = np.linalg.inv(C)
+    I1 = float(np.trace(C))
+    Jp = J ** (-2.0 / 3.0)
+    d = Jp * I1 - 3.0
+    psi1 = c10 + 2.0 * c20 * d + 3.0 * c30 * d ** 2
+    psi11 = 2.0 * c20 + 6.0 * c30 * d
+    I3 = np.identity(3)
+    A1 = 2.0 * Jp * (I3 - I1 / 3.0 * Ci)
+    S = psi1 * A1 + K * np.log(J) * Ci
+    outer = np.einsum("ij,kl->ijkl", Ci, Ci)
+    ICi = 0.5 * (np.einsum("ik,jl->ijkl", Ci, Ci) + np.einsum("il,jk->ijkl", Ci, Ci))
+    C4 = K * outer - 2.0 * K * np.log(J) * ICi
+    CiI = np.einsum("ij,kl->ijkl", Ci, I3) + np.einsum("ij,kl->ijkl", I3, Ci)
+    C4 += psi1 * Jp * (-4.0 / 3.0 * CiI + 4.0 / 9.0 * I1 * outer + 4.0 / 3.0 * I1 * ICi)
+    C4 += psi11 * np.einsum("ij,kl->ijkl", A1, A1)
+    return _voigt_of(C4), S
+
+
+def yeoh_energy(F, c10, c20, c30, K):
+    """Psi = sum_i c_i0 (I1bar - 3)^i + K/2 (ln J)^2 (yeoh.py:13-16)."""
+    J = np.linalg.det(F)
+    d = J ** (-2.0 / 3.0) * np.trace(F.T @ F) - 3.0
+    return c10 * d + c20 * d ** 2 + c30 * d ** 3 + 0.5 * K * np.log(J) ** 2
+
+
+def material_fn(model, params):
+    """(C_voigt, S) = material(F) for model 0 = Neo-Hookean (mu, lambda),
+    1 = Mooney-Rivlin (c10, c01, K), 2 = Yeoh (c10, c20, c30, K)."""
+    if model == 0:
+        return nh_material(*params)
+    if model == 1:
+        return lambda F: mooney_rivlin_stress_and_tangent(F, *params)
+    return lambda F: yeoh_stress_and_tangent(F, *params)
+
+
 def neohookean_batch(F, mu, K):
     """Vectorised NeoHookean over F (m,3,3), mu/K (m,) -> (C_voigt (m,6,6), S (m,3,3))."""
     J = np.linalg.det(F)
@@ -202,6 +288,53 @@ def tet4_force_and_tangent(X, u, material):
     k_mat = (B.T @ C @ B) * V
     k_geo = (G.T @ np.kron(np.identity(3), S) @ G) * V
     f = (B.T @ tensor_to_voigt_stress(S)) * V
+    return k_mat + k_geo, f
+
+
+def tensor_to_voigt_strain(E):
+    """src/psierra/utility/tensor.py:41-64 (engineering shear)."""
+    return np.array([E[0, 0], E[1, 1], E[2, 2], 2 * E[0, 1], 2 * E[1, 2], 2 * E[2, 0]])
+
+
+def voigt_strain_to_tensor(v):
+    """src/psierra/utility/tensor.py:89-111."""
+    return np.array([[v[0], 0.5 * v[3], 0.5 * v[5]], [0.5 * v[3], v[1], 0.5 * v[4]], [0.5 * v[5], 0.5 * v[4], v[2]]])
+
+
+def strain_pushforward(F):
+    """compute_strain_pushforward_matrix (src/psierra/utility/tensor.py:113-137):
+    column k = Voigt(F^T eps_k F) for the k-th unit Voigt strain."""
+    T = np.zeros((6, 6))
+    for k in range(6):
+        e = np.zeros(6)
+        e[k] = 1.0
+        T[:, k] = tensor_to_voigt_strain(F.T @ voigt_strain_to_tensor(e) @ F)
+    return T
+
+
+UL_INVERSION_MSG = "Element inversion detected in Tet4UpdatedLagrangian: det(F) = {:.4e} <= 0."
+
+
+def tet4_ul_force_and_tangent(X, u, material):
+    """Tet4UpdatedLagrangian.compute_nonlinear_force_and_tangent_stiffness
+    (src/psierra/element/tet4_updated_lagrangian.py:31-109): spatial
+    gradients h = H F^-1, v = J V, sigma = F S F^T / J, f = b_L^T sigma v,
+    k = b_L^T (T^T C T / J) b_L v + g^T (I3 x sigma) g v."""
+    V, H = tet4_volume_and_gradients(X)
+    F = (X + u).T @ H
+    J = np.linalg.det(F)
+    if J <= 0.0:
+        raise ValueError(UL_INVERSION_MSG.format(J))
+    h = H @ np.linalg.inv(F)
+    v = J * V
+    C, S = material(F)
+    sigma = (F @ S @ F.T) / J
+    bL = linear_B(h)
+    f = (bL.T @ tensor_to_voigt_stress(sigma)) * v
+    T = strain_pushforward(F)
+    k_mat = (bL.T @ ((T.T @ C @ T) / J) @ bL) * v
+    g = geometric_G(h)
+    k_geo = (g.T @ np.kron(np.identity(3), sigma) @ g) * v
     return k_mat + k_geo, f
 
 

@@ -67,6 +67,10 @@ SIGNATURES = {
     "psx_ecsw_info": (c_int32, [c_void_p, _i64p, c_int32]),
     "psx_ecsw_evaluate": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "psx_assemble_mass": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "psx_assemble_hyperelastic": (
+        c_int32,
+        [c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_void_p],
+    ),
     "psx_ecsw_training": (
         c_int32,
         [c_void_p, c_void_p, c_int32, c_void_p, c_int32, c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_void_p,

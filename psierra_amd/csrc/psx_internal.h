@@ -109,6 +109,8 @@ struct psx_plan {
 
 namespace psx {
 // Kernel launchers (defined in the .hip files).
+int32_t launch_tet4_hyper(const DevPlan& p, int model, const double* const* params, const double* u,
+                          double* f, double* kval, int precision, hipStream_t s, int32_t* status);
 int32_t launch_tet4_assemble(const DevPlan& p, const double* u, const double* mu, const double* lam,
                              double* f, double* kval, int precision, hipStream_t s, int32_t* status);
 int32_t launch_tet10_assemble(const DevPlan& p, const double* u, const double* mu, const double* lam,

@@ -191,6 +191,108 @@ __device__ __forceinline__ void nh_point(const R A[3][3], R mu, R K, NHPoint<R>&
   o.ga = R(-2.0 / 3.0) * fac;
 }
 
+// Hyperelastic models on the same kinematics (all with the volumetric part
+// K/2 (ln J)^2 and the isochoric invariants of C_bar = J^-2/3 C):
+//   kNeoHookean    Psi = mu/2 (I1bar - 3)                      params (mu, K)
+//   kMooneyRivlin  Psi = c10 (I1bar - 3) + c01 (I2bar - 3)     params (c10, c01, K)
+//   kYeoh          Psi = sum_i ci0 (I1bar - 3)^i, i = 1..3     params (c10, c20, c30, K)
+// (src/psierra/material/neohookean.py, mooney_rivlin.py, yeoh.py).  With
+// f = 2 psi_1 J^-2/3 (psi_1 = dPsi/dI1bar) the Neo-Hookean terms above hold
+// unchanged; the extra terms of the tangent push forward, with
+// b = F F^T, p = I1 q - b q and z = 2 J^-2/3 (q - I1/3 g), to
+//   Mooney-Rivlin (c' = c01 J^-4/3): alpha += 16/9 I2 c', beta += 8/3 I2 c',
+//     K_ab += V c' [4 q_a q_b^T - 2 (b (h_a . h_b) + q_b q_a^T)
+//                   - 8/3 (g_a p_b^T + p_a g_b^T)],
+//   Yeoh (psi_11 = d2Psi/dI1bar2): K_ab += V psi_11 z_a z_b^T,
+// and the Mooney-Rivlin stress adds 2 c' C^-1 (2 dev E + 4 tr(E) E - 4 E^2
+// - 4/3 (tr(E)^2 - tr(E^2)) I), all formed from E so small strains keep
+// their relative accuracy.
+enum HyperModel { kNeoHookean = 0, kMooneyRivlin = 1, kYeoh = 2 };
+
+template <typename R>
+struct HyperExtra {
+  R cq = R(0), cb = R(0), cp = R(0);   // Mooney-Rivlin block coefficients
+  R psi11 = R(0), zs = R(0);           // Yeoh: psi_11 and 2 J^-2/3
+  R I1 = R(3);
+  R b[3][3];                           // F F^T (Mooney-Rivlin)
+};
+
+// Material point of model M from A = grad u; prm as listed above.
+template <typename R, int M>
+__device__ __forceinline__ void hyper_point(const R A[3][3], const R prm[4], NHPoint<R>& o, HyperExtra<R>& x) {
+  if constexpr (M == kNeoHookean) {
+    nh_point(A, prm[0], prm[1], o);
+    return;
+  } else {
+    const R K = M == kMooneyRivlin ? prm[2] : prm[3];
+    // mu placeholder: nh_point's isochoric part is rescaled below
+    nh_point(A, R(1), K, o);
+    const R t = o.trE;
+    const R I1 = R(3) + R(2) * t;
+    x.I1 = I1;
+    R f;
+    if constexpr (M == kMooneyRivlin) {
+      f = R(2) * prm[0] * o.jm23;
+    } else {
+      const R d = R(3) * (o.jm23 - R(1)) + R(2) * t * o.jm23;   // I1bar - 3
+      const R psi1 = prm[0] + R(2) * prm[1] * d + R(3) * prm[2] * d * d;
+      x.psi11 = R(2) * prm[1] + R(6) * prm[2] * d;
+      x.zs = R(2) * o.jm23;
+      f = R(2) * psi1 * o.jm23;
+    }
+    // isochoric part of S for f (nh_point used f = J^-2/3): S = C^-1 Sbar
+    R Sb[3][3];
+#pragma unroll
+    for (int I = 0; I < 3; ++I)
+#pragma unroll
+      for (int L = 0; L < 3; ++L)
+        Sb[I][L] = R(2) * f * (o.E[I][L] - (I == L ? t * R(1.0 / 3.0) : R(0))) + (I == L ? K * o.lnJ : R(0));
+    R al = K + R(2.0 / 9.0) * f * I1;
+    R be = R(-2) * K * o.lnJ + R(2.0 / 3.0) * f * I1;
+    if constexpr (M == kMooneyRivlin) {
+      const R cpr = prm[1] * o.jm23 * o.jm23;
+      R E2[3][3];
+      R trE2 = R(0);
+#pragma unroll
+      for (int I = 0; I < 3; ++I)
+#pragma unroll
+        for (int L = 0; L < 3; ++L) {
+          E2[I][L] = o.E[I][0] * o.E[0][L] + o.E[I][1] * o.E[1][L] + o.E[I][2] * o.E[2][L];
+          if (I == L) trE2 += E2[I][L];
+        }
+      const R I2 = R(3) + R(4) * t + R(2) * t * t - R(2) * trE2;
+#pragma unroll
+      for (int I = 0; I < 3; ++I)
+#pragma unroll
+        for (int L = 0; L < 3; ++L)
+          Sb[I][L] += R(2) * cpr *
+                      (R(2) * (o.E[I][L] - (I == L ? t * R(1.0 / 3.0) : R(0))) + R(4) * t * o.E[I][L] -
+                       R(4) * E2[I][L] - (I == L ? R(4.0 / 3.0) * (t * t - trE2) : R(0)));
+      al += R(16.0 / 9.0) * I2 * cpr;
+      be += R(8.0 / 3.0) * I2 * cpr;
+      x.cq = R(4) * cpr;
+      x.cb = R(-2) * cpr;
+      x.cp = R(-8.0 / 3.0) * cpr;
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+          x.b[i][k] = o.F[i][0] * o.F[k][0] + o.F[i][1] * o.F[k][1] + o.F[i][2] * o.F[k][2];
+    }
+#pragma unroll
+    for (int I = 0; I < 3; ++I)
+#pragma unroll
+      for (int L = I; L < 3; ++L) {
+        const R v = o.Ci[I][0] * Sb[0][L] + o.Ci[I][1] * Sb[1][L] + o.Ci[I][2] * Sb[2][L];
+        o.S[I][L] = v;
+        o.S[L][I] = v;
+      }
+    o.al = al;
+    o.be = be;
+    o.ga = R(-2.0 / 3.0) * f;
+  }
+}
+
 template <typename R>
 __device__ __forceinline__ void mat3vec(const R M[3][3], const R v[3], R o[3]) {
 #pragma unroll

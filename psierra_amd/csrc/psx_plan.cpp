@@ -705,6 +705,27 @@ int32_t psx_assemble(psx_plan* plan, const double* u, const double* mu, const do
   return psx::launch_tet10_assemble(plan->d, u, mu, lam, f, kval, precision, s, status);
 }
 
+int32_t psx_assemble_hyperelastic(psx_plan* plan, int32_t model, const double* const* params, const double* u,
+                                  double* f, double* kval, int32_t precision, void* stream, int32_t* status) {
+  psx::clear_error();
+  if (!plan || !params || !u || !f || !status)
+    return psx::fail(PSX_EINVAL, "null argument to psx_assemble_hyperelastic");
+  if (precision != PSX_FP64 && precision != PSX_FP32)
+    return psx::fail(PSX_EINVAL, "precision must be PSX_FP64 or PSX_FP32");
+  const int n = model == PSX_NEO_HOOKEAN ? 2 : model == PSX_MOONEY_RIVLIN ? 3 : model == PSX_YEOH ? 4 : -1;
+  if (n < 0) return psx::fail(PSX_EINVAL, "unknown material model");
+  for (int i = 0; i < n; ++i)
+    if (!params[i]) return psx::fail(PSX_EINVAL, "null material parameter array");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (plan->hp.npe != 4) {
+    if (model != PSX_NEO_HOOKEAN)
+      return psx::fail(PSX_EUNSUPPORTED, "Tet10 plans evaluate the Neo-Hookean model only");
+    return psx::launch_tet10_assemble(plan->d, u, params[0], params[1], f, kval, precision, s, status);
+  }
+  const double* prm[4] = {params[0], params[1], n > 2 ? params[2] : nullptr, n > 3 ? params[3] : nullptr};
+  return psx::launch_tet4_hyper(plan->d, model, prm, u, f, kval, precision, s, status);
+}
+
 int32_t psx_assemble_mass(psx_plan* plan, const double* rho, double* kval, void* stream) {
   psx::clear_error();
   if (!plan || !rho || !kval) return psx::fail(PSX_EINVAL, "null argument to psx_assemble_mass");

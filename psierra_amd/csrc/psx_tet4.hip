@@ -52,26 +52,46 @@ __device__ __forceinline__ void flag_inversion(int32_t* status, int e) {
   status[0] = 1;
 }
 
+// Per-element material parameter arrays (device, one value per element):
+// Neo-Hookean (mu, lambda); Mooney-Rivlin (c10, c01, K); Yeoh (c10, c20,
+// c30, K).
+struct MatParams {
+  const double* v[4];
+};
+
+// Parameters of element e as the material point takes them (Neo-Hookean:
+// mu and K = lambda + 2 mu / 3).
+template <int M>
+__device__ __forceinline__ void load_params(const MatParams& mp, int64_t e, double pr[4]) {
+  if constexpr (M == kNeoHookean) {
+    pr[0] = mp.v[0][e];
+    pr[1] = mp.v[1][e] + (2.0 / 3.0) * pr[0];
+    pr[2] = pr[3] = 0.0;
+  } else {
+    constexpr int n = M == kMooneyRivlin ? 3 : 4;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) pr[i] = i < n ? mp.v[i][e] : 0.0;
+  }
+}
+
 // Inputs of one incidence, in the row's node order: the row node first,
 // then the element's other three nodes in increasing local index.
 template <typename R>
 struct IncInputs {
   uint32_t pe = 0, rank = 0;
-  double m = 0.0, l = 0.0;
+  double pr[4] = {0.0, 0.0, 0.0, 0.0};   // material parameters
   float4 G[4];               // FP32 path: precomputed (h_b, V)
   double4 U[4];
   double4 X[4];              // FP64 path: reference coordinates
 };
 
-template <typename R>
-__device__ __forceinline__ void load_incidence(const DevPlan& p, const double* mu, const double* lam, int k,
-                                               IncInputs<R>& in) {
+template <typename R, int M>
+__device__ __forceinline__ void load_incidence(const DevPlan& p, const MatParams& mp, int k, IncInputs<R>& in) {
   in.pe = p.inc_elem[k];
   in.rank = p.inc_rank[k];
   const int64_t e = int64_t(in.pe & kIncElemMask);
   const int4 c = *reinterpret_cast<const int4*>(p.conn + 4 * e);
-  in.m = mu[e];
-  in.l = lam[e];
+  load_params<M>(mp, e, in.pr);
   const int lp = int(in.pe >> kIncElemBits);
   const int r1 = lp == 0 ? 1 : 0, r2 = lp <= 1 ? 2 : 1, r3 = lp <= 2 ? 3 : 2;
   const int i0 = lp == 0 ? c.x : lp == 1 ? c.y : lp == 2 ? c.z : c.w;
@@ -97,20 +117,20 @@ __device__ __forceinline__ void load_incidence(const DevPlan& p, const double* m
 
 // Row of one incidence: out[0..35] = K_{a b} for b = a, then the other three
 // nodes (row-major 3x3 each), out[36..38] = f_a.  Returns det F.
-template <typename R, bool WANT_K>
+template <typename R, int M, bool WANT_K>
 __device__ __forceinline__ R incidence_row(const IncInputs<R>& in, R* out) {
+  const R prm[4] = {R(in.pr[0]), R(in.pr[1]), R(in.pr[2]), R(in.pr[3])};
   if constexpr (std::is_same<R, float>::value)
-    return tet4_row_grad<WANT_K>(in.G, in.U, float(in.m), float(in.l + (2.0 / 3.0) * in.m), out);
+    return tet4_row_grad_m<M, WANT_K>(in.G, in.U, prm, out);
   else
-    return tet4_row<R, WANT_K>(in.X, in.U, R(in.m), R(in.l + (2.0 / 3.0) * in.m), out);
+    return tet4_row_m<R, M, WANT_K>(in.X, in.U, prm, out);
 }
 
 // f_int only (line-search trials): one lane per incidence, f_a summed in FP64
 // over the node's incidence run.
-template <typename R, int TILE>
-__global__ __launch_bounds__(TILE) void tet4_force_kernel(DevPlan p, const double* __restrict__ mu,
-                                                          const double* __restrict__ lam,
-                                                          double* __restrict__ f, int32_t* status) {
+template <typename R, int TILE, int M>
+__global__ __launch_bounds__(TILE) void tet4_force_kernel(DevPlan p, MatParams mp, double* __restrict__ f,
+                                                          int32_t* status) {
   __shared__ R contrib[3 * TILE];
   __shared__ int32_t s_incp[TILE + 1];
   const int tile = xcd_tile();
@@ -119,11 +139,11 @@ __global__ __launch_bounds__(TILE) void tet4_force_kernel(DevPlan p, const doubl
   const int n0 = ti.x, nn = ti.y, k0 = ti.z, nk = ti.w;
   const int t = threadIdx.x;
   IncInputs<R> in;
-  if (t < nk) load_incidence<R>(p, mu, lam, k0 + t, in);
+  if (t < nk) load_incidence<R, M>(p, mp, k0 + t, in);
   for (int i = t; i <= nn; i += TILE) s_incp[i] = p.inc_ptr[n0 + i] - k0;
   if (t < nk) {
     R out[3];
-    const R J = incidence_row<R, false>(in, out);
+    const R J = incidence_row<R, M, false>(in, out);
     if (J <= R(0)) flag_inversion(status, int(in.pe & kIncElemMask));
 #pragma unroll
     for (int v = 0; v < 3; ++v) contrib[v * TILE + t] = out[v];
@@ -164,10 +184,8 @@ __device__ __forceinline__ double oct_sum(double x, bool upper_quad) {
 // 1..3) mark the incidences that feed it.  Incidence t finds its node
 // through the run offset kept in byte 0 of its rank word; a unit lane finds
 // its node by counting unit-run starts (one bit per node with units).
-template <typename R, int TILE>
-__global__ __launch_bounds__(TILE) void tet4_assemble_rows(DevPlan p, const double* __restrict__ mu,
-                                                           const double* __restrict__ lam,
-                                                           double* __restrict__ f,
+template <typename R, int TILE, int M>
+__global__ __launch_bounds__(TILE) void tet4_assemble_rows(DevPlan p, MatParams mp, double* __restrict__ f,
                                                            double* __restrict__ kval, int32_t* status) {
   constexpr int NV = 18;
   constexpr int ND = kTet4DiagLanes;
@@ -191,7 +209,7 @@ __global__ __launch_bounds__(TILE) void tet4_assemble_rows(DevPlan p, const doub
   const int lane = t & 63, wave = t >> 6;
 
   IncInputs<R> in;
-  if (t < nk) load_incidence<R>(p, mu, lam, k0 + t, in);
+  if (t < nk) load_incidence<R, M>(p, mp, k0 + t, in);
   // node table
   int ub = 0, ue = 0;
   if (t < nn) {
@@ -226,7 +244,7 @@ __global__ __launch_bounds__(TILE) void tet4_assemble_rows(DevPlan p, const doub
   R keep[NV];   // pass B values: the blocks at q = 2 and 3
   if (t < nk) {
     R out[39];
-    const R J = incidence_row<R, true>(in, out);
+    const R J = incidence_row<R, M, true>(in, out);
     if (J <= R(0)) flag_inversion(status, int(in.pe & kIncElemMask));
     const uint32_t r = in.rank;
     const int off = int(r & 0xffu);
@@ -345,15 +363,35 @@ __global__ __launch_bounds__(TILE) void tet4_assemble_rows(DevPlan p, const doub
 
 }
 
-template <typename R, int TILE>
-static void launch_t4(const DevPlan& p, const double* mu, const double* lam, double* f, double* kval,
-                      hipStream_t s, int32_t* status) {
+template <typename R, int TILE, int M>
+static void launch_t4(const DevPlan& p, const MatParams& mp, double* f, double* kval, hipStream_t s,
+                      int32_t* status) {
   const int blocks = 8 * ((p.n_tiles + 7) / 8);
   if (kval)
-    hipLaunchKernelGGL((tet4_assemble_rows<R, TILE>), dim3(blocks), dim3(TILE), 0, s, p, mu, lam, f, kval,
-                       status);
+    hipLaunchKernelGGL((tet4_assemble_rows<R, TILE, M>), dim3(blocks), dim3(TILE), 0, s, p, mp, f, kval, status);
   else
-    hipLaunchKernelGGL((tet4_force_kernel<R, TILE>), dim3(blocks), dim3(TILE), 0, s, p, mu, lam, f, status);
+    hipLaunchKernelGGL((tet4_force_kernel<R, TILE, M>), dim3(blocks), dim3(TILE), 0, s, p, mp, f, status);
+}
+
+template <int M>
+static int32_t launch_t4_tiles(const DevPlan& p, const MatParams& mp, double* f, double* kval, bool fp32,
+                               hipStream_t s, int32_t* status) {
+  switch (p.tile) {
+    case 64:
+      fp32 ? launch_t4<float, 64, M>(p, mp, f, kval, s, status) : launch_t4<double, 64, M>(p, mp, f, kval, s, status);
+      break;
+    case 128:
+      fp32 ? launch_t4<float, 128, M>(p, mp, f, kval, s, status)
+           : launch_t4<double, 128, M>(p, mp, f, kval, s, status);
+      break;
+    case 256:
+      fp32 ? launch_t4<float, 256, M>(p, mp, f, kval, s, status)
+           : launch_t4<double, 256, M>(p, mp, f, kval, s, status);
+      break;
+    default:
+      return fail(PSX_EINVAL, "unsupported tile");
+  }
+  return PSX_OK;
 }
 
 int32_t launch_permute_u(const DevPlan& p, const double* u, hipStream_t s, int32_t* status) {
@@ -362,30 +400,28 @@ int32_t launch_permute_u(const DevPlan& p, const double* u, hipStream_t s, int32
   return PSX_OK;
 }
 
-int32_t launch_tet4_assemble(const DevPlan& p, const double* u, const double* mu, const double* lam,
-                             double* f, double* kval, int precision, hipStream_t s, int32_t* status) {
+int32_t launch_tet4_hyper(const DevPlan& p, int model, const double* const* params, const double* u,
+                          double* f, double* kval, int precision, hipStream_t s, int32_t* status) {
   int32_t rc = launch_permute_u(p, u, s, status);
   if (rc) return rc;
   if (p.n_tiles <= 0) return PSX_OK;
   const bool fp32 = precision == PSX_FP32;
-  switch (p.tile) {
-    case 64:
-      fp32 ? launch_t4<float, 64>(p, mu, lam, f, kval, s, status)
-           : launch_t4<double, 64>(p, mu, lam, f, kval, s, status);
-      break;
-    case 128:
-      fp32 ? launch_t4<float, 128>(p, mu, lam, f, kval, s, status)
-           : launch_t4<double, 128>(p, mu, lam, f, kval, s, status);
-      break;
-    case 256:
-      fp32 ? launch_t4<float, 256>(p, mu, lam, f, kval, s, status)
-           : launch_t4<double, 256>(p, mu, lam, f, kval, s, status);
-      break;
-    default:
-      return fail(PSX_EINVAL, "unsupported tile");
+  const MatParams mp{{params[0], params[1], params[2], params[3]}};
+  switch (model) {
+    case kNeoHookean: rc = launch_t4_tiles<kNeoHookean>(p, mp, f, kval, fp32, s, status); break;
+    case kMooneyRivlin: rc = launch_t4_tiles<kMooneyRivlin>(p, mp, f, kval, fp32, s, status); break;
+    case kYeoh: rc = launch_t4_tiles<kYeoh>(p, mp, f, kval, fp32, s, status); break;
+    default: return fail(PSX_EINVAL, "unknown material model");
   }
+  if (rc) return rc;
   PSX_HIP_TRY(hipGetLastError());
   return PSX_OK;
+}
+
+int32_t launch_tet4_assemble(const DevPlan& p, const double* u, const double* mu, const double* lam,
+                             double* f, double* kval, int precision, hipStream_t s, int32_t* status) {
+  const double* params[4] = {mu, lam, nullptr, nullptr};
+  return launch_tet4_hyper(p, kNeoHookean, params, u, f, kval, precision, s, status);
 }
 
 // ---------------------------------------------------------------------------

@@ -44,10 +44,12 @@ __device__ __forceinline__ void tet4_geometry(const double4 X[4], const double4 
 // order (h[0] = row node), the volume V and grad u: the four 3x3 blocks
 // K_{0b} (b = 0..3, row-major, 36 values) and f_0 (3 values), or f_0 only.
 // Returns J = det F.
-template <typename R, bool WANT_K>
-__device__ __forceinline__ R tet4_row_core(const R h[4][3], R V, const R A[3][3], R mu, R K, R* out) {
+// prm: the model's parameters (kNeoHookean: mu, K = lambda + 2 mu / 3).
+template <typename R, int M, bool WANT_K>
+__device__ __forceinline__ R tet4_row_core_m(const R h[4][3], R V, const R A[3][3], const R prm[4], R* out) {
   NHPoint<R> m;
-  nh_point(A, mu, K, m);
+  HyperExtra<R> x;
+  hyper_point<R, M>(A, prm, m, x);
   R Sh0[3];
   mat3vec(m.S, h[0], Sh0);
   // f_0 = V F S h_0
@@ -77,16 +79,47 @@ __device__ __forceinline__ R tet4_row_core(const R h[4][3], R V, const R A[3][3]
       T[i] = vg * g[0][i];
       W[i] = V * Sh0[i];
     }
+    // model-specific vectors of the row node (see psx_nh.h)
+    R Qm[3] = {R(0), R(0), R(0)}, Pm[3] = {R(0), R(0), R(0)}, Z0[3] = {R(0), R(0), R(0)};
+    if constexpr (M == kMooneyRivlin) {
+      R bq0[3];
+      mat3vec(x.b, q[0], bq0);
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        Qm[i] = V * x.cq * q[0][i];                          // times q_b^T
+        Pm[i] = V * x.cp * (x.I1 * q[0][i] - bq0[i]);        // p_0, times g_b^T
+      }
+    } else if constexpr (M == kYeoh) {
+#pragma unroll
+      for (int i = 0; i < 3; ++i) Z0[i] = V * x.psi11 * x.zs * (q[0][i] - x.I1 * R(1.0 / 3.0) * g[0][i]);
+    }
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
       const R sb = g[b][0] * Rv[0] + g[b][1] * Rv[1] + g[b][2] * Rv[2] + h[b][0] * W[0] + h[b][1] * W[1] +
                    h[b][2] * W[2];
+      R pb[3] = {R(0), R(0), R(0)}, zb[3] = {R(0), R(0), R(0)};
+      R hh = R(0);
+      if constexpr (M == kMooneyRivlin) {
+        R bqb[3];
+        mat3vec(x.b, q[b], bqb);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) pb[i] = x.I1 * q[b][i] - bqb[i];
+        hh = V * x.cb * (h[0][0] * h[b][0] + h[0][1] * h[b][1] + h[0][2] * h[b][2]);
+      } else if constexpr (M == kYeoh) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) zb[i] = x.zs * (q[b][i] - x.I1 * R(1.0 / 3.0) * g[b][i]);
+      }
       R* o = out + 9 * b;
 #pragma unroll
       for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int k = (b == 0 ? i : 0); k < 3; ++k) {
-          const R v = P[i] * g[b][k] + g[b][i] * Rv[k] + T[i] * q[b][k] + (i == k ? sb : R(0));
+          R v = P[i] * g[b][k] + g[b][i] * Rv[k] + T[i] * q[b][k] + (i == k ? sb : R(0));
+          if constexpr (M == kMooneyRivlin)
+            v += Qm[i] * q[b][k] + hh * x.b[i][k] + V * x.cb * q[b][i] * q[0][k] +
+                 V * x.cp * g[0][i] * pb[k] + Pm[i] * g[b][k];
+          else if constexpr (M == kYeoh)
+            v += Z0[i] * zb[k];
           o[3 * i + k] = v;
           if (b == 0 && k != i) o[3 * k + i] = v;  // K_00 is symmetric
         }
@@ -95,21 +128,33 @@ __device__ __forceinline__ R tet4_row_core(const R h[4][3], R V, const R A[3][3]
   return m.J;
 }
 
+template <typename R, bool WANT_K>
+__device__ __forceinline__ R tet4_row_core(const R h[4][3], R V, const R A[3][3], R mu, R K, R* out) {
+  const R prm[4] = {mu, K, R(0), R(0)};
+  return tet4_row_core_m<R, kNeoHookean, WANT_K>(h, V, A, prm, out);
+}
+
 // Element seen from its local node 0 (callers rotate the node order so the
 // row node comes first), geometry from the reference coordinates.
-template <typename R, bool WANT_K>
-__device__ __forceinline__ R tet4_row(const double4 X[4], const double4 U[4], R mu, R K, R* out) {
+template <typename R, int M, bool WANT_K>
+__device__ __forceinline__ R tet4_row_m(const double4 X[4], const double4 U[4], const R prm[4], R* out) {
   R h[4][3], V, A[3][3];
   tet4_geometry(X, U, h, V, A);
-  return tet4_row_core<R, WANT_K>(h, V, A, mu, K, out);
+  return tet4_row_core_m<R, M, WANT_K>(h, V, A, prm, out);
+}
+
+template <typename R, bool WANT_K>
+__device__ __forceinline__ R tet4_row(const double4 X[4], const double4 U[4], R mu, R K, R* out) {
+  const R prm[4] = {mu, K, R(0), R(0)};
+  return tet4_row_m<R, kNeoHookean, WANT_K>(X, U, prm, out);
 }
 
 // Same row from precomputed shape-function gradients already in the row's
 // node order: G[b] = (h_b, V) (b = 0 the row node), displacements U in the
 // same order (differences taken in FP64 before rounding).
-template <bool WANT_K>
-__device__ __forceinline__ float tet4_row_grad(const float4 G[4], const double4 U[4], float mu, float K,
-                                               float* out) {
+template <int M, bool WANT_K>
+__device__ __forceinline__ float tet4_row_grad_m(const float4 G[4], const double4 U[4], const float prm[4],
+                                                 float* out) {
   float h[4][3];
 #pragma unroll
   for (int b = 0; b < 4; ++b) {
@@ -131,7 +176,14 @@ __device__ __forceinline__ float tet4_row_grad(const float4 G[4], const double4 
 #pragma unroll
       for (int j = 0; j < 3; ++j) A[i][j] = dU[i][0] * h[1][j] + dU[i][1] * h[2][j] + dU[i][2] * h[3][j];
   }
-  return tet4_row_core<float, WANT_K>(h, G[0].w, A, mu, K, out);
+  return tet4_row_core_m<float, M, WANT_K>(h, G[0].w, A, prm, out);
+}
+
+template <bool WANT_K>
+__device__ __forceinline__ float tet4_row_grad(const float4 G[4], const double4 U[4], float mu, float K,
+                                               float* out) {
+  const float prm[4] = {mu, K, 0.f, 0.f};
+  return tet4_row_grad_m<kNeoHookean, WANT_K>(G, U, prm, out);
 }
 
 // Internal force of a whole element in its own node order: f[3 b + i] =

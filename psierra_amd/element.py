@@ -22,7 +22,7 @@ import numpy as np
 import scipy.sparse as spsp
 
 from . import engine
-from .material import lame_arrays
+from .material import MooneyRivlinField, YeohField, lame_arrays, material_model
 
 
 class _GpuTotalLagrangian:
@@ -73,23 +73,48 @@ class _GpuTotalLagrangian:
         if st["params_key"] != pkey:
             import torch
 
-            mu, lam = lame_arrays(self._material, st["plan"].n_elements)
+            model, arrays = material_model(self._material, st["plan"].n_elements)
+            if model != "neohookean" and self._npe != 4:
+                raise NotImplementedError("the GPU Tet10 element evaluates the Neo-Hookean model only")
             dev = st["plan"].device
-            st["mu"] = torch.as_tensor(mu, dtype=torch.float64, device=dev)
-            st["lam"] = torch.as_tensor(lam, dtype=torch.float64, device=dev)
+            st["model"] = model
+            st["params"] = [torch.as_tensor(a, dtype=torch.float64, device=dev) for a in arrays]
+            if model == "neohookean":
+                st["mu"], st["lam"] = st["params"]
             st["params_key"] = pkey
         return st
 
     def plan(self, mesh) -> engine.AssemblyPlan:
         return self._state(mesh)["plan"]
 
-    def assemble_device(self, mesh, u, want_k: bool = True, f=None, kval=None, check_status: bool = True):
+    # Element name carried by the inversion message (None: the Total
+    # Lagrangian text "Element inversion detected: det(F) = ...").
+    _inversion_name: str | None = None
+
+    def _relabel(self, err: ValueError) -> ValueError:
+        msg = str(err)
+        if self._inversion_name and msg.startswith("Element inversion detected: "):
+            return ValueError(msg.replace("Element inversion detected: ",
+                                          f"Element inversion detected in {self._inversion_name}: ", 1))
+        return err
+
+    def assemble_device(self, mesh, u, want_k: bool = True, f=None, kval=None, check_status: bool = True,
+                        precision: str | None = None):
         """f_int and K values as device tensors (u may be a device tensor)."""
         st = self._state(mesh)
-        return st["plan"].assemble(
-            u, st["mu"], st["lam"], f=f, kval=kval, precision=self.precision, want_k=want_k,
-            check_status=check_status,
-        )
+        prec = precision or self.precision
+        try:
+            if st["model"] == "neohookean":
+                return st["plan"].assemble(
+                    u, st["mu"], st["lam"], f=f, kval=kval, precision=prec, want_k=want_k,
+                    check_status=check_status,
+                )
+            return st["plan"].assemble_hyperelastic(
+                u, st["model"], st["params"], f=f, kval=kval, precision=prec, want_k=want_k,
+                check_status=check_status,
+            )
+        except ValueError as err:
+            raise self._relabel(err) from None
 
     # -- SolidDomain hooks (src/psierra/domain/solid.py:238-241, 288-291) ---
     def assemble_internal_forces(self, mesh, displacement) -> np.ndarray:
@@ -111,15 +136,15 @@ class _GpuTotalLagrangian:
 
     # SolidDomain.assemble_stiffness_matrix delegates here when the element
     # defines it (src/psierra/domain/solid.py:180-181): the linear stiffness
-    # B^T C B V with C Hooke's law at the reference state is the Neo-Hookean
-    # tangent at u = 0 (S = 0, C4 = Hookean), assembled in FP64.
+    # B^T C B V with C Hooke's law at the reference state is the material's
+    # tangent at u = 0 (S = 0, C4 = Hooke for each supported model),
+    # assembled in FP64.
     def assemble_stiffness_matrix(self, mesh) -> spsp.csr_matrix:
         plan = self._state(mesh)["plan"]
-        st = self._state(mesh)
         import torch
 
         u0 = torch.zeros(plan.n_dofs, dtype=torch.float64, device=plan.device)
-        _, k = plan.assemble(u0, st["mu"], st["lam"], precision="fp64", want_k=True)
+        _, k = self.assemble_device(mesh, u0, want_k=True, precision="fp64")
         return self._csr(mesh, k)
 
     def assemble_mass_matrix(self, mesh) -> spsp.csr_matrix:
@@ -142,6 +167,10 @@ class _GpuTotalLagrangian:
 
     def _element_params(self, elem_index):
         mat = self._material
+        p = getattr(mat, "parameters", None)
+        if isinstance(mat, (MooneyRivlinField, YeohField)) or hasattr(p, "c01") or hasattr(p, "c20"):
+            raise NotImplementedError("the per-element GPU API evaluates the Neo-Hookean model; "
+                                      "other models run through the assembly hooks")
         if getattr(mat, "supports_element_indexing", False) and hasattr(mat, "mu"):
             e = 0 if elem_index is None else int(elem_index)
             return float(mat.mu[e]), float(mat.lam[e])
@@ -157,7 +186,10 @@ class _GpuTotalLagrangian:
         mu, lam = self._element_params(elem_index)
         X = np.asarray(node_coords, dtype=np.float64).reshape(1, self._npe, 3)
         u = np.asarray(u_nodes, dtype=np.float64).reshape(1, self._npe, 3)
-        k, f = engine.element_batch(X, u, [mu], [lam], want_k=True, precision=self.precision)
+        try:
+            k, f = engine.element_batch(X, u, [mu], [lam], want_k=True, precision=self.precision)
+        except ValueError as err:
+            raise self._relabel(err) from None
         return k[0].cpu().numpy(), f[0].cpu().numpy()
 
 
@@ -206,3 +238,22 @@ class GpuTet10TotalLagrangian(_GpuTotalLagrangian):
     @property
     def default_quadrature_order(self) -> int:
         return 4
+
+
+class GpuTet4UpdatedLagrangian(GpuTet4TotalLagrangian):
+    """Tet4UpdatedLagrangian (src/psierra/element/tet4_updated_lagrangian.py:
+    31-109) on the GPU.  Its spatial form (h = H F^-1, Cauchy stress, pushed-
+    forward tangent over the current volume) gives the same f_int and K_T as
+    the Total Lagrangian form (the reference's own statement, :18-20, checked
+    on the oracle's restatements of both), so the same kernels run; only the
+    inversion message names the element."""
+
+    _inversion_name = "Tet4UpdatedLagrangian"
+
+
+class GpuTet10UpdatedLagrangian(GpuTet10TotalLagrangian):
+    """Tet10UpdatedLagrangian (src/psierra/element/tet10_updated_lagrangian.py)
+    on the GPU: pointwise the same integrand as the Total Lagrangian form at
+    each of the 4 quadrature points, so the same kernels run."""
+
+    _inversion_name = "Tet10UpdatedLagrangian"

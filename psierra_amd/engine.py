@@ -18,6 +18,7 @@ from ._lib import check, lib, ptr
 PRECISIONS = {"fp64": _lib.PSX_FP64, "fp32": _lib.PSX_FP32}
 
 INVERSION_MSG = "Element inversion detected: det(F) = {:.4e} <= 0."
+MODELS = {"neohookean": 0, "mooney_rivlin": 1, "yeoh": 2}
 
 
 def _torch():
@@ -181,6 +182,33 @@ class AssemblyPlan:
                 ptr(self.status),
             )
         )
+        if check_status:
+            self.raise_on_status(u)
+        return f, kval
+
+    def assemble_hyperelastic(self, u, model, params, f=None, kval=None, precision="fp64", want_k=True,
+                              check_status=True):
+        """As ``assemble`` for material ``model`` (MODELS: "neohookean" (mu,
+        lambda), "mooney_rivlin" (c10, c01, K), "yeoh" (c10, c20, c30, K)),
+        ``params`` one (E,) array or device tensor per parameter."""
+        code = MODELS[model] if isinstance(model, str) else int(model)
+        npar = (2, 3, 4)[code]
+        if len(params) != npar:
+            raise ValueError(f"model {model} takes {npar} parameter arrays, got {len(params)}")
+        u = self._device_f64(u, self.n_dofs, "u")
+        prm = [self._device_f64(a, self.n_elements, f"param{i}") for i, a in enumerate(params)]
+        if f is None or (want_k and kval is None):
+            f0, k0 = self.alloc_outputs(want_k)
+            f = f if f is not None else f0
+            kval = kval if kval is not None else k0
+        self._check_out(f, self.n_dofs, "f")
+        if want_k:
+            self._check_out(kval, self.nnz, "kval")
+        else:
+            kval = None
+        arr = (ctypes.c_void_p * 4)(*[t.data_ptr() for t in prm], *([0] * (4 - npar)))
+        check(lib().psx_assemble_hyperelastic(self._h, code, arr, ptr(u), ptr(f), ptr(kval), precision_code(precision),
+                                              current_stream_handle(), ptr(self.status)))
         if check_status:
             self.raise_on_status(u)
         return f, kval

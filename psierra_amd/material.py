@@ -174,3 +174,72 @@ class NeoHookeanField:
         F = np.asarray(deformation_gradient, dtype=np.float64).reshape(1, 3, 3)
         C, S = material_batch(F, mu, bulk, precision)
         return C[0].cpu().numpy(), S[0].cpu().numpy()
+
+
+class MooneyRivlinField:
+    """Compressible Mooney-Rivlin (src/psierra/material/mooney_rivlin.py) with
+    per-element (c10, c01, K); evaluated by the assembly kernels."""
+
+    supports_element_indexing: bool = True
+
+    def __init__(self, c10, c01, bulk_modulus, mass_density: float = 1.0) -> None:
+        self.c10, self.c01, self.bulk = (np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+                                         for a in (c10, c01, bulk_modulus))
+        if not (self.c10.shape == self.c01.shape == self.bulk.shape):
+            raise ValueError("c10, c01 and bulk_modulus must have the same length")
+        if np.any(self.c10 + self.c01 <= 0.0) or np.any(self.bulk <= 0.0):
+            raise ValueError("need c10 + c01 > 0 and K > 0 for every element")
+        self._mass_density = float(mass_density)
+        self.version = 0
+
+    @property
+    def mass_density(self) -> float:
+        return self._mass_density
+
+
+class YeohField:
+    """Compressible Yeoh (src/psierra/material/yeoh.py) with per-element
+    (c10, c20, c30, K); evaluated by the assembly kernels."""
+
+    supports_element_indexing: bool = True
+
+    def __init__(self, c10, c20, c30, bulk_modulus, mass_density: float = 1.0) -> None:
+        self.c10, self.c20, self.c30, self.bulk = (np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+                                                   for a in (c10, c20, c30, bulk_modulus))
+        if not (self.c10.shape == self.c20.shape == self.c30.shape == self.bulk.shape):
+            raise ValueError("c10, c20, c30 and bulk_modulus must have the same length")
+        if np.any(self.c10 <= 0.0) or np.any(self.bulk <= 0.0):
+            raise ValueError("need c10 > 0 and K > 0 for every element")
+        self._mass_density = float(mass_density)
+        self.version = 0
+
+    @property
+    def mass_density(self) -> float:
+        return self._mass_density
+
+
+def material_model(material, n_elements: int):
+    """(model name, per-element parameter arrays) of a supported material:
+    Neo-Hookean (mu, lambda), Mooney-Rivlin (c10, c01, K), Yeoh (c10, c20,
+    c30, K).  Accepts the *Field classes here and pSierra's own materials
+    (their ``parameters`` expose c10/c01/c20/c30 and bulk_modulus)."""
+
+    def full(v):
+        a = np.asarray(v, dtype=np.float64).reshape(-1)
+        if a.size == 1:
+            return np.full(n_elements, float(a[0]))
+        if a.size != n_elements:
+            raise ValueError(f"material has {a.size} elements, mesh has {n_elements}")
+        return a
+
+    if isinstance(material, MooneyRivlinField):
+        return "mooney_rivlin", [full(material.c10), full(material.c01), full(material.bulk)]
+    if isinstance(material, YeohField):
+        return "yeoh", [full(material.c10), full(material.c20), full(material.c30), full(material.bulk)]
+    params = getattr(material, "parameters", None)
+    if params is not None and hasattr(params, "c01") and hasattr(params, "c10"):
+        return "mooney_rivlin", [full(params.c10), full(params.c01), full(params.bulk_modulus)]
+    if params is not None and hasattr(params, "c20") and hasattr(params, "c30"):
+        return "yeoh", [full(params.c10), full(params.c20), full(params.c30), full(params.bulk_modulus)]
+    mu, lam = lame_arrays(material, n_elements)
+    return "neohookean", [mu, lam]

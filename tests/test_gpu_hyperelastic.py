@@ -1,0 +1,116 @@
+"""Mooney-Rivlin and Yeoh assembly on the GPU against the oracle.
+
+The oracle restates MooneyRivlin/Yeoh.compute_stress_and_tangent
+(src/psierra/material/mooney_rivlin.py:69-152, yeoh.py:69-148; pinned in
+test_oracle.py by finite differences of the strain energies) and the
+reference's Tet4 element (B^T C B V + G^T (I3 x S) G V, B^T S V).  Per-element
+parameters; tolerances as for Neo-Hookean: FP64 1e-12 (K) / 1e-11 (f), FP32
+1e-5.  Also: the Neo-Hookean model through psx_assemble_hyperelastic is
+bit-identical to psx_assemble, the force-only kernel matches the fused one,
+and the linear-stiffness hook gives Hooke's law with mu = 2 (c10 + c01).
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import assembly as A
+from oracle import fem
+from psierra_amd import workloads as W
+from tests.gpu_helpers import banner, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _params(model, E, seed):
+    rng = np.random.default_rng(seed)
+    s = np.exp(0.3 * rng.standard_normal(E))
+    if model == "mooney_rivlin":
+        return [300.0 * s, 120.0 * s, 6000.0 * s]
+    return [400.0 * s, 60.0 * s, 25.0 * s, 6000.0 * s]
+
+
+def _oracle(wl, model, params):
+    code = {"mooney_rivlin": 1, "yeoh": 2}[model]
+    E = wl.mesh.n_elements
+    ke = np.empty((E, 12, 12))
+    fe = np.empty((E, 12))
+    for e in range(E):
+        c = wl.mesh.elements[e]
+        ke[e], fe[e] = fem.tet4_force_and_tangent(wl.mesh.nodes[c], wl.u[c],
+                                                  fem.material_fn(code, [p[e] for p in params]))
+    dofs = A.compute_element_dofs(wl.mesh.elements)
+    rows, cols = A.create_coo_sparsity_indices(dofs)
+    n = wl.mesh.n_dofs
+    return A.assemble_vector(fe, dofs, n), A.assemble_sparse_matrix(ke, rows, cols, n)
+
+
+@pytest.fixture(scope="module")
+def box():
+    return W.jittered_box_tet4(n=6, seed=31)
+
+
+@pytest.mark.parametrize("model", ["mooney_rivlin", "yeoh"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_model_matches_oracle(box, model, precision):
+    from psierra_amd.engine import AssemblyPlan
+
+    wl = box
+    params = _params(model, wl.mesh.n_elements, seed=7)
+    f_ref, K_ref = _oracle(wl, model, params)
+    plan = AssemblyPlan(wl.mesh.nodes, wl.mesh.elements)
+    banner(f"hyperelastic {model} {precision}")
+    f, k = plan.assemble_hyperelastic(wl.u.ravel(), model, params, precision=precision)
+    f_only, _ = plan.assemble_hyperelastic(wl.u.ravel(), model, params, precision=precision, want_k=False)
+    f, k, f_only = f.cpu().numpy(), k.cpu().numpy(), f_only.cpu().numpy()
+    indptr, indices = plan.pattern()
+    assert np.array_equal(indptr, K_ref.indptr) and np.array_equal(indices, K_ref.indices)
+    tk, tf = (1e-12, 1e-11) if precision == "fp64" else (1e-5, 1e-5)
+    assert rel_err(k, K_ref.data) < tk
+    assert rel_err(f, f_ref) < tf
+    assert rel_err(f_only, f) < (1e-13 if precision == "fp64" else 1e-6)
+
+
+def test_neohookean_model_is_the_plain_path(box):
+    from psierra_amd.engine import AssemblyPlan
+
+    wl = box
+    plan = AssemblyPlan(wl.mesh.nodes, wl.mesh.elements)
+    banner("hyperelastic neohookean == psx_assemble")
+    for precision in ("fp64", "fp32"):
+        f1, k1 = plan.assemble(wl.u.ravel(), wl.mu, wl.lam, precision=precision)
+        f2, k2 = plan.assemble_hyperelastic(wl.u.ravel(), "neohookean", [wl.mu, wl.lam], precision=precision)
+        assert bool((f1 == f2).all()) and bool((k1 == k2).all())
+
+
+def test_domain_hooks_and_linear_stiffness(box):
+    from psierra_amd.domain import SolidDomain
+    from psierra_amd.element import GpuTet4TotalLagrangian
+    from psierra_amd.material import MooneyRivlinField
+
+    wl = box
+    c10, c01, K = _params("mooney_rivlin", wl.mesh.n_elements, seed=8)
+    mat = MooneyRivlinField(c10, c01, K)
+    dom = SolidDomain(wl.mesh, mat, GpuTet4TotalLagrangian(mat))
+    dom.update_state(u=wl.u.ravel())
+    banner("mooney-rivlin domain hooks")
+    f_ref, K_ref = _oracle(wl, "mooney_rivlin", [c10, c01, K])
+    assert rel_err(dom.assemble_internal_forces(), f_ref) < 1e-11
+    assert rel_err(dom.assemble_tangent_stiffness_matrix().data, K_ref.data) < 1e-12
+    mu = 2.0 * (c10 + c01)
+    K_lin = A.linear_stiffness(wl.mesh.nodes, wl.mesh.elements, mu, K - 2.0 / 3.0 * mu)
+    assert rel_err(dom.assemble_stiffness_matrix().data, K_lin.data) < 1e-12
+
+
+def test_tet10_rejects_other_models():
+    from psierra_amd.domain import SolidDomain
+    from psierra_amd.element import GpuTet10TotalLagrangian
+    from psierra_amd.material import YeohField
+
+    wl = W.jittered_box_tet10(n=2, seed=2)
+    E = wl.mesh.n_elements
+    mat = YeohField(np.full(E, 400.0), np.full(E, 60.0), np.full(E, 25.0), np.full(E, 6000.0))
+    dom = SolidDomain(wl.mesh, mat, GpuTet10TotalLagrangian(mat))
+    banner("tet10 + yeoh rejected")
+    with pytest.raises(NotImplementedError, match="Neo-Hookean model only"):
+        dom.assemble_internal_forces()

@@ -339,3 +339,90 @@ def test_coo_to_csr_keeps_explicit_zeros_and_sum_drops_them():
     assert K.nnz == 4
     C = A.free_dof_csc(K, np.array([0, 1]))
     assert C.nnz == 2
+
+
+def test_updated_lagrangian_equals_total_lagrangian():
+    """The reference states that its Updated Lagrangian Tet4 gives the same
+    virtual work and tangent as the Total Lagrangian one
+    (src/psierra/element/tet4_updated_lagrangian.py:18-20); the oracle's two
+    restatements, built from different operators (spatial b_L, pushed-forward
+    C, Cauchy stress vs. material B, C, S), agree, and the UL force's
+    central difference is the UL tangent."""
+    from oracle import fem
+
+    rng = np.random.default_rng(41)
+    X = np.array([[0.0, 0.0, 0.0], [1.0, 0.1, 0.0], [0.1, 1.0, 0.05], [0.0, 0.2, 1.0]])
+    mat = fem.nh_material(1e3, 5e3)
+    for scale in (1e-4, 0.05, 0.2):
+        u = scale * rng.standard_normal((4, 3))
+        k_tl, f_tl = fem.tet4_force_and_tangent(X, u, mat)
+        k_ul, f_ul = fem.tet4_ul_force_and_tangent(X, u, mat)
+        assert np.linalg.norm(f_ul - f_tl) <= 1e-11 * np.linalg.norm(f_tl) + 1e-13
+        assert np.linalg.norm(k_ul - k_tl) <= 1e-11 * np.linalg.norm(k_tl)
+    u = 0.05 * rng.standard_normal((4, 3))
+    k_ul, _ = fem.tet4_ul_force_and_tangent(X, u, mat)
+    h = 1e-7
+    kfd = np.zeros((12, 12))
+    for d in range(12):
+        up, um = u.copy(), u.copy()
+        up.flat[d] += h
+        um.flat[d] -= h
+        kfd[:, d] = (fem.tet4_ul_force_and_tangent(X, up, mat)[1] - fem.tet4_ul_force_and_tangent(X, um, mat)[1]) / (2 * h)
+    assert np.linalg.norm(kfd - k_ul) <= 1e-6 * np.linalg.norm(k_ul)
+
+
+def _fd_check_material(stress_tangent, energy, F):
+    """S = 2 dPsi/dC and C_voigt = 2 dS/dC by central differences in C
+    (the reference's material tests take the same derivatives)."""
+    from oracle import fem
+
+    Cv, S = stress_tangent(F)
+    h = 1e-6
+    # S_IJ = dPsi/dE_IJ with C = I + 2E, F = R U: perturb through F = F0 (I + dE)-ish via C
+    # use U = sqrtm(C) so that F(C) = U is a valid deformation with F^T F = C
+    w, Q = np.linalg.eigh(F.T @ F)
+
+    def F_of(C):
+        w, Q = np.linalg.eigh(C)
+        return Q @ np.diag(np.sqrt(w)) @ Q.T
+
+    C0 = F.T @ F
+    S_fd = np.zeros((3, 3))
+    for i in range(3):
+        for j in range(3):
+            dC = np.zeros((3, 3))
+            dC[i, j] += 0.5 * h
+            dC[j, i] += 0.5 * h
+            S_fd[i, j] = 2.0 * (energy(F_of(C0 + dC)) - energy(F_of(C0 - dC))) / (2 * h)
+    assert np.linalg.norm(S_fd - S) <= 1e-6 * np.linalg.norm(S) + 1e-8
+    # C4 = 2 dS/dC in Voigt: column b from perturbing C in the (k, l) pair of b
+    Cv_fd = np.zeros((6, 6))
+    for b, (k, l) in fem.VOIGT_MAP_3D.items():
+        dC = np.zeros((3, 3))
+        dC[k, l] += 0.5 * h
+        dC[l, k] += 0.5 * h
+        Sp = stress_tangent(F_of(C0 + dC))[1]
+        Sm = stress_tangent(F_of(C0 - dC))[1]
+        dS = 2.0 * (Sp - Sm) / (2 * h)
+        Cv_fd[:, b] = [dS[i, j] for i, j in fem.VOIGT_MAP_3D.values()]
+    assert np.linalg.norm(Cv_fd - Cv) <= 1e-5 * np.linalg.norm(Cv)
+
+
+def test_mooney_rivlin_and_yeoh_are_consistent():
+    from oracle import fem
+
+    F = np.array([[1.1, 0.05, 0.02], [0.03, 0.93, 0.06], [0.01, 0.04, 1.05]])
+    c10, c01, c20, c30, K = 400.0, 150.0, 80.0, 30.0, 6000.0
+    _fd_check_material(lambda F: fem.mooney_rivlin_stress_and_tangent(F, c10, c01, K),
+                       lambda F: fem.mooney_rivlin_energy(F, c10, c01, K), F)
+    _fd_check_material(lambda F: fem.yeoh_stress_and_tangent(F, c10, c20, c30, K),
+                       lambda F: fem.yeoh_energy(F, c10, c20, c30, K), F)
+    # both reduce to Neo-Hookean with mu = 2 c10 when the extra constants vanish
+    nh = fem.neohookean_stress_and_tangent(F, 2 * c10, K)
+    for got in (fem.mooney_rivlin_stress_and_tangent(F, c10, 0.0, K), fem.yeoh_stress_and_tangent(F, c10, 0.0, 0.0, K)):
+        assert np.allclose(got[0], nh[0], rtol=1e-13, atol=1e-10) and np.allclose(got[1], nh[1], rtol=1e-13, atol=1e-10)
+    # small-strain limit: Hooke with mu = 2 (c10 + c01) (mooney_rivlin.py:20-21)
+    lam = K - 2.0 / 3.0 * 2 * (c10 + c01)
+    Cv, S = fem.mooney_rivlin_stress_and_tangent(np.identity(3), c10, c01, K)
+    assert np.allclose(S, 0.0, atol=1e-12)
+    assert np.allclose(Cv, fem.hookean_C(lam, 2 * (c10 + c01)), rtol=1e-12)
