@@ -3,7 +3,7 @@
 // Global assembly is owner-computes over node rows.  The plan orders nodes
 // along a Morton curve (locality) and packs consecutive nodes into tiles of
 // at most TILE element incidences and at most TILE reduction lanes
-// (4 per node + one per (node, slot) unit).  One work-group handles one tile:
+// (8 per node + one per (node, slot) unit).  One work-group handles one tile:
 //   phase 1: lane t takes incidence t (element e seen from one of its nodes
 //            a): it gathers the element's nodal displacements, evaluates the
 //            element in registers (FP64, or FP32 from precomputed gradients)
@@ -11,9 +11,9 @@
 //            K_aa (6 values, symmetric), f_a (3) and the three off-diagonal
 //            blocks K_ab (27);
 //   phase 2: every reduction lane sums, in FP64 and in increasing incidence
-//            order, one list of 9-value contributions: four lanes per node
-//            take a quarter of the node's incidence run each (K_aa and f_a,
-//            combined across the four lanes in a fixed order), one lane per
+//            order, one list of 9-value contributions: eight lanes per node
+//            take an eighth of the node's incidence run each (K_aa and f_a,
+//            combined across the eight lanes in a fixed order), one lane per
 //            off-diagonal CSR block walks the incidences that feed it.  The
 //            lists have similar lengths, so the lanes of a wave stay in step.
 //            Each lane then writes its 3x3 block (or f_a) into the CSR
