@@ -54,6 +54,10 @@ __device__ __forceinline__ int unit_owner(const unsigned long long* starts, cons
   return nz[c - 1];
 }
 
+// Index of the lowest set bit of a nonzero mask.
+__device__ __forceinline__ int low_bit(uint32_t m) { return __ffs(int(m)) - 1; }
+__device__ __forceinline__ int low_bit(unsigned long long m) { return __ffsll(static_cast<long long>(m)) - 1; }
+
 // Bits [b, e) of a 64-bit word, e - b < 64.
 __device__ __forceinline__ unsigned long long bit_range(int b, int e) {
   return ((1ull << (e - b)) - 1ull) << b;

@@ -54,6 +54,7 @@ void fill_csr(const HostPattern& hp, int32_t* indptr, int32_t* indices);
 // Device view of a plan, passed by value to kernels.
 struct DevPlan {
   int32_t E, N, npe, n_tiles, tile;
+  int32_t max_inc;                // most incidences of any node
   int32_t conn_stride;            // ints per element in conn (4 or 12)
   const int32_t* conn;            // internal node ids, AoS [E][conn_stride]
   const double4* Xi;              // reference coordinates, internal order

@@ -309,10 +309,11 @@ int32_t make_tiles(psx_plan* plan, int32_t tile) {
   // per (node, slot) unit) per tile.
   // Tet10 tiles also hold at most 64 nodes (their node lanes are one wave).
   std::vector<int32_t> tiles{0};
-  int64_t start_inc = 0, lanes = 0;
+  int64_t start_inc = 0, lanes = 0, max_inc = 0;
   const int64_t max_nodes = plan->hp.npe == 4 ? int64_t(tile) : 64;
   for (int64_t i = 0; i < N; ++i) {
     int64_t cnt = plan->inc_ptr_h[i + 1] - plan->inc_ptr_h[i];
+    max_inc = std::max(max_inc, cnt);
     if (cnt > std::min<int64_t>(tile, kMaxNodeIncidences))
       return fail(PSX_EUNSUPPORTED, "node with " + std::to_string(cnt) +
                                         " incident elements exceeds the supported " +
@@ -347,6 +348,7 @@ int32_t make_tiles(psx_plan* plan, int32_t tile) {
   plan->d.tile_info = dinfo;
   plan->d.n_tiles = int32_t(tiles.size() - 1);
   plan->d.tile = tile;
+  plan->d.max_inc = int32_t(max_inc);
   return PSX_OK;
 }
 

@@ -184,7 +184,11 @@ __device__ __forceinline__ double oct_sum(double x, bool upper_quad) {
 // 1..3) mark the incidences that feed it.  Incidence t finds its node
 // through the run offset kept in byte 0 of its rank word; a unit lane finds
 // its node by counting unit-run starts (one bit per node with units).
-template <typename R, int TILE, int M>
+//
+// Feed masks hold one bit per incidence of the node's run: MaskT is uint32_t
+// when no node has more than 32 incidences (smaller LDS and cheaper bit
+// scans), unsigned long long otherwise.
+template <typename R, int TILE, int M, typename MaskT>
 __global__ __launch_bounds__(TILE) void tet4_assemble_rows(DevPlan p, MatParams mp, double* __restrict__ f,
                                                            double* __restrict__ kval, int32_t* status) {
   constexpr int NV = 18;
@@ -192,7 +196,7 @@ __global__ __launch_bounds__(TILE) void tet4_assemble_rows(DevPlan p, MatParams 
   constexpr int NN = TILE / ND;
   constexpr int NW = TILE / 64;
   __shared__ R contrib[NV * TILE];
-  __shared__ unsigned long long s_mask[3 * TILE];
+  __shared__ MaskT s_mask[3 * TILE];
   __shared__ unsigned long long s_ustart[NW];
   __shared__ int32_t s_incp[NN + 1];
   __shared__ int32_t s_uoff[NN + 1];
@@ -235,7 +239,7 @@ __global__ __launch_bounds__(TILE) void tet4_assemble_rows(DevPlan p, MatParams 
   }
   if (t < NW) s_ustart[t] = 0ull;
 #pragma unroll
-  for (int q = 0; q < 3; ++q) s_mask[q * TILE + t] = 0ull;
+  for (int q = 0; q < 3; ++q) s_mask[q * TILE + t] = MaskT(0);
   __syncthreads();
 
   // ---- phase 1: incidence t
@@ -250,7 +254,7 @@ __global__ __launch_bounds__(TILE) void tet4_assemble_rows(DevPlan p, MatParams 
     const int off = int(r & 0xffu);
     const int j = s_first[t - off];
     const int ubase = s_uoff[j] - u0;
-    const unsigned long long bit = 1ull << off;
+    const MaskT bit = MaskT(1) << off;
 #pragma unroll
     for (int q = 1; q < 4; ++q) atomicOr(&s_mask[(q - 1) * TILE + ubase + int((r >> (8 * q)) & 0xffu)], bit);
 #pragma unroll
@@ -265,14 +269,14 @@ __global__ __launch_bounds__(TILE) void tet4_assemble_rows(DevPlan p, MatParams 
   const int units = s_uoff[nn] - u0;
   int j = 0, kb = 0, slot = 0;
   bool diag = false, offd = false;
-  unsigned long long m1 = 0ull, m2 = 0ull, m3 = 0ull;
+  MaskT m1 = 0, m2 = 0, m3 = 0;
   if (t < nd) {
     // (node, eighth of its run): K_aa upper triangle and f_a
     j = t / ND;
     const unsigned part = unsigned(t) % ND;
     kb = s_incp[j];
     const unsigned L = unsigned(s_incp[j + 1] - kb);
-    m1 = bit_range(int((part * L) / ND), int(((part + 1) * L) / ND));   // <= 8 bits
+    m1 = MaskT(bit_range(int((part * L) / ND), int(((part + 1) * L) / ND)));   // <= 8 bits
     diag = true;
   } else if (t < nd + units) {
     // (node, off-diagonal slot)
@@ -292,14 +296,14 @@ __global__ __launch_bounds__(TILE) void tet4_assemble_rows(DevPlan p, MatParams 
   for (int c = 0; c < 9; ++c) acc[c] = 0.0;
   // Sum the 9 values at rows row0.. of the incidences in `msk`, two at a
   // time, in increasing incidence order; rowb(a) picks the rows per bit.
-  auto walk = [&](unsigned long long msk, auto rowb) {
+  auto walk = [&](MaskT msk, auto rowb) {
     const R* base = contrib + kb;
     while (msk) {
-      const int a = __ffsll(static_cast<long long>(msk)) - 1;
-      msk &= msk - 1ull;
-      const bool two = msk != 0ull;
-      const int b = two ? __ffsll(static_cast<long long>(msk)) - 1 : a;
-      msk &= msk - 1ull;
+      const int a = low_bit(msk);
+      msk &= msk - 1;
+      const bool two = msk != 0;
+      const int b = two ? low_bit(msk) : a;
+      msk &= msk - 1;
       const R* sa = base + rowb(a) * TILE + a;
       const R* sb = base + rowb(b) * TILE + b;
       R va[9], vb[9];
@@ -350,7 +354,7 @@ __global__ __launch_bounds__(TILE) void tet4_assemble_rows(DevPlan p, MatParams 
   __syncthreads();
   // pass B: blocks at q = 2 (rows 0..8) and q = 3 (rows 9..17)
   if (offd) {
-    walk(m2 | m3, [&](int a) { return ((m2 >> a) & 1ull) ? 0 : 9; });
+    walk(m2 | m3, [&](int a) { return ((m2 >> a) & 1) ? 0 : 9; });
     // each 3x3 block once; a unit lane's rows are three consecutive values,
     // so neighbouring slots give contiguous runs
     const int d = s_uoff[j + 1] - s_uoff[j];
@@ -367,8 +371,12 @@ template <typename R, int TILE, int M>
 static void launch_t4(const DevPlan& p, const MatParams& mp, double* f, double* kval, hipStream_t s,
                       int32_t* status) {
   const int blocks = 8 * ((p.n_tiles + 7) / 8);
-  if (kval)
-    hipLaunchKernelGGL((tet4_assemble_rows<R, TILE, M>), dim3(blocks), dim3(TILE), 0, s, p, mp, f, kval, status);
+  if (kval && p.max_inc <= 32)
+    hipLaunchKernelGGL((tet4_assemble_rows<R, TILE, M, uint32_t>), dim3(blocks), dim3(TILE), 0, s, p, mp, f, kval,
+                       status);
+  else if (kval)
+    hipLaunchKernelGGL((tet4_assemble_rows<R, TILE, M, unsigned long long>), dim3(blocks), dim3(TILE), 0, s, p, mp,
+                       f, kval, status);
   else
     hipLaunchKernelGGL((tet4_force_kernel<R, TILE, M>), dim3(blocks), dim3(TILE), 0, s, p, mp, f, status);
 }

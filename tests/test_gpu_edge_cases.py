@@ -2,8 +2,9 @@
 
 * nodes that no element references (empty CSR rows, f = 0 exactly) and a
   one-element mesh;
-* a node shared by 64 elements, the most incidences a node may have (its
-  feed masks are 64-bit), and the plan's rejection of a 65th;
+* a node shared by 32, 33 and 64 elements (the kernel's feed masks are
+  32-bit up to 32 incidences per node, 64-bit above; 64 is the most a node
+  may have), and the plan's rejection of a 65th;
 * zero displacement: f = 0 bit-exactly and K equal to the small-strain
   stiffness of the oracle (S = 0, C = Hookean);
 * every supported tile size gives the same bits on a mesh whose node runs
@@ -76,9 +77,12 @@ def test_isolated_nodes_and_single_element(precision):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
-def test_node_with_64_incidences(precision):
-    wl = _fan(64, seed=8)
-    banner(f"edge: fan of 64 elements {precision}")
+@pytest.mark.parametrize("n_elem", [32, 33, 64])
+def test_node_with_many_incidences(precision, n_elem):
+    # 32 is the largest run the 32-bit feed masks take; 33 and 64 use the
+    # 64-bit masks
+    wl = _fan(n_elem, seed=8)
+    banner(f"edge: fan of {n_elem} elements {precision}")
     plan, _, _, _ = _check(wl, precision)
     assert plan.info["tile"] == 256
 
