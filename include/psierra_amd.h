@@ -128,6 +128,39 @@ int32_t psx_assemble_hyperelastic(psx_plan* plan, int32_t model, const double* c
                                   const double* u, double* f, double* kval, int32_t precision,
                                   void* stream, int32_t* status);
 
+/* ---- F-bar Tet4 assembly (device) ------------------------------------------
+ * Replaces Tet4Fbar's domain hooks (src/psierra/element/tet4_fbar.py:
+ * prepare_for_assembly :226-255, assemble_internal_forces :264-330,
+ * assemble_tangent_stiffness_matrix :332-459) for a Tet4 plan and the
+ * compressible Neo-Hookean material with per-element (mu, lambda).
+ *
+ * psx_fbar_create builds the pattern of K_dev + K_vol: node pairs that share
+ * an element or a neighbouring node (the nodal patches of :86-109), laid out
+ * like psx_plan_pattern (node rows contiguous, columns ascending).  The plan
+ * must outlive the F-bar object.  psx_fbar_pattern fills indptr (3N+1) and
+ * indices (nnz, psx_fbar_nnz) on the host.
+ *
+ * psx_fbar_assemble: u (3N), mu, lam (n_elements) on the device; bulk is the
+ * single K of Tet4Fbar._get_bulk_modulus (:112-118) used by K_vol.  Writes
+ * f (3N) and, when kval != NULL, every value of kval (psx_fbar_nnz) -- the
+ * reference's K_dev + K_vol with explicit zeros kept (its CSR sum drops
+ * exact-zero sums).  Element inversion as psx_assemble (element index in
+ * status[1]).  Working memory is part of the F-bar object: calls on one
+ * object must not overlap. */
+typedef struct psx_fbar psx_fbar;
+int32_t psx_fbar_create(psx_plan* plan, psx_fbar** out);
+int32_t psx_fbar_destroy(psx_fbar* fbar);
+int32_t psx_fbar_nnz(const psx_fbar* fbar, int64_t* nnz);
+int32_t psx_fbar_pattern(const psx_fbar* fbar, int32_t* indptr, int32_t* indices);
+int32_t psx_fbar_assemble(psx_fbar* fbar, const double* u, const double* mu, const double* lam, double bulk,
+                          double* f, double* kval, int32_t precision, void* stream, int32_t* status);
+
+/* Free-dof CSC over the F-bar pattern: as psx_csc_create (below), for the
+ * kval of psx_fbar_assemble (the reference's K_tan[free][:, free].tocsc(),
+ * src/psierra/analysis/nonlinear_static.py:354-355). */
+struct psx_csc;
+int32_t psx_fbar_csc_create(const psx_fbar* fbar, const int64_t* free_dofs, int64_t n_free, struct psx_csc** out);
+
 /* ---- Consistent mass matrix (device, Tet4 and Tet10, FP64) ---------------
  * Replaces SolidDomain.assemble_mass_matrix (src/psierra/domain/solid.py:
  * 208-228) with Tet4Base.compute_mass_matrix

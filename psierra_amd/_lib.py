@@ -67,6 +67,15 @@ SIGNATURES = {
     "psx_ecsw_info": (c_int32, [c_void_p, _i64p, c_int32]),
     "psx_ecsw_evaluate": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "psx_assemble_mass": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "psx_fbar_create": (c_int32, [c_void_p, POINTER(c_void_p)]),
+    "psx_fbar_destroy": (c_int32, [c_void_p]),
+    "psx_fbar_nnz": (c_int32, [c_void_p, _i64p]),
+    "psx_fbar_pattern": (c_int32, [c_void_p, c_void_p, c_void_p]),
+    "psx_fbar_csc_create": (c_int32, [c_void_p, c_void_p, c_int64, POINTER(c_void_p)]),
+    "psx_fbar_assemble": (
+        c_int32,
+        [c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_void_p, c_void_p, c_int32, c_void_p, c_void_p],
+    ),
     "psx_assemble_hyperelastic": (
         c_int32,
         [c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_void_p],

@@ -68,7 +68,7 @@ class NonlinearStaticAnalysis:
             raise ValueError("All degrees of freedom are constrained. Nothing to solve.")
         f_ext = np.zeros(n) if f_ext is None else np.asarray(f_ext, dtype=np.float64)
         ls = line_search if line_search is not None else ContactAwareLineSearch()
-        system = FreeDofSystem(plan, free)
+        system = element.free_dof_system(mesh, free) if hasattr(element, "free_dof_system") else FreeDofSystem(plan, free)
         free_t = system.free_dev
         u = torch.as_tensor(dom.u.copy(), device=dev)
         f_ext_free = torch.as_tensor(f_ext[free], device=dev)

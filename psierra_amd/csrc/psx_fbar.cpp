@@ -1,0 +1,229 @@
+// Host side of the F-bar Tet4 assembly: the distance-2 pattern of
+// K_dev + K_vol and the device layout the F-bar kernels read.
+//
+// Tet4Fbar (src/psierra/element/tet4_fbar.py:86-109) adds, for every node a,
+// the block K V0_a g_a g_a^T over the dofs of a's nodal patch (the nodes of
+// the elements around a).  Two nodes are coupled when they share an element
+// (K_dev) or a patch (K_vol), i.e. when they are at most two element edges
+// apart; that is the row set built here, per node, in ascending user id.
+#include <algorithm>
+#include <thread>
+
+#include "psx_internal.h"
+
+namespace psx {
+
+namespace {
+
+template <typename Fn>
+void parallel_nodes(int64_t n, Fn fn) {
+  unsigned T = std::min(std::max(1u, std::thread::hardware_concurrency()), 16u);
+  if (n < 4096 || T == 1) {
+    fn(int64_t(0), n);
+    return;
+  }
+  std::vector<std::thread> th;
+  const int64_t chunk = (n + T - 1) / T;
+  for (unsigned t = 0; t < T; ++t) {
+    const int64_t b = t * chunk, e = std::min<int64_t>(n, b + chunk);
+    if (b >= e) break;
+    th.emplace_back([=] { fn(b, e); });
+  }
+  for (auto& x : th) x.join();
+}
+
+template <typename T>
+int32_t to_device(psx_fbar* fb, const T* host, size_t count, T** dev) {
+  const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+  void* p = nullptr;
+  PSX_HIP_TRY(hipMalloc(&p, bytes));
+  fb->allocations.push_back(p);
+  fb->device_bytes += int64_t(bytes);
+  if (host && count) PSX_HIP_TRY(hipMemcpy(p, host, count * sizeof(T), hipMemcpyHostToDevice));
+  *dev = static_cast<T*>(p);
+  return PSX_OK;
+}
+
+void free_all(psx_fbar* fb) {
+  for (void* p : fb->allocations) (void)hipFree(p);
+  fb->allocations.clear();
+}
+
+int32_t build(psx_plan* plan, psx_fbar* fb) {
+  const HostPattern& hp = plan->hp;
+  const int64_t N = hp.N;
+  // distance-2 adjacency (user order): union of the neighbours' neighbours
+  std::vector<int32_t> deg2(N);
+  auto collect = [&](int64_t a, std::vector<int32_t>& buf) {
+    buf.clear();
+    for (int32_t i = hp.adj_ptr[a]; i < hp.adj_ptr[a + 1]; ++i) {
+      const int32_t b = hp.adj[i];
+      buf.insert(buf.end(), hp.adj.begin() + hp.adj_ptr[b], hp.adj.begin() + hp.adj_ptr[b + 1]);
+    }
+    std::sort(buf.begin(), buf.end());
+    buf.erase(std::unique(buf.begin(), buf.end()), buf.end());
+  };
+  parallel_nodes(N, [&](int64_t b, int64_t e) {
+    std::vector<int32_t> buf;
+    for (int64_t a = b; a < e; ++a) {
+      collect(a, buf);
+      deg2[a] = int32_t(buf.size());
+    }
+  });
+  fb->adj2_ptr.assign(N + 1, 0);
+  for (int64_t a = 0; a < N; ++a) fb->adj2_ptr[a + 1] = fb->adj2_ptr[a] + deg2[a];
+  fb->nnz2 = 9 * int64_t(fb->adj2_ptr[N]);
+  if (fb->nnz2 >= (int64_t(1) << 31)) return fail(PSX_EUNSUPPORTED, "F-bar nnz >= 2^31 is not supported (int32 CSR)");
+  fb->adj2.resize(fb->adj2_ptr[N]);
+  parallel_nodes(N, [&](int64_t b, int64_t e) {
+    std::vector<int32_t> buf;
+    for (int64_t a = b; a < e; ++a) {
+      collect(a, buf);
+      std::copy(buf.begin(), buf.end(), fb->adj2.begin() + fb->adj2_ptr[a]);
+    }
+  });
+
+  // internal-order layout
+  const std::vector<int32_t>& perm = plan->perm_h;   // internal -> user
+  std::vector<int32_t> inv(N);
+  for (int64_t i = 0; i < N; ++i) inv[perm[i]] = int32_t(i);
+  std::vector<int32_t> uoff(N + 1, 0), uoff2(N + 1, 0);
+  for (int64_t i = 0; i < N; ++i) {
+    const int32_t a = perm[i];
+    uoff[i + 1] = uoff[i] + (hp.adj_ptr[a + 1] - hp.adj_ptr[a]);
+    uoff2[i + 1] = uoff2[i] + deg2[a];
+  }
+  std::vector<int32_t> nbr1_user(uoff[N]), nbr1_int(uoff[N]), slot2(uoff[N]), nbr2_user(uoff2[N]);
+  std::vector<int4> rec2(N);
+  parallel_nodes(N, [&](int64_t b, int64_t e) {
+    for (int64_t i = b; i < e; ++i) {
+      const int32_t a = perm[i];
+      const int32_t* l1 = hp.adj.data() + hp.adj_ptr[a];
+      const int32_t n1 = hp.adj_ptr[a + 1] - hp.adj_ptr[a];
+      const int32_t* l2 = fb->adj2.data() + fb->adj2_ptr[a];
+      const int32_t n2 = deg2[a];
+      int32_t self2 = 0;
+      for (int32_t s = 0, t = 0; s < n1; ++s) {
+        while (l2[t] < l1[s]) ++t;   // l1 is a subset of l2, both ascending
+        nbr1_user[uoff[i] + s] = l1[s];
+        nbr1_int[uoff[i] + s] = inv[l1[s]];
+        slot2[uoff[i] + s] = t;
+        if (l1[s] == a) self2 = t;
+      }
+      std::copy(l2, l2 + n2, nbr2_user.begin() + uoff2[i]);
+      rec2[i] = make_int4(int32_t(9 * int64_t(fb->adj2_ptr[a])), n2, self2, 0);
+    }
+  });
+
+  FbarDev& d = fb->d;
+  int32_t* p1u = nullptr;
+  int32_t* p1i = nullptr;
+  int32_t* pu2 = nullptr;
+  int32_t* pn2 = nullptr;
+  int32_t* ps2 = nullptr;
+  int4* pr2 = nullptr;
+  int32_t rc;
+  if ((rc = to_device(fb, nbr1_user.data(), nbr1_user.size(), &p1u))) return rc;
+  if ((rc = to_device(fb, nbr1_int.data(), nbr1_int.size(), &p1i))) return rc;
+  if ((rc = to_device(fb, uoff2.data(), uoff2.size(), &pu2))) return rc;
+  if ((rc = to_device(fb, nbr2_user.data(), nbr2_user.size(), &pn2))) return rc;
+  if ((rc = to_device(fb, slot2.data(), slot2.size(), &ps2))) return rc;
+  if ((rc = to_device(fb, rec2.data(), rec2.size(), &pr2))) return rc;
+  d.nbr1_user = p1u;
+  d.nbr1_int = p1i;
+  d.uoff2 = pu2;
+  d.nbr2_user = pn2;
+  d.slot2 = ps2;
+  d.node_rec2 = pr2;
+  const size_t E = size_t(hp.E);
+  if ((rc = to_device<double>(fb, nullptr, E, &d.qv))) return rc;
+  if ((rc = to_device<double>(fb, nullptr, E, &d.vjm1))) return rc;
+  if ((rc = to_device<double>(fb, nullptr, 12 * E, &d.wq))) return rc;
+  if ((rc = to_device<double>(fb, nullptr, size_t(N), &d.v0))) return rc;
+  if ((rc = to_device<double>(fb, nullptr, size_t(N), &d.jm1))) return rc;
+  if ((rc = to_device<double>(fb, nullptr, 3 * size_t(uoff[N]), &d.g))) return rc;
+  return PSX_OK;
+}
+
+}  // namespace
+
+}  // namespace psx
+
+extern "C" {
+
+int32_t psx_fbar_create(psx_plan* plan, psx_fbar** out) {
+  psx::clear_error();
+  if (!plan || !out) return psx::fail(PSX_EINVAL, "plan/out is null");
+  if (plan->hp.npe != 4) return psx::fail(PSX_EUNSUPPORTED, "F-bar needs a Tet4 plan");
+  psx_fbar* fb = new psx_fbar();
+  fb->plan = plan;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  (void)hipSetDevice(plan->device);
+  const int32_t rc = psx::build(plan, fb);
+  (void)hipSetDevice(dev);
+  if (rc) {
+    psx::free_all(fb);
+    delete fb;
+    return rc;
+  }
+  *out = fb;
+  return PSX_OK;
+}
+
+int32_t psx_fbar_destroy(psx_fbar* fbar) {
+  psx::clear_error();
+  if (!fbar) return PSX_OK;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  (void)hipSetDevice(fbar->plan->device);
+  psx::free_all(fbar);
+  (void)hipSetDevice(dev);
+  delete fbar;
+  return PSX_OK;
+}
+
+int32_t psx_fbar_nnz(const psx_fbar* fbar, int64_t* nnz) {
+  psx::clear_error();
+  if (!fbar || !nnz) return psx::fail(PSX_EINVAL, "null argument");
+  *nnz = fbar->nnz2;
+  return PSX_OK;
+}
+
+int32_t psx_fbar_pattern(const psx_fbar* fbar, int32_t* indptr, int32_t* indices) {
+  psx::clear_error();
+  if (!fbar || !indptr || !indices) return psx::fail(PSX_EINVAL, "null argument");
+  const int64_t N = fbar->plan->hp.N;
+  psx::parallel_nodes(N, [&](int64_t b, int64_t e) {
+    for (int64_t a = b; a < e; ++a) {
+      const int32_t d = fbar->adj2_ptr[a + 1] - fbar->adj2_ptr[a];
+      const int64_t base = 9 * int64_t(fbar->adj2_ptr[a]);
+      for (int r = 0; r < 3; ++r) {
+        indptr[3 * a + r] = int32_t(base + 3 * r * d);
+        int32_t* o = indices + base + 3 * r * d;
+        for (int32_t s = 0; s < d; ++s)
+          for (int c = 0; c < 3; ++c) o[3 * s + c] = 3 * fbar->adj2[fbar->adj2_ptr[a] + s] + c;
+      }
+    }
+  });
+  indptr[3 * N] = int32_t(fbar->nnz2);
+  return PSX_OK;
+}
+
+int32_t psx_fbar_csc_create(const psx_fbar* fbar, const int64_t* free_dofs, int64_t n_free, psx_csc** out) {
+  psx::clear_error();
+  if (!fbar) return psx::fail(PSX_EINVAL, "fbar is null");
+  return psx::create_csc_adj(fbar->adj2_ptr, fbar->adj2, fbar->plan->hp.N, free_dofs, n_free, out);
+}
+
+int32_t psx_fbar_assemble(psx_fbar* fbar, const double* u, const double* mu, const double* lam, double bulk,
+                          double* f, double* kval, int32_t precision, void* stream, int32_t* status) {
+  psx::clear_error();
+  if (!fbar || !u || !mu || !lam || !f || !status) return psx::fail(PSX_EINVAL, "null argument to psx_fbar_assemble");
+  if (precision != PSX_FP64 && precision != PSX_FP32)
+    return psx::fail(PSX_EINVAL, "precision must be PSX_FP64 or PSX_FP32");
+  return psx::launch_tet4_fbar(fbar->plan->d, fbar->d, u, mu, lam, bulk, f, kval, precision,
+                               static_cast<hipStream_t>(stream), status);
+}
+
+}  // extern "C"

@@ -50,6 +50,15 @@ struct HostPattern {
 
 int32_t build_host_pattern(const int64_t* conn, int64_t E, int npe, int64_t N, HostPattern& out);
 void fill_csr(const HostPattern& hp, int32_t* indptr, int32_t* indices);
+}  // namespace psx
+
+struct psx_csc;
+
+namespace psx {
+// Free-dof CSC of the node-block CSR whose node rows are given by an
+// adjacency (user ids ascending per node, node a's row contiguous).
+int32_t create_csc_adj(const std::vector<int32_t>& adj_ptr, const std::vector<int32_t>& adj, int64_t N,
+                       const int64_t* free_dofs, int64_t n_free, psx_csc** out);
 
 // Device view of a plan, passed by value to kernels.
 struct DevPlan {
@@ -74,6 +83,31 @@ struct DevPlan {
   double4* wi;                    // scratch: adjoint cotangent in internal order
 };
 
+// F-bar store layout in the distance-2 pattern of K_dev + K_vol (device view):
+// per internal node {first value, row length in blocks, diagonal slot, 0},
+// per (node, slot) unit of the element pattern its slot in the wider row.
+struct FbarView {
+  const int4* node_rec2 = nullptr;
+  const int32_t* slot2 = nullptr;
+};
+
+// F-bar working set (device): the wider pattern's layout and the per-call
+// element and node quantities of the nodal volume averaging.
+struct FbarDev {
+  const int32_t* nbr1_user = nullptr;  // per element-pattern unit: neighbour's user id (slot order)
+  const int32_t* nbr1_int = nullptr;   // per element-pattern unit: neighbour's internal id
+  const int32_t* uoff2 = nullptr;      // N+1: prefix of distance-2 row lengths, internal order
+  const int32_t* nbr2_user = nullptr;  // per distance-2 unit: user id (ascending per row)
+  const int4* node_rec2 = nullptr;     // N: {first value, row length, diagonal slot, 0}
+  const int32_t* slot2 = nullptr;      // per element-pattern unit: slot in the distance-2 row
+  double* qv = nullptr;                // per element: V / 4
+  double* vjm1 = nullptr;              // per element: V (J - 1) / 4
+  double* wq = nullptr;                // per element: 12 values V J / 4 F h_b
+  double* v0 = nullptr;                // per internal node: nodal reference volume
+  double* jm1 = nullptr;               // per internal node: J_node - 1
+  double* g = nullptr;                 // per element-pattern unit: volumetric gradient (3)
+};
+
 // Structural CSC of K[free][:, free] (device view).
 struct DevCsc {
   int32_t n_free;
@@ -91,6 +125,15 @@ struct psx_csc {
   int device = 0;
   int64_t n_free = 0, nnz_struct = 0;
   psx::DevCsc d{};
+  std::vector<void*> allocations;
+};
+
+struct psx_fbar {
+  psx_plan* plan = nullptr;            // borrowed: must outlive this object
+  std::vector<int32_t> adj2_ptr, adj2; // distance-2 adjacency, user order
+  int64_t nnz2 = 0;
+  int64_t device_bytes = 0;
+  psx::FbarDev d{};
   std::vector<void*> allocations;
 };
 
@@ -114,6 +157,13 @@ int32_t launch_tet4_hyper(const DevPlan& p, int model, const double* const* para
                           double* f, double* kval, int precision, hipStream_t s, int32_t* status);
 int32_t launch_tet4_assemble(const DevPlan& p, const double* u, const double* mu, const double* lam,
                              double* f, double* kval, int precision, hipStream_t s, int32_t* status);
+int32_t launch_t4_fbar_rows(const DevPlan& p, const double* mu, const double* lam, const double* jm1_node,
+                            const FbarView& fv, double* f, double* kval, int precision, hipStream_t s,
+                            int32_t* status);
+int32_t launch_permute_u(const DevPlan& p, const double* u, hipStream_t s, int32_t* status);
+int32_t launch_tet4_fbar(const DevPlan& p, const FbarDev& fb, const double* u, const double* mu,
+                         const double* lam, double bulk, double* f, double* kval, int precision, hipStream_t s,
+                         int32_t* status);
 int32_t launch_tet10_assemble(const DevPlan& p, const double* u, const double* mu, const double* lam,
                               double* f, double* kval, int precision, hipStream_t s, int32_t* status);
 int32_t launch_tet4_vjp(const DevPlan& p, const double* u, const double* mu, const double* lam,

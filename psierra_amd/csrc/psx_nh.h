@@ -207,7 +207,7 @@ __device__ __forceinline__ void nh_point(const R A[3][3], R mu, R K, NHPoint<R>&
 // and the Mooney-Rivlin stress adds 2 c' C^-1 (2 dev E + 4 tr(E) E - 4 E^2
 // - 4/3 (tr(E)^2 - tr(E^2)) I), all formed from E so small strains keep
 // their relative accuracy.
-enum HyperModel { kNeoHookean = 0, kMooneyRivlin = 1, kYeoh = 2 };
+enum HyperModel { kNeoHookean = 0, kMooneyRivlin = 1, kYeoh = 2, kFbarNeoHookean = 3 };
 
 template <typename R>
 struct HyperExtra {

@@ -545,11 +545,15 @@ int32_t create_plan(const int64_t* conn, int64_t E, int npe, const double* X, in
   return PSX_OK;
 }
 
-int32_t create_csc(const psx_plan* plan, const int64_t* free_dofs, int64_t n_free, psx_csc** out) {
+int32_t create_csc_adj(const std::vector<int32_t>& adj_ptr, const std::vector<int32_t>& adj, int64_t N,
+                       const int64_t* free_dofs, int64_t n_free, psx_csc** out) {
   if (!out) return fail(PSX_EINVAL, "out is null");
   *out = nullptr;
-  const HostPattern& hp = plan->hp;
-  const int64_t n = 3 * hp.N;
+  struct {
+    const std::vector<int32_t>& adj_ptr;
+    const std::vector<int32_t>& adj;
+  } hp{adj_ptr, adj};
+  const int64_t n = 3 * N;
   if (n_free < 0 || n_free > n || (n_free > 0 && !free_dofs))
     return fail(PSX_EINVAL, "bad free dof list");
   std::vector<int32_t> newidx(n, -1), freev(n_free);
@@ -613,6 +617,10 @@ int32_t create_csc(const psx_plan* plan, const int64_t* free_dofs, int64_t n_fre
   csc->d.partial = static_cast<double*>(dpart);
   *out = csc;
   return PSX_OK;
+}
+
+int32_t create_csc(const psx_plan* plan, const int64_t* free_dofs, int64_t n_free, psx_csc** out) {
+  return create_csc_adj(plan->hp.adj_ptr, plan->hp.adj, plan->hp.N, free_dofs, n_free, out);
 }
 
 int32_t retile(psx_plan* plan, int32_t tile) {

@@ -378,7 +378,6 @@ __global__ __launch_bounds__(TILE) void tet10_assemble_rows(DevPlan p, const dou
   }
 }
 
-int32_t launch_permute_u(const DevPlan& p, const double* u, hipStream_t s, int32_t* status);
 
 template <typename R, int TILE>
 static void launch_t10(const DevPlan& p, const double* mu, const double* lam, double* f, double* kval,

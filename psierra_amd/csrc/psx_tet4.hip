@@ -54,7 +54,7 @@ __device__ __forceinline__ void flag_inversion(int32_t* status, int e) {
 
 // Per-element material parameter arrays (device, one value per element):
 // Neo-Hookean (mu, lambda); Mooney-Rivlin (c10, c01, K); Yeoh (c10, c20,
-// c30, K).
+// c30, K); F-bar Neo-Hookean (mu, lambda, and J_node - 1 per internal node).
 struct MatParams {
   const double* v[4];
 };
@@ -63,7 +63,7 @@ struct MatParams {
 // mu and K = lambda + 2 mu / 3).
 template <int M>
 __device__ __forceinline__ void load_params(const MatParams& mp, int64_t e, double pr[4]) {
-  if constexpr (M == kNeoHookean) {
+  if constexpr (M == kNeoHookean || M == kFbarNeoHookean) {
     pr[0] = mp.v[0][e];
     pr[1] = mp.v[1][e] + (2.0 / 3.0) * pr[0];
     pr[2] = pr[3] = 0.0;
@@ -92,6 +92,12 @@ __device__ __forceinline__ void load_incidence(const DevPlan& p, const MatParams
   const int64_t e = int64_t(in.pe & kIncElemMask);
   const int4 c = *reinterpret_cast<const int4*>(p.conn + 4 * e);
   load_params<M>(mp, e, in.pr);
+  if constexpr (M == kFbarNeoHookean) {
+    // J_bar - 1: mean over the element's nodes in element order
+    // (np.mean(J_node[elements], axis=1), tet4_fbar.py:396)
+    const double* jn = mp.v[2];
+    in.pr[2] = (((jn[c.x] + jn[c.y]) + jn[c.z]) + jn[c.w]) * 0.25;
+  }
   const int lp = int(in.pe >> kIncElemBits);
   const int r1 = lp == 0 ? 1 : 0, r2 = lp <= 1 ? 2 : 1, r3 = lp <= 2 ? 3 : 2;
   const int i0 = lp == 0 ? c.x : lp == 1 ? c.y : lp == 2 ? c.z : c.w;
@@ -188,13 +194,27 @@ __device__ __forceinline__ double oct_sum(double x, bool upper_quad) {
 // Feed masks hold one bit per incidence of the node's run: MaskT is uint32_t
 // when no node has more than 32 incidences (smaller LDS and cheaper bit
 // scans), unsigned long long otherwise.
+//
+// F-bar (M = kFbarNeoHookean) writes into the wider pattern of K_dev + K_vol:
+// fv gives each node's first value, row length and diagonal slot there, and
+// each (node, slot) unit its slot in that row; the blocks are added to the
+// K_vol values already stored.
 template <typename R, int TILE, int M, typename MaskT>
-__global__ __launch_bounds__(TILE) void tet4_assemble_rows(DevPlan p, MatParams mp, double* __restrict__ f,
-                                                           double* __restrict__ kval, int32_t* status) {
+__global__ __launch_bounds__(TILE) void tet4_assemble_rows(DevPlan p, MatParams mp, FbarView fv,
+                                                           double* __restrict__ f, double* __restrict__ kval,
+                                                           int32_t* status) {
+  constexpr bool FBAR = M == kFbarNeoHookean;
   constexpr int NV = 18;
   constexpr int ND = kTet4DiagLanes;
   constexpr int NN = TILE / ND;
   constexpr int NW = TILE / 64;
+  // store of one CSR value: a plain write, or an add onto K_vol (F-bar)
+  auto put = [](double* dst, double v) {
+    if constexpr (FBAR)
+      *dst += v;
+    else
+      *dst = v;
+  };
   __shared__ R contrib[NV * TILE];
   __shared__ MaskT s_mask[3 * TILE];
   __shared__ unsigned long long s_ustart[NW];
@@ -204,6 +224,7 @@ __global__ __launch_bounds__(TILE) void tet4_assemble_rows(DevPlan p, MatParams 
   __shared__ uint8_t s_self[NN];
   __shared__ uint8_t s_nz[NN];
   __shared__ uint8_t s_first[TILE];
+  __shared__ int32_t s_f2[FBAR ? 2 * NN : 1];   // F-bar: diagonal block offset, row length
 
   const int tile = xcd_tile();
   if (tile >= p.n_tiles) return;
@@ -223,6 +244,12 @@ __global__ __launch_bounds__(TILE) void tet4_assemble_rows(DevPlan p, MatParams 
     s_uoff[t] = r0.y;
     s_kofs[t] = r0.z;
     s_self[t] = uint8_t(r0.w);
+    if constexpr (FBAR) {
+      const int4 r2 = fv.node_rec2[n0 + t];
+      s_kofs[t] = r2.x;
+      s_f2[t] = r2.x + 3 * r2.z;
+      s_f2[NN + t] = r2.y;
+    }
     if (t == nn - 1) {
       s_incp[nn] = ke;
       s_uoff[nn] = r1.y;
@@ -330,7 +357,7 @@ __global__ __launch_bounds__(TILE) void tet4_assemble_rows(DevPlan p, MatParams 
 #pragma unroll
     for (int c = 0; c < 9; ++c) acc[c] = oct_sum(acc[c], upper);
     const int part = int(unsigned(t) % ND);
-    const int d = s_uoff[j + 1] - s_uoff[j];
+    const int d = FBAR ? s_f2[NN + j] : s_uoff[j + 1] - s_uoff[j];
     if (part == 3) {
       const int64_t a = p.perm[n0 + j];
 #pragma unroll
@@ -340,10 +367,10 @@ __global__ __launch_bounds__(TILE) void tet4_assemble_rows(DevPlan p, MatParams 
       const double e0 = part == 0 ? acc[0] : part == 1 ? acc[1] : acc[2];
       const double e1 = part == 0 ? acc[1] : part == 1 ? acc[3] : acc[4];
       const double e2 = part == 0 ? acc[2] : part == 1 ? acc[4] : acc[5];
-      double* dst = kval + s_kofs[j] + part * 3 * d + 3 * int(s_self[j]);
-      dst[0] = e0;
-      dst[1] = e1;
-      dst[2] = e2;
+      double* dst = kval + (FBAR ? s_f2[j] : s_kofs[j] + 3 * int(s_self[j])) + part * 3 * d;
+      put(dst, e0);
+      put(dst + 1, e1);
+      put(dst + 2, e2);
     }
   }
   __syncthreads();   // pass A reads are done
@@ -357,48 +384,62 @@ __global__ __launch_bounds__(TILE) void tet4_assemble_rows(DevPlan p, MatParams 
     walk(m2 | m3, [&](int a) { return ((m2 >> a) & 1) ? 0 : 9; });
     // each 3x3 block once; a unit lane's rows are three consecutive values,
     // so neighbouring slots give contiguous runs
-    const int d = s_uoff[j + 1] - s_uoff[j];
-    double* dst = kval + s_kofs[j] + 3 * slot;
+    const int d = FBAR ? s_f2[NN + j] : s_uoff[j + 1] - s_uoff[j];
+    const int col = FBAR ? fv.slot2[s_uoff[j] + slot] : slot;
+    double* dst = kval + s_kofs[j] + 3 * col;
 #pragma unroll
     for (int r = 0; r < 3; ++r)
 #pragma unroll
-      for (int c = 0; c < 3; ++c) dst[r * 3 * d + c] = acc[3 * r + c];
+      for (int c = 0; c < 3; ++c) put(dst + r * 3 * d + c, acc[3 * r + c]);
   }
 
 }
 
 template <typename R, int TILE, int M>
-static void launch_t4(const DevPlan& p, const MatParams& mp, double* f, double* kval, hipStream_t s,
-                      int32_t* status) {
+static void launch_t4(const DevPlan& p, const MatParams& mp, const FbarView& fv, double* f, double* kval,
+                      hipStream_t s, int32_t* status) {
   const int blocks = 8 * ((p.n_tiles + 7) / 8);
   if (kval && p.max_inc <= 32)
-    hipLaunchKernelGGL((tet4_assemble_rows<R, TILE, M, uint32_t>), dim3(blocks), dim3(TILE), 0, s, p, mp, f, kval,
-                       status);
+    hipLaunchKernelGGL((tet4_assemble_rows<R, TILE, M, uint32_t>), dim3(blocks), dim3(TILE), 0, s, p, mp, fv, f,
+                       kval, status);
   else if (kval)
     hipLaunchKernelGGL((tet4_assemble_rows<R, TILE, M, unsigned long long>), dim3(blocks), dim3(TILE), 0, s, p, mp,
-                       f, kval, status);
+                       fv, f, kval, status);
   else
     hipLaunchKernelGGL((tet4_force_kernel<R, TILE, M>), dim3(blocks), dim3(TILE), 0, s, p, mp, f, status);
 }
 
 template <int M>
-static int32_t launch_t4_tiles(const DevPlan& p, const MatParams& mp, double* f, double* kval, bool fp32,
-                               hipStream_t s, int32_t* status) {
+static int32_t launch_t4_tiles(const DevPlan& p, const MatParams& mp, const FbarView& fv, double* f, double* kval,
+                               bool fp32, hipStream_t s, int32_t* status) {
   switch (p.tile) {
     case 64:
-      fp32 ? launch_t4<float, 64, M>(p, mp, f, kval, s, status) : launch_t4<double, 64, M>(p, mp, f, kval, s, status);
+      fp32 ? launch_t4<float, 64, M>(p, mp, fv, f, kval, s, status)
+           : launch_t4<double, 64, M>(p, mp, fv, f, kval, s, status);
       break;
     case 128:
-      fp32 ? launch_t4<float, 128, M>(p, mp, f, kval, s, status)
-           : launch_t4<double, 128, M>(p, mp, f, kval, s, status);
+      fp32 ? launch_t4<float, 128, M>(p, mp, fv, f, kval, s, status)
+           : launch_t4<double, 128, M>(p, mp, fv, f, kval, s, status);
       break;
     case 256:
-      fp32 ? launch_t4<float, 256, M>(p, mp, f, kval, s, status)
-           : launch_t4<double, 256, M>(p, mp, f, kval, s, status);
+      fp32 ? launch_t4<float, 256, M>(p, mp, fv, f, kval, s, status)
+           : launch_t4<double, 256, M>(p, mp, fv, f, kval, s, status);
       break;
     default:
       return fail(PSX_EINVAL, "unsupported tile");
   }
+  return PSX_OK;
+}
+
+// F-bar rows (K_dev added onto the K_vol values in kval, and f) or f only.
+int32_t launch_t4_fbar_rows(const DevPlan& p, const double* mu, const double* lam, const double* jm1_node,
+                            const FbarView& fv, double* f, double* kval, int precision, hipStream_t s,
+                            int32_t* status) {
+  if (p.n_tiles <= 0) return PSX_OK;
+  const MatParams mp{{mu, lam, jm1_node, nullptr}};
+  const int32_t rc = launch_t4_tiles<kFbarNeoHookean>(p, mp, fv, f, kval, precision == PSX_FP32, s, status);
+  if (rc) return rc;
+  PSX_HIP_TRY(hipGetLastError());
   return PSX_OK;
 }
 
@@ -416,9 +457,9 @@ int32_t launch_tet4_hyper(const DevPlan& p, int model, const double* const* para
   const bool fp32 = precision == PSX_FP32;
   const MatParams mp{{params[0], params[1], params[2], params[3]}};
   switch (model) {
-    case kNeoHookean: rc = launch_t4_tiles<kNeoHookean>(p, mp, f, kval, fp32, s, status); break;
-    case kMooneyRivlin: rc = launch_t4_tiles<kMooneyRivlin>(p, mp, f, kval, fp32, s, status); break;
-    case kYeoh: rc = launch_t4_tiles<kYeoh>(p, mp, f, kval, fp32, s, status); break;
+    case kNeoHookean: rc = launch_t4_tiles<kNeoHookean>(p, mp, FbarView{}, f, kval, fp32, s, status); break;
+    case kMooneyRivlin: rc = launch_t4_tiles<kMooneyRivlin>(p, mp, FbarView{}, f, kval, fp32, s, status); break;
+    case kYeoh: rc = launch_t4_tiles<kYeoh>(p, mp, FbarView{}, f, kval, fp32, s, status); break;
     default: return fail(PSX_EINVAL, "unknown material model");
   }
   if (rc) return rc;

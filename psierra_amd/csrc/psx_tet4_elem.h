@@ -40,25 +40,73 @@ __device__ __forceinline__ void tet4_geometry(const double4 X[4], const double4 
     for (int j = 0; j < 3; ++j) A[i][j] = dU[i][0] * h[1][j] + dU[i][1] * h[2][j] + dU[i][2] * h[3][j];
 }
 
-// Row 0 of an element, given the shape-function gradients in the row's node
-// order (h[0] = row node), the volume V and grad u: the four 3x3 blocks
-// K_{0b} (b = 0..3, row-major, 36 values) and f_0 (3 values), or f_0 only.
-// Returns J = det F.
-// prm: the model's parameters (kNeoHookean: mu, K = lambda + 2 mu / 3).
-template <typename R, int M, bool WANT_K>
-__device__ __forceinline__ R tet4_row_core_m(const R h[4][3], R V, const R A[3][3], const R prm[4], R* out) {
+// F-bar row (Tet4Fbar's domain assembly, src/psierra/element/tet4_fbar.py:
+// 264-459): Neo-Hookean at F_bar = r F, r = (J_bar / J)^(1/3), with
+// prm = (mu, K, J_bar - 1).  With g = F^-T h, q = F h, binv = F^-T F^-1,
+// T = binv g, g1 = g - tr(C^-1)/3 q and the F_bar material point
+// (fb = mu J_bar^-2/3, I1_bar = r^2 I1, c1 = K + 2/9 fb I1_bar,
+//  c2 = -2 K ln J_bar + 2/3 fb I1_bar, S_bar = C^-1 (2 fb dev E + K ln J_bar / r^2 I)):
+//   f_a  = V [ r^2 F S_bar h_a + (1 - r^2) tr(S_bar)/3 q_a ]
+//   K_ab = V [ c1 g1_a g1_b^T + c2 ((g_b g_a^T + (g_a . g_b) I)/2
+//              - (T_a q_b^T + q_a T_b^T)/3 + tr(C^-2)/9 q_a q_b^T)
+//              + (h_a . S_bar h_b) I ].
+// This is the reference's (r^2 B_dev)^T (C(F_bar) - K_bulk m m^T) (r^2 B_dev) V
+// + G^T (I x S_bar) G V: the deviatoric strain rows have zero trace, so the
+// K_bulk m m^T term and the C^-1 (x) I terms of C(F_bar) drop out, and the r^4
+// of B_bar cancels the r^-4 that C(F_bar) carries through C_bar^-1 = C^-1 / r^2.
+// f_a is B_bar^T S_bar V with B_bar = r^2 B_dev + m b_vol^T.  The nodal
+// volumetric stiffness K_vol is assembled by its own kernel.  S_bar is formed
+// from E (dev E_bar = r^2 dev E), so small strains keep their relative accuracy.
+template <typename R, bool WANT_K>
+__device__ __forceinline__ R tet4_fbar_row_core(const R h[4][3], R V, const R A[3][3], const R prm[4], R* out) {
   NHPoint<R> m;
-  HyperExtra<R> x;
-  hyper_point<R, M>(A, prm, m, x);
-  R Sh0[3];
-  mat3vec(m.S, h[0], Sh0);
-  // f_0 = V F S h_0
-  R FSh[3];
-  mat3vec(m.F, Sh0, FSh);
+  nh_point(A, prm[0], prm[1], m);   // kinematics (F, F^-T, C^-1, E, J, ln J)
+  const R mu = prm[0], K = prm[1];
+  const R lnJb = d_log1p(prm[2]);
+  const R r2 = d_exp(R(2.0 / 3.0) * (lnJb - m.lnJ));
+  const R fb = mu * d_exp(R(-2.0 / 3.0) * lnJb);
+  const R vol = K * lnJb * d_rcp(r2);
+  R Sb[3][3], S[3][3];
 #pragma unroll
-  for (int i = 0; i < 3; ++i) out[(WANT_K ? 36 : 0) + i] = V * FSh[i];
+  for (int I = 0; I < 3; ++I)
+#pragma unroll
+    for (int L = 0; L < 3; ++L)
+      Sb[I][L] = R(2) * fb * (m.E[I][L] - (I == L ? m.trE * R(1.0 / 3.0) : R(0))) + (I == L ? vol : R(0));
+#pragma unroll
+  for (int I = 0; I < 3; ++I)
+#pragma unroll
+    for (int L = I; L < 3; ++L) {
+      const R v = m.Ci[I][0] * Sb[0][L] + m.Ci[I][1] * Sb[1][L] + m.Ci[I][2] * Sb[2][L];
+      S[I][L] = v;
+      S[L][I] = v;
+    }
+  const R trS3 = (S[0][0] + S[1][1] + S[2][2]) * R(1.0 / 3.0);
+  R Sh0[3], FSh[3], q0[3];
+  mat3vec(S, h[0], Sh0);
+  mat3vec(m.F, Sh0, FSh);
+  mat3vec(m.F, h[0], q0);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) out[(WANT_K ? 36 : 0) + i] = V * (r2 * FSh[i] + (R(1) - r2) * trS3 * q0[i]);
   if (WANT_K) {
-    R g[4][3], q[4][3];
+    const R I1b = r2 * (R(3) + R(2) * m.trE);
+    const R c1 = K + R(2.0 / 9.0) * fb * I1b;
+    const R c2 = R(-2) * K * lnJb + R(2.0 / 3.0) * fb * I1b;
+    const R trCi3 = (m.Ci[0][0] + m.Ci[1][1] + m.Ci[2][2]) * R(1.0 / 3.0);
+    R trC2 = R(0);
+#pragma unroll
+    for (int I = 0; I < 3; ++I)
+#pragma unroll
+      for (int L = 0; L < 3; ++L) trC2 += m.Ci[I][L] * m.Ci[I][L];
+    R binv[3][3];   // F^-T F^-1 = Gm Gm^T
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int k = i; k < 3; ++k) {
+        const R v = m.Gm[i][0] * m.Gm[k][0] + m.Gm[i][1] * m.Gm[k][1] + m.Gm[i][2] * m.Gm[k][2];
+        binv[i][k] = v;
+        binv[k][i] = v;
+      }
+    R g[4][3], q[4][3], T[4][3], g1[4][3];
 #pragma unroll
     for (int b = 1; b < 4; ++b) {
       mat3vec(m.Gm, h[b], g[b]);
@@ -69,63 +117,132 @@ __device__ __forceinline__ R tet4_row_core_m(const R h[4][3], R V, const R A[3][
       g[0][c] = -(g[1][c] + g[2][c] + g[3][c]);
       q[0][c] = -(q[1][c] + q[2][c] + q[3][c]);
     }
-    // K_0b = P g_b^T + g_b Rv^T + T q_b^T + (g_b . Rv + h_b . W) I
-    R P[3], Rv[3], T[3], W[3];
-    const R va = V * m.al, vb = R(0.5) * V * m.be, vg = V * m.ga;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      mat3vec(binv, g[b], T[b]);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) g1[b][c] = g[b][c] - trCi3 * q[b][c];
+    }
+    // K_0b = P1 g1_b^T + g_b Rv^T + P2 q_b^T + Q0 T_b^T + (g_b . Rv + h_b . W) I
+    R P1[3], Rv[3], P2[3], Q0[3], W[3];
+    const R v1 = V * c1, v2 = V * c2;
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-      P[i] = va * g[0][i] + vg * q[0][i];
-      Rv[i] = vb * g[0][i];
-      T[i] = vg * g[0][i];
+      P1[i] = v1 * g1[0][i];
+      Rv[i] = R(0.5) * v2 * g[0][i];
+      P2[i] = v2 * (trC2 * R(1.0 / 9.0) * q[0][i] - R(1.0 / 3.0) * T[0][i]);
+      Q0[i] = R(-1.0 / 3.0) * v2 * q[0][i];
       W[i] = V * Sh0[i];
-    }
-    // model-specific vectors of the row node (see psx_nh.h)
-    R Qm[3] = {R(0), R(0), R(0)}, Pm[3] = {R(0), R(0), R(0)}, Z0[3] = {R(0), R(0), R(0)};
-    if constexpr (M == kMooneyRivlin) {
-      R bq0[3];
-      mat3vec(x.b, q[0], bq0);
-#pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        Qm[i] = V * x.cq * q[0][i];                          // times q_b^T
-        Pm[i] = V * x.cp * (x.I1 * q[0][i] - bq0[i]);        // p_0, times g_b^T
-      }
-    } else if constexpr (M == kYeoh) {
-#pragma unroll
-      for (int i = 0; i < 3; ++i) Z0[i] = V * x.psi11 * x.zs * (q[0][i] - x.I1 * R(1.0 / 3.0) * g[0][i]);
     }
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
       const R sb = g[b][0] * Rv[0] + g[b][1] * Rv[1] + g[b][2] * Rv[2] + h[b][0] * W[0] + h[b][1] * W[1] +
                    h[b][2] * W[2];
-      R pb[3] = {R(0), R(0), R(0)}, zb[3] = {R(0), R(0), R(0)};
-      R hh = R(0);
-      if constexpr (M == kMooneyRivlin) {
-        R bqb[3];
-        mat3vec(x.b, q[b], bqb);
-#pragma unroll
-        for (int i = 0; i < 3; ++i) pb[i] = x.I1 * q[b][i] - bqb[i];
-        hh = V * x.cb * (h[0][0] * h[b][0] + h[0][1] * h[b][1] + h[0][2] * h[b][2]);
-      } else if constexpr (M == kYeoh) {
-#pragma unroll
-        for (int i = 0; i < 3; ++i) zb[i] = x.zs * (q[b][i] - x.I1 * R(1.0 / 3.0) * g[b][i]);
-      }
       R* o = out + 9 * b;
 #pragma unroll
       for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int k = (b == 0 ? i : 0); k < 3; ++k) {
-          R v = P[i] * g[b][k] + g[b][i] * Rv[k] + T[i] * q[b][k] + (i == k ? sb : R(0));
-          if constexpr (M == kMooneyRivlin)
-            v += Qm[i] * q[b][k] + hh * x.b[i][k] + V * x.cb * q[b][i] * q[0][k] +
-                 V * x.cp * g[0][i] * pb[k] + Pm[i] * g[b][k];
-          else if constexpr (M == kYeoh)
-            v += Z0[i] * zb[k];
+          const R v = P1[i] * g1[b][k] + g[b][i] * Rv[k] + P2[i] * q[b][k] + Q0[i] * T[b][k] + (i == k ? sb : R(0));
           o[3 * i + k] = v;
-          if (b == 0 && k != i) o[3 * k + i] = v;  // K_00 is symmetric
+          if (b == 0 && k != i) o[3 * k + i] = v;   // K_00 is symmetric
         }
     }
   }
   return m.J;
+}
+
+// Row 0 of an element, given the shape-function gradients in the row's node
+// order (h[0] = row node), the volume V and grad u: the four 3x3 blocks
+// K_{0b} (b = 0..3, row-major, 36 values) and f_0 (3 values), or f_0 only.
+// Returns J = det F.
+// prm: the model's parameters (kNeoHookean: mu, K = lambda + 2 mu / 3;
+// kFbarNeoHookean: mu, K, J_bar - 1).
+template <typename R, int M, bool WANT_K>
+__device__ __forceinline__ R tet4_row_core_m(const R h[4][3], R V, const R A[3][3], const R prm[4], R* out) {
+  if constexpr (M == kFbarNeoHookean) {
+    return tet4_fbar_row_core<R, WANT_K>(h, V, A, prm, out);
+  } else {
+    NHPoint<R> m;
+    HyperExtra<R> x;
+    hyper_point<R, M>(A, prm, m, x);
+    R Sh0[3];
+    mat3vec(m.S, h[0], Sh0);
+    // f_0 = V F S h_0
+    R FSh[3];
+    mat3vec(m.F, Sh0, FSh);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) out[(WANT_K ? 36 : 0) + i] = V * FSh[i];
+    if (WANT_K) {
+      R g[4][3], q[4][3];
+#pragma unroll
+      for (int b = 1; b < 4; ++b) {
+        mat3vec(m.Gm, h[b], g[b]);
+        mat3vec(m.F, h[b], q[b]);
+      }
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        g[0][c] = -(g[1][c] + g[2][c] + g[3][c]);
+        q[0][c] = -(q[1][c] + q[2][c] + q[3][c]);
+      }
+      // K_0b = P g_b^T + g_b Rv^T + T q_b^T + (g_b . Rv + h_b . W) I
+      R P[3], Rv[3], T[3], W[3];
+      const R va = V * m.al, vb = R(0.5) * V * m.be, vg = V * m.ga;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        P[i] = va * g[0][i] + vg * q[0][i];
+        Rv[i] = vb * g[0][i];
+        T[i] = vg * g[0][i];
+        W[i] = V * Sh0[i];
+      }
+      // model-specific vectors of the row node (see psx_nh.h)
+      R Qm[3] = {R(0), R(0), R(0)}, Pm[3] = {R(0), R(0), R(0)}, Z0[3] = {R(0), R(0), R(0)};
+      if constexpr (M == kMooneyRivlin) {
+        R bq0[3];
+        mat3vec(x.b, q[0], bq0);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          Qm[i] = V * x.cq * q[0][i];                          // times q_b^T
+          Pm[i] = V * x.cp * (x.I1 * q[0][i] - bq0[i]);        // p_0, times g_b^T
+        }
+      } else if constexpr (M == kYeoh) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) Z0[i] = V * x.psi11 * x.zs * (q[0][i] - x.I1 * R(1.0 / 3.0) * g[0][i]);
+      }
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const R sb = g[b][0] * Rv[0] + g[b][1] * Rv[1] + g[b][2] * Rv[2] + h[b][0] * W[0] + h[b][1] * W[1] +
+                     h[b][2] * W[2];
+        R pb[3] = {R(0), R(0), R(0)}, zb[3] = {R(0), R(0), R(0)};
+        R hh = R(0);
+        if constexpr (M == kMooneyRivlin) {
+          R bqb[3];
+          mat3vec(x.b, q[b], bqb);
+#pragma unroll
+          for (int i = 0; i < 3; ++i) pb[i] = x.I1 * q[b][i] - bqb[i];
+          hh = V * x.cb * (h[0][0] * h[b][0] + h[0][1] * h[b][1] + h[0][2] * h[b][2]);
+        } else if constexpr (M == kYeoh) {
+#pragma unroll
+          for (int i = 0; i < 3; ++i) zb[i] = x.zs * (q[b][i] - x.I1 * R(1.0 / 3.0) * g[b][i]);
+        }
+        R* o = out + 9 * b;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int k = (b == 0 ? i : 0); k < 3; ++k) {
+            R v = P[i] * g[b][k] + g[b][i] * Rv[k] + T[i] * q[b][k] + (i == k ? sb : R(0));
+            if constexpr (M == kMooneyRivlin)
+              v += Qm[i] * q[b][k] + hh * x.b[i][k] + V * x.cb * q[b][i] * q[0][k] +
+                   V * x.cp * g[0][i] * pb[k] + Pm[i] * g[b][k];
+            else if constexpr (M == kYeoh)
+              v += Z0[i] * zb[k];
+            o[3 * i + k] = v;
+            if (b == 0 && k != i) o[3 * k + i] = v;  // K_00 is symmetric
+          }
+      }
+    }
+    return m.J;
+  }
 }
 
 template <typename R, bool WANT_K>

@@ -86,7 +86,6 @@ __global__ __launch_bounds__(TILE) void tet4_vjp_kernel(DevPlan p, const double*
   }
 }
 
-int32_t launch_permute_u(const DevPlan& p, const double* u, hipStream_t s, int32_t* status);
 
 template <typename R, int TILE>
 static void launch_vjp_t(const DevPlan& p, const double* mu, const double* lam, double* gu, double* gmu,

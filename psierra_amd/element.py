@@ -87,6 +87,12 @@ class _GpuTotalLagrangian:
     def plan(self, mesh) -> engine.AssemblyPlan:
         return self._state(mesh)["plan"]
 
+    def free_dof_system(self, mesh, free_dofs):
+        """Device free-dof CSC / residual helper over this element's K pattern."""
+        from .solver import FreeDofSystem
+
+        return FreeDofSystem(self.plan(mesh), free_dofs)
+
     # Element name carried by the inversion message (None: the Total
     # Lagrangian text "Element inversion detected: det(F) = ...").
     _inversion_name: str | None = None
@@ -257,3 +263,96 @@ class GpuTet10UpdatedLagrangian(GpuTet10TotalLagrangian):
     each of the 4 quadrature points, so the same kernels run."""
 
     _inversion_name = "Tet10UpdatedLagrangian"
+
+
+class GpuTet4Fbar(GpuTet4TotalLagrangian):
+    """Tet4Fbar (src/psierra/element/tet4_fbar.py:17-459) on the GPU for the
+    compressible Neo-Hookean material with per-element (mu, lambda).
+
+    The domain hooks run ``engine.FbarAssembly``: nodal volume averaging
+    J_bar, F_bar = (J_bar / J)^(1/3) F, f = B_bar^T S(F_bar) V and the
+    tangent K_dev + K_vol of tet4_fbar.py:332-459, which couples nodes up to
+    two element edges apart.  ``bulk_modulus`` is the single K of the
+    volumetric part; by default it is taken as Tet4Fbar._get_bulk_modulus
+    does (tet4_fbar.py:112-118): the material's ``parameters.bulk_modulus``,
+    else from its small-strain constitutive matrix, else the common
+    lambda + 2 mu / 3 of a uniform per-element material.
+
+    The per-element API with J_bar = J (no smoothed value given) is the
+    Total Lagrangian element (ratio 1, B_bar = B), as in the reference; an
+    explicit smoothed J_bar is served through the domain hooks only.
+    """
+
+    supports_element_indexing = True
+
+    def __init__(self, material, precision: str = "fp64", tile: int = 0, bulk_modulus: float | None = None) -> None:
+        super().__init__(material, precision=precision, tile=tile)
+        self.bulk_modulus = bulk_modulus
+
+    def _bulk(self, st) -> float:
+        if self.bulk_modulus is not None:
+            return float(self.bulk_modulus)
+        mat = self._material
+        for attr in ("_parameters", "parameters"):
+            prm = getattr(mat, attr, None)
+            if prm is not None and hasattr(prm, "bulk_modulus"):
+                return float(prm.bulk_modulus)
+        if hasattr(mat, "compute_constitutive_matrix"):
+            C = np.asarray(mat.compute_constitutive_matrix(), dtype=np.float64)
+            return float(np.sum(C[:3, :3])) / 9.0
+        mu = st["mu"].cpu().numpy()
+        lam = st["lam"].cpu().numpy()
+        K = lam + (2.0 / 3.0) * mu
+        if K.size and np.all(K == K[0]):
+            return float(K[0])
+        raise ValueError("GpuTet4Fbar needs bulk_modulus= for a material with per-element bulk moduli")
+
+    def _fbar(self, mesh):
+        st = self._state(mesh)
+        if st["model"] != "neohookean":
+            raise NotImplementedError("the GPU F-bar element evaluates the Neo-Hookean model only")
+        if "fbar" not in st:
+            st["fbar"] = engine.FbarAssembly(st["plan"])
+        return st
+
+    def assemble_device(self, mesh, u, want_k: bool = True, f=None, kval=None, check_status: bool = True,
+                        precision: str | None = None):
+        st = self._fbar(mesh)
+        return st["fbar"].assemble(u, st["mu"], st["lam"], self._bulk(st), f=f, kval=kval,
+                                   precision=precision or self.precision, want_k=want_k, check_status=check_status)
+
+    def free_dof_system(self, mesh, free_dofs):
+        from .solver import FreeDofSystem
+
+        st = self._fbar(mesh)
+        return FreeDofSystem(st["plan"], free_dofs, fbar=st["fbar"])
+
+    def prepare_for_assembly(self, mesh, displacement) -> None:
+        """The smoothed J_bar is recomputed on the device by every assembly."""
+        self._fbar(mesh)
+
+    def prepare_for_linear_assembly(self, mesh) -> None:
+        self._fbar(mesh)
+
+    def _csr(self, mesh, kval) -> spsp.csr_matrix:
+        fb = self._fbar(mesh)["fbar"]
+        indptr, indices = fb.pattern()
+        n = fb.n_dofs
+        K = spsp.csr_matrix((kval.cpu().numpy(), indices.copy(), indptr.copy()), shape=(n, n))
+        # the reference returns the CSR sum K_dev + K_vol, which drops exact zeros
+        K.eliminate_zeros()
+        return K
+
+    def assemble_tangent_stiffness_matrix(self, mesh, displacement) -> spsp.csr_matrix:
+        _, k = self.assemble_device(mesh, np.asarray(displacement, dtype=np.float64).reshape(-1), want_k=True)
+        return self._csr(mesh, k)
+
+    def assemble_mass_matrix(self, mesh) -> spsp.csr_matrix:
+        return GpuTet4TotalLagrangian._csr(self, mesh, self._state(mesh)["plan"].assemble_mass(
+            float(self._material.mass_density)))
+
+    def compute_nonlinear_force_and_tangent_stiffness(self, node_coords, u_nodes, elem_index=None, **kw):
+        J_bar = kw.pop("J_bar", None)
+        if J_bar is not None:
+            raise NotImplementedError("an explicit smoothed J_bar is evaluated through the F-bar assembly hooks")
+        return super().compute_nonlinear_force_and_tangent_stiffness(node_coords, u_nodes, elem_index, **kw)

@@ -262,6 +262,87 @@ class AssemblyPlan:
             raise ValueError(f"{name} must have {n} entries on {self.device}")
 
 
+FBAR_INVERSION_MSG = "Element inversion detected at element {}: det(F) = {:.4e} <= 0."
+
+
+class FbarAssembly:
+    """Tet4Fbar's domain assembly on the GPU (src/psierra/element/tet4_fbar.py:
+    226-459) over a Tet4 ``AssemblyPlan`` (kept alive by this object).
+
+    ``assemble`` returns f (3N) and the values of K_dev + K_vol in the CSR
+    order of ``pattern()``: node pairs that share an element or a
+    neighbouring node, explicit zeros kept.  Compressible Neo-Hookean with
+    per-element (mu, lambda); ``bulk`` is the single K of the volumetric
+    part (Tet4Fbar._get_bulk_modulus, tet4_fbar.py:112-118).
+    """
+
+    def __init__(self, plan: AssemblyPlan):
+        if plan.npe != 4:
+            raise ValueError("F-bar needs a Tet4 plan")
+        self.plan = plan
+        handle = ctypes.c_void_p()
+        check(lib().psx_fbar_create(plan._h, ctypes.byref(handle)))
+        self._h = handle
+        n = ctypes.c_int64()
+        check(lib().psx_fbar_nnz(handle, ctypes.byref(n)))
+        self.nnz = int(n.value)
+        self.n_dofs = plan.n_dofs
+        self.device = plan.device
+        self._indptr = None
+        self._indices = None
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                lib().psx_fbar_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    def pattern(self) -> tuple[np.ndarray, np.ndarray]:
+        if self._indptr is None:
+            indptr = np.empty(self.n_dofs + 1, dtype=np.int32)
+            indices = np.empty(self.nnz, dtype=np.int32)
+            check(lib().psx_fbar_pattern(self._h, ptr(indptr), ptr(indices)))
+            self._indptr, self._indices = indptr, indices
+        return self._indptr, self._indices
+
+    def assemble(self, u, mu, lam, bulk, f=None, kval=None, precision="fp64", want_k=True, check_status=True):
+        """f_int (3N) and, with ``want_k``, the K values (nnz) as device tensors."""
+        torch = _torch()
+        plan = self.plan
+        u = plan._device_f64(u, self.n_dofs, "u")
+        mu = plan._device_f64(mu, plan.n_elements, "mu")
+        lam = plan._device_f64(lam, plan.n_elements, "lam")
+        if f is None:
+            f = torch.empty(self.n_dofs, dtype=torch.float64, device=self.device)
+        plan._check_out(f, self.n_dofs, "f")
+        if want_k:
+            if kval is None:
+                kval = torch.empty(self.nnz, dtype=torch.float64, device=self.device)
+            plan._check_out(kval, self.nnz, "kval")
+        else:
+            kval = None
+        check(
+            lib().psx_fbar_assemble(
+                self._h, ptr(u), ptr(mu), ptr(lam), float(bulk), ptr(f), ptr(kval), precision_code(precision),
+                current_stream_handle(), ptr(plan.status),
+            )
+        )
+        if check_status:
+            st = plan.status.cpu().numpy()
+            if st[0] != 0:
+                e = int(st[1])
+                conn = plan.elements[e]
+                X = plan.nodes[conn]
+                ue = u.reshape(-1, 3)[torch.as_tensor(conn, device=self.device)].cpu().numpy()
+                Dinv = np.linalg.inv((X[1:] - X[0]).T)
+                H = np.vstack([-Dinv.sum(axis=0), Dinv])
+                raise ValueError(FBAR_INVERSION_MSG.format(e, _det3(np.identity(3) + ue.T @ H)))
+        return f, kval
+
+
 def element_batch(X, u, mu, lam, want_k=True, precision="fp64"):
     """Per-element k (n, nde, nde) and f (n, nde) on the device (no assembly)."""
     torch = _torch()

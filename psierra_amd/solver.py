@@ -29,12 +29,18 @@ class DirectSolver:
 
 
 class FreeDofSystem:
-    def __init__(self, plan: AssemblyPlan, free_dofs: np.ndarray) -> None:
+    def __init__(self, plan: AssemblyPlan, free_dofs: np.ndarray, fbar=None) -> None:
+        """``fbar``: an ``engine.FbarAssembly`` whose (wider) pattern the K
+        values follow, instead of the plan's element pattern."""
         torch = _torch()
         self.plan = plan
         self.free_dofs = np.ascontiguousarray(free_dofs, dtype=np.int64)
         h = ctypes.c_void_p()
-        check(lib().psx_csc_create(plan._h, ptr(self.free_dofs), self.free_dofs.size, ctypes.byref(h)))
+        if fbar is None:
+            check(lib().psx_csc_create(plan._h, ptr(self.free_dofs), self.free_dofs.size, ctypes.byref(h)))
+        else:
+            check(lib().psx_fbar_csc_create(fbar._h, ptr(self.free_dofs), self.free_dofs.size, ctypes.byref(h)))
+        self._fbar = fbar   # keeps the pattern owner alive
         self._h = h
         nf, nnz = ctypes.c_int64(), ctypes.c_int64()
         check(lib().psx_csc_info(h, ctypes.byref(nf), ctypes.byref(nnz)))
