@@ -98,7 +98,7 @@ def bench_tet10(args):
     from psierra_amd import workloads as W
     from psierra_amd.engine import AssemblyPlan
 
-    wl = W.jittered_box_tet10(n=args.n if args.n != 100 else 40, seed=2)
+    wl = W.jittered_box_tet10(n=args.n or 40, seed=2)
     plan = AssemblyPlan(wl.mesh.nodes, wl.mesh.elements)
     dev = plan.device
     u = torch.as_tensor(wl.u.reshape(-1), device=dev)
@@ -121,6 +121,49 @@ def bench_tet10(args):
         "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
         "config": {"workload": wl.name, "elements": E, "nodes": N, "nnz": nnz, "fp64_plain_ms": res["fp64"],
                    "speedup_fp32_vs_fp64_plain": res["fp64"] / res["fp32"], "tile": plan.info["tile"]},
+        "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": ach / HBM_PEAK_GBPS, "traffic": None},
+        "cpu_baseline": None,
+    }
+
+
+def bench_fbar(args):
+    """F-bar Tet4 on a jittered box (default 48^3 hexes, 663k tets): f_int and
+    K_dev + K_vol on the distance-2 pattern, the whole device pipeline."""
+    import torch
+
+    from psierra_amd import workloads as W
+    from psierra_amd.engine import AssemblyPlan, FbarAssembly
+
+    n = args.n or 48
+    wl = W.jittered_box_tet4(n=n, seed=1)
+    t0 = time.perf_counter()
+    plan = AssemblyPlan(wl.mesh.nodes, wl.mesh.elements)
+    fb = FbarAssembly(plan)
+    t_setup = time.perf_counter() - t0
+    dev = plan.device
+    u = torch.as_tensor(wl.u.reshape(-1), device=dev)
+    mu = torch.as_tensor(wl.mu, device=dev)
+    lam = torch.as_tensor(wl.lam, device=dev)
+    bulk = float(np.mean(wl.lam + 2.0 / 3.0 * wl.mu))
+    f = torch.empty(plan.n_dofs, dtype=torch.float64, device=dev)
+    k = torch.empty(fb.nnz, dtype=torch.float64, device=dev)
+    res = {}
+    for prec in ("fp64", "fp32"):
+        res[prec] = _event_time(
+            lambda: fb.assemble(u, mu, lam, bulk, f=f, kval=k, precision=prec, check_status=False),
+            args.steps, args.warmup)
+    fb.assemble(u, mu, lam, bulk, f=f, kval=k, precision="fp32")   # raises on inversion
+    E, N = plan.n_elements, plan.n_nodes
+    b_alg = algorithmic_bytes(E, N, 4, fb.nnz, coord_bytes=4, param_bytes=4)
+    ach = b_alg / (res["fp32"] * 1e-3) / 1e9
+    return {
+        "metric": "Tet4 F-bar NeoHookean f_int+K_T assembly elements/s",
+        "value": E / (res["fp32"] * 1e-3), "unit": "elements/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": res["fp32"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": wl.name + "_fbar", "elements": E, "nodes": N, "nnz": fb.nnz,
+                   "element_pattern_nnz": plan.nnz, "fp64_plain_ms": res["fp64"], "setup_s": t_setup},
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": ach / HBM_PEAK_GBPS, "traffic": None},
         "cpu_baseline": None,
@@ -237,12 +280,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--n", type=int, default=100, help="hexes per box side (100 -> 6.0M tets)")
+    ap.add_argument("--n", type=int, default=None,
+                    help="hexes per box side (default: tet4 100 -> 6.0M tets, tet10 40, fbar 48)")
     ap.add_argument("--cpu-sample", type=int, default=80000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--tile", type=int, default=0)
-    ap.add_argument("--workload", choices=("tet4", "tet10", "vjp", "ecsw", "ecsw_train"), default="tet4")
+    ap.add_argument("--workload", choices=("tet4", "tet10", "vjp", "ecsw", "ecsw_train", "fbar"), default="tet4")
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -251,13 +295,15 @@ def main():
         )
         sys.exit(2)
     if args.workload != "tet4":
-        fn = {"tet10": bench_tet10, "vjp": bench_vjp, "ecsw": bench_ecsw, "ecsw_train": bench_ecsw_train}[
+        fn = {"tet10": bench_tet10, "vjp": bench_vjp, "ecsw": bench_ecsw, "ecsw_train": bench_ecsw_train,
+              "fbar": bench_fbar}[
             args.workload]
         print(json.dumps(fn(args)), flush=True)
         return
 
     from psierra_amd import workloads as W
 
+    args.n = args.n or 100
     t0 = time.perf_counter()
     wl = W.jittered_box_tet4(n=args.n, seed=1)
     t_build = time.perf_counter() - t0

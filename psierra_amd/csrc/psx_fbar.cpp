@@ -87,13 +87,12 @@ int32_t build(psx_plan* plan, psx_fbar* fb) {
   const std::vector<int32_t>& perm = plan->perm_h;   // internal -> user
   std::vector<int32_t> inv(N);
   for (int64_t i = 0; i < N; ++i) inv[perm[i]] = int32_t(i);
-  std::vector<int32_t> uoff(N + 1, 0), uoff2(N + 1, 0);
+  std::vector<int32_t> uoff(N + 1, 0);
   for (int64_t i = 0; i < N; ++i) {
     const int32_t a = perm[i];
     uoff[i + 1] = uoff[i] + (hp.adj_ptr[a + 1] - hp.adj_ptr[a]);
-    uoff2[i + 1] = uoff2[i] + deg2[a];
   }
-  std::vector<int32_t> nbr1_user(uoff[N]), nbr1_int(uoff[N]), slot2(uoff[N]), nbr2_user(uoff2[N]);
+  std::vector<int32_t> slot2(uoff[N]);
   std::vector<int4> rec2(N);
   parallel_nodes(N, [&](int64_t b, int64_t e) {
     for (int64_t i = b; i < e; ++i) {
@@ -101,47 +100,75 @@ int32_t build(psx_plan* plan, psx_fbar* fb) {
       const int32_t* l1 = hp.adj.data() + hp.adj_ptr[a];
       const int32_t n1 = hp.adj_ptr[a + 1] - hp.adj_ptr[a];
       const int32_t* l2 = fb->adj2.data() + fb->adj2_ptr[a];
-      const int32_t n2 = deg2[a];
       int32_t self2 = 0;
       for (int32_t s = 0, t = 0; s < n1; ++s) {
         while (l2[t] < l1[s]) ++t;   // l1 is a subset of l2, both ascending
-        nbr1_user[uoff[i] + s] = l1[s];
-        nbr1_int[uoff[i] + s] = inv[l1[s]];
         slot2[uoff[i] + s] = t;
         if (l1[s] == a) self2 = t;
       }
-      std::copy(l2, l2 + n2, nbr2_user.begin() + uoff2[i]);
-      rec2[i] = make_int4(int32_t(9 * int64_t(fb->adj2_ptr[a])), n2, self2, 0);
+      rec2[i] = make_int4(int32_t(9 * int64_t(fb->adj2_ptr[a])), deg2[a], self2, 0);
+    }
+  });
+  int32_t max_deg2 = 0;
+  for (int64_t a = 0; a < N; ++a) max_deg2 = std::max(max_deg2, deg2[a]);
+  if (max_deg2 > 65535) return fail(PSX_EUNSUPPORTED, "F-bar row with more than 65535 node blocks");
+
+  // K_vol[p, q] = K sum_a V0_a g_a[p] g_a[q]^T over the common element
+  // neighbours a: per (p, a) the units of a (contiguous), the unit (a, p)
+  // and, for each neighbour q of a, q's slot in p's distance-2 row.
+  std::vector<int64_t> moff(uoff[N] + 1, 0);
+  for (int64_t i = 0; i < N; ++i) {
+    const int32_t a0 = perm[i];
+    for (int32_t t1 = hp.adj_ptr[a0], s = 0; t1 < hp.adj_ptr[a0 + 1]; ++t1, ++s) {
+      const int32_t a = hp.adj[t1];
+      moff[uoff[i] + s + 1] = moff[uoff[i] + s] + (hp.adj_ptr[a + 1] - hp.adj_ptr[a]);
+    }
+  }
+  const int64_t n_map = moff[uoff[N]];
+  if (n_map >= (int64_t(1) << 31)) return fail(PSX_EUNSUPPORTED, "F-bar volumetric slot map exceeds 2^31 entries");
+  std::vector<int4> kv_unit(uoff[N]);
+  std::vector<uint16_t> kv_slot(n_map);
+  parallel_nodes(N, [&](int64_t b, int64_t e) {
+    for (int64_t i = b; i < e; ++i) {
+      const int32_t a0 = perm[i];
+      const int32_t* l2 = fb->adj2.data() + fb->adj2_ptr[a0];
+      for (int32_t t1 = hp.adj_ptr[a0], s = 0; t1 < hp.adj_ptr[a0 + 1]; ++t1, ++s) {
+        const int32_t a = hp.adj[t1];
+        const int32_t* la = hp.adj.data() + hp.adj_ptr[a];
+        const int32_t na = hp.adj_ptr[a + 1] - hp.adj_ptr[a];
+        const int32_t ub = uoff[inv[a]];
+        const int32_t ap = ub + int32_t(std::lower_bound(la, la + na, a0) - la);
+        const int64_t m0 = moff[uoff[i] + s];
+        kv_unit[uoff[i] + s] = make_int4(ub, na, ap, int32_t(m0));
+        for (int32_t q = 0, t = 0; q < na; ++q) {
+          while (l2[t] < la[q]) ++t;
+          kv_slot[m0 + q] = uint16_t(t);
+        }
+      }
     }
   });
 
   FbarDev& d = fb->d;
-  int32_t* p1u = nullptr;
-  int32_t* p1i = nullptr;
-  int32_t* pu2 = nullptr;
-  int32_t* pn2 = nullptr;
   int32_t* ps2 = nullptr;
   int4* pr2 = nullptr;
+  int4* pku = nullptr;
+  uint16_t* pks = nullptr;
   int32_t rc;
-  if ((rc = to_device(fb, nbr1_user.data(), nbr1_user.size(), &p1u))) return rc;
-  if ((rc = to_device(fb, nbr1_int.data(), nbr1_int.size(), &p1i))) return rc;
-  if ((rc = to_device(fb, uoff2.data(), uoff2.size(), &pu2))) return rc;
-  if ((rc = to_device(fb, nbr2_user.data(), nbr2_user.size(), &pn2))) return rc;
   if ((rc = to_device(fb, slot2.data(), slot2.size(), &ps2))) return rc;
   if ((rc = to_device(fb, rec2.data(), rec2.size(), &pr2))) return rc;
-  d.nbr1_user = p1u;
-  d.nbr1_int = p1i;
-  d.uoff2 = pu2;
-  d.nbr2_user = pn2;
+  if ((rc = to_device(fb, kv_unit.data(), kv_unit.size(), &pku))) return rc;
+  if ((rc = to_device(fb, kv_slot.data(), kv_slot.size(), &pks))) return rc;
   d.slot2 = ps2;
   d.node_rec2 = pr2;
+  d.kv_unit = pku;
+  d.kv_slot = pks;
+  d.max_deg2 = max_deg2;
   const size_t E = size_t(hp.E);
   if ((rc = to_device<double>(fb, nullptr, E, &d.qv))) return rc;
   if ((rc = to_device<double>(fb, nullptr, E, &d.vjm1))) return rc;
   if ((rc = to_device<double>(fb, nullptr, 12 * E, &d.wq))) return rc;
-  if ((rc = to_device<double>(fb, nullptr, size_t(N), &d.v0))) return rc;
   if ((rc = to_device<double>(fb, nullptr, size_t(N), &d.jm1))) return rc;
-  if ((rc = to_device<double>(fb, nullptr, 3 * size_t(uoff[N]), &d.g))) return rc;
+  if ((rc = to_device<double4>(fb, nullptr, size_t(uoff[N]), &d.g))) return rc;
   return PSX_OK;
 }
 

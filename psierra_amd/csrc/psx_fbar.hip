@@ -6,27 +6,30 @@
 //      small strains stay accurate), V_e / 4, V_e (J_e - 1) / 4 and
 //      w_e q_b = V_e J_e / 4 F h_b for the element's four nodes (q_b is the
 //      node's column of div_e = B_0 + B_1 + B_2, the volumetric row of B);
-//   2. fbar_node_kernel, one wave per node a, walking its incidences in
-//      order: V0_a = sum V_e / 4, J_node - 1 = sum V_e (J_e - 1) / 4 / V0_a
-//      and, with K, the volumetric gradient g_a over a's element neighbours,
-//      g_a[b] = sum_{e in a, b} w_e q_b / V0_a (one lane per neighbour b);
-//   3. fbar_kvol_kernel, one wave per node p, one lane per slot q of p's
-//      distance-2 row: K_vol[p, q] = K_bulk sum_a V0_a g_a[p] g_a[q]^T over
-//      the element neighbours a of p (ascending), written to every value of
-//      the row;
+//   2. fbar_node_kernel, one wave per node a, its incidences staged in LDS:
+//      V0_a = sum V_e / 4, J_node - 1 = sum V_e (J_e - 1) / 4 / V0_a and,
+//      with K, the volumetric gradient g_a over a's element neighbours,
+//      g_a[b] = sum_{e in a, b} w_e q_b / V0_a (one lane per neighbour b,
+//      incidences in run order);
+//   3. fbar_kvol_kernel, one wave per node p: its distance-2 row of
+//      K_vol[p, q] = K_bulk sum_a V0_a g_a[p] g_a[q]^T over the common
+//      neighbours a (ascending; q's slots in p's row listed with the plan),
+//      accumulated in LDS and written to every value of the row;
 //   4. the Tet4 row kernel with the F-bar material point (J_bar from step 2,
 //      deviatoric tangent and f), adding K_dev onto the K_vol values.
-// Every sum runs in an order fixed by the mesh; nothing uses atomics except
-// the inversion flag.
+// Every sum runs in an order fixed by the mesh (wave sums in a fixed
+// butterfly order); nothing uses atomics except the inversion flag.
+#include <algorithm>
 #include <climits>
 
+#include "psx_device.h"
 #include "psx_internal.h"
 #include "psx_tet4_elem.h"
 
 namespace psx {
 
 __global__ __launch_bounds__(256) void fbar_element_kernel(DevPlan p, FbarDev fb, int32_t* status) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int e = xcd_tile() * 256 + int(threadIdx.x);
   if (e >= p.E) return;
   const int4 c = *reinterpret_cast<const int4*>(p.conn + 4 * e);
   const double4 X[4] = {p.Xi[c.x], p.Xi[c.y], p.Xi[c.z], p.Xi[c.w]};
@@ -63,108 +66,126 @@ __global__ __launch_bounds__(256) void fbar_element_kernel(DevPlan p, FbarDev fb
 // local order (the order of the rank word's bytes 1..3).
 __device__ __forceinline__ int other_local(int lp, int q) { return (q - 1) + ((q - 1) >= lp ? 1 : 0); }
 
+// Sum over the 64 lanes of a wave, the same bits in every lane (butterfly
+// in a fixed order).
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) x += __shfl_xor(x, m, 64);
+  return x;
+}
+
+// One wave per node a (a node has at most 64 incidences): its incidences'
+// element quantities are loaded in one round, one lane each, and staged in
+// LDS; V0_a and J_node - 1 are wave sums, and each lane of a's element
+// neighbours b walks the staged incidences in run order for g_a[b].
 __global__ __launch_bounds__(256) void fbar_node_kernel(DevPlan p, FbarDev fb, int want_g) {
-  const int a = blockIdx.x * 4 + int(threadIdx.x >> 6);
+  __shared__ double s_wq[4][64][12];
+  __shared__ uint32_t s_rank[4][64];
+  __shared__ uint8_t s_lp[4][64];
+  const int w = int(threadIdx.x >> 6);
+  const int a = xcd_tile() * 4 + w;
   const int lane = threadIdx.x & 63;
   if (a >= p.N) return;
   const int4 rec = p.node_rec[a];
-  const int kb = rec.x, ke = p.node_rec[a + 1].x;
-  double sv = 0.0, sj = 0.0;
-  for (int k = kb; k < ke; ++k) {
-    const int e = int(p.inc_elem[k] & kIncElemMask);
-    sv += fb.qv[e];
-    sj += fb.vjm1[e];
-  }
-  if (lane == 0) {
-    fb.v0[a] = sv;
-    fb.jm1[a] = sv > 0.0 ? sj / sv : 0.0;
-  }
-  if (!want_g) return;
-  const int ub = rec.y, deg = p.node_rec[a + 1].y - ub, self = rec.w;
-  const double iv = sv > 0.0 ? 1.0 / sv : 0.0;
-  for (int s = lane; s < deg; s += 64) {
-    double g0 = 0.0, g1 = 0.0, g2 = 0.0;
-    for (int k = kb; k < ke; ++k) {
-      const uint32_t pe = p.inc_elem[k], r = p.inc_rank[k];
-      const int lp = int(pe >> kIncElemBits);
-      int loc = -1;
-      if (s == self)
-        loc = lp;
-      else
+  const int4 rec1 = p.node_rec[a + 1];
+  const int kb = rec.x, L = rec1.x - rec.x;
+  double qv = 0.0, vj = 0.0;
+  if (lane < L) {
+    const uint32_t pe = p.inc_elem[kb + lane];
+    const int64_t e = int64_t(pe & kIncElemMask);
+    qv = fb.qv[e];
+    vj = fb.vjm1[e];
+    if (want_g) {
+      s_rank[w][lane] = p.inc_rank[kb + lane];
+      s_lp[w][lane] = uint8_t(pe >> kIncElemBits);
+      const double2* src = reinterpret_cast<const double2*>(fb.wq + 12 * e);
 #pragma unroll
-        for (int q = 1; q < 4; ++q)
-          if (int((r >> (8 * q)) & 0xffu) == s) loc = other_local(lp, q);
-      if (loc >= 0) {
-        const double* w = fb.wq + 12 * int64_t(pe & kIncElemMask) + 3 * loc;
-        g0 += w[0];
-        g1 += w[1];
-        g2 += w[2];
+      for (int i = 0; i < 6; ++i) {
+        const double2 v = src[i];
+        s_wq[w][lane][2 * i] = v.x;
+        s_wq[w][lane][2 * i + 1] = v.y;
       }
     }
-    double* g = fb.g + 3 * int64_t(ub + s);
-    g[0] = g0 * iv;
-    g[1] = g1 * iv;
-    g[2] = g2 * iv;
   }
-}
-
-// Position of `key` in the ascending list v[0..n), or n if absent.
-__device__ __forceinline__ int find_sorted(const int32_t* v, int n, int32_t key) {
-  int lo = 0, len = n;
-  while (len > 0) {
-    const int half = len >> 1;
-    if (v[lo + half] < key) {
-      lo += half + 1;
-      len -= half + 1;
-    } else {
-      len = half;
-    }
-  }
-  return (lo < n && v[lo] == key) ? lo : n;
-}
-
-__global__ __launch_bounds__(256) void fbar_kvol_kernel(DevPlan p, FbarDev fb, double bulk, double* kval) {
-  const int pn = blockIdx.x * 4 + int(threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (pn >= p.N) return;
-  const int p_user = p.perm[pn];
-  const int u1b = p.uoff[pn], u1e = p.uoff[pn + 1];
-  const int u2b = fb.uoff2[pn];
-  const int4 rec2 = fb.node_rec2[pn];
-  const int deg2 = rec2.y;
-  for (int s2 = lane; s2 - lane < deg2; s2 += 64) {
-    const bool active = s2 < deg2;
-    const int32_t q_user = active ? fb.nbr2_user[u2b + s2] : -1;
-    double acc[9];
+  const double sv = wave_sum(qv);
+  const double sj = wave_sum(vj);
+  if (lane == 0) fb.jm1[a] = sv > 0.0 ? sj / sv : 0.0;
+  if (!want_g) return;
+  __builtin_amdgcn_wave_barrier();
+  const int ub = rec.y, deg = rec1.y - ub, self = rec.w;
+  const double iv = sv > 0.0 ? 1.0 / sv : 0.0;
+  // four lanes per neighbour, each over a quarter of the run, combined as
+  // (q0 + q1) + (q2 + q3)
+  const int part = lane & 3;
+  const int k0 = (part * L) >> 2, k1 = ((part + 1) * L) >> 2;
+  for (int base = 0; base < deg; base += 16) {
+    const int s = base + (lane >> 2);
+    double g0 = 0.0, g1 = 0.0, g2 = 0.0;
+    if (s < deg) {
+      for (int k = k0; k < k1; ++k) {
+        const uint32_t r = s_rank[w][k];
+        const int lp = s_lp[w][k];
+        int loc = -1;
+        if (s == self)
+          loc = lp;
+        else
 #pragma unroll
-    for (int i = 0; i < 9; ++i) acc[i] = 0.0;
-    for (int u = u1b; u < u1e; ++u) {
-      const int an = fb.nbr1_int[u];
-      const int ab = p.uoff[an], n = p.uoff[an + 1] - ab;
-      const int32_t* list = fb.nbr1_user + ab;
-      const int pp = find_sorted(list, n, p_user);   // p is a neighbour of a: always found
-      if (pp >= n) continue;
-      const double w = fb.v0[an];
-      const double* gp = fb.g + 3 * int64_t(ab + pp);
-      const double wp[3] = {w * gp[0], w * gp[1], w * gp[2]};
-      if (active) {
-        const int pq = find_sorted(list, n, q_user);
-        if (pq < n) {
-          const double* gq = fb.g + 3 * int64_t(ab + pq);
-#pragma unroll
-          for (int r = 0; r < 3; ++r)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) acc[3 * r + c] += wp[r] * gq[c];
+          for (int q = 1; q < 4; ++q)
+            if (int((r >> (8 * q)) & 0xffu) == s) loc = other_local(lp, q);
+        if (loc >= 0) {
+          g0 += s_wq[w][k][3 * loc];
+          g1 += s_wq[w][k][3 * loc + 1];
+          g2 += s_wq[w][k][3 * loc + 2];
         }
       }
     }
-    if (active) {
-      double* dst = kval + rec2.x + 3 * s2;
+    g0 = quad_sum(g0);
+    g1 = quad_sum(g1);
+    g2 = quad_sum(g2);
+    if (part == 0 && s < deg) fb.g[ub + s] = make_double4(g0 * iv, g1 * iv, g2 * iv, sv);
+  }
+}
+
+// One wave per node p: K_vol[p, q] = K sum_a V0_a g_a[p] g_a[q]^T for the
+// whole distance-2 row of p, accumulated in LDS.  The common neighbours a
+// are taken in ascending order (p's element-pattern units); for each, the
+// lanes read a's units and their slots in p's row contiguously (one lane per
+// neighbour q of a) and add the 3x3 term into q's slot -- distinct q, distinct
+// slots, so each slot sums its terms in a-order.  The row is then written
+// with consecutive lanes on consecutive values.
+__global__ __launch_bounds__(256) void fbar_kvol_kernel(DevPlan p, FbarDev fb, int waves, double bulk, double* kval) {
+  extern __shared__ double s_acc[];
+  const int w = int(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int pn = xcd_tile() * waves + w;
+  if (pn >= p.N) return;
+  const int4 rec2 = fb.node_rec2[pn];
+  const int deg2 = rec2.y, n9 = 9 * deg2;
+  double* acc = s_acc + size_t(w) * 9 * fb.max_deg2;
+  for (int i = lane; i < n9; i += 64) acc[i] = 0.0;
+  __builtin_amdgcn_wave_barrier();
+  const int u1b = p.uoff[pn], u1e = p.uoff[pn + 1];
+  for (int u = u1b; u < u1e; ++u) {
+    const int4 ku = fb.kv_unit[u];   // {first unit of a, deg a, unit (a, p), first slot entry}
+    const double4 gp = fb.g[ku.z];
+    const double wp[3] = {gp.w * gp.x, gp.w * gp.y, gp.w * gp.z};
+    for (int j = lane; j < ku.y; j += 64) {
+      const double4 gq = fb.g[ku.x + j];
+      double* a9 = acc + 9 * int(fb.kv_slot[ku.w + j]);
+      const double q3[3] = {gq.x, gq.y, gq.z};
 #pragma unroll
       for (int r = 0; r < 3; ++r)
 #pragma unroll
-        for (int c = 0; c < 3; ++c) dst[r * 3 * deg2 + c] = bulk * acc[3 * r + c];
+        for (int c = 0; c < 3; ++c) a9[3 * r + c] += wp[r] * q3[c];
     }
+    __builtin_amdgcn_wave_barrier();
+  }
+  // CSR order of p's rows: value r * 3 deg2 + 3 s + c
+  const int row = 3 * deg2;
+  for (int i = lane; i < n9; i += 64) {
+    const int r = i / row, rem = i - r * row;
+    const int s2 = rem / 3, c = rem - 3 * s2;
+    kval[rec2.x + i] = bulk * acc[9 * s2 + 3 * r + c];
   }
 }
 
@@ -173,10 +194,17 @@ int32_t launch_tet4_fbar(const DevPlan& p, const FbarDev& fb, const double* u, c
                          int32_t* status) {
   int32_t rc = launch_permute_u(p, u, s, status);
   if (rc) return rc;
-  if (p.E > 0) hipLaunchKernelGGL(fbar_element_kernel, dim3((p.E + 255) / 256), dim3(256), 0, s, p, fb, status);
-  const int node_blocks = (p.N + 3) / 4;
-  hipLaunchKernelGGL(fbar_node_kernel, dim3(node_blocks), dim3(256), 0, s, p, fb, kval ? 1 : 0);
-  if (kval) hipLaunchKernelGGL(fbar_kvol_kernel, dim3(node_blocks), dim3(256), 0, s, p, fb, bulk, kval);
+  // grids in multiples of 8 work-groups for the XCD-contiguous mapping
+  auto grid = [](int64_t items, int per_group) { return dim3(unsigned(8 * ((items + 8 * per_group - 1) / (8 * per_group)))); };
+  if (p.E > 0) hipLaunchKernelGGL(fbar_element_kernel, grid(p.E, 256), dim3(256), 0, s, p, fb, status);
+  hipLaunchKernelGGL(fbar_node_kernel, grid(p.N, 4), dim3(256), 0, s, p, fb, kval ? 1 : 0);
+  if (kval && fb.max_deg2 > 0) {
+    // up to four nodes (one wave each) per work-group within 64 KiB of LDS
+    const size_t per_wave = size_t(9) * fb.max_deg2 * sizeof(double);
+    const int waves = int(std::max<size_t>(1, std::min<size_t>(4, (size_t(64) << 10) / per_wave)));
+    hipLaunchKernelGGL(fbar_kvol_kernel, grid(p.N, waves), dim3(64 * waves), per_wave * waves, s, p, fb, waves, bulk,
+                       kval);
+  }
   PSX_HIP_TRY(hipGetLastError());
   const FbarView fv{fb.node_rec2, fb.slot2};
   return launch_t4_fbar_rows(p, mu, lam, fb.jm1, fv, f, kval, precision, s, status);

@@ -94,18 +94,17 @@ struct FbarView {
 // F-bar working set (device): the wider pattern's layout and the per-call
 // element and node quantities of the nodal volume averaging.
 struct FbarDev {
-  const int32_t* nbr1_user = nullptr;  // per element-pattern unit: neighbour's user id (slot order)
-  const int32_t* nbr1_int = nullptr;   // per element-pattern unit: neighbour's internal id
-  const int32_t* uoff2 = nullptr;      // N+1: prefix of distance-2 row lengths, internal order
-  const int32_t* nbr2_user = nullptr;  // per distance-2 unit: user id (ascending per row)
   const int4* node_rec2 = nullptr;     // N: {first value, row length, diagonal slot, 0}
   const int32_t* slot2 = nullptr;      // per element-pattern unit: slot in the distance-2 row
+  const int4* kv_unit = nullptr;       // per element-pattern unit (p, a): {first unit of a, deg a,
+                                       //   unit (a, p), first entry of the slot map}
+  const uint16_t* kv_slot = nullptr;   // per (p, a, q in N1(a)): slot of q in p's distance-2 row
+  int32_t max_deg2 = 0;                // longest distance-2 row (blocks)
   double* qv = nullptr;                // per element: V / 4
   double* vjm1 = nullptr;              // per element: V (J - 1) / 4
   double* wq = nullptr;                // per element: 12 values V J / 4 F h_b
-  double* v0 = nullptr;                // per internal node: nodal reference volume
   double* jm1 = nullptr;               // per internal node: J_node - 1
-  double* g = nullptr;                 // per element-pattern unit: volumetric gradient (3)
+  double4* g = nullptr;                // per element-pattern unit (a, b): {g_a[b], V0_a}
 };
 
 // Structural CSC of K[free][:, free] (device view).
