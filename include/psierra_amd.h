@@ -118,11 +118,10 @@ int32_t psx_assemble(psx_plan* plan, const double* u, const double* mu,
  * src/psierra/material/mooney_rivlin.py:69-152 (Psi = c10 (I1bar - 3) +
  * c01 (I2bar - 3) + K/2 (ln J)^2) and src/psierra/material/yeoh.py:69-148
  * (Psi = c10 (I1bar - 3) + c20 (I1bar - 3)^2 + c30 (I1bar - 3)^3 +
- * K/2 (ln J)^2) under Tet4TotalLagrangian (the same call sites as
- * psx_assemble).  params: host array of device pointers, one value per
- * element each -- PSX_NEO_HOOKEAN: mu, lambda (as psx_assemble);
- * PSX_MOONEY_RIVLIN: c10, c01, K; PSX_YEOH: c10, c20, c30, K.  Tet10 plans
- * take PSX_NEO_HOOKEAN only. */
+ * K/2 (ln J)^2) under Tet4TotalLagrangian and Tet10TotalLagrangian (the
+ * same call sites as psx_assemble).  params: host array of device pointers,
+ * one value per element each -- PSX_NEO_HOOKEAN: mu, lambda (as
+ * psx_assemble); PSX_MOONEY_RIVLIN: c10, c01, K; PSX_YEOH: c10, c20, c30, K. */
 enum { PSX_NEO_HOOKEAN = 0, PSX_MOONEY_RIVLIN = 1, PSX_YEOH = 2 };
 int32_t psx_assemble_hyperelastic(psx_plan* plan, int32_t model, const double* const* params,
                                   const double* u, double* f, double* kval, int32_t precision,

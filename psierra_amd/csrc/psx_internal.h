@@ -163,6 +163,8 @@ int32_t launch_permute_u(const DevPlan& p, const double* u, hipStream_t s, int32
 int32_t launch_tet4_fbar(const DevPlan& p, const FbarDev& fb, const double* u, const double* mu,
                          const double* lam, double bulk, double* f, double* kval, int precision, hipStream_t s,
                          int32_t* status);
+int32_t launch_tet10_hyper(const DevPlan& p, int model, const double* const* params, const double* u, double* f,
+                           double* kval, int precision, hipStream_t s, int32_t* status);
 int32_t launch_tet10_assemble(const DevPlan& p, const double* u, const double* mu, const double* lam,
                               double* f, double* kval, int precision, hipStream_t s, int32_t* status);
 int32_t launch_tet4_vjp(const DevPlan& p, const double* u, const double* mu, const double* lam,

@@ -293,6 +293,28 @@ __device__ __forceinline__ void hyper_point(const R A[3][3], const R prm[4], NHP
   }
 }
 
+// Per-element material parameter arrays (device, one value per element):
+// Neo-Hookean (mu, lambda); Mooney-Rivlin (c10, c01, K); Yeoh (c10, c20,
+// c30, K); F-bar Neo-Hookean (mu, lambda, and J_node - 1 per internal node).
+struct MatParams {
+  const double* v[4];
+};
+
+// Parameters of element e as the material point takes them (Neo-Hookean:
+// mu and K = lambda + 2 mu / 3).
+template <int M>
+__device__ __forceinline__ void load_params(const MatParams& mp, int64_t e, double pr[4]) {
+  if constexpr (M == kNeoHookean || M == kFbarNeoHookean) {
+    pr[0] = mp.v[0][e];
+    pr[1] = mp.v[1][e] + (2.0 / 3.0) * pr[0];
+    pr[2] = pr[3] = 0.0;
+  } else {
+    constexpr int n = M == kMooneyRivlin ? 3 : 4;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) pr[i] = i < n ? mp.v[i][e] : 0.0;
+  }
+}
+
 template <typename R>
 __device__ __forceinline__ void mat3vec(const R M[3][3], const R v[3], R o[3]) {
 #pragma unroll
@@ -321,6 +343,38 @@ __device__ __forceinline__ void nh_block(const NHPoint<R>& m, R w, const R ga_[3
 #pragma unroll
     for (int k = 0; k < 3; ++k)
       out[3 * i + k] = P[i] * gb[k] + gb[i] * Rv[k] + T[i] * qb[k] + (i == k ? d : R(0));
+}
+
+// One 3x3 tangent block K_ab of model M (weight w): the Neo-Hookean terms of
+// nh_block plus the model's extra terms (see hyper_point); ha = h_a.
+template <typename R, int M>
+__device__ __forceinline__ void hyper_block(const NHPoint<R>& m, const HyperExtra<R>& x, R w, const R ga_[3],
+                                            const R qa_[3], const R ha[3], const R Sha[3], const R gb[3],
+                                            const R qb[3], const R hb[3], R out[9]) {
+  nh_block(m, w, ga_, qa_, Sha, gb, qb, hb, out);
+  if constexpr (M == kMooneyRivlin) {
+    R bqa[3], bqb[3];
+    mat3vec(x.b, qa_, bqa);
+    mat3vec(x.b, qb, bqb);
+    const R hh = w * x.cb * (ha[0] * hb[0] + ha[1] * hb[1] + ha[2] * hb[2]);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+        out[3 * i + k] += w * x.cq * qa_[i] * qb[k] + hh * x.b[i][k] + w * x.cb * qb[i] * qa_[k] +
+                          w * x.cp * (ga_[i] * (x.I1 * qb[k] - bqb[k]) + (x.I1 * qa_[i] - bqa[i]) * gb[k]);
+  } else if constexpr (M == kYeoh) {
+    R za[3], zb[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      za[i] = x.zs * (qa_[i] - x.I1 * R(1.0 / 3.0) * ga_[i]);
+      zb[i] = x.zs * (qb[i] - x.I1 * R(1.0 / 3.0) * gb[i]);
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) out[3 * i + k] += w * x.psi11 * za[i] * zb[k];
+  }
 }
 
 // dS/dmu at fixed lambda (K = lambda + 2 mu / 3):

@@ -727,12 +727,8 @@ int32_t psx_assemble_hyperelastic(psx_plan* plan, int32_t model, const double* c
   for (int i = 0; i < n; ++i)
     if (!params[i]) return psx::fail(PSX_EINVAL, "null material parameter array");
   hipStream_t s = static_cast<hipStream_t>(stream);
-  if (plan->hp.npe != 4) {
-    if (model != PSX_NEO_HOOKEAN)
-      return psx::fail(PSX_EUNSUPPORTED, "Tet10 plans evaluate the Neo-Hookean model only");
-    return psx::launch_tet10_assemble(plan->d, u, params[0], params[1], f, kval, precision, s, status);
-  }
   const double* prm[4] = {params[0], params[1], n > 2 ? params[2] : nullptr, n > 3 ? params[3] : nullptr};
+  if (plan->hp.npe != 4) return psx::launch_tet10_hyper(plan->d, model, prm, u, f, kval, precision, s, status);
   return psx::launch_tet4_hyper(plan->d, model, prm, u, f, kval, precision, s, status);
 }
 

@@ -43,13 +43,13 @@ __host__ __device__ constexpr double tet10_dn(int q, int a, int j) {
   }
 }
 
-// Row p of a Tet10 element.  dX, dU: node coordinates/displacements minus
-// node 0's (index 0 is zero).  sink(v, value) receives the 90 block values
+// Row p of a Tet10 element with material model M (prm as load_params
+// gives them).  dX, dU: node coordinates/displacements minus node 0's
+// (index 0 is zero).  sink(v, value) receives the 90 block values
 // (v = 9 b + 3 i + k) then the 3 force values (v = 90 + i) once per
-// quadrature point; the caller sums them.  Returns min over points of det F
-// and sets jbad when det J <= 0 at some point.
-template <typename R, bool WANT_K, class Sink>
-__device__ __forceinline__ R tet10_row(const R dX[10][3], const R dU[10][3], R mu, R K, int p,
+// quadrature point; the caller sums them.  Returns min over points of det F.
+template <typename R, int M, bool WANT_K, class Sink>
+__device__ __forceinline__ R tet10_row(const R dX[10][3], const R dU[10][3], const R prm[4], int p,
                                        Sink&& sink) {
   R Jmin = R(1);
 #pragma unroll
@@ -96,7 +96,8 @@ __device__ __forceinline__ R tet10_row(const R dX[10][3], const R dU[10][3], R m
       }
     }
     NHPoint<R> m;
-    nh_point(A, mu, K, m);
+    HyperExtra<R> x;
+    hyper_point<R, M>(A, prm, m, x);
     Jmin = fmin(Jmin, m.J);
     R Shp[3], FSh[3];
     mat3vec(m.S, hp, Shp);
@@ -116,7 +117,7 @@ __device__ __forceinline__ R tet10_row(const R dX[10][3], const R dU[10][3], R m
                   R(tet10_dn(q, b, 2)) * adj[2][j]) * idet;
         mat3vec(m.Gm, h, gb);
         mat3vec(m.F, h, qb);
-        nh_block(m, w, gp, qp, Shp, gb, qb, h, blk);
+        hyper_block<R, M>(m, x, w, gp, qp, hp, Shp, gb, qb, h, blk);
 #pragma unroll
         for (int c = 0; c < 9; ++c) sink(9 * b + c, blk[c]);
       }
@@ -155,10 +156,9 @@ __device__ __forceinline__ void tet10_gather(const DevPlan& p, int e, R dX[10][3
 }
 
 // f_int only: one lane per incidence, f_a summed in FP64 over the run.
-template <typename R, int TILE>
-__global__ __launch_bounds__(TILE) void tet10_force_kernel(DevPlan p, const double* __restrict__ mu,
-                                                           const double* __restrict__ lam,
-                                                           double* __restrict__ f, int32_t* status) {
+template <typename R, int TILE, int M>
+__global__ __launch_bounds__(TILE) void tet10_force_kernel(DevPlan p, MatParams mp, double* __restrict__ f,
+                                                           int32_t* status) {
   __shared__ R contrib[3 * TILE];
   __shared__ int32_t s_incp[TILE + 1];
   const int tile = xcd_tile();
@@ -173,10 +173,11 @@ __global__ __launch_bounds__(TILE) void tet10_force_kernel(DevPlan p, const doub
     const int lp = int(pe >> kIncElemBits);
     R dX[10][3], dU[10][3];
     tet10_gather<R>(p, e, dX, dU);
-    const double m = mu[e], l = lam[e];
+    double pr[4];
+    load_params<M>(mp, e, pr);
+    const R prm[4] = {R(pr[0]), R(pr[1]), R(pr[2]), R(pr[3])};
     R fa[3] = {R(0), R(0), R(0)};
-    const R J = tet10_row<R, false>(dX, dU, R(m), R(l + (2.0 / 3.0) * m), lp,
-                                    [&](int v, R val) { fa[v - 90] += val; });
+    const R J = tet10_row<R, M, false>(dX, dU, prm, lp, [&](int v, R val) { fa[v - 90] += val; });
     if (J <= R(0)) {
       atomicMin(&status[1], e);
       status[0] = 1;
@@ -218,10 +219,8 @@ __device__ __forceinline__ int tet10_rank_pos(uint32_t w0, uint32_t w1, uint32_t
 // incidence order, passing over the lanes up to kTet10LanePasses times, and
 // write each 3x3 block once, three consecutive values per row.  A tile has
 // at most 64 nodes (the plan's rule), so the node lanes are one wave.
-template <typename R, int TILE>
-__global__ __launch_bounds__(TILE) void tet10_assemble_rows(DevPlan p, const double* __restrict__ mu,
-                                                            const double* __restrict__ lam,
-                                                            double* __restrict__ f,
+template <typename R, int TILE, int M>
+__global__ __launch_bounds__(TILE) void tet10_assemble_rows(DevPlan p, MatParams mp, double* __restrict__ f,
                                                             double* __restrict__ kval, int32_t* status) {
   constexpr int NV = 93;
   constexpr int UMAX = kTet10LanePasses * TILE;   // reduction lanes per tile
@@ -282,13 +281,14 @@ __global__ __launch_bounds__(TILE) void tet10_assemble_rows(DevPlan p, const dou
     for (int i = 0; i < 3; ++i) w[i] = p.inc_rank[3 * int64_t(k0 + t) + i];
     R dX[10][3], dU[10][3];
     tet10_gather<R>(p, e, dX, dU);
-    const double m = mu[e], l = lam[e];
+    double pr[4];
+    load_params<M>(mp, e, pr);
+    const R prm[4] = {R(pr[0]), R(pr[1]), R(pr[2]), R(pr[3])};
     // the four quadrature points accumulate in LDS (keeps 93 accumulators
     // out of registers)
 #pragma unroll
     for (int v = 0; v < NV; ++v) contrib[v * TILE + t] = R(0);
-    const R J = tet10_row<R, true>(dX, dU, R(m), R(l + (2.0 / 3.0) * m), lp,
-                                   [&](int v, R val) { contrib[v * TILE + t] += val; });
+    const R J = tet10_row<R, M, true>(dX, dU, prm, lp, [&](int v, R val) { contrib[v * TILE + t] += val; });
     if (J <= R(0)) {
       atomicMin(&status[1], e);
       status[0] = 1;
@@ -379,37 +379,55 @@ __global__ __launch_bounds__(TILE) void tet10_assemble_rows(DevPlan p, const dou
 }
 
 
-template <typename R, int TILE>
-static void launch_t10(const DevPlan& p, const double* mu, const double* lam, double* f, double* kval,
-                       hipStream_t s, int32_t* status) {
+template <typename R, int TILE, int M>
+static void launch_t10(const DevPlan& p, const MatParams& mp, double* f, double* kval, hipStream_t s,
+                       int32_t* status) {
   const int blocks = 8 * ((p.n_tiles + 7) / 8);
   if (kval)
-    hipLaunchKernelGGL((tet10_assemble_rows<R, TILE>), dim3(blocks), dim3(TILE), 0, s, p, mu, lam, f, kval,
-                       status);
+    hipLaunchKernelGGL((tet10_assemble_rows<R, TILE, M>), dim3(blocks), dim3(TILE), 0, s, p, mp, f, kval, status);
   else
-    hipLaunchKernelGGL((tet10_force_kernel<R, TILE>), dim3(blocks), dim3(TILE), 0, s, p, mu, lam, f, status);
+    hipLaunchKernelGGL((tet10_force_kernel<R, TILE, M>), dim3(blocks), dim3(TILE), 0, s, p, mp, f, status);
 }
 
-int32_t launch_tet10_assemble(const DevPlan& p, const double* u, const double* mu, const double* lam,
-                              double* f, double* kval, int precision, hipStream_t s, int32_t* status) {
-  int32_t rc = launch_permute_u(p, u, s, status);
-  if (rc) return rc;
-  if (p.n_tiles <= 0) return PSX_OK;
-  const bool fp32 = precision == PSX_FP32;
+template <int M>
+static int32_t launch_t10_tiles(const DevPlan& p, const MatParams& mp, double* f, double* kval, bool fp32,
+                                hipStream_t s, int32_t* status) {
   switch (p.tile) {
     case 64:
-      fp32 ? launch_t10<float, 64>(p, mu, lam, f, kval, s, status)
-           : launch_t10<double, 64>(p, mu, lam, f, kval, s, status);
+      fp32 ? launch_t10<float, 64, M>(p, mp, f, kval, s, status) : launch_t10<double, 64, M>(p, mp, f, kval, s, status);
       break;
     case 128:
-      fp32 ? launch_t10<float, 128>(p, mu, lam, f, kval, s, status)
-           : launch_t10<double, 128>(p, mu, lam, f, kval, s, status);
+      fp32 ? launch_t10<float, 128, M>(p, mp, f, kval, s, status)
+           : launch_t10<double, 128, M>(p, mp, f, kval, s, status);
       break;
     default:
       return fail(PSX_EINVAL, "Tet10 supports tiles of 64 or 128 incidences");
   }
+  return PSX_OK;
+}
+
+int32_t launch_tet10_hyper(const DevPlan& p, int model, const double* const* params, const double* u, double* f,
+                           double* kval, int precision, hipStream_t s, int32_t* status) {
+  int32_t rc = launch_permute_u(p, u, s, status);
+  if (rc) return rc;
+  if (p.n_tiles <= 0) return PSX_OK;
+  const bool fp32 = precision == PSX_FP32;
+  const MatParams mp{{params[0], params[1], params[2], params[3]}};
+  switch (model) {
+    case kNeoHookean: rc = launch_t10_tiles<kNeoHookean>(p, mp, f, kval, fp32, s, status); break;
+    case kMooneyRivlin: rc = launch_t10_tiles<kMooneyRivlin>(p, mp, f, kval, fp32, s, status); break;
+    case kYeoh: rc = launch_t10_tiles<kYeoh>(p, mp, f, kval, fp32, s, status); break;
+    default: return fail(PSX_EINVAL, "unknown material model");
+  }
+  if (rc) return rc;
   PSX_HIP_TRY(hipGetLastError());
   return PSX_OK;
+}
+
+int32_t launch_tet10_assemble(const DevPlan& p, const double* u, const double* mu, const double* lam,
+                              double* f, double* kval, int precision, hipStream_t s, int32_t* status) {
+  const double* params[4] = {mu, lam, nullptr, nullptr};
+  return launch_tet10_hyper(p, kNeoHookean, params, u, f, kval, precision, s, status);
 }
 
 // Element batch: one lane per element, the 30x30 tangent and f.
@@ -446,8 +464,7 @@ __global__ __launch_bounds__(64) void tet10_element_batch_kernel(int n, const do
       return;
     }
   }
-  const R m = R(mu[e]);
-  const R Kb = R(lam[e] + (2.0 / 3.0) * mu[e]);
+  const R prm[4] = {R(mu[e]), R(lam[e] + (2.0 / 3.0) * mu[e]), R(0), R(0)};
   double* ko = k ? k + 900 * int64_t(e) : nullptr;
   double* fo = f + 30 * int64_t(e);
   for (int p = 0; p < 10; ++p) {
@@ -457,7 +474,7 @@ __global__ __launch_bounds__(64) void tet10_element_batch_kernel(int n, const do
         for (int c = 0; c < 30; ++c) ko[30 * (3 * p + i) + c] = 0.0;
     R J;
     if (ko)
-      J = tet10_row<R, true>(dX, dU, m, Kb, p, [&](int v, R val) {
+      J = tet10_row<R, kNeoHookean, true>(dX, dU, prm, p, [&](int v, R val) {
         if (v >= 90) {
           fo[3 * p + (v - 90)] += double(val);
         } else {
@@ -466,7 +483,7 @@ __global__ __launch_bounds__(64) void tet10_element_batch_kernel(int n, const do
         }
       });
     else
-      J = tet10_row<R, false>(dX, dU, m, Kb, p, [&](int v, R val) {
+      J = tet10_row<R, kNeoHookean, false>(dX, dU, prm, p, [&](int v, R val) {
         if (v >= 90) fo[3 * p + (v - 90)] += double(val);
       });
     if (p == 0 && J <= R(0)) {

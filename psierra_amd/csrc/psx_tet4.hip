@@ -52,28 +52,6 @@ __device__ __forceinline__ void flag_inversion(int32_t* status, int e) {
   status[0] = 1;
 }
 
-// Per-element material parameter arrays (device, one value per element):
-// Neo-Hookean (mu, lambda); Mooney-Rivlin (c10, c01, K); Yeoh (c10, c20,
-// c30, K); F-bar Neo-Hookean (mu, lambda, and J_node - 1 per internal node).
-struct MatParams {
-  const double* v[4];
-};
-
-// Parameters of element e as the material point takes them (Neo-Hookean:
-// mu and K = lambda + 2 mu / 3).
-template <int M>
-__device__ __forceinline__ void load_params(const MatParams& mp, int64_t e, double pr[4]) {
-  if constexpr (M == kNeoHookean || M == kFbarNeoHookean) {
-    pr[0] = mp.v[0][e];
-    pr[1] = mp.v[1][e] + (2.0 / 3.0) * pr[0];
-    pr[2] = pr[3] = 0.0;
-  } else {
-    constexpr int n = M == kMooneyRivlin ? 3 : 4;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) pr[i] = i < n ? mp.v[i][e] : 0.0;
-  }
-}
-
 // Inputs of one incidence, in the row's node order: the row node first,
 // then the element's other three nodes in increasing local index.
 template <typename R>

@@ -74,8 +74,6 @@ class _GpuTotalLagrangian:
             import torch
 
             model, arrays = material_model(self._material, st["plan"].n_elements)
-            if model != "neohookean" and self._npe != 4:
-                raise NotImplementedError("the GPU Tet10 element evaluates the Neo-Hookean model only")
             dev = st["plan"].device
             st["model"] = model
             st["params"] = [torch.as_tensor(a, dtype=torch.float64, device=dev) for a in arrays]

@@ -3,7 +3,8 @@
 The oracle restates MooneyRivlin/Yeoh.compute_stress_and_tangent
 (src/psierra/material/mooney_rivlin.py:69-152, yeoh.py:69-148; pinned in
 test_oracle.py by finite differences of the strain energies) and the
-reference's Tet4 element (B^T C B V + G^T (I3 x S) G V, B^T S V).  Per-element
+reference's Tet4 and Tet10 elements (B^T C B V + G^T (I3 x S) G V, B^T S V;
+Tet10 summed over the 4-point rule).  Per-element
 parameters; tolerances as for Neo-Hookean: FP64 1e-12 (K) / 1e-11 (f), FP32
 1e-5.  Also: the Neo-Hookean model through psx_assemble_hyperelastic is
 bit-identical to psx_assemble, the force-only kernel matches the fused one,
@@ -32,13 +33,13 @@ def _params(model, E, seed):
 
 def _oracle(wl, model, params):
     code = {"mooney_rivlin": 1, "yeoh": 2}[model]
-    E = wl.mesh.n_elements
-    ke = np.empty((E, 12, 12))
-    fe = np.empty((E, 12))
+    E, npe = wl.mesh.elements.shape
+    element = fem.tet4_force_and_tangent if npe == 4 else fem.tet10_force_and_tangent
+    ke = np.empty((E, 3 * npe, 3 * npe))
+    fe = np.empty((E, 3 * npe))
     for e in range(E):
         c = wl.mesh.elements[e]
-        ke[e], fe[e] = fem.tet4_force_and_tangent(wl.mesh.nodes[c], wl.u[c],
-                                                  fem.material_fn(code, [p[e] for p in params]))
+        ke[e], fe[e] = element(wl.mesh.nodes[c], wl.u[c], fem.material_fn(code, [p[e] for p in params]))
     dofs = A.compute_element_dofs(wl.mesh.elements)
     rows, cols = A.create_coo_sparsity_indices(dofs)
     n = wl.mesh.n_dofs
@@ -102,15 +103,38 @@ def test_domain_hooks_and_linear_stiffness(box):
     assert rel_err(dom.assemble_stiffness_matrix().data, K_lin.data) < 1e-12
 
 
-def test_tet10_rejects_other_models():
+@pytest.mark.parametrize("model", ["mooney_rivlin", "yeoh"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_tet10_model_matches_oracle(model, precision):
+    from psierra_amd.engine import AssemblyPlan
+
+    wl = W.jittered_box_tet10(n=3, seed=32)
+    params = _params(model, wl.mesh.n_elements, seed=9)
+    f_ref, K_ref = _oracle(wl, model, params)
+    plan = AssemblyPlan(wl.mesh.nodes, wl.mesh.elements)
+    banner(f"hyperelastic tet10 {model} {precision}")
+    f, k = plan.assemble_hyperelastic(wl.u.ravel(), model, params, precision=precision)
+    f_only, _ = plan.assemble_hyperelastic(wl.u.ravel(), model, params, precision=precision, want_k=False)
+    f, k, f_only = f.cpu().numpy(), k.cpu().numpy(), f_only.cpu().numpy()
+    indptr, indices = plan.pattern()
+    assert np.array_equal(indptr, K_ref.indptr) and np.array_equal(indices, K_ref.indices)
+    tk, tf = (1e-12, 1e-11) if precision == "fp64" else (1e-5, 1e-5)
+    assert rel_err(k, K_ref.data) < tk
+    assert rel_err(f, f_ref) < tf
+    assert rel_err(f_only, f) < (1e-13 if precision == "fp64" else 1e-6)
+
+
+def test_tet10_domain_hooks_with_yeoh():
     from psierra_amd.domain import SolidDomain
     from psierra_amd.element import GpuTet10TotalLagrangian
     from psierra_amd.material import YeohField
 
     wl = W.jittered_box_tet10(n=2, seed=2)
-    E = wl.mesh.n_elements
-    mat = YeohField(np.full(E, 400.0), np.full(E, 60.0), np.full(E, 25.0), np.full(E, 6000.0))
+    c10, c20, c30, K = _params("yeoh", wl.mesh.n_elements, seed=10)
+    mat = YeohField(c10, c20, c30, K)
     dom = SolidDomain(wl.mesh, mat, GpuTet10TotalLagrangian(mat))
-    banner("tet10 + yeoh rejected")
-    with pytest.raises(NotImplementedError, match="Neo-Hookean model only"):
-        dom.assemble_internal_forces()
+    dom.update_state(u=wl.u.ravel())
+    banner("tet10 + yeoh domain hooks")
+    f_ref, K_ref = _oracle(wl, "yeoh", [c10, c20, c30, K])
+    assert rel_err(dom.assemble_internal_forces(), f_ref) < 1e-11
+    assert rel_err(dom.assemble_tangent_stiffness_matrix().data, K_ref.data) < 1e-12
