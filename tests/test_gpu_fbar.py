@@ -180,3 +180,39 @@ def test_fbar_relieves_locking_in_newton_solve():
             free = mgr.free_dofs
             assert np.linalg.norm((f_int - f_ext)[free]) < 1e-4 * np.linalg.norm(f_ext[free])
     assert defl["fbar"] > 1.15 * defl["tl"]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_fbar_isolated_nodes_and_single_element(precision):
+    from psierra_amd.mesh import TetMesh
+    from psierra_amd.workloads import Workload
+
+    nodes = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1], [5, 5, 5], [6, 5, 5.0]])
+    conn = np.array([[0, 1, 2, 3]], dtype=np.int64)
+    rng = np.random.default_rng(41)
+    wl = Workload("fbar_one", TetMesh(nodes, conn), 0.02 * rng.standard_normal(nodes.shape),
+                  np.array([1e3]), np.array([5e3]))
+    banner(f"fbar: single element + isolated nodes {precision}")
+    fb, f, k = _gpu(wl, precision)
+    f_ref = FB.internal_forces(nodes, conn, wl.u.reshape(-1), wl.mu, wl.lam)
+    K_ref = FB.tangent_stiffness(nodes, conn, wl.u.reshape(-1), wl.mu, wl.lam, BULK)
+    tf, tk = (1e-11, 1e-12) if precision == "fp64" else (1e-5, 1e-5)
+    assert rel_err(f, f_ref) < tf and _k_err(fb, k, K_ref) < tk
+    assert np.all(f[12:] == 0.0)
+    indptr, _ = fb.pattern()
+    assert np.all(np.diff(indptr)[12:] == 0)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_fbar_node_with_33_incidences(precision):
+    # a fan of 33 elements around node 0: the 64-bit feed masks of the row kernel
+    from tests.test_gpu_edge_cases import _fan
+
+    wl = _fan(33, seed=42)
+    banner(f"fbar: fan of 33 elements {precision}")
+    fb, f, k = _gpu(wl, precision)
+    X, conn, u = wl.mesh.nodes, wl.mesh.elements, wl.u.reshape(-1)
+    f_ref = FB.internal_forces(X, conn, u, wl.mu, wl.lam)
+    K_ref = FB.tangent_stiffness(X, conn, u, wl.mu, wl.lam, BULK)
+    tf, tk = (1e-11, 1e-12) if precision == "fp64" else (1e-5, 1e-5)
+    assert rel_err(f, f_ref) < tf and _k_err(fb, k, K_ref) < tk
