@@ -47,7 +47,10 @@ class NonlinearStaticAnalysis:
         self.linear_solver = linear_solver if linear_solver is not None else DirectSolver()
 
     def solve(self, dirichlet, f_ext=None, n_steps=10, nr_iterations=25, nr_tol=1e-4, verbose=False,
-              line_search=None) -> NonlinearStaticResult:
+              line_search=None, use_graphs: bool = False) -> NonlinearStaticResult:
+        """``use_graphs``: run every f_int and K_T assembly of the loop as a
+        replay of a HIP graph captured once per solve (same kernels, same
+        results)."""
         import torch
 
         dom = self.domain
@@ -74,9 +77,23 @@ class NonlinearStaticAnalysis:
         f_ext_free = torch.as_tensor(f_ext[free], device=dev)
         result = NonlinearStaticResult(dom.u.copy(), np.zeros(n), False, 0)
 
-        def force(u_vec):
-            f, _ = element.assemble_device(mesh, u_vec, want_k=False)
-            return f
+        if use_graphs:
+            g_force = element.graph_assembly(mesh, want_k=False)
+            g_tangent = element.graph_assembly(mesh, want_k=True)
+
+            def force(u_vec):
+                return g_force.replay(u_vec, check_status=True)[0]
+
+            def tangent(u_vec):
+                return g_tangent.replay(u_vec, check_status=True)[1]
+        else:
+
+            def force(u_vec):
+                f, _ = element.assemble_device(mesh, u_vec, want_k=False)
+                return f
+
+            def tangent(u_vec):
+                return element.assemble_device(mesh, u_vec, want_k=True)[1]
 
         for step in range(1, n_steps + 1):
             lf = step / n_steps
@@ -107,8 +124,7 @@ class NonlinearStaticAnalysis:
                 if rnorm < tol:
                     converged = True
                     break
-                _, kval = element.assemble_device(mesh, u, want_k=True)
-                K_ff = system.csc(kval)
+                K_ff = system.csc(tangent(u))
                 try:
                     du = self.linear_solver.solve(K_ff, r.cpu().numpy())
                 except (RuntimeError, np.linalg.LinAlgError):

@@ -120,6 +120,36 @@ class _GpuTotalLagrangian:
         except ValueError as err:
             raise self._relabel(err) from None
 
+    def _nnz(self, mesh) -> int:
+        return self._state(mesh)["plan"].nnz
+
+    def _raise_status(self, mesh, u) -> None:
+        try:
+            self._state(mesh)["plan"].raise_on_status(u)
+        except ValueError as err:
+            raise self._relabel(err) from None
+
+    def graph_assembly(self, mesh, want_k: bool = True, precision: str | None = None):
+        """This element's assembly on ``mesh`` captured in a HIP graph
+        (``graph.GraphedCall``): ``replay(u, check_status)`` returns the device
+        (f, kval) buffers, overwritten by the next replay."""
+        import torch
+
+        from .graph import GraphedCall
+
+        plan = self._state(mesh)["plan"]
+        dev = plan.device
+        u = torch.zeros(plan.n_dofs, dtype=torch.float64, device=dev)
+        f = torch.empty(plan.n_dofs, dtype=torch.float64, device=dev)
+        k = torch.empty(self._nnz(mesh), dtype=torch.float64, device=dev) if want_k else None
+        prec = precision or self.precision
+        self.assemble_device(mesh, u, want_k=want_k, f=f, kval=k, check_status=False, precision=prec)
+
+        def launch():
+            self.assemble_device(mesh, u, want_k=want_k, f=f, kval=k, check_status=False, precision=prec)
+
+        return GraphedCall(launch, u, (f, k), check=lambda uu: self._raise_status(mesh, uu))
+
     # -- SolidDomain hooks (src/psierra/domain/solid.py:238-241, 288-291) ---
     def assemble_internal_forces(self, mesh, displacement) -> np.ndarray:
         f, _ = self.assemble_device(mesh, np.asarray(displacement, dtype=np.float64).reshape(-1), want_k=False)
@@ -311,12 +341,22 @@ class GpuTet4Fbar(GpuTet4TotalLagrangian):
             raise NotImplementedError("the GPU F-bar element evaluates the Neo-Hookean model only")
         if "fbar" not in st:
             st["fbar"] = engine.FbarAssembly(st["plan"])
+        key = (st["params_key"], self.bulk_modulus)
+        if st.get("bulk_key") != key:
+            st["bulk"] = self._bulk(st)
+            st["bulk_key"] = key
         return st
+
+    def _nnz(self, mesh) -> int:
+        return self._fbar(mesh)["fbar"].nnz
+
+    def _raise_status(self, mesh, u) -> None:
+        self._fbar(mesh)["fbar"].raise_on_status(u)
 
     def assemble_device(self, mesh, u, want_k: bool = True, f=None, kval=None, check_status: bool = True,
                         precision: str | None = None):
         st = self._fbar(mesh)
-        return st["fbar"].assemble(u, st["mu"], st["lam"], self._bulk(st), f=f, kval=kval,
+        return st["fbar"].assemble(u, st["mu"], st["lam"], st["bulk"], f=f, kval=kval,
                                    precision=precision or self.precision, want_k=want_k, check_status=check_status)
 
     def free_dof_system(self, mesh, free_dofs):

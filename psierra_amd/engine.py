@@ -331,16 +331,22 @@ class FbarAssembly:
             )
         )
         if check_status:
-            st = plan.status.cpu().numpy()
-            if st[0] != 0:
-                e = int(st[1])
-                conn = plan.elements[e]
-                X = plan.nodes[conn]
-                ue = u.reshape(-1, 3)[torch.as_tensor(conn, device=self.device)].cpu().numpy()
-                Dinv = np.linalg.inv((X[1:] - X[0]).T)
-                H = np.vstack([-Dinv.sum(axis=0), Dinv])
-                raise ValueError(FBAR_INVERSION_MSG.format(e, _det3(np.identity(3) + ue.T @ H)))
+            self.raise_on_status(u)
         return f, kval
+
+    def raise_on_status(self, u):
+        """Tet4Fbar's exception for the lowest inverted element of the last call."""
+        torch = _torch()
+        plan = self.plan
+        st = plan.status.cpu().numpy()
+        if st[0] != 0:
+            e = int(st[1])
+            conn = plan.elements[e]
+            X = plan.nodes[conn]
+            ue = u.reshape(-1, 3)[torch.as_tensor(conn, device=self.device)].cpu().numpy()
+            Dinv = np.linalg.inv((X[1:] - X[0]).T)
+            H = np.vstack([-Dinv.sum(axis=0), Dinv])
+            raise ValueError(FBAR_INVERSION_MSG.format(e, _det3(np.identity(3) + ue.T @ H)))
 
 
 def element_batch(X, u, mu, lam, want_k=True, precision="fp64"):

@@ -6,6 +6,10 @@ a HIP graph through torch's graph API; ``replay()`` re-runs it on whatever
 the captured input buffers hold, which is what a Newton loop needs: copy the
 new iterate into ``u`` and replay.
 
+``GraphedCall`` does the same for any assembly call (the elements'
+``graph_assembly`` builds one per mesh for the Newton loop of
+``psierra_amd.analysis``).
+
 ``autotune_tile`` times the work-group tile sizes the kernels are built for
 on the caller's own plan and inputs and keeps the fastest.
 """
@@ -14,6 +18,39 @@ from __future__ import annotations
 import numpy as np
 
 from .engine import AssemblyPlan, _torch
+
+
+class GraphedCall:
+    """A HIP graph of ``launch()``, which reads the device buffer ``u`` and
+    writes ``outputs`` (no allocation, memset or host sync inside).
+    ``replay(u)`` copies the new iterate into ``u`` and replays; ``check``
+    (optional) raises for a bad status after a replay."""
+
+    def __init__(self, launch, u, outputs, check=None, warmup: int = 2):
+        torch = _torch()
+        self.u = u
+        self.outputs = outputs
+        self._launch = launch
+        self._check = check
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            for _ in range(warmup):
+                launch()
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            launch()
+        torch.cuda.synchronize()
+
+    def replay(self, u=None, check_status: bool = False):
+        if u is not None:
+            self.u.copy_(u.reshape(-1))
+        self.graph.replay()
+        if check_status and self._check is not None:
+            self._check(self.u)
+        return self.outputs
 
 
 class GraphedAssembly:

@@ -119,3 +119,50 @@ def test_loaded_cantilever_solve_matches_oracle():
     assert rel_err(res.displacement, u_ref) < 1e-8
     assert res.iteration_history == iters
     assert abs(res.displacement[3 * tip + 2].mean()) > 1e-3
+
+
+@pytest.mark.parametrize("element_name", ["tl_fp32", "fbar"])
+def test_graph_replayed_newton_solve_equals_eager(element_name):
+    # every f_int / K_T assembly of the loop replayed from a HIP graph, with
+    # the loop's own eager work (residuals, CSC extraction, line search) in
+    # between: the iterates must be the eager solve's bit for bit
+    from psierra_amd.analysis import NonlinearStaticAnalysis
+    from psierra_amd.boundary import DirichletConstraint, DirichletManager, select_nodes_by_plane
+    from psierra_amd.domain import SolidDomain
+    from psierra_amd.element import GpuTet4Fbar, GpuTet4TotalLagrangian
+    from psierra_amd.material import IsotropicElasticParameters, NeoHookean, NeoHookeanField
+
+    if element_name == "tl_fp32":
+        wl = W.cantilever_tet4()
+        mesh = wl.mesh
+        mgr = _clamped(wl)
+        tip = select_nodes_by_plane(mesh, "yz", "max")
+        f_ext = np.zeros(mesh.n_dofs)
+        f_ext[3 * tip + 2] = -0.05 / len(tip)
+        mat = NeoHookeanField(wl.mu, wl.lam)
+        make = lambda: GpuTet4TotalLagrangian(mat, precision="fp32")
+    else:
+        # the nearly incompressible beam of the F-bar locking test
+        from tests.test_gpu_fbar import _beam
+
+        mesh = _beam()
+        params = IsotropicElasticParameters()
+        params.set_from_youngs_poisson(youngs_modulus=1e5, poissons_ratio=0.499, mass_density=1000.0)
+        mat = NeoHookean(params)
+        mgr = DirichletManager(mesh.n_dofs)
+        mgr.add_constraint(DirichletConstraint(select_nodes_by_plane(mesh, "yz", "min"), (0, 1, 2), 0.0))
+        tip = select_nodes_by_plane(mesh, "yz", "max")
+        f_ext = np.zeros(mesh.n_dofs)
+        f_ext[3 * tip + 1] = 5.0 / len(tip)
+        make = lambda: GpuTet4Fbar(mat)
+    results = []
+    for use_graphs in (False, True):
+        dom = SolidDomain(mesh, mat, make())
+        banner(f"newton {element_name} graphs={use_graphs}")
+        res = NonlinearStaticAnalysis(dom).solve(mgr, f_ext=f_ext, n_steps=2, nr_iterations=15, nr_tol=1e-4,
+                                                 use_graphs=use_graphs)
+        assert res.converged and res.steps_completed == 2
+        results.append(res)
+    assert np.array_equal(results[0].displacement, results[1].displacement)
+    assert results[0].iteration_history == results[1].iteration_history
+    assert results[0].residual_history == results[1].residual_history
