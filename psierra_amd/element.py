@@ -29,7 +29,9 @@ class _GpuTotalLagrangian:
     supports_element_indexing: bool = True
     _npe = 4
 
-    def __init__(self, material, precision: str = "fp64", tile: int = 0) -> None:
+    def __init__(self, material, precision: str = "fp64", tile=0) -> None:
+        """tile: work-group tile (0 = the plan's default), or "auto" to time
+        the supported tiles on the first assembly and keep the fastest."""
         engine.precision_code(precision)
         self._material = material
         self.precision = precision
@@ -67,7 +69,9 @@ class _GpuTotalLagrangian:
             elements = np.asarray(mesh.elements, dtype=np.int64)
             if elements.shape[1] != self._npe:
                 raise ValueError(f"{type(self).__name__} needs {self._npe}-node elements")
-            st = {"mesh": mesh, "plan": engine.AssemblyPlan(mesh.nodes, elements, self.tile), "params_key": None}
+            tile = 0 if self.tile == "auto" else int(self.tile)
+            st = {"mesh": mesh, "plan": engine.AssemblyPlan(mesh.nodes, elements, tile), "params_key": None,
+                  "tuned": self.tile != "auto"}
             self._cache = {key: st}
         pkey = (id(self._material), getattr(self._material, "version", 0))
         if st["params_key"] != pkey:
@@ -102,10 +106,36 @@ class _GpuTotalLagrangian:
                                           f"Element inversion detected in {self._inversion_name}: ", 1))
         return err
 
+    def _autotune(self, mesh, u, precision, reps: int = 3) -> None:
+        """Time each supported tile on this mesh and input and keep the
+        fastest (every tile gives the same bits)."""
+        import torch
+
+        st = self._state(mesh)
+        st["tuned"] = True
+        plan = st["plan"]
+        tiles = (64, 128, 256) if plan.npe == 4 else (64, 128)
+        timings = {}
+        for tile in tiles:
+            plan.set_tile(tile)
+            self.assemble_device(mesh, u, want_k=True, precision=precision)
+            torch.cuda.synchronize()
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            for _ in range(reps):
+                self.assemble_device(mesh, u, want_k=True, check_status=False, precision=precision)
+            b.record()
+            torch.cuda.synchronize()
+            timings[tile] = a.elapsed_time(b) / reps
+        st["tile_timings"] = timings
+        plan.set_tile(min(timings, key=timings.get))
+
     def assemble_device(self, mesh, u, want_k: bool = True, f=None, kval=None, check_status: bool = True,
                         precision: str | None = None):
         """f_int and K values as device tensors (u may be a device tensor)."""
         st = self._state(mesh)
+        if not st["tuned"]:
+            self._autotune(mesh, u, precision or self.precision)
         prec = precision or self.precision
         try:
             if st["model"] == "neohookean":
@@ -313,7 +343,7 @@ class GpuTet4Fbar(GpuTet4TotalLagrangian):
 
     supports_element_indexing = True
 
-    def __init__(self, material, precision: str = "fp64", tile: int = 0, bulk_modulus: float | None = None) -> None:
+    def __init__(self, material, precision: str = "fp64", tile=0, bulk_modulus: float | None = None) -> None:
         super().__init__(material, precision=precision, tile=tile)
         self.bulk_modulus = bulk_modulus
 
@@ -356,6 +386,8 @@ class GpuTet4Fbar(GpuTet4TotalLagrangian):
     def assemble_device(self, mesh, u, want_k: bool = True, f=None, kval=None, check_status: bool = True,
                         precision: str | None = None):
         st = self._fbar(mesh)
+        if not st["tuned"]:
+            self._autotune(mesh, u, precision or self.precision)
         return st["fbar"].assemble(u, st["mu"], st["lam"], st["bulk"], f=f, kval=kval,
                                    precision=precision or self.precision, want_k=want_k, check_status=check_status)
 

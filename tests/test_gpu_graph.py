@@ -53,3 +53,24 @@ def test_autotune_picks_a_supported_tile():
                             torch.as_tensor(wl.lam, device=dev))
     assert best in (64, 128, 256) and plan.info["tile"] == best
     assert all(np.isfinite(v) and v > 0 for v in t.values())
+
+
+@pytest.mark.parametrize("kind", ["tl", "fbar"])
+def test_element_autotuned_tile(kind):
+    from psierra_amd.element import GpuTet4Fbar, GpuTet4TotalLagrangian
+    from psierra_amd.material import NeoHookeanField
+
+    wl = W.jittered_box_tet4(n=8, seed=22)
+    mu = np.full(wl.mesh.n_elements, 1.0e3)
+    mat = NeoHookeanField(mu, 5.0 * mu)
+    cls = GpuTet4TotalLagrangian if kind == "tl" else GpuTet4Fbar
+    banner(f"autotuned tile {kind}")
+    auto = cls(mat, precision="fp32", tile="auto")
+    fixed = cls(mat, precision="fp32")
+    f1, k1 = auto.assemble_device(wl.mesh, wl.u.ravel())
+    f2, k2 = fixed.assemble_device(wl.mesh, wl.u.ravel())
+    st = auto._state(wl.mesh)
+    assert set(st["tile_timings"]) == {64, 128, 256}
+    assert st["plan"].info["tile"] == min(st["tile_timings"], key=st["tile_timings"].get)
+    # every tile gives the same bits
+    assert bool((f1 == f2).all()) and bool((k1 == k2).all())
