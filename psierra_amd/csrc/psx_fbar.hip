@@ -11,7 +11,7 @@
 //      with K, the volumetric gradient g_a over a's element neighbours,
 //      g_a[b] = sum_{e in a, b} w_e q_b / V0_a (one lane per neighbour b,
 //      incidences in run order);
-//   3. fbar_kvol_kernel, one wave per node p: its distance-2 row of
+//   3. fbar_kvol_kernel, half a wave per node p: its distance-2 row of
 //      K_vol[p, q] = K_bulk sum_a V0_a g_a[p] g_a[q]^T over the common
 //      neighbours a (ascending; q's slots in p's row listed with the plan),
 //      accumulated in LDS and written to every value of the row;
@@ -146,7 +146,7 @@ __global__ __launch_bounds__(256) void fbar_node_kernel(DevPlan p, FbarDev fb, i
   }
 }
 
-// One wave per node p: K_vol[p, q] = K sum_a V0_a g_a[p] g_a[q]^T for the
+// Half a wave per node p: K_vol[p, q] = K sum_a V0_a g_a[p] g_a[q]^T for the
 // whole distance-2 row of p, accumulated in LDS.  The common neighbours a
 // are taken in ascending order (p's element-pattern units); for each, the
 // lanes read a's units and their slots in p's row contiguously (one lane per
@@ -155,21 +155,23 @@ __global__ __launch_bounds__(256) void fbar_node_kernel(DevPlan p, FbarDev fb, i
 // with consecutive lanes on consecutive values.
 __global__ __launch_bounds__(256) void fbar_kvol_kernel(DevPlan p, FbarDev fb, int waves, double bulk, double* kval) {
   extern __shared__ double s_acc[];
+  // two nodes per wave, 32 lanes each (a node's neighbours rarely exceed 32)
   const int w = int(threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  const int pn = xcd_tile() * waves + w;
+  const int half = int(threadIdx.x >> 5) & 1;
+  const int lane = threadIdx.x & 31;
+  const int pn = 2 * (xcd_tile() * waves + w) + half;
   if (pn >= p.N) return;
   const int4 rec2 = fb.node_rec2[pn];
   const int deg2 = rec2.y, n9 = 9 * deg2;
-  double* acc = s_acc + size_t(w) * 9 * fb.max_deg2;
-  for (int i = lane; i < n9; i += 64) acc[i] = 0.0;
+  double* acc = s_acc + size_t(2 * w + half) * 9 * fb.max_deg2;
+  for (int i = lane; i < n9; i += 32) acc[i] = 0.0;
   __builtin_amdgcn_wave_barrier();
   const int u1b = p.uoff[pn], u1e = p.uoff[pn + 1];
   for (int u = u1b; u < u1e; ++u) {
     const int4 ku = fb.kv_unit[u];   // {first unit of a, deg a, unit (a, p), first slot entry}
     const double4 gp = fb.g[ku.z];
     const double wp[3] = {gp.w * gp.x, gp.w * gp.y, gp.w * gp.z};
-    for (int j = lane; j < ku.y; j += 64) {
+    for (int j = lane; j < ku.y; j += 32) {
       const double4 gq = fb.g[ku.x + j];
       double* a9 = acc + 9 * int(fb.kv_slot[ku.w + j]);
       const double q3[3] = {gq.x, gq.y, gq.z};
@@ -182,7 +184,7 @@ __global__ __launch_bounds__(256) void fbar_kvol_kernel(DevPlan p, FbarDev fb, i
   }
   // CSR order of p's rows: value r * 3 deg2 + 3 s + c
   const int row = 3 * deg2;
-  for (int i = lane; i < n9; i += 64) {
+  for (int i = lane; i < n9; i += 32) {
     const int r = i / row, rem = i - r * row;
     const int s2 = rem / 3, c = rem - 3 * s2;
     kval[rec2.x + i] = bulk * acc[9 * s2 + 3 * r + c];
@@ -199,11 +201,11 @@ int32_t launch_tet4_fbar(const DevPlan& p, const FbarDev& fb, const double* u, c
   if (p.E > 0) hipLaunchKernelGGL(fbar_element_kernel, grid(p.E, 256), dim3(256), 0, s, p, fb, status);
   hipLaunchKernelGGL(fbar_node_kernel, grid(p.N, 4), dim3(256), 0, s, p, fb, kval ? 1 : 0);
   if (kval && fb.max_deg2 > 0) {
-    // up to four nodes (one wave each) per work-group within 64 KiB of LDS
-    const size_t per_wave = size_t(9) * fb.max_deg2 * sizeof(double);
+    // up to four waves (two nodes each) per work-group within 64 KiB of LDS
+    const size_t per_wave = size_t(2 * 9) * fb.max_deg2 * sizeof(double);
     const int waves = int(std::max<size_t>(1, std::min<size_t>(4, (size_t(64) << 10) / per_wave)));
-    hipLaunchKernelGGL(fbar_kvol_kernel, grid(p.N, waves), dim3(64 * waves), per_wave * waves, s, p, fb, waves, bulk,
-                       kval);
+    hipLaunchKernelGGL(fbar_kvol_kernel, grid(p.N, 2 * waves), dim3(64 * waves), per_wave * waves, s, p, fb, waves,
+                       bulk, kval);
   }
   PSX_HIP_TRY(hipGetLastError());
   const FbarView fv{fb.node_rec2, fb.slot2};
