@@ -148,6 +148,11 @@ int32_t psx_assemble_hyperelastic(psx_plan* plan, int32_t model, const double* c
  * object must not overlap. */
 typedef struct psx_fbar psx_fbar;
 int32_t psx_fbar_create(psx_plan* plan, psx_fbar** out);
+/* The same pattern from the Tet4 connectivity alone (host only, no GPU):
+ * conn n_elements x 4 node ids (row-major int64). */
+int32_t psx_fbar_pattern_nnz(const int64_t* conn, int64_t n_elements, int64_t n_nodes, int64_t* nnz);
+int32_t psx_fbar_pattern_build(const int64_t* conn, int64_t n_elements, int64_t n_nodes, int32_t* indptr,
+                               int32_t* indices);
 int32_t psx_fbar_destroy(psx_fbar* fbar);
 int32_t psx_fbar_nnz(const psx_fbar* fbar, int64_t* nnz);
 int32_t psx_fbar_pattern(const psx_fbar* fbar, int32_t* indptr, int32_t* indices);

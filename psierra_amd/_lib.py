@@ -71,6 +71,8 @@ SIGNATURES = {
     "psx_fbar_destroy": (c_int32, [c_void_p]),
     "psx_fbar_nnz": (c_int32, [c_void_p, _i64p]),
     "psx_fbar_pattern": (c_int32, [c_void_p, c_void_p, c_void_p]),
+    "psx_fbar_pattern_nnz": (c_int32, [c_void_p, c_int64, c_int64, _i64p]),
+    "psx_fbar_pattern_build": (c_int32, [c_void_p, c_int64, c_int64, c_void_p, c_void_p]),
     "psx_fbar_csc_create": (c_int32, [c_void_p, c_void_p, c_int64, POINTER(c_void_p)]),
     "psx_fbar_assemble": (
         c_int32,

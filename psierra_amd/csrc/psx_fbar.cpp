@@ -49,11 +49,12 @@ void free_all(psx_fbar* fb) {
   fb->allocations.clear();
 }
 
-int32_t build(psx_plan* plan, psx_fbar* fb) {
-  const HostPattern& hp = plan->hp;
+// Distance-2 node adjacency (user order): for each node, the sorted union of
+// its element neighbours' element neighbours.
+int32_t build_distance2(const HostPattern& hp, std::vector<int32_t>& adj2_ptr, std::vector<int32_t>& adj2,
+                        std::vector<int32_t>& deg2) {
   const int64_t N = hp.N;
-  // distance-2 adjacency (user order): union of the neighbours' neighbours
-  std::vector<int32_t> deg2(N);
+  deg2.assign(N, 0);
   auto collect = [&](int64_t a, std::vector<int32_t>& buf) {
     buf.clear();
     for (int32_t i = hp.adj_ptr[a]; i < hp.adj_ptr[a + 1]; ++i) {
@@ -70,18 +71,45 @@ int32_t build(psx_plan* plan, psx_fbar* fb) {
       deg2[a] = int32_t(buf.size());
     }
   });
-  fb->adj2_ptr.assign(N + 1, 0);
-  for (int64_t a = 0; a < N; ++a) fb->adj2_ptr[a + 1] = fb->adj2_ptr[a] + deg2[a];
-  fb->nnz2 = 9 * int64_t(fb->adj2_ptr[N]);
-  if (fb->nnz2 >= (int64_t(1) << 31)) return fail(PSX_EUNSUPPORTED, "F-bar nnz >= 2^31 is not supported (int32 CSR)");
-  fb->adj2.resize(fb->adj2_ptr[N]);
+  adj2_ptr.assign(N + 1, 0);
+  for (int64_t a = 0; a < N; ++a) adj2_ptr[a + 1] = adj2_ptr[a] + deg2[a];
+  if (9 * int64_t(adj2_ptr[N]) >= (int64_t(1) << 31))
+    return fail(PSX_EUNSUPPORTED, "F-bar nnz >= 2^31 is not supported (int32 CSR)");
+  adj2.resize(adj2_ptr[N]);
   parallel_nodes(N, [&](int64_t b, int64_t e) {
     std::vector<int32_t> buf;
     for (int64_t a = b; a < e; ++a) {
       collect(a, buf);
-      std::copy(buf.begin(), buf.end(), fb->adj2.begin() + fb->adj2_ptr[a]);
+      std::copy(buf.begin(), buf.end(), adj2.begin() + adj2_ptr[a]);
     }
   });
+  return PSX_OK;
+}
+
+void fill_csr2(int64_t N, const std::vector<int32_t>& adj2_ptr, const std::vector<int32_t>& adj2, int32_t* indptr,
+               int32_t* indices) {
+  parallel_nodes(N, [&](int64_t b, int64_t e) {
+    for (int64_t a = b; a < e; ++a) {
+      const int32_t d = adj2_ptr[a + 1] - adj2_ptr[a];
+      const int64_t base = 9 * int64_t(adj2_ptr[a]);
+      for (int r = 0; r < 3; ++r) {
+        indptr[3 * a + r] = int32_t(base + 3 * r * d);
+        int32_t* o = indices + base + 3 * r * d;
+        for (int32_t s = 0; s < d; ++s)
+          for (int c = 0; c < 3; ++c) o[3 * s + c] = 3 * adj2[adj2_ptr[a] + s] + c;
+      }
+    }
+  });
+  indptr[3 * N] = int32_t(9 * int64_t(adj2_ptr[N]));
+}
+
+int32_t build(psx_plan* plan, psx_fbar* fb) {
+  const HostPattern& hp = plan->hp;
+  const int64_t N = hp.N;
+  std::vector<int32_t> deg2;
+  int32_t rc0 = build_distance2(hp, fb->adj2_ptr, fb->adj2, deg2);
+  if (rc0) return rc0;
+  fb->nnz2 = 9 * int64_t(fb->adj2_ptr[N]);
 
   // internal-order layout
   const std::vector<int32_t>& perm = plan->perm_h;   // internal -> user
@@ -220,20 +248,30 @@ int32_t psx_fbar_nnz(const psx_fbar* fbar, int64_t* nnz) {
 int32_t psx_fbar_pattern(const psx_fbar* fbar, int32_t* indptr, int32_t* indices) {
   psx::clear_error();
   if (!fbar || !indptr || !indices) return psx::fail(PSX_EINVAL, "null argument");
-  const int64_t N = fbar->plan->hp.N;
-  psx::parallel_nodes(N, [&](int64_t b, int64_t e) {
-    for (int64_t a = b; a < e; ++a) {
-      const int32_t d = fbar->adj2_ptr[a + 1] - fbar->adj2_ptr[a];
-      const int64_t base = 9 * int64_t(fbar->adj2_ptr[a]);
-      for (int r = 0; r < 3; ++r) {
-        indptr[3 * a + r] = int32_t(base + 3 * r * d);
-        int32_t* o = indices + base + 3 * r * d;
-        for (int32_t s = 0; s < d; ++s)
-          for (int c = 0; c < 3; ++c) o[3 * s + c] = 3 * fbar->adj2[fbar->adj2_ptr[a] + s] + c;
-      }
-    }
-  });
-  indptr[3 * N] = int32_t(fbar->nnz2);
+  psx::fill_csr2(fbar->plan->hp.N, fbar->adj2_ptr, fbar->adj2, indptr, indices);
+  return PSX_OK;
+}
+
+int32_t psx_fbar_pattern_nnz(const int64_t* conn, int64_t n_elements, int64_t n_nodes, int64_t* nnz) {
+  psx::clear_error();
+  if (!nnz) return psx::fail(PSX_EINVAL, "nnz is null");
+  psx::HostPattern hp;
+  std::vector<int32_t> ptr, adj2, deg2;
+  int32_t rc = psx::build_host_pattern(conn, n_elements, 4, n_nodes, hp);
+  if (rc || (rc = psx::build_distance2(hp, ptr, adj2, deg2))) return rc;
+  *nnz = 9 * int64_t(ptr[n_nodes]);
+  return PSX_OK;
+}
+
+int32_t psx_fbar_pattern_build(const int64_t* conn, int64_t n_elements, int64_t n_nodes, int32_t* indptr,
+                               int32_t* indices) {
+  psx::clear_error();
+  if (!indptr || !indices) return psx::fail(PSX_EINVAL, "indptr/indices is null");
+  psx::HostPattern hp;
+  std::vector<int32_t> ptr, adj2, deg2;
+  int32_t rc = psx::build_host_pattern(conn, n_elements, 4, n_nodes, hp);
+  if (rc || (rc = psx::build_distance2(hp, ptr, adj2, deg2))) return rc;
+  psx::fill_csr2(n_nodes, ptr, adj2, indptr, indices);
   return PSX_OK;
 }
 

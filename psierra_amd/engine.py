@@ -53,6 +53,21 @@ def pattern_host(elements: np.ndarray, n_nodes: int) -> tuple[np.ndarray, np.nda
     return indptr, indices
 
 
+def fbar_pattern_host(elements: np.ndarray, n_nodes: int) -> tuple[np.ndarray, np.ndarray]:
+    """CSR (indptr, indices) of Tet4Fbar's K_dev + K_vol (node pairs sharing
+    an element or a neighbouring node), built natively on the host."""
+    conn = np.ascontiguousarray(elements, dtype=np.int64)
+    E, npe = conn.shape
+    if npe != 4:
+        raise ValueError("the F-bar pattern is defined for Tet4 connectivity")
+    nnz = ctypes.c_int64(0)
+    check(lib().psx_fbar_pattern_nnz(ptr(conn), E, n_nodes, ctypes.byref(nnz)))
+    indptr = np.empty(3 * n_nodes + 1, dtype=np.int32)
+    indices = np.empty(nnz.value, dtype=np.int32)
+    check(lib().psx_fbar_pattern_build(ptr(conn), E, n_nodes, ptr(indptr), ptr(indices)))
+    return indptr, indices
+
+
 def _det3(M):
     return float(np.linalg.det(M))
 

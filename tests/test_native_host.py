@@ -81,3 +81,20 @@ def test_last_error_is_per_call():
     ok = np.array([[0, 1, 2, 3]], np.int64)
     assert L.psx_pattern_nnz(_lib.ptr(ok), 1, 4, 4, ctypes.byref(nnz)) == 0
     assert _lib.last_error() == "" and nnz.value == 144
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_fbar_pattern_bit_exact_vs_oracle(seed):
+    from oracle import fbar as FB
+    from psierra_amd.engine import fbar_pattern_host
+
+    n_nodes, conn = _random_mesh(seed, 4)
+    ip_ref, ix_ref = FB.distance2_pattern(conn, n_nodes)
+    ip, ix = fbar_pattern_host(conn, n_nodes)
+    assert ip.dtype == np.int32 and ix.dtype == np.int32
+    assert np.array_equal(ip, ip_ref)
+    assert np.array_equal(ix, ix_ref)
+    # it contains the element pattern
+    ip1, ix1 = pattern_host(conn, n_nodes)
+    for r in range(0, 3 * n_nodes, 7):
+        assert set(ix1[ip1[r]:ip1[r + 1]]) <= set(ix[ip[r]:ip[r + 1]])
