@@ -142,15 +142,24 @@ __global__ __launch_bounds__(256) void ecsw_project_kernel(DevEcsw d, const doub
         for (int i = 0; i < 3; ++i) s_f[12 * el + 3 * a + i] = out[36 + i];
       }
       __syncthreads();
-      // B_e = (w k_e) Phi_e
-      for (int o = t; o < nb * kPhi; o += 256) {
-        const int e = o / kPhi, r = (o / kRank) % 12, j = o % kRank;
-        const float* kr = s_k + 144 * e + 12 * r;
+      // B_e = (w k_e) Phi_e: thread (e, j) forms column j of B_e, its 12
+      // Phi values held in registers; the k_e rows are the same for the 64
+      // lanes of a wave (LDS broadcast)
+      for (int o = t; o < nb * kRank; o += 256) {
+        const int e = o >> 6, j = o & (kRank - 1);
         const float* ph = s_phi + e * kPhi + j;
-        float s = 0.0f;
+        float pc[12];
 #pragma unroll
-        for (int c = 0; c < 12; ++c) s += kr[c] * ph[c * kRank];
-        s_B[o] = s;
+        for (int c = 0; c < 12; ++c) pc[c] = ph[c * kRank];
+        const float* ke = s_k + 144 * e;
+        float* bo = s_B + e * kPhi + j;
+#pragma unroll
+        for (int r = 0; r < 12; ++r) {
+          float s = 0.0f;
+#pragma unroll
+          for (int c = 0; c < 12; ++c) s += ke[12 * r + c] * pc[c];
+          bo[r * kRank] = s;
+        }
       }
       if (t < kRank) {
         for (int e = 0; e < nb; ++e)
