@@ -11,12 +11,13 @@
 // The tangent projection is one GEMM per instance: with A = [Phi_e] and
 // B = [w_e k_e Phi_e] stacked over the RID (12 |RID| x 64 each),
 // K_r = A^T B.  Every work item (an instance and a chunk of its RID) runs in
-// a persistent launch; per batch of EB elements a work-group stages Phi_e in
-// LDS (the next batch's Phi_e is already in flight into registers),
-// evaluates u_e = Phi_e u_r, the element rows of k_e and f_e (FP32, one lane
-// per (element, row node)), forms w k_e Phi_e, and its four waves
-// accumulate the four 32x32 quadrants of A^T B with v_mfma_f32_32x32x2f32
-// (FP32 in, FP32 accumulate).  Each work item writes its 64x64 partial; a
+// a persistent launch.  Per super-batch of SB elements a work-group forms
+// u_e = Phi_e u_r and the element rows of k_e and f_e (FP32, one lane per
+// (element, row node), all SB elements at once); then per batch of EB
+// elements it stages Phi_e in LDS (the next batch's Phi_e is already in
+// flight into registers), forms w k_e Phi_e, and its four waves accumulate
+// the four 32x32 quadrants of A^T B with v_mfma_f32_32x32x2f32 (FP32 in,
+// FP32 accumulate).  Each work item writes its 64x64 partial; a
 // second kernel adds the partials of an instance in a fixed order, 16x16
 // tile pairs per work-group, symmetrises (model.py:317) and writes FP64.
 #include <algorithm>
@@ -31,7 +32,8 @@
 namespace psx {
 
 constexpr int kRank = 64;
-constexpr int kEB = 8;        // elements staged per batch
+constexpr int kEB = 8;        // elements staged per matrix-core batch
+constexpr int kSB = 32;       // elements whose rows are evaluated together (4 lanes each)
 constexpr int kChunk = 128;   // RID elements per work item
 constexpr int kPhi = 12 * kRank;
 
@@ -54,9 +56,9 @@ __global__ __launch_bounds__(256) void ecsw_project_kernel(DevEcsw d, const doub
   constexpr int kPre = kEB * kPhi / 4 / 256;   // float4 of Phi per thread per batch
   __shared__ float s_phi[kEB * kPhi];
   __shared__ float s_B[kEB * kPhi];
-  __shared__ float s_k[kEB * 144];
-  __shared__ float s_f[kEB * 12];
-  __shared__ float s_ue[kEB * 12];
+  __shared__ float s_k[kSB * 144];
+  __shared__ float s_f[kSB * 12];
+  __shared__ float s_ue[kSB * 12];
   __shared__ float s_ur[kRank];
   const int t = threadIdx.x;
   const int lane = t & 63;
@@ -79,33 +81,30 @@ __global__ __launch_bounds__(256) void ecsw_project_kernel(DevEcsw d, const doub
 #pragma unroll
     for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
     float fr = 0.0f;
-    for (int b0 = 0; b0 < cnt; b0 += kEB) {
-      const int nb = min(kEB, cnt - b0);
-      __syncthreads();   // the previous batch's MFMA reads of s_phi / s_B are done
-#pragma unroll
-      for (int i = 0; i < kPre; ++i)
-        if (t + 256 * i < nb * (kPhi / 4)) reinterpret_cast<float4*>(s_phi)[t + 256 * i] = pre[i];
-      __syncthreads();
-      if (b0 + kEB < cnt) {
-        // the next batch's Phi_e, in flight during this batch
-        const int n4 = min(kEB, cnt - b0 - kEB) * (kPhi / 4);
-        const float4* src = reinterpret_cast<const float4*>(d.phi + int64_t(e0 + b0 + kEB) * kPhi);
-#pragma unroll
-        for (int i = 0; i < kPre; ++i)
-          pre[i] = t + 256 * i < n4 ? src[t + 256 * i] : make_float4(0.f, 0.f, 0.f, 0.f);
-      }
-      if (t < nb * 12) {
-        const float* row = s_phi + (t / 12) * kPhi + (t % 12) * kRank;
+    for (int s0 = 0; s0 < cnt; s0 += kSB) {
+      const int ns = min(kSB, cnt - s0);
+      __syncthreads();   // s_ur is written; the previous super-batch's reads of s_k / s_f are done
+      // u_e = Phi_e u_r for the super-batch, one (element, dof) per thread,
+      // Phi rows read from global memory (they are staged again, from L2,
+      // for the matrix-core batches below)
+      for (int o = t; o < ns * 12; o += 256) {
+        const float4* row = reinterpret_cast<const float4*>(d.phi + int64_t(e0 + s0) * kPhi + int64_t(o) * kRank);
         float s = 0.0f;
-#pragma unroll 16
-        for (int j = 0; j < kRank; ++j) s += row[j] * s_ur[j];
-        s_ue[t] = s;
+#pragma unroll 4
+        for (int j4 = 0; j4 < kRank / 4; ++j4) {
+          const float4 v = row[j4];
+          s += v.x * s_ur[4 * j4];
+          s += v.y * s_ur[4 * j4 + 1];
+          s += v.z * s_ur[4 * j4 + 2];
+          s += v.w * s_ur[4 * j4 + 3];
+        }
+        s_ue[o] = s;
       }
       __syncthreads();
-      if (t < nb * 4) {
-        // row a of element el: K_{a b} for the four b and f_a
+      // the element rows of k_e and f_e, one (element, row node) per thread
+      if (t < ns * 4) {
         const int el = t >> 2, a = t & 3;
-        const int e = e0 + b0 + el;
+        const int e = e0 + s0 + el;
         const int r1 = a == 0 ? 1 : 0, r2 = a <= 1 ? 2 : 1, r3 = a <= 2 ? 3 : 2;
         const int rot[4] = {a, r1, r2, r3};
         const float4 prm = d.geom[5 * int64_t(e) + 4];
@@ -141,40 +140,56 @@ __global__ __launch_bounds__(256) void ecsw_project_kernel(DevEcsw d, const doub
 #pragma unroll
         for (int i = 0; i < 3; ++i) s_f[12 * el + 3 * a + i] = out[36 + i];
       }
-      __syncthreads();
-      // B_e = (w k_e) Phi_e: thread (e, j) forms column j of B_e, its 12
-      // Phi values held in registers; the k_e rows are the same for the 64
-      // lanes of a wave (LDS broadcast)
-      for (int o = t; o < nb * kRank; o += 256) {
-        const int e = o >> 6, j = o & (kRank - 1);
-        const float* ph = s_phi + e * kPhi + j;
-        float pc[12];
+      for (int b0 = s0; b0 < s0 + ns; b0 += kEB) {
+        const int nb = min(kEB, cnt - b0);
+        const int lb = b0 - s0;   // first element of the batch within the super-batch
+        __syncthreads();   // the rows are written; the previous batch's MFMA reads of s_phi / s_B are done
 #pragma unroll
-        for (int c = 0; c < 12; ++c) pc[c] = ph[c * kRank];
-        const float* ke = s_k + 144 * e;
-        float* bo = s_B + e * kPhi + j;
+        for (int i = 0; i < kPre; ++i)
+          if (t + 256 * i < nb * (kPhi / 4)) reinterpret_cast<float4*>(s_phi)[t + 256 * i] = pre[i];
+        __syncthreads();
+        if (b0 + kEB < cnt) {
+          // the next batch's Phi_e, in flight during this batch
+          const int n4 = min(kEB, cnt - b0 - kEB) * (kPhi / 4);
+          const float4* src = reinterpret_cast<const float4*>(d.phi + int64_t(e0 + b0 + kEB) * kPhi);
 #pragma unroll
-        for (int r = 0; r < 12; ++r) {
-          float s = 0.0f;
-#pragma unroll
-          for (int c = 0; c < 12; ++c) s += ke[12 * r + c] * pc[c];
-          bo[r * kRank] = s;
+          for (int i = 0; i < kPre; ++i)
+            pre[i] = t + 256 * i < n4 ? src[t + 256 * i] : make_float4(0.f, 0.f, 0.f, 0.f);
         }
-      }
-      if (t < kRank) {
-        for (int e = 0; e < nb; ++e)
+        // B_e = (w k_e) Phi_e: thread (e, j) forms column j of B_e, its 12
+        // Phi values held in registers; the k_e rows are the same for the 64
+        // lanes of a wave (LDS broadcast)
+        for (int o = t; o < nb * kRank; o += 256) {
+          const int e = o >> 6, j = o & (kRank - 1);
+          const float* ph = s_phi + e * kPhi + j;
+          float pc[12];
 #pragma unroll
-          for (int r = 0; r < 12; ++r) fr += s_f[12 * e + r] * s_phi[e * kPhi + r * kRank + t];
-      }
-      __syncthreads();
-      // A^T B on the matrix cores: quadrant (qI, qJ), K-steps of 2 rows.
-      for (int e = 0; e < nb; ++e) {
+          for (int c = 0; c < 12; ++c) pc[c] = ph[c * kRank];
+          const float* ke = s_k + 144 * (lb + e);
+          float* bo = s_B + e * kPhi + j;
 #pragma unroll
-        for (int kp = 0; kp < 6; ++kp) {
-          const int row = 2 * kp + (lane >> 5);
-          const float av = s_phi[e * kPhi + row * kRank + 32 * qI + (lane & 31)];
-          const float bv = s_B[e * kPhi + row * kRank + 32 * qJ + (lane & 31)];
-          acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, acc, 0, 0, 0);
+          for (int r = 0; r < 12; ++r) {
+            float s = 0.0f;
+#pragma unroll
+            for (int c = 0; c < 12; ++c) s += ke[12 * r + c] * pc[c];
+            bo[r * kRank] = s;
+          }
+        }
+        if (t < kRank) {
+          for (int e = 0; e < nb; ++e)
+#pragma unroll
+            for (int r = 0; r < 12; ++r) fr += s_f[12 * (lb + e) + r] * s_phi[e * kPhi + r * kRank + t];
+        }
+        __syncthreads();
+        // A^T B on the matrix cores: quadrant (qI, qJ), K-steps of 2 rows.
+        for (int e = 0; e < nb; ++e) {
+#pragma unroll
+          for (int kp = 0; kp < 6; ++kp) {
+            const int row = 2 * kp + (lane >> 5);
+            const float av = s_phi[e * kPhi + row * kRank + 32 * qI + (lane & 31)];
+            const float bv = s_B[e * kPhi + row * kRank + 32 * qJ + (lane & 31)];
+            acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, acc, 0, 0, 0);
+          }
         }
       }
     }
