@@ -15,9 +15,10 @@
 // u_e = Phi_e u_r and the element rows of k_e and f_e (FP32, one lane per
 // (element, row node), all SB elements at once); then per batch of EB
 // elements it stages Phi_e in LDS (the next batch's Phi_e is already in
-// flight into registers), forms w k_e Phi_e, and its four waves accumulate
-// the four 32x32 quadrants of A^T B with v_mfma_f32_32x32x2f32 (FP32 in,
-// FP32 accumulate).  Each work item writes its 64x64 partial; a
+// flight into registers) and each of its four waves accumulates a 64x16
+// column strip of A^T B with v_mfma_f32_16x16x4f32 (FP32 in, FP32
+// accumulate), forming its own 16 columns of w k_e Phi_e in registers as
+// the matrix-core operand.  Each work item writes its 64x64 partial; a
 // second kernel adds the partials of an instance in a fixed order, 16x16
 // tile pairs per work-group, symmetrises (model.py:317) and writes FP64.
 #include <algorithm>
@@ -33,11 +34,11 @@ namespace psx {
 
 constexpr int kRank = 64;
 constexpr int kEB = 8;        // elements staged per matrix-core batch
-constexpr int kSB = 32;       // elements whose rows are evaluated together (4 lanes each)
+constexpr int kSB = 16;       // elements whose rows are evaluated together (4 lanes each)
 constexpr int kChunk = 128;   // RID elements per work item
 constexpr int kPhi = 12 * kRank;
 
-typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 struct DevEcsw {
   int32_t n_inst, n_elem, n_items;
@@ -51,19 +52,20 @@ struct DevEcsw {
   float* partF;              // [n_items][64]
 };
 
-__global__ __launch_bounds__(256) void ecsw_project_kernel(DevEcsw d, const double* __restrict__ ur_all,
-                                                           int32_t* status) {
+// 35.6 KB of LDS per work-group lets four groups share a CU; the register
+// budget is held to match (four waves per SIMD).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void ecsw_project_kernel(
+    DevEcsw d, const double* __restrict__ ur_all, int32_t* status) {
   constexpr int kPre = kEB * kPhi / 4 / 256;   // float4 of Phi per thread per batch
-  __shared__ float s_phi[kEB * kPhi];
-  __shared__ float s_B[kEB * kPhi];
-  __shared__ float s_k[kSB * 144];
+  __shared__ __align__(16) float s_phi[kEB * kPhi];
+  __shared__ __align__(16) float s_k[kSB * 144];
   __shared__ float s_f[kSB * 12];
   __shared__ float s_ue[kSB * 12];
   __shared__ float s_ur[kRank];
   const int t = threadIdx.x;
   const int lane = t & 63;
   const int wv = t >> 6;
-  const int qI = wv >> 1, qJ = wv & 1;
+  const int kq = lane >> 4, x = lane & 15;   // matrix-core operand row (k) and column
   for (int item = blockIdx.x; item < d.n_items; item += gridDim.x) {
     const int inst = d.item_inst[item];
     const int e0 = d.item_e0[item];
@@ -77,9 +79,11 @@ __global__ __launch_bounds__(256) void ecsw_project_kernel(DevEcsw d, const doub
     }
     __syncthreads();   // the previous item's reads of s_ur are done
     if (t < kRank) s_ur[t] = float(ur_all[int64_t(inst) * kRank + t]);
-    floatx16 acc;
+    floatx4 acc[4];
 #pragma unroll
-    for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
+    for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[mt][r] = 0.0f;
     float fr = 0.0f;
     for (int s0 = 0; s0 < cnt; s0 += kSB) {
       const int ns = min(kSB, cnt - s0);
@@ -143,7 +147,7 @@ __global__ __launch_bounds__(256) void ecsw_project_kernel(DevEcsw d, const doub
       for (int b0 = s0; b0 < s0 + ns; b0 += kEB) {
         const int nb = min(kEB, cnt - b0);
         const int lb = b0 - s0;   // first element of the batch within the super-batch
-        __syncthreads();   // the rows are written; the previous batch's MFMA reads of s_phi / s_B are done
+        __syncthreads();   // the rows are written; the previous batch's reads of s_phi are done
 #pragma unroll
         for (int i = 0; i < kPre; ++i)
           if (t + 256 * i < nb * (kPhi / 4)) reinterpret_cast<float4*>(s_phi)[t + 256 * i] = pre[i];
@@ -156,51 +160,56 @@ __global__ __launch_bounds__(256) void ecsw_project_kernel(DevEcsw d, const doub
           for (int i = 0; i < kPre; ++i)
             pre[i] = t + 256 * i < n4 ? src[t + 256 * i] : make_float4(0.f, 0.f, 0.f, 0.f);
         }
-        // B_e = (w k_e) Phi_e: thread (e, j) forms column j of B_e, its 12
-        // Phi values held in registers; the k_e rows are the same for the 64
-        // lanes of a wave (LDS broadcast)
-        for (int o = t; o < nb * kRank; o += 256) {
-          const int e = o >> 6, j = o & (kRank - 1);
-          const float* ph = s_phi + e * kPhi + j;
-          float pc[12];
-#pragma unroll
-          for (int c = 0; c < 12; ++c) pc[c] = ph[c * kRank];
-          const float* ke = s_k + 144 * (lb + e);
-          float* bo = s_B + e * kPhi + j;
-#pragma unroll
-          for (int r = 0; r < 12; ++r) {
-            float s = 0.0f;
-#pragma unroll
-            for (int c = 0; c < 12; ++c) s += ke[12 * r + c] * pc[c];
-            bo[r * kRank] = s;
-          }
-        }
         if (t < kRank) {
           for (int e = 0; e < nb; ++e)
 #pragma unroll
             for (int r = 0; r < 12; ++r) fr += s_f[12 * (lb + e) + r] * s_phi[e * kPhi + r * kRank + t];
         }
-        __syncthreads();
-        // A^T B on the matrix cores: quadrant (qI, qJ), K-steps of 2 rows.
+        // A^T B on the matrix cores, wave wv owning columns 16 wv .. 16 wv + 15.
+        // Per element and K-step ks (dof rows 4 ks .. 4 ks + 3), lane (kq, x)
+        // supplies B[4 ks + kq][16 wv + x] = sum_c (w k_e)[4 ks + kq][c] Phi_e[c][16 wv + x]
+        // (its Phi column in registers) and, for M-tile mt, A = Phi_e[4 ks + kq][4 x + mt]
+        // (one 16-byte LDS read for the four tiles): tile mt accumulates
+        // K_r[4 m + mt][16 wv + n].
         for (int e = 0; e < nb; ++e) {
+          const float* ph = s_phi + e * kPhi;
+          const float* ke = s_k + 144 * (lb + e);
+          float pc[12];
 #pragma unroll
-          for (int kp = 0; kp < 6; ++kp) {
-            const int row = 2 * kp + (lane >> 5);
-            const float av = s_phi[e * kPhi + row * kRank + 32 * qI + (lane & 31)];
-            const float bv = s_B[e * kPhi + row * kRank + 32 * qJ + (lane & 31)];
-            acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, acc, 0, 0, 0);
+          for (int c = 0; c < 12; ++c) pc[c] = ph[c * kRank + 16 * wv + x];
+#pragma unroll
+          for (int ks = 0; ks < 3; ++ks) {
+            const int r = 4 * ks + kq;
+            const float4* kr = reinterpret_cast<const float4*>(ke + 12 * r);
+            const float4 k0 = kr[0], k1 = kr[1], k2 = kr[2];
+            float bv = k0.x * pc[0];
+            bv += k0.y * pc[1];
+            bv += k0.z * pc[2];
+            bv += k0.w * pc[3];
+            bv += k1.x * pc[4];
+            bv += k1.y * pc[5];
+            bv += k1.z * pc[6];
+            bv += k1.w * pc[7];
+            bv += k2.x * pc[8];
+            bv += k2.y * pc[9];
+            bv += k2.z * pc[10];
+            bv += k2.w * pc[11];
+            const float4 av = reinterpret_cast<const float4*>(ph + r * kRank)[x];
+            acc[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.x, bv, acc[0], 0, 0, 0);
+            acc[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.y, bv, acc[1], 0, 0, 0);
+            acc[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.z, bv, acc[2], 0, 0, 0);
+            acc[3] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.w, bv, acc[3], 0, 0, 0);
           }
         }
       }
     }
-    // 32x32 accumulator layout: register r of lane l holds
-    // row 8 (r / 4) + 4 (l / 32) + r % 4, column l % 32.
+    // 16x16 accumulator layout: register r of lane l holds row
+    // m = 4 (l / 16) + r, column n = l % 16 of the tile, i.e. K_r row 4 m + mt.
     float* out = d.partK + int64_t(item) * kRank * kRank;
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      const int i = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-      out[(32 * qI + i) * kRank + 32 * qJ + (lane & 31)] = acc[r];
-    }
+    for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) out[(16 * kq + 4 * r + mt) * kRank + 16 * wv + x] = acc[mt][r];
     if (t < kRank) d.partF[int64_t(item) * kRank + t] = fr;
   }
 }
@@ -224,6 +233,8 @@ __global__ __launch_bounds__(256) void ecsw_reduce_kernel(DevEcsw d, double* __r
   const int i = t >> 4, j = t & 15;
   const int i0 = d.inst_item0[inst], i1 = d.inst_item0[inst + 1];
   double a = 0.0, b = 0.0;
+  // the loads run ahead of the in-order FP64 sums
+#pragma unroll 8
   for (int it = i0; it < i1; ++it) {
     const float* P = d.partK + int64_t(it) * kRank * kRank;
     a += double(P[(16 * I + i) * kRank + 16 * J + j]);
@@ -238,6 +249,7 @@ __global__ __launch_bounds__(256) void ecsw_reduce_kernel(DevEcsw d, double* __r
   K[(16 * J + j) * kRank + 16 * I + i] = v;
   if (I == 0 && J == 0 && t < kRank) {
     double s = 0.0;
+#pragma unroll 8
     for (int it = i0; it < i1; ++it) s += double(d.partF[int64_t(it) * kRank + t]);
     f_r[int64_t(inst) * kRank + t] = s;
   }
@@ -348,7 +360,11 @@ int32_t ecsw_create(int32_t n_inst, const int64_t* rid_offsets, const double* Xe
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, h->device) == hipSuccess && prop.multiProcessorCount > 0)
     cus = prop.multiProcessorCount;
-  h->grid = int(std::min<size_t>(n_items, size_t(cus) * 2));
+  // as many resident work-groups as the kernel's LDS and registers allow
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ecsw_project_kernel, 256, 0) != hipSuccess || per_cu < 1)
+    per_cu = 2;
+  h->grid = int(std::min<size_t>(n_items, size_t(cus) * size_t(per_cu)));
   *out = h;
   return PSX_OK;
 }
