@@ -54,6 +54,36 @@ def cpu_baseline(wl, sample_elems, seed=0):
     return sample_elems / t_loop, sample_elems / t_vec, t_loop, t_vec
 
 
+def single_thread():
+    """numpy and its BLAS limited to one thread, so a CPU baseline's core
+    count is 1."""
+    from threadpoolctl import threadpool_limits
+
+    return threadpool_limits(1)
+
+
+def cpu_entry(value, unit, sample, **extra):
+    """The ``cpu_baseline`` object of a bench line: the oracle (the FP64
+    restatement of the reference, "port") timed on one host core."""
+    out = {"value": value, "unit": unit, "cores": 1, "kind": "port", "sample": sample,
+           "cpu_model": platform.processor() or platform.machine(), "numpy": np.__version__}
+    out.update(extra)
+    return out
+
+
+def cpu_until(fn, items, min_s):
+    """Run fn over items until at least min_s seconds have passed (one item at
+    least); returns (work units done, seconds, items used)."""
+    done, used = 0, 0
+    t0 = time.perf_counter()
+    for it in items:
+        done += fn(it)
+        used += 1
+        if time.perf_counter() - t0 >= min_s:
+            break
+    return done, time.perf_counter() - t0, used
+
+
 PROFILE_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles")
 
 
@@ -123,8 +153,18 @@ def bench_tet10(args):
                    "speedup_fp32_vs_fp64_plain": res["fp64"] / res["fp32"], "tile": plan.info["tile"]},
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": ach / HBM_PEAK_GBPS, "traffic": None},
-        "cpu_baseline": None,
+        "cpu_baseline": None if args.no_cpu else tet10_cpu(wl),
     }
+
+
+def tet10_cpu(wl, sample=3000):
+    with single_thread():
+        loop_eps, vec_eps, t_loop, t_vec = cpu_baseline(wl, sample)
+    return cpu_entry(loop_eps, "elements/s",
+                     f"{sample} seeded elements of the same Tet10 mesh; FP64 oracle per-element loop (4-point "
+                     f"rule, element evaluated twice, add.at, coo.tocsr) took {t_loop:.2f}s; vectorised FP64 "
+                     f"numpy on the same sample: {vec_eps:.4g} elements/s ({t_vec:.2f}s)",
+                     vectorised_value=vec_eps)
 
 
 def bench_fbar(args):
@@ -166,8 +206,27 @@ def bench_fbar(args):
                    "element_pattern_nnz": plan.nnz, "fp64_plain_ms": res["fp64"], "setup_s": t_setup},
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": ach / HBM_PEAK_GBPS, "traffic": None},
-        "cpu_baseline": None,
+        "cpu_baseline": None if args.no_cpu else fbar_cpu(),
     }
+
+
+def fbar_cpu(n=20):
+    """Tet4Fbar's domain assembly is vectorised numpy in the reference
+    (tet4_fbar.py:264-459); the oracle restates it the same way."""
+    from oracle import fbar as FB
+    from psierra_amd import workloads as W
+
+    wl = W.jittered_box_tet4(n=n, seed=1)
+    bulk = float(np.mean(wl.lam + 2.0 / 3.0 * wl.mu))
+    m = wl.mesh
+    with single_thread():
+        t0 = time.perf_counter()
+        FB.internal_forces(m.nodes, m.elements, wl.u, wl.mu, wl.lam)
+        FB.tangent_stiffness(m.nodes, m.elements, wl.u, wl.mu, wl.lam, bulk)
+        t = time.perf_counter() - t0
+    return cpu_entry(m.n_elements / t, "elements/s",
+                     f"the same generator at {n}^3 hexes ({m.n_elements} tets); FP64 oracle f_int and "
+                     f"K_dev + K_vol (vectorised numpy, scipy sparse) took {t:.2f}s")
 
 
 def bench_vjp(args):
@@ -192,8 +251,20 @@ def bench_vjp(args):
         "warmup": args.warmup, "ms_per_step": res["fp32"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
         "config": {"workload": wl.name, "elements": E, "fp64_ms": res["fp64"]},
-        "roofline": None, "cpu_baseline": None,
+        "roofline": None, "cpu_baseline": None if args.no_cpu else vjp_cpu(wl),
     }
+
+
+def vjp_cpu(wl, sample=20000):
+    """The reference has no adjoint (its parameter gradients are finite
+    differences); the CPU path to K_T w is assembling K_T, timed here."""
+    with single_thread():
+        loop_eps, vec_eps, t_loop, t_vec = cpu_baseline(wl, sample)
+    return cpu_entry(loop_eps, "elements/s",
+                     f"{sample} seeded elements of the same mesh; K_T by the FP64 oracle per-element loop "
+                     f"(add.at, coo.tocsr) took {t_loop:.2f}s, vectorised {vec_eps:.4g} elements/s; the sparse "
+                     f"product K_T w and the parameter gradients are not included",
+                     vectorised_value=vec_eps)
 
 
 def bench_ecsw(args):
@@ -224,8 +295,23 @@ def bench_ecsw(args):
                    "total_gflop": flop_total / 1e9, "setup_s": t_build},
         "roofline": {"bound": "mfma", "achieved": tf, "peak": 157.3, "unit": "TFLOP/s", "frac": tf / 157.3,
                      "traffic": None},
-        "cpu_baseline": None,
+        "cpu_baseline": None if args.no_cpu else ecsw_cpu(insts),
     }
+
+
+def ecsw_cpu(insts, min_s=4.0):
+    from oracle import ecsw as EC
+
+    def one(ins):
+        EC.evaluate_forces_and_stiffness(ins["mesh"].nodes, ins["mesh"].elements, ins["basis"], ins["rid"],
+                                         ins["weights"], ins["mu"], ins["lam"], ins["u_r"])
+        return len(ins["rid"])
+
+    with single_thread():
+        done, t, used = cpu_until(one, insts, min_s)
+    return cpu_entry(done / t, "elements/s",
+                     f"the first {used} of the same instances ({done} RID elements); FP64 oracle of "
+                     f"evaluate_forces_and_stiffness (vectorised numpy over each RID) took {t:.2f}s")
 
 
 def bench_ecsw_train(args):
@@ -271,8 +357,27 @@ def bench_ecsw_train(args):
                    "A_bytes": 8 * rows * E},
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": ach / HBM_PEAK_GBPS, "traffic": None},
-        "cpu_baseline": None,
+        "cpu_baseline": None if args.no_cpu else ecsw_train_cpu(r, m),
     }
+
+
+def ecsw_train_cpu(r, m, n=12):
+    from oracle import ecsw as EC
+    from psierra_amd import workloads as W
+
+    wl = W.vjp_box_tet4(n=n, seed=3)
+    rng = np.random.default_rng(6)
+    nd = wl.mesh.n_dofs
+    phi = np.linalg.qr(rng.standard_normal((nd, r)))[0]
+    snaps = 1e-4 * rng.standard_normal((nd, m)) + wl.u.reshape(-1, 1)
+    with single_thread():
+        t0 = time.perf_counter()
+        EC.training_matrix(wl.mesh.nodes, wl.mesh.elements, phi, snaps, wl.mu, wl.lam, include_stiffness=True)
+        t = time.perf_counter() - t0
+    E = wl.mesh.n_elements
+    return cpu_entry(E * m / t, "element-snapshots/s",
+                     f"the same recipe on a {n}^3 box ({E} tets, rank {r}, {m} snapshots, stiffness rows); "
+                     f"FP64 oracle of ECSWTrainingMatrix.build took {t:.2f}s")
 
 
 def main():
@@ -312,7 +417,8 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        loop_eps, vec_eps, t_loop, t_vec = cpu_baseline(wl, args.cpu_sample)
+        with single_thread():
+            loop_eps, vec_eps, t_loop, t_vec = cpu_baseline(wl, args.cpu_sample)
         cpu = {
             "value": loop_eps,
             "unit": "elements/s",

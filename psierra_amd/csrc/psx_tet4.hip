@@ -296,6 +296,21 @@ __global__ __launch_bounds__(TILE) void tet4_assemble_rows(DevPlan p, MatParams 
       m3 = s_mask[2 * TILE + u];
     }
   }
+  // F-bar: the K_vol values this unit adds onto, loaded now so their latency
+  // hides behind the sums
+  double old[FBAR ? 9 : 1];
+  double* odst = kval;
+  int od = 0;
+  if constexpr (FBAR) {
+    if (offd) {
+      od = s_f2[NN + j];
+      odst = kval + s_kofs[j] + 3 * fv.slot2[u0 + t - nd];
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) old[3 * r + c] = odst[r * 3 * od + c];
+    }
+  }
   double acc[9];
 #pragma unroll
   for (int c = 0; c < 9; ++c) acc[c] = 0.0;
@@ -362,13 +377,19 @@ __global__ __launch_bounds__(TILE) void tet4_assemble_rows(DevPlan p, MatParams 
     walk(m2 | m3, [&](int a) { return ((m2 >> a) & 1) ? 0 : 9; });
     // each 3x3 block once; a unit lane's rows are three consecutive values,
     // so neighbouring slots give contiguous runs
-    const int d = FBAR ? s_f2[NN + j] : s_uoff[j + 1] - s_uoff[j];
-    const int col = FBAR ? fv.slot2[s_uoff[j] + slot] : slot;
-    double* dst = kval + s_kofs[j] + 3 * col;
+    if constexpr (FBAR) {
 #pragma unroll
-    for (int r = 0; r < 3; ++r)
+      for (int r = 0; r < 3; ++r)
 #pragma unroll
-      for (int c = 0; c < 3; ++c) put(dst + r * 3 * d + c, acc[3 * r + c]);
+        for (int c = 0; c < 3; ++c) odst[r * 3 * od + c] = old[3 * r + c] + acc[3 * r + c];
+    } else {
+      const int d = s_uoff[j + 1] - s_uoff[j];
+      double* dst = kval + s_kofs[j] + 3 * slot;
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) put(dst + r * 3 * d + c, acc[3 * r + c]);
+    }
   }
 
 }
