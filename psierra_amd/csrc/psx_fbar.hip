@@ -14,7 +14,8 @@
 //   3. fbar_kvol_kernel, half a wave per node p: its distance-2 row of
 //      K_vol[p, q] = K_bulk sum_a V0_a g_a[p] g_a[q]^T over the common
 //      neighbours a (ascending; q's slots in p's row listed with the plan),
-//      accumulated in LDS and written to every value of the row;
+//      accumulated in LDS and written to every value of the row; p's unit
+//      records are read in one round and each unit's feeds one unit ahead;
 //   4. the Tet4 row kernel with the F-bar material point (J_bar from step 2,
 //      deviatoric tangent and f), adding K_dev onto the K_vol values.
 // Every sum runs in an order fixed by the mesh (wave sums in a fixed
@@ -160,35 +161,100 @@ __global__ __launch_bounds__(256) void fbar_kvol_kernel(DevPlan p, FbarDev fb, i
   const int half = int(threadIdx.x >> 5) & 1;
   const int lane = threadIdx.x & 31;
   const int pn = 2 * (xcd_tile() * waves + w) + half;
-  if (pn >= p.N) return;
-  const int4 rec2 = fb.node_rec2[pn];
+  const bool live = pn < p.N;
+  if (__ballot(live) == 0ull) return;   // the wave's lanes read each other's records below
+  int4 rec2 = make_int4(0, 0, 0, 0);
+  int u1b = 0, u1e = 0;
+  if (live) {
+    rec2 = fb.node_rec2[pn];
+    u1b = p.uoff[pn];
+    u1e = p.uoff[pn + 1];
+  }
   const int deg2 = rec2.y, n9 = 9 * deg2;
   double* acc = s_acc + size_t(2 * w + half) * 9 * fb.max_deg2;
   for (int i = lane; i < n9; i += 32) acc[i] = 0.0;
   __builtin_amdgcn_wave_barrier();
-  const int u1b = p.uoff[pn], u1e = p.uoff[pn + 1];
-  for (int u = u1b; u < u1e; ++u) {
-    const int4 ku = fb.kv_unit[u];   // {first unit of a, deg a, unit (a, p), first slot entry}
-    const double4 gp = fb.g[ku.z];
-    const double wp[3] = {gp.w * gp.x, gp.w * gp.y, gp.w * gp.z};
-    for (int j = lane; j < ku.y; j += 32) {
-      const double4 gq = fb.g[ku.x + j];
-      double* a9 = acc + 9 * int(fb.kv_slot[ku.w + j]);
-      const double q3[3] = {gq.x, gq.y, gq.z};
-#pragma unroll
-      for (int r = 0; r < 3; ++r)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) a9[3 * r + c] += wp[r] * q3[c];
+  const int nu = u1e - u1b;   // p's element neighbours a (units (p, a))
+  // both halves step through max(nu) units together
+  const int nmax = __builtin_amdgcn_readfirstlane(max(__builtin_amdgcn_readlane(nu, 0), __builtin_amdgcn_readlane(nu, 32)));
+  for (int cb = 0; cb < nmax; cb += 32) {
+    // the chunk's unit records {first unit of a, deg a, unit (a, p), first
+    // slot entry} and g_a[p], one lane each, read in one round
+    int4 kl = make_int4(0, 0, 0, 0);
+    double4 gl = make_double4(0.0, 0.0, 0.0, 0.0);
+    if (cb + lane < nu) {
+      kl = fb.kv_unit[u1b + cb + lane];
+      gl = fb.g[kl.z];
     }
-    __builtin_amdgcn_wave_barrier();
+    const int cnt = min(32, nmax - cb);
+    // unit i of this half, broadcast from lane i of the half
+    auto unit = [&](int i) {
+      const int l0 = i, l1 = 32 + i;
+      const int4 k0 = make_int4(__builtin_amdgcn_readlane(kl.x, l0), __builtin_amdgcn_readlane(kl.y, l0),
+                                __builtin_amdgcn_readlane(kl.z, l0), __builtin_amdgcn_readlane(kl.w, l0));
+      const int4 k1 = make_int4(__builtin_amdgcn_readlane(kl.x, l1), __builtin_amdgcn_readlane(kl.y, l1),
+                                __builtin_amdgcn_readlane(kl.z, l1), __builtin_amdgcn_readlane(kl.w, l1));
+      int4 k = half ? k1 : k0;
+      if (cb + i >= nu) k.y = 0;   // past this half's units: no feeds
+      return k;
+    };
+    auto bcast = [&](double x, int i) {
+      const long long v = __double_as_longlong(x);
+      const int lo = int(v), hi = int(v >> 32);
+      const int rlo = half ? __builtin_amdgcn_readlane(lo, 32 + i) : __builtin_amdgcn_readlane(lo, i);
+      const int rhi = half ? __builtin_amdgcn_readlane(hi, 32 + i) : __builtin_amdgcn_readlane(hi, i);
+      return __longlong_as_double((static_cast<long long>(rhi) << 32) | static_cast<unsigned>(rlo));
+    };
+    // feeds of unit i (lane j < deg a: g_a[q_j] and q_j's slot in p's row),
+    // loaded one unit ahead of their use
+    int4 ku = unit(0);
+    double4 gq = make_double4(0.0, 0.0, 0.0, 0.0);
+    int sl = 0;
+    if (lane < ku.y) {
+      gq = fb.g[ku.x + lane];
+      sl = fb.kv_slot[ku.w + lane];
+    }
+    for (int i = 0; i < cnt; ++i) {
+      const double gw = bcast(gl.w, i);
+      const double wp[3] = {gw * bcast(gl.x, i), gw * bcast(gl.y, i), gw * bcast(gl.z, i)};
+      int4 kn = make_int4(0, 0, 0, 0);
+      double4 gqn = make_double4(0.0, 0.0, 0.0, 0.0);
+      int sln = 0;
+      if (i + 1 < cnt) {
+        kn = unit(i + 1);
+        if (lane < kn.y) {
+          gqn = fb.g[kn.x + lane];
+          sln = fb.kv_slot[kn.w + lane];
+        }
+      }
+      for (int j = lane; j < ku.y; j += 32) {
+        double4 g = gq;
+        int s = sl;
+        if (j != lane) {   // a neighbour with more than 32 element neighbours
+          g = fb.g[ku.x + j];
+          s = fb.kv_slot[ku.w + j];
+        }
+        double* a9 = acc + 9 * s;
+        const double q3[3] = {g.x, g.y, g.z};
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) a9[3 * r + c] += wp[r] * q3[c];
+      }
+      __builtin_amdgcn_wave_barrier();
+      ku = kn;
+      gq = gqn;
+      sl = sln;
+    }
   }
   // CSR order of p's rows: value r * 3 deg2 + 3 s + c
   const int row = 3 * deg2;
-  for (int i = lane; i < n9; i += 32) {
-    const int r = i / row, rem = i - r * row;
-    const int s2 = rem / 3, c = rem - 3 * s2;
-    kval[rec2.x + i] = bulk * acc[9 * s2 + 3 * r + c];
-  }
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+    for (int k = lane; k < row; k += 32) {
+      const int s2 = k / 3, c = k - 3 * s2;
+      kval[rec2.x + r * row + k] = bulk * acc[9 * s2 + 3 * r + c];
+    }
 }
 
 int32_t launch_tet4_fbar(const DevPlan& p, const FbarDev& fb, const double* u, const double* mu,
