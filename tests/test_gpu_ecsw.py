@@ -50,3 +50,32 @@ def test_single_instance_model(instances):
     f1, K1 = model.evaluate_forces_and_stiffness(inst["u_r"])
     f2, K2 = BatchedECSW([inst]).evaluate(inst["u_r"][None])
     assert np.array_equal(f1, f2[0].cpu().numpy()) and np.array_equal(K1, K2[0].cpu().numpy())
+
+
+def test_rid_sizes_at_batch_and_chunk_edges(instances):
+    """RIDs of 1, 5, 8, 17, 128 and 129 elements: below one matrix-core batch,
+    exactly one, across a 16-element row batch, exactly one 128-element work
+    item and one past it; each equal to the oracle."""
+    from psierra_amd.ecsw import BatchedECSW
+
+    banner("ecsw rid edges")
+    big = max(instances, key=lambda inst: inst["mesh"].n_elements)
+    rng = np.random.default_rng(7)
+    cases = []
+    for n in (1, 5, 8, 17, 128, 129):
+        assert n <= big["mesh"].n_elements
+        inst = dict(big)
+        inst["rid"] = np.sort(rng.choice(big["mesh"].n_elements, n, replace=False)).astype(np.int64)
+        inst["weights"] = rng.uniform(0.5, 2.0, n)
+        cases.append(inst)
+    b = BatchedECSW(cases)
+    f, K = b.evaluate(np.stack([c["u_r"] for c in cases]))
+    f, K = f.cpu().numpy(), K.cpu().numpy()
+    for i, c in enumerate(cases):
+        m = c["mesh"]
+        fr, Kr = OE.evaluate_forces_and_stiffness(m.nodes, m.elements, c["basis"], c["rid"], c["weights"],
+                                                  c["mu"], c["lam"], c["u_r"])
+        ek, ef = rel_err(K[i], Kr), rel_err(f[i], fr)
+        print(f"rid {len(c['rid'])}: K_r rel {ek:.2e}, f_r rel {ef:.2e}")
+        assert ek < 1e-4 and ef < 1e-4
+        assert np.array_equal(K[i], K[i].T)
