@@ -39,6 +39,12 @@ __device__ __forceinline__ double quad_sum(double x) {
   return x + quad_dpp<0x4E>(x);
 }
 
+// The same for a float (all four lanes of the quad active).
+__device__ __forceinline__ float quad_sum(float x) {
+  x += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0xB1, 0xF, 0xF, true));
+  return x + __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0x4E, 0xF, 0xF, true));
+}
+
 // Node owning unit u: bit b of starts[] marks the first unit of a node with
 // units, nz[] lists those nodes in order.
 template <int NW>

@@ -10,6 +10,7 @@
 // gradients of src/psierra/element/tet10_base.py:117-179 and the rule of
 // src/psierra/element/quadrature.py:36-50.
 #include <climits>
+#include <type_traits>
 
 #include "psx_device.h"
 #include "psx_internal.h"
@@ -43,85 +44,121 @@ __host__ __device__ constexpr double tet10_dn(int q, int a, int j) {
   }
 }
 
+// d N_a / d xi_j at the point with barycentric coordinates L (L[0] =
+// 1 - L[1] - L[2] - L[3]): the expressions of tet10_dn with the point given
+// at run time.
+__device__ __forceinline__ double tet10_dn_at(const double L[4], int a, int j) {
+  switch (a) {
+    case 0: return -(4.0 * L[0] - 1.0);
+    case 1: return j == 0 ? 4.0 * L[1] - 1.0 : 0.0;
+    case 2: return j == 1 ? 4.0 * L[2] - 1.0 : 0.0;
+    case 3: return j == 2 ? 4.0 * L[3] - 1.0 : 0.0;
+    case 4: return j == 0 ? 4.0 * (L[0] - L[1]) : -4.0 * L[1];
+    case 5: return j == 0 ? 4.0 * L[2] : j == 1 ? 4.0 * L[1] : 0.0;
+    case 6: return j == 1 ? 4.0 * (L[0] - L[2]) : -4.0 * L[2];
+    case 7: return j == 2 ? 4.0 * (L[0] - L[3]) : -4.0 * L[3];
+    case 8: return j == 0 ? 4.0 * L[3] : j == 2 ? 4.0 * L[1] : 0.0;
+    default: return j == 1 ? 4.0 * L[3] : j == 2 ? 4.0 * L[2] : 0.0;
+  }
+}
+
+// Barycentric coordinates of quadrature point q (0..3).
+__device__ __forceinline__ void tet10_qp_coords(int q, double L[4]) {
+  L[1] = q == 1 ? kQb : kQa;
+  L[2] = q == 2 ? kQb : kQa;
+  L[3] = q == 3 ? kQb : kQa;
+  L[0] = 1.0 - L[1] - L[2] - L[3];
+}
+
 // Row p of a Tet10 element with material model M (prm as load_params
-// gives them).  dX, dU: node coordinates/displacements minus node 0's
-// (index 0 is zero).  sink(v, value) receives the 90 block values
-// (v = 9 b + 3 i + k) then the 3 force values (v = 90 + i) once per
-// quadrature point; the caller sums them.  Returns min over points of det F.
+// gives them) at one quadrature point L.  dX, dU: node coordinates /
+// displacements minus node 0's (index 0 is zero).  sink(v, value) receives
+// the 90 block values (v = 9 b + 3 i + k) then the 3 force values
+// (v = 90 + i) of the point.  Returns det F at the point.
+template <typename R, int M, bool WANT_K, class Sink>
+__device__ __forceinline__ R tet10_qp(const R dX[10][3], const R dU[10][3], const R prm[4], int p,
+                                      const double L[4], Sink&& sink) {
+  R Jm[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      R s = R(0);
+#pragma unroll
+      for (int a = 1; a < 10; ++a) s += dX[a][i] * R(tet10_dn_at(L, a, j));
+      Jm[i][j] = s;
+    }
+  R adj[3][3];
+  adj3(Jm, adj);
+  const R det = Jm[0][0] * adj[0][0] + Jm[0][1] * adj[1][0] + Jm[0][2] * adj[2][0];
+  const R idet = R(1) / det;
+  const R w = det * R(1.0 / 24.0);
+  // h_a[j] = sum_k dN(a,k) Jinv[k][j]; A = sum_a dU_a (x) h_a
+  R A[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) A[i][j] = R(0);
+  R hp[3] = {R(0), R(0), R(0)};
+#pragma unroll
+  for (int a = 0; a < 10; ++a) {
+    R h[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      h[j] = (R(tet10_dn_at(L, a, 0)) * adj[0][j] + R(tet10_dn_at(L, a, 1)) * adj[1][j] +
+              R(tet10_dn_at(L, a, 2)) * adj[2][j]) * idet;
+    if (a > 0) {
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) A[i][j] += dU[a][i] * h[j];
+    }
+    if (a == p) {
+      hp[0] = h[0];
+      hp[1] = h[1];
+      hp[2] = h[2];
+    }
+  }
+  NHPoint<R> m;
+  HyperExtra<R> x;
+  hyper_point<R, M>(A, prm, m, x);
+  R Shp[3], FSh[3];
+  mat3vec(m.S, hp, Shp);
+  mat3vec(m.F, Shp, FSh);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) sink(90 + i, w * FSh[i]);
+  if (WANT_K) {
+    R gp[3], qp[3];
+    mat3vec(m.Gm, hp, gp);
+    mat3vec(m.F, hp, qp);
+#pragma unroll
+    for (int b = 0; b < 10; ++b) {
+      R h[3], gb[3], qb[3], blk[9];
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        h[j] = (R(tet10_dn_at(L, b, 0)) * adj[0][j] + R(tet10_dn_at(L, b, 1)) * adj[1][j] +
+                R(tet10_dn_at(L, b, 2)) * adj[2][j]) * idet;
+      mat3vec(m.Gm, h, gb);
+      mat3vec(m.F, h, qb);
+      hyper_block<R, M>(m, x, w, gp, qp, hp, Shp, gb, qb, h, blk);
+#pragma unroll
+      for (int c = 0; c < 9; ++c) sink(9 * b + c, blk[c]);
+    }
+  }
+  return m.J;
+}
+
+// Row p summed over the four quadrature points (sink receives each point's
+// values in turn; the caller sums them).  Returns min over points of det F.
 template <typename R, int M, bool WANT_K, class Sink>
 __device__ __forceinline__ R tet10_row(const R dX[10][3], const R dU[10][3], const R prm[4], int p,
                                        Sink&& sink) {
   R Jmin = R(1);
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    R Jm[3][3];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int j = 0; j < 3; ++j) {
-        R s = R(0);
-#pragma unroll
-        for (int a = 1; a < 10; ++a) s += dX[a][i] * R(tet10_dn(q, a, j));
-        Jm[i][j] = s;
-      }
-    R adj[3][3];
-    adj3(Jm, adj);
-    const R det = Jm[0][0] * adj[0][0] + Jm[0][1] * adj[1][0] + Jm[0][2] * adj[2][0];
-    const R idet = R(1) / det;
-    const R w = det * R(1.0 / 24.0);
-    // h_a[j] = sum_k dN(q,a,k) Jinv[k][j]; A = sum_a dU_a (x) h_a
-    R A[3][3];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int j = 0; j < 3; ++j) A[i][j] = R(0);
-    R hp[3] = {R(0), R(0), R(0)};
-#pragma unroll
-    for (int a = 0; a < 10; ++a) {
-      R h[3];
-#pragma unroll
-      for (int j = 0; j < 3; ++j)
-        h[j] = (R(tet10_dn(q, a, 0)) * adj[0][j] + R(tet10_dn(q, a, 1)) * adj[1][j] +
-                R(tet10_dn(q, a, 2)) * adj[2][j]) * idet;
-      if (a > 0) {
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-          for (int j = 0; j < 3; ++j) A[i][j] += dU[a][i] * h[j];
-      }
-      if (a == p) {
-        hp[0] = h[0];
-        hp[1] = h[1];
-        hp[2] = h[2];
-      }
-    }
-    NHPoint<R> m;
-    HyperExtra<R> x;
-    hyper_point<R, M>(A, prm, m, x);
-    Jmin = fmin(Jmin, m.J);
-    R Shp[3], FSh[3];
-    mat3vec(m.S, hp, Shp);
-    mat3vec(m.F, Shp, FSh);
-#pragma unroll
-    for (int i = 0; i < 3; ++i) sink(90 + i, w * FSh[i]);
-    if (WANT_K) {
-      R gp[3], qp[3];
-      mat3vec(m.Gm, hp, gp);
-      mat3vec(m.F, hp, qp);
-#pragma unroll
-      for (int b = 0; b < 10; ++b) {
-        R h[3], gb[3], qb[3], blk[9];
-#pragma unroll
-        for (int j = 0; j < 3; ++j)
-          h[j] = (R(tet10_dn(q, b, 0)) * adj[0][j] + R(tet10_dn(q, b, 1)) * adj[1][j] +
-                  R(tet10_dn(q, b, 2)) * adj[2][j]) * idet;
-        mat3vec(m.Gm, h, gb);
-        mat3vec(m.F, h, qb);
-        hyper_block<R, M>(m, x, w, gp, qp, hp, Shp, gb, qb, h, blk);
-#pragma unroll
-        for (int c = 0; c < 9; ++c) sink(9 * b + c, blk[c]);
-      }
-    }
+    double L[4];
+    tet10_qp_coords(q, L);
+    Jmin = fmin(Jmin, tet10_qp<R, M, WANT_K>(dX, dU, prm, p, L, sink));
   }
   return Jmin;
 }
@@ -153,6 +190,78 @@ __device__ __forceinline__ void tet10_gather(const DevPlan& p, int e, R dX[10][3
       dU[a][2] = R(Ua.z - U0.z);
     }
   }
+}
+
+// DPP broadcast of lane SRC of each quad to the quad's four lanes.
+template <int SRC>
+__device__ __forceinline__ float quad_bcast(float x) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), SRC * 0x55, 0xF, 0xF, false));
+}
+template <int SRC>
+__device__ __forceinline__ double quad_bcast(double x) {
+  const unsigned long long v = static_cast<unsigned long long>(__double_as_longlong(x));
+  const unsigned lo = unsigned(__builtin_amdgcn_mov_dpp(int(unsigned(v)), SRC * 0x55, 0xF, 0xF, false));
+  const unsigned hi = unsigned(__builtin_amdgcn_mov_dpp(int(unsigned(v >> 32)), SRC * 0x55, 0xF, 0xF, false));
+  return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
+}
+
+// tet10_gather for the quad of lanes that shares one incidence (all four
+// lanes active): lane qp loads nodes qp, qp + 4, qp + 8 (besides node 0),
+// forms their differences, and the quad exchanges them.
+template <typename R>
+__device__ __forceinline__ void tet10_gather_quad(const DevPlan& p, int e, int qp, R dX[10][3], R dU[10][3]) {
+  const int32_t* c = p.conn + 12 * int64_t(e);
+  const int4 c0 = *reinterpret_cast<const int4*>(c);
+  const int4 c1 = *reinterpret_cast<const int4*>(c + 4);
+  const int2 c2 = *reinterpret_cast<const int2*>(c + 8);
+  const int ids[10] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w, c2.x, c2.y};
+  const double4 X0 = p.Xi[ids[0]];
+  const double4 U0 = p.ui[ids[0]];
+  R mine[3][6];
+#pragma unroll
+  for (int s = 0; s < 3; ++s) {
+    // node qp + 4 s of this lane (node 0 and nodes past 9 give zeros)
+    const int a = qp + 4 * s;
+    const int id = s == 0 ? (qp == 1 ? ids[1] : qp == 2 ? ids[2] : ids[3])
+                          : s == 1 ? (qp == 0 ? ids[4] : qp == 1 ? ids[5] : qp == 2 ? ids[6] : ids[7])
+                                   : (qp == 0 ? ids[8] : ids[9]);
+    if (a >= 1 && a < 10) {
+      const double4 Xa = p.Xi[id];
+      const double4 Ua = p.ui[id];
+      mine[s][0] = R(Xa.x - X0.x);
+      mine[s][1] = R(Xa.y - X0.y);
+      mine[s][2] = R(Xa.z - X0.z);
+      mine[s][3] = R(Ua.x - U0.x);
+      mine[s][4] = R(Ua.y - U0.y);
+      mine[s][5] = R(Ua.z - U0.z);
+    } else {
+#pragma unroll
+      for (int k = 0; k < 6; ++k) mine[s][k] = R(0);
+    }
+  }
+  dX[0][0] = dX[0][1] = dX[0][2] = R(0);
+  dU[0][0] = dU[0][1] = dU[0][2] = R(0);
+  auto take = [&](auto src, int a, const R* v) {
+    constexpr int S = decltype(src)::value;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      dX[a][k] = quad_bcast<S>(v[k]);
+      dU[a][k] = quad_bcast<S>(v[3 + k]);
+    }
+  };
+  using L0 = std::integral_constant<int, 0>;
+  using L1 = std::integral_constant<int, 1>;
+  using L2 = std::integral_constant<int, 2>;
+  using L3 = std::integral_constant<int, 3>;
+  take(L1{}, 1, mine[0]);
+  take(L2{}, 2, mine[0]);
+  take(L3{}, 3, mine[0]);
+  take(L0{}, 4, mine[1]);
+  take(L1{}, 5, mine[1]);
+  take(L2{}, 6, mine[1]);
+  take(L3{}, 7, mine[1]);
+  take(L0{}, 8, mine[2]);
+  take(L1{}, 9, mine[2]);
 }
 
 // f_int only: one lane per incidence, f_a summed in FP64 over the run.
@@ -219,10 +328,15 @@ __device__ __forceinline__ int tet10_rank_pos(uint32_t w0, uint32_t w1, uint32_t
 // incidence order, passing over the lanes up to kTet10LanePasses times, and
 // write each 3x3 block once, three consecutive values per row.  A tile has
 // at most 64 nodes (the plan's rule), so the node lanes are one wave.
+// f_int + K_T: four threads per incidence of the tile, one per quadrature
+// point; each value of the row is summed over the four points across the
+// quad ((q0 + q1) + (q2 + q3), DPP) and written to LDS once, so the LDS per
+// incidence (93 values) is shared by four waves' worth of threads.
 template <typename R, int TILE, int M>
-__global__ __launch_bounds__(TILE) void tet10_assemble_rows(DevPlan p, MatParams mp, double* __restrict__ f,
-                                                            double* __restrict__ kval, int32_t* status) {
+__global__ __launch_bounds__(4 * TILE) void tet10_assemble_rows(DevPlan p, MatParams mp, double* __restrict__ f,
+                                                                double* __restrict__ kval, int32_t* status) {
   constexpr int NV = 93;
+  constexpr int NT = 4 * TILE;                     // threads
   constexpr int UMAX = kTet10LanePasses * TILE;   // reduction lanes per tile
   constexpr int NN = 64;
   constexpr int NW = UMAX / 64;
@@ -265,50 +379,54 @@ __global__ __launch_bounds__(TILE) void tet10_assemble_rows(DevPlan p, MatParams
     const unsigned long long bal = __ballot(has);
     if (has) s_nz[__popcll(bal & ((1ull << lane) - 1ull))] = uint8_t(t);
   }
-  for (int i = t; i < NW; i += TILE) s_ustart[i] = 0ull;
-  for (int i = t; i < UMAX; i += TILE) s_mask[i] = 0ull;
+  for (int i = t; i < NW; i += NT) s_ustart[i] = 0ull;
+  for (int i = t; i < UMAX; i += NT) s_mask[i] = 0ull;
   __syncthreads();
 
-  // ---- phase 1
+  // ---- phase 1: incidence it, quadrature point qp (a quad of lanes per incidence)
   const int u0 = s_uoff[0];
   if (t < nn && ue > ub) atomicOr(&s_ustart[(ub - u0) >> 6], 1ull << ((ub - u0) & 63));
-  if (t < nk) {
-    const uint32_t pe = p.inc_elem[k0 + t];
+  const int it = t >> 2, qp = t & 3;
+  if (it < nk) {   // the same for the four lanes of a quad
+    const uint32_t pe = p.inc_elem[k0 + it];
     const int e = int(pe & kIncElemMask);
     const int lp = int(pe >> kIncElemBits);
-    uint32_t w[3];
-#pragma unroll
-    for (int i = 0; i < 3; ++i) w[i] = p.inc_rank[3 * int64_t(k0 + t) + i];
     R dX[10][3], dU[10][3];
-    tet10_gather<R>(p, e, dX, dU);
+    tet10_gather_quad<R>(p, e, qp, dX, dU);
     double pr[4];
     load_params<M>(mp, e, pr);
     const R prm[4] = {R(pr[0]), R(pr[1]), R(pr[2]), R(pr[3])};
-    // the four quadrature points accumulate in LDS (keeps 93 accumulators
-    // out of registers)
-#pragma unroll
-    for (int v = 0; v < NV; ++v) contrib[v * TILE + t] = R(0);
-    const R J = tet10_row<R, M, true>(dX, dU, prm, lp, [&](int v, R val) { contrib[v * TILE + t] += val; });
+    double L[4];
+    tet10_qp_coords(qp, L);
+    // the quad's four lanes hold the same sum and store it to the same place
+    // (no per-value branch)
+    const R J = tet10_qp<R, M, true>(dX, dU, prm, lp, L,
+                                     [&](int v, R val) { contrib[v * TILE + it] = quad_sum(val); });
     if (J <= R(0)) {
       atomicMin(&status[1], e);
       status[0] = 1;
     }
+    if (qp == 0) {
+      uint32_t w[3];
 #pragma unroll
-    for (int i = 0; i < 3; ++i) rk[i * TILE + t] = w[i];
-    const int off = int((w[2] >> 16) & 0xffu);
-    const int j = s_first[t - off];
-    const int ubase = s_uoff[j] - u0;
-    const unsigned long long bit = 1ull << off;
+      for (int i = 0; i < 3; ++i) w[i] = p.inc_rank[3 * int64_t(k0 + it) + i];
 #pragma unroll
-    for (int q = 0; q < 10; ++q)
-      if (q != lp) atomicOr(&s_mask[ubase + int((w[q >> 2] >> (8 * (q & 3))) & 0xffu)], bit);
+      for (int i = 0; i < 3; ++i) rk[i * TILE + it] = w[i];
+      const int off = int((w[2] >> 16) & 0xffu);
+      const int j = s_first[it - off];
+      const int ubase = s_uoff[j] - u0;
+      const unsigned long long bit = 1ull << off;
+#pragma unroll
+      for (int q = 0; q < 10; ++q)
+        if (q != lp) atomicOr(&s_mask[ubase + int((w[q >> 2] >> (8 * (q & 3))) & 0xffu)], bit);
+    }
   }
   __syncthreads();
 
   // ---- phase 2
   const int ni = 4 * nn;
   const int lanes = ni + s_uoff[nn] - u0;
-  for (int base = 0; base < lanes; base += TILE) {
+  for (int base = 0; base < lanes; base += NT) {
     const int w = base + t;
     int j = 0, kb = 0, slot = 0;
     bool diag = false, offd = false;
@@ -384,7 +502,8 @@ static void launch_t10(const DevPlan& p, const MatParams& mp, double* f, double*
                        int32_t* status) {
   const int blocks = 8 * ((p.n_tiles + 7) / 8);
   if (kval)
-    hipLaunchKernelGGL((tet10_assemble_rows<R, TILE, M>), dim3(blocks), dim3(TILE), 0, s, p, mp, f, kval, status);
+    hipLaunchKernelGGL((tet10_assemble_rows<R, TILE, M>), dim3(blocks), dim3(4 * TILE), 0, s, p, mp, f, kval,
+                       status);
   else
     hipLaunchKernelGGL((tet10_force_kernel<R, TILE, M>), dim3(blocks), dim3(TILE), 0, s, p, mp, f, status);
 }
