@@ -380,6 +380,81 @@ def ecsw_train_cpu(r, m, n=12):
                      f"FP64 oracle of ECSWTrainingMatrix.build took {t:.2f}s")
 
 
+def clamped_free_dofs(mesh):
+    """Free DOFs of a box clamped on its x = min face (all three components)."""
+    from psierra_amd.boundary import select_nodes_by_plane
+
+    nodes = select_nodes_by_plane(mesh, "yz", "min")
+    mask = np.ones(mesh.n_dofs, dtype=bool)
+    mask[(3 * nodes[:, None] + np.arange(3)).ravel()] = False
+    return np.nonzero(mask)[0]
+
+
+def bench_csc(args):
+    """Free-DOF CSC extraction on the Tet4 box clamped at x = 0: K[free][:, free]
+    with exact zeros dropped, from the FP32-mode K values, on the device."""
+    import torch
+
+    from psierra_amd import workloads as W
+    from psierra_amd._lib import check, lib, ptr
+    from psierra_amd.engine import AssemblyPlan, current_stream_handle
+    from psierra_amd.solver import FreeDofSystem
+
+    n = args.n or 100
+    wl = W.jittered_box_tet4(n=n, seed=1)
+    plan = AssemblyPlan(wl.mesh.nodes, wl.mesh.elements)
+    dev = plan.device
+    u = torch.as_tensor(wl.u.reshape(-1), device=dev)
+    mu = torch.as_tensor(wl.mu, device=dev)
+    lam = torch.as_tensor(wl.lam, device=dev)
+    f, k = plan.assemble(u, mu, lam, precision="fp32")
+    free = clamped_free_dofs(wl.mesh)
+    fs = FreeDofSystem(plan, free)
+
+    def call():
+        check(lib().psx_csc_extract(fs._h, ptr(k), 1, ptr(fs.colptr), ptr(fs.rowind), ptr(fs.vals),
+                                    current_stream_handle()))
+
+    ms = _event_time(call, args.steps, args.warmup)
+    kept = int(fs.colptr[-1].item())
+    # read the kept K values, write their values, row indices and the column pointers
+    b_alg = 8.0 * kept + 8.0 * kept + 4.0 * kept + 4.0 * (fs.n_free + 1)
+    ach = b_alg / (ms * 1e-3) / 1e9
+    return {
+        "metric": "Free-DOF CSC extraction K[free][:, free] (exact zeros dropped), nonzeros/s",
+        "value": kept / (ms * 1e-3), "unit": "nonzeros/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": wl.name + "_clamped_x0", "n_free": int(fs.n_free), "structural_nnz": int(fs.nnz_struct),
+                   "kept_nnz": kept},
+        "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": ach / HBM_PEAK_GBPS, "traffic": None},
+        "cpu_baseline": None if args.no_cpu else csc_cpu(),
+    }
+
+
+def csc_cpu(n=48):
+    """scipy's K_total = K_tan + K_contact; K_total[free][:, free].tocsc()
+    (nonlinear_static.py:354-355) on the 48^3 box, K from the oracle's vectorised
+    assembly."""
+    import scipy.sparse as spsp
+
+    from oracle import assembly as A
+    from psierra_amd import workloads as W
+
+    wl = W.jittered_box_tet4(n=n, seed=1)
+    m = wl.mesh
+    with single_thread():
+        _, K = A.assemble(m.nodes, m.elements, wl.u, wl.mu, wl.lam)
+        free = clamped_free_dofs(m)
+        t0 = time.perf_counter()
+        C = A.free_dof_csc(K, free)
+        t = time.perf_counter() - t0
+    return cpu_entry(C.nnz / t, "nonzeros/s",
+                     f"the same generator at {n}^3 hexes ({K.nnz} nonzeros, {C.nnz} kept in K[free][:, free]); "
+                     f"scipy CSR '+' and fancy slicing to CSC took {t:.2f}s")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -391,7 +466,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--tile", type=int, default=0)
-    ap.add_argument("--workload", choices=("tet4", "tet10", "vjp", "ecsw", "ecsw_train", "fbar"), default="tet4")
+    ap.add_argument("--workload", choices=("tet4", "tet10", "vjp", "ecsw", "ecsw_train", "fbar", "csc"),
+                    default="tet4")
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -401,7 +477,7 @@ def main():
         sys.exit(2)
     if args.workload != "tet4":
         fn = {"tet10": bench_tet10, "vjp": bench_vjp, "ecsw": bench_ecsw, "ecsw_train": bench_ecsw_train,
-              "fbar": bench_fbar}[
+              "fbar": bench_fbar, "csc": bench_csc}[
             args.workload]
         print(json.dumps(fn(args)), flush=True)
         return

@@ -4,12 +4,16 @@
 //   K_total = K_tan + K_contact            (CSR '+', drops exact-zero sums)
 //   K_tan_free = K_total[free][:, free].tocsc()
 // (src/psierra/analysis/nonlinear_static.py:354-355) and hands it to SuperLU
-// (src/psierra/solver/sparse_linear_solver.py:16-19).  Here the structural
-// CSC of K[free][:, free] is built once on the host (psx_csc_create): column
-// j' = free dof j holds, in ascending order, the free rows i that share an
-// element with j, and the CSR position of K[i][j].  Per Newton iteration
-// psx_csc_extract gathers the values, drops exact zeros (as the CSR '+'
-// does) and compacts the columns with a deterministic three-pass scan.
+// (src/psierra/solver/sparse_linear_solver.py:16-19).  Here the columns of
+// node b (3b, 3b+1, 3b+2) are built by one wave: lane (s, r) takes row 3a + r
+// of b's s-th neighbour a (ascending), whose three values K[3a+r][3b..3b+2]
+// are consecutive in the CSR values (one 24-byte read serves the three
+// columns).  A ballot per column marks the kept entries (free row, and
+// nonzero when exact zeros are dropped, as the CSR '+' does); a first pass
+// counts them per column, a scan gives the column pointers, and a second
+// pass writes each column's rows in ascending order at the positions the
+// ballots give.  Everything is in a fixed order: the output is the same on
+// every run.
 //
 // psx_free_residual forms r = f_ext_free - f_int[free]
 // (nonlinear_static.py:322-323) and sum(r^2) in FP64 with a fixed-order
@@ -19,15 +23,6 @@
 namespace psx {
 
 constexpr int kCscBlock = 256;
-
-__device__ __forceinline__ int col_count(const DevCsc& c, const double* __restrict__ kval, int j,
-                                         int drop) {
-  const int b = c.colptr_s[j], e = c.colptr_s[j + 1];
-  if (!drop) return e - b;
-  int n = 0;
-  for (int q = b; q < e; ++q) n += kval[c.src[q]] != 0.0;
-  return n;
-}
 
 __device__ __forceinline__ int block_exclusive_scan(int v, int* sh) {
   // Hillis-Steele over kCscBlock lanes; returns the exclusive prefix of v.
@@ -45,12 +40,76 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int* sh) {
   return incl - v;
 }
 
-__global__ __launch_bounds__(kCscBlock) void csc_count_kernel(DevCsc c, const double* __restrict__ kval,
-                                                              int drop, int* __restrict__ blocksum) {
+// One wave per column node b, taken in locality order.  FILL = false: the kept entries of each free
+// column of b into colptr[column].  FILL = true: colptr[column] holds the
+// column's offset within its 256-column block and blockoff the blocks'
+// offsets; the column's rows and values are written and colptr[column] is
+// set to the final start.
+template <bool FILL>
+__global__ __launch_bounds__(kCscBlock) void csc_node_kernel(DevCsc c, const double* __restrict__ kval, int drop,
+                                                             const int* __restrict__ blockoff,
+                                                             int* __restrict__ colptr, int* __restrict__ rowind,
+                                                             double* __restrict__ vals) {
+  const int lane = threadIdx.x & 63;
+  const int bi = blockIdx.x * (kCscBlock / 64) + int(threadIdx.x >> 6);
+  if (bi >= c.N) return;
+  const int b = c.order[bi];
+  int jc[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) jc[k] = c.newidx[3 * b + k];
+  if (jc[0] < 0 && jc[1] < 0 && jc[2] < 0) return;
+  const int u0 = c.adj_ptr[b], items = 3 * (c.adj_ptr[b + 1] - u0);
+  int start[3] = {0, 0, 0};
+  if (FILL) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      if (jc[k] >= 0) start[k] = colptr[jc[k]] + blockoff[jc[k] / kCscBlock];
+  }
+  int kept[3] = {0, 0, 0};
+  const unsigned long long below = (1ull << lane) - 1ull;
+  for (int base = 0; base < items; base += 64) {
+    const int l = base + lane;
+    int ri = -1;
+    double v[3] = {0.0, 0.0, 0.0};
+    if (l < items) {
+      const int s = l / 3, r = l - 3 * s;
+      const int4 un = c.unit[u0 + s];
+      ri = c.newidx[3 * un.z + r];
+      if (ri >= 0) {
+        const double* src = kval + un.x + r * un.y;
+        v[0] = src[0];
+        v[1] = src[1];
+        v[2] = src[2];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const bool keep = ri >= 0 && jc[k] >= 0 && (!drop || v[k] != 0.0);
+      const unsigned long long m = __ballot(keep);
+      if (FILL && keep) {
+        const int o = start[k] + kept[k] + __popcll(m & below);
+        rowind[o] = ri;
+        vals[o] = v[k];
+      }
+      kept[k] += __popcll(m);
+    }
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      if (jc[k] >= 0) colptr[jc[k]] = FILL ? start[k] : kept[k];
+  }
+}
+
+// colptr[j] (the column counts) -> the exclusive prefix within each block of
+// 256 columns; blocksum[block] = the block's total.
+__global__ __launch_bounds__(kCscBlock) void csc_scan_local_kernel(int n_free, int* __restrict__ colptr,
+                                                                   int* __restrict__ blocksum) {
   __shared__ int sh[kCscBlock];
   const int j = blockIdx.x * kCscBlock + threadIdx.x;
-  const int v = j < c.n_free ? col_count(c, kval, j, drop) : 0;
+  const int v = j < n_free ? colptr[j] : 0;
   const int ex = block_exclusive_scan(v, sh);
+  if (j < n_free) colptr[j] = ex;
   if (threadIdx.x == kCscBlock - 1) blocksum[blockIdx.x] = ex + v;
 }
 
@@ -72,36 +131,17 @@ __global__ __launch_bounds__(kCscBlock) void csc_scan_blocks_kernel(int nblocks,
   if (threadIdx.x == 0) colptr[n_free] = carry;
 }
 
-__global__ __launch_bounds__(kCscBlock) void csc_fill_kernel(DevCsc c, const double* __restrict__ kval,
-                                                             int drop, const int* __restrict__ blockoff,
-                                                             int* __restrict__ colptr,
-                                                             int* __restrict__ rowind,
-                                                             double* __restrict__ vals) {
-  __shared__ int sh[kCscBlock];
-  const int j = blockIdx.x * kCscBlock + threadIdx.x;
-  const int v = j < c.n_free ? col_count(c, kval, j, drop) : 0;
-  const int start = blockoff[blockIdx.x] + block_exclusive_scan(v, sh);
-  if (j >= c.n_free) return;
-  colptr[j] = start;
-  int o = start;
-  for (int q = c.colptr_s[j]; q < c.colptr_s[j + 1]; ++q) {
-    const double x = kval[c.src[q]];
-    if (drop && x == 0.0) continue;
-    rowind[o] = c.row_s[q];
-    vals[o] = x;
-    ++o;
-  }
-}
-
 int32_t launch_csc_extract(const DevCsc& c, const double* kval, int drop, int* colptr, int* rowind,
                            double* vals, hipStream_t s) {
   const int nb = (c.n_free + kCscBlock - 1) / kCscBlock;
   if (nb == 0) return PSX_OK;
-  hipLaunchKernelGGL(csc_count_kernel, dim3(nb), dim3(kCscBlock), 0, s, c, kval, drop, c.scratch);
-  hipLaunchKernelGGL(csc_scan_blocks_kernel, dim3(1), dim3(kCscBlock), 0, s, nb, c.scratch, colptr,
-                     c.n_free);
-  hipLaunchKernelGGL(csc_fill_kernel, dim3(nb), dim3(kCscBlock), 0, s, c, kval, drop, c.scratch, colptr,
-                     rowind, vals);
+  const int node_blocks = (c.N + kCscBlock / 64 - 1) / (kCscBlock / 64);
+  hipLaunchKernelGGL(csc_node_kernel<false>, dim3(node_blocks), dim3(kCscBlock), 0, s, c, kval, drop,
+                     static_cast<const int*>(nullptr), colptr, rowind, vals);
+  hipLaunchKernelGGL(csc_scan_local_kernel, dim3(nb), dim3(kCscBlock), 0, s, c.n_free, colptr, c.scratch);
+  hipLaunchKernelGGL(csc_scan_blocks_kernel, dim3(1), dim3(kCscBlock), 0, s, nb, c.scratch, colptr, c.n_free);
+  hipLaunchKernelGGL(csc_node_kernel<true>, dim3(node_blocks), dim3(kCscBlock), 0, s, c, kval, drop, c.scratch,
+                     colptr, rowind, vals);
   PSX_HIP_TRY(hipGetLastError());
   return PSX_OK;
 }

@@ -278,7 +278,8 @@ int32_t psx_fbar_pattern_build(const int64_t* conn, int64_t n_elements, int64_t 
 int32_t psx_fbar_csc_create(const psx_fbar* fbar, const int64_t* free_dofs, int64_t n_free, psx_csc** out) {
   psx::clear_error();
   if (!fbar) return psx::fail(PSX_EINVAL, "fbar is null");
-  return psx::create_csc_adj(fbar->adj2_ptr, fbar->adj2, fbar->plan->hp.N, free_dofs, n_free, out);
+  return psx::create_csc_adj(fbar->adj2_ptr, fbar->adj2, fbar->plan->perm_h, fbar->plan->hp.N, free_dofs, n_free,
+                             out);
 }
 
 int32_t psx_fbar_assemble(psx_fbar* fbar, const double* u, const double* mu, const double* lam, double bulk,

@@ -57,7 +57,8 @@ struct psx_csc;
 namespace psx {
 // Free-dof CSC of the node-block CSR whose node rows are given by an
 // adjacency (user ids ascending per node, node a's row contiguous).
-int32_t create_csc_adj(const std::vector<int32_t>& adj_ptr, const std::vector<int32_t>& adj, int64_t N,
+int32_t create_csc_adj(const std::vector<int32_t>& adj_ptr, const std::vector<int32_t>& adj,
+                       const std::vector<int32_t>& order, int64_t N,
                        const int64_t* free_dofs, int64_t n_free, psx_csc** out);
 
 // Device view of a plan, passed by value to kernels.
@@ -110,9 +111,11 @@ struct FbarDev {
 // Structural CSC of K[free][:, free] (device view).
 struct DevCsc {
   int32_t n_free;
-  const int32_t* colptr_s;  // n_free + 1
-  const int32_t* row_s;     // free-numbered rows, ascending per column
-  const int32_t* src;       // CSR value position of each entry
+  int32_t N;                // nodes
+  const int32_t* order;     // N: column nodes in locality order (waves close in time share cache lines)
+  const int32_t* adj_ptr;   // N + 1: node b's neighbours a (ascending) are units adj_ptr[b]..
+  const int4* unit;         // per unit (b, a): {K position of K[3a][3b], row stride 3 deg(a), a, 0}
+  const int32_t* newidx;    // 3N: free index of each dof, -1 if prescribed
   const int32_t* free;      // free dof ids (ascending)
   int32_t* scratch;         // per-block counts / offsets
   double* partial;          // per-block residual partial sums

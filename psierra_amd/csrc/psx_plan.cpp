@@ -545,8 +545,9 @@ int32_t create_plan(const int64_t* conn, int64_t E, int npe, const double* X, in
   return PSX_OK;
 }
 
-int32_t create_csc_adj(const std::vector<int32_t>& adj_ptr, const std::vector<int32_t>& adj, int64_t N,
-                       const int64_t* free_dofs, int64_t n_free, psx_csc** out) {
+int32_t create_csc_adj(const std::vector<int32_t>& adj_ptr, const std::vector<int32_t>& adj,
+                       const std::vector<int32_t>& order, int64_t N, const int64_t* free_dofs, int64_t n_free,
+                       psx_csc** out) {
   if (!out) return fail(PSX_EINVAL, "out is null");
   *out = nullptr;
   struct {
@@ -570,22 +571,24 @@ int32_t create_csc_adj(const std::vector<int32_t>& adj_ptr, const std::vector<in
     const int32_t* nb = hp.adj.data() + hp.adj_ptr[a];
     return int32_t(std::lower_bound(nb, nb + deg(a), int32_t(b)) - nb);
   };
-  std::vector<int32_t> colptr(n_free + 1, 0), rows, src;
-  rows.reserve(size_t(n_free) * 48);
-  src.reserve(size_t(n_free) * 48);
-  for (int64_t k = 0; k < n_free; ++k) {
-    const int64_t j = freev[k], a = j / 3, cj = j % 3;
-    for (int32_t t = hp.adj_ptr[a]; t < hp.adj_ptr[a + 1]; ++t) {
-      const int64_t b = hp.adj[t];
-      for (int c = 0; c < 3; ++c) {
-        const int64_t i = 3 * b + c;
-        if (newidx[i] < 0) continue;
-        rows.push_back(newidx[i]);
-        src.push_back(int32_t(9 * int64_t(hp.adj_ptr[b]) + 3 * c * deg(b) + 3 * rank(b, a) + cj));
-      }
+  // Column node b's unit (b, a) for each neighbour a: where block K[3a.., 3b..]
+  // starts in the CSR values and a's row stride.  The structural count of
+  // K[free][:, free] is kept for the output sizes.
+  const int64_t n_units = int64_t(hp.adj.size());
+  std::vector<int4> unit(size_t(std::max<int64_t>(n_units, 1)));
+  int64_t nnz_struct = 0;
+  for (int64_t b = 0; b < N; ++b) {
+    int64_t free_cols = 0;
+    for (int c = 0; c < 3; ++c) free_cols += newidx[3 * b + c] >= 0;
+    for (int32_t t = hp.adj_ptr[b]; t < hp.adj_ptr[b + 1]; ++t) {
+      const int64_t a = hp.adj[t];
+      unit[t] = make_int4(int32_t(9 * int64_t(hp.adj_ptr[a]) + 3 * rank(a, b)), int32_t(3 * deg(a)), int32_t(a), 0);
+      int64_t free_rows = 0;
+      for (int r = 0; r < 3; ++r) free_rows += newidx[3 * a + r] >= 0;
+      nnz_struct += free_cols * free_rows;
     }
-    colptr[k + 1] = int32_t(rows.size());
   }
+  if (nnz_struct >= (int64_t(1) << 31)) return fail(PSX_EINVAL, "free-dof matrix has 2^31 or more entries");
   auto* csc = new psx_csc();
   auto bail = [&](int32_t rc) {
     for (void* p : csc->allocations) (void)hipFree(p);
@@ -594,24 +597,28 @@ int32_t create_csc_adj(const std::vector<int32_t>& adj_ptr, const std::vector<in
   };
   (void)hipGetDevice(&csc->device);
   csc->n_free = n_free;
-  csc->nnz_struct = int64_t(rows.size());
+  csc->nnz_struct = nnz_struct;
   auto up = [&](const void* h, size_t bytes, void** d) -> int32_t {
     PSX_HIP_TRY(hipMalloc(d, std::max<size_t>(bytes, 16)));
     csc->allocations.push_back(*d);
     if (bytes && h) PSX_HIP_TRY(hipMemcpy(*d, h, bytes, hipMemcpyHostToDevice));
     return PSX_OK;
   };
-  void *dcol, *drow, *dsrc, *dfree, *dscr, *dpart;
+  if (int64_t(order.size()) != N) return bail(fail(PSX_EINVAL, "node order has the wrong length"));
+  void *dord, *dptr, *dunit, *dnew, *dfree, *dscr, *dpart;
   const size_t nb = size_t((n_free + 255) / 256) + 1;
   int32_t rc;
-  if ((rc = up(colptr.data(), colptr.size() * 4, &dcol)) || (rc = up(rows.data(), rows.size() * 4, &drow)) ||
-      (rc = up(src.data(), src.size() * 4, &dsrc)) || (rc = up(freev.data(), freev.size() * 4, &dfree)) ||
+  if ((rc = up(order.data(), size_t(N) * 4, &dord)) || (rc = up(hp.adj_ptr.data(), size_t(N + 1) * 4, &dptr)) ||
+      (rc = up(unit.data(), size_t(n_units) * sizeof(int4), &dunit)) ||
+      (rc = up(newidx.data(), newidx.size() * 4, &dnew)) || (rc = up(freev.data(), freev.size() * 4, &dfree)) ||
       (rc = up(nullptr, nb * 4, &dscr)) || (rc = up(nullptr, nb * 8, &dpart)))
     return bail(rc);
   csc->d.n_free = int32_t(n_free);
-  csc->d.colptr_s = static_cast<int32_t*>(dcol);
-  csc->d.row_s = static_cast<int32_t*>(drow);
-  csc->d.src = static_cast<int32_t*>(dsrc);
+  csc->d.N = int32_t(N);
+  csc->d.order = static_cast<int32_t*>(dord);
+  csc->d.adj_ptr = static_cast<int32_t*>(dptr);
+  csc->d.unit = static_cast<int4*>(dunit);
+  csc->d.newidx = static_cast<int32_t*>(dnew);
   csc->d.free = static_cast<int32_t*>(dfree);
   csc->d.scratch = static_cast<int32_t*>(dscr);
   csc->d.partial = static_cast<double*>(dpart);
@@ -620,7 +627,7 @@ int32_t create_csc_adj(const std::vector<int32_t>& adj_ptr, const std::vector<in
 }
 
 int32_t create_csc(const psx_plan* plan, const int64_t* free_dofs, int64_t n_free, psx_csc** out) {
-  return create_csc_adj(plan->hp.adj_ptr, plan->hp.adj, plan->hp.N, free_dofs, n_free, out);
+  return create_csc_adj(plan->hp.adj_ptr, plan->hp.adj, plan->perm_h, plan->hp.N, free_dofs, n_free, out);
 }
 
 int32_t retile(psx_plan* plan, int32_t tile) {

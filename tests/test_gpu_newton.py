@@ -166,3 +166,40 @@ def test_graph_replayed_newton_solve_equals_eager(element_name):
     assert np.array_equal(results[0].displacement, results[1].displacement)
     assert results[0].iteration_history == results[1].iteration_history
     assert results[0].residual_history == results[1].residual_history
+
+
+def test_csc_random_free_set_element_and_fbar_patterns():
+    """A random fifth of the DOFs prescribed (so nodes have partly free rows and
+    columns) and a few K values zeroed: the device CSC of the element pattern and
+    of the F-bar pattern equal scipy's (K + empty)[free][:, free].tocsc() bit for
+    bit."""
+    import scipy.sparse as spsp
+    import torch
+
+    from psierra_amd.engine import AssemblyPlan, FbarAssembly
+    from psierra_amd.solver import FreeDofSystem
+
+    wl = W.jittered_box_tet4(n=6, seed=5)
+    plan = AssemblyPlan(wl.mesh.nodes, wl.mesh.elements)
+    dev = plan.device
+    rng = np.random.default_rng(11)
+    n = plan.n_dofs
+    free = np.setdiff1d(np.arange(n), rng.choice(n, n // 5, replace=False))
+    u, mu, lam = (torch.as_tensor(x, device=dev) for x in (wl.u.ravel(), wl.mu, wl.lam))
+    banner("csc random free set")
+    _, k = plan.assemble(u, mu, lam)
+    k[torch.as_tensor(rng.choice(plan.nnz, 200, replace=False), device=dev)] = 0.0
+    indptr, indices = plan.pattern()
+    ref = A.free_dof_csc(spsp.csr_matrix((k.cpu().numpy(), indices, indptr), shape=(n, n)), free)
+    got = FreeDofSystem(plan, free).csc(k)
+    assert np.array_equal(got.indptr, ref.indptr) and np.array_equal(got.indices, ref.indices)
+    assert np.array_equal(got.data, ref.data)
+
+    fb = FbarAssembly(plan)
+    _, k2 = fb.assemble(u, mu, lam, float(np.mean(wl.lam + 2.0 / 3.0 * wl.mu)))
+    k2[torch.as_tensor(rng.choice(fb.nnz, 200, replace=False), device=dev)] = 0.0
+    indptr2, indices2 = fb.pattern()
+    ref2 = A.free_dof_csc(spsp.csr_matrix((k2.cpu().numpy(), indices2, indptr2), shape=(n, n)), free)
+    got2 = FreeDofSystem(plan, free, fbar=fb).csc(k2)
+    assert np.array_equal(got2.indptr, ref2.indptr) and np.array_equal(got2.indices, ref2.indices)
+    assert np.array_equal(got2.data, ref2.data)
