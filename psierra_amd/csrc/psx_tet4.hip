@@ -111,7 +111,7 @@ __device__ __forceinline__ R incidence_row(const IncInputs<R>& in, R* out) {
 }
 
 // f_int only (line-search trials): one lane per incidence, f_a summed in FP64
-// over the node's incidence run.
+// over the node's incidence run by one lane per (node, component).
 template <typename R, int TILE, int M>
 __global__ __launch_bounds__(TILE) void tet4_force_kernel(DevPlan p, MatParams mp, double* __restrict__ f,
                                                           int32_t* status) {
@@ -133,14 +133,12 @@ __global__ __launch_bounds__(TILE) void tet4_force_kernel(DevPlan p, MatParams m
     for (int v = 0; v < 3; ++v) contrib[v * TILE + t] = out[v];
   }
   __syncthreads();
-  for (int j = t; j < nn; j += TILE) {
-    double fa[3] = {0.0, 0.0, 0.0};
-    for (int kk = s_incp[j]; kk < s_incp[j + 1]; ++kk)
-#pragma unroll
-      for (int c = 0; c < 3; ++c) fa[c] += double(contrib[c * TILE + kk]);
-    const int64_t a = p.perm[n0 + j];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) f[3 * a + c] = fa[c];
+  // one lane per (node, component): the node's run summed in incidence order
+  for (int jc = t; jc < 3 * nn; jc += TILE) {
+    const int j = jc / 3, c = jc - 3 * j;
+    double fa = 0.0;
+    for (int kk = s_incp[j]; kk < s_incp[j + 1]; ++kk) fa += double(contrib[c * TILE + kk]);
+    f[3 * int64_t(p.perm[n0 + j]) + c] = fa;
   }
 }
 
