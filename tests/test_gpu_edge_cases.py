@@ -87,6 +87,32 @@ def test_node_with_many_incidences(precision, n_elem):
     assert plan.info["tile"] == 256
 
 
+@pytest.mark.parametrize("n_elem", [21, 64])
+def test_free_csc_of_high_degree_node(n_elem):
+    """The free-DOF CSC when a column node has more than 21 neighbours (its
+    3 x neighbours (row, neighbour) pairs take several passes of a wave): bit for
+    bit scipy's (K + empty)[free][:, free].tocsc(), all DOFs free and a random
+    third prescribed."""
+    import scipy.sparse as spsp
+    import torch
+
+    from psierra_amd.solver import FreeDofSystem
+
+    wl = _fan(n_elem, seed=12)
+    banner(f"edge: free CSC of a fan of {n_elem} elements")
+    plan, _, k, _ = _check(wl, "fp64")
+    kt = torch.as_tensor(k, device=plan.device)
+    indptr, indices = plan.pattern()
+    n = plan.n_dofs
+    K = spsp.csr_matrix((k, indices, indptr), shape=(n, n))
+    rng = np.random.default_rng(13)
+    for free in (np.arange(n), np.setdiff1d(np.arange(n), rng.choice(n, n // 3, replace=False))):
+        ref = A.free_dof_csc(K, free)
+        got = FreeDofSystem(plan, free).csc(kt)
+        assert np.array_equal(got.indptr, ref.indptr) and np.array_equal(got.indices, ref.indices)
+        assert np.array_equal(got.data, ref.data)
+
+
 def test_node_with_65_incidences_is_rejected():
     from psierra_amd.engine import AssemblyPlan
 
