@@ -264,7 +264,9 @@ __device__ __forceinline__ void tet10_gather_quad(const DevPlan& p, int e, int q
   take(L1{}, 9, mine[2]);
 }
 
-// f_int only: one lane per incidence, f_a summed in FP64 over the run.
+// f_int only, one lane per incidence over the four quadrature points (the
+// FP32 form: its registers and LDS leave room for enough waves), f_a summed
+// in FP64 over the node's run.
 template <typename R, int TILE, int M>
 __global__ __launch_bounds__(TILE) void tet10_force_kernel(DevPlan p, MatParams mp, double* __restrict__ f,
                                                            int32_t* status) {
@@ -303,6 +305,52 @@ __global__ __launch_bounds__(TILE) void tet10_force_kernel(DevPlan p, MatParams 
     const int64_t a = p.perm[n0 + j];
 #pragma unroll
     for (int c = 0; c < 3; ++c) f[3 * a + c] = fs[c];
+  }
+}
+
+// f_int only, four lanes per incidence (one per quadrature point, the quad
+// summing f_a with DPP; the FP64 form, whose one-lane version is held to few
+// waves by its registers), f_a summed in FP64 over the node's run by one lane
+// per (node, component).
+template <typename R, int TILE, int M>
+__global__ __launch_bounds__(4 * TILE) void tet10_force_quad_kernel(DevPlan p, MatParams mp, double* __restrict__ f,
+                                                               int32_t* status) {
+  constexpr int NT = 4 * TILE;
+  __shared__ R contrib[3 * TILE];
+  __shared__ int32_t s_incp[TILE + 1];
+  const int tile = xcd_tile();
+  if (tile >= p.n_tiles) return;
+  const int4 ti = load_uniform(p.tile_info, tile);
+  const int n0 = ti.x, nn = ti.y, k0 = ti.z, nk = ti.w;
+  const int t = threadIdx.x;
+  for (int i = t; i <= nn; i += NT) s_incp[i] = p.node_rec[n0 + i].x - k0;
+  const int it = t >> 2, qp = t & 3;
+  if (it < nk) {   // the same for the four lanes of a quad
+    const uint32_t pe = p.inc_elem[k0 + it];
+    const int e = int(pe & kIncElemMask);
+    const int lp = int(pe >> kIncElemBits);
+    R dX[10][3], dU[10][3];
+    tet10_gather_quad<R>(p, e, qp, dX, dU);
+    double pr[4];
+    load_params<M>(mp, e, pr);
+    const R prm[4] = {R(pr[0]), R(pr[1]), R(pr[2]), R(pr[3])};
+    double L[4];
+    tet10_qp_coords(qp, L);
+    R fa[3] = {R(0), R(0), R(0)};
+    const R J = tet10_qp<R, M, false>(dX, dU, prm, lp, L, [&](int v, R val) { fa[v - 90] = val; });
+    if (J <= R(0)) {
+      atomicMin(&status[1], e);
+      status[0] = 1;
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) contrib[c * TILE + it] = quad_sum(fa[c]);
+  }
+  __syncthreads();
+  for (int jc = t; jc < 3 * nn; jc += NT) {
+    const int j = jc / 3, c = jc - 3 * j;
+    double fs = 0.0;
+    for (int kk = s_incp[j]; kk < s_incp[j + 1]; ++kk) fs += double(contrib[c * TILE + kk]);
+    f[3 * int64_t(p.perm[n0 + j]) + c] = fs;
   }
 }
 
@@ -503,6 +551,9 @@ static void launch_t10(const DevPlan& p, const MatParams& mp, double* f, double*
   const int blocks = 8 * ((p.n_tiles + 7) / 8);
   if (kval)
     hipLaunchKernelGGL((tet10_assemble_rows<R, TILE, M>), dim3(blocks), dim3(4 * TILE), 0, s, p, mp, f, kval,
+                       status);
+  else if constexpr (std::is_same<R, double>::value)
+    hipLaunchKernelGGL((tet10_force_quad_kernel<R, TILE, M>), dim3(blocks), dim3(4 * TILE), 0, s, p, mp, f,
                        status);
   else
     hipLaunchKernelGGL((tet10_force_kernel<R, TILE, M>), dim3(blocks), dim3(TILE), 0, s, p, mp, f, status);
