@@ -6,7 +6,7 @@
 //      small strains stay accurate), V_e / 4, V_e (J_e - 1) / 4 and
 //      w_e q_b = V_e J_e / 4 F h_b for the element's four nodes (q_b is the
 //      node's column of div_e = B_0 + B_1 + B_2, the volumetric row of B);
-//   2. fbar_node_kernel, one wave per node a, its incidences staged in LDS:
+//   2. fbar_node_kernel, a wave (or half a wave) per node a, its incidences staged in LDS:
 //      V0_a = sum V_e / 4, J_node - 1 = sum V_e (J_e - 1) / 4 / V0_a and,
 //      with K, the volumetric gradient g_a over a's element neighbours,
 //      g_a[b] = sum_{e in a, b} w_e q_b / V0_a (one lane per neighbour b,
@@ -67,26 +67,23 @@ __global__ __launch_bounds__(256) void fbar_element_kernel(DevPlan p, FbarDev fb
 // local order (the order of the rank word's bytes 1..3).
 __device__ __forceinline__ int other_local(int lp, int q) { return (q - 1) + ((q - 1) >= lp ? 1 : 0); }
 
-// Sum over the 64 lanes of a wave, the same bits in every lane (butterfly
-// in a fixed order).
-__device__ __forceinline__ double wave_sum(double x) {
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) x += __shfl_xor(x, m, 64);
-  return x;
-}
-
-// One wave per node a (a node has at most 64 incidences): its incidences'
-// element quantities are loaded in one round, one lane each, and staged in
-// LDS; V0_a and J_node - 1 are wave sums, and each lane of a's element
-// neighbours b walks the staged incidences in run order for g_a[b].
+// NPW nodes per wave, W = 64 / NPW lanes each (a node has at most W
+// incidences: NPW = 2 when no node has more than 32, else 1): a node's
+// incidences' element quantities are loaded in one round, one lane each, and
+// staged in LDS; V0_a and J_node - 1 are sums over the node's lanes in a
+// fixed butterfly order, and each lane of a's element neighbours b walks the
+// staged incidences in run order for g_a[b].
+template <int NPW>
 __global__ __launch_bounds__(256) void fbar_node_kernel(DevPlan p, FbarDev fb, int want_g) {
-  __shared__ double s_wq[4][64][12];
-  __shared__ uint32_t s_rank[4][64];
-  __shared__ uint8_t s_lp[4][64];
+  constexpr int W = 64 / NPW;
+  __shared__ double s_wq[4 * NPW][W][12];
+  __shared__ uint32_t s_rank[4 * NPW][W];
+  __shared__ uint8_t s_lp[4 * NPW][W];
   const int w = int(threadIdx.x >> 6);
-  const int a = xcd_tile() * 4 + w;
-  const int lane = threadIdx.x & 63;
-  if (a >= p.N) return;
+  const int sub = int(threadIdx.x & 63) / W, lane = int(threadIdx.x & 63) % W;
+  const int g = w * NPW + sub;   // this node's LDS slot
+  const int a = (xcd_tile() * 4 + w) * NPW + sub;
+  if (a >= p.N) return;   // the lanes below only exchange values within a node's group
   const int4 rec = p.node_rec[a];
   const int4 rec1 = p.node_rec[a + 1];
   const int kb = rec.x, L = rec1.x - rec.x;
@@ -97,19 +94,24 @@ __global__ __launch_bounds__(256) void fbar_node_kernel(DevPlan p, FbarDev fb, i
     qv = fb.qv[e];
     vj = fb.vjm1[e];
     if (want_g) {
-      s_rank[w][lane] = p.inc_rank[kb + lane];
-      s_lp[w][lane] = uint8_t(pe >> kIncElemBits);
+      s_rank[g][lane] = p.inc_rank[kb + lane];
+      s_lp[g][lane] = uint8_t(pe >> kIncElemBits);
       const double2* src = reinterpret_cast<const double2*>(fb.wq + 12 * e);
 #pragma unroll
       for (int i = 0; i < 6; ++i) {
         const double2 v = src[i];
-        s_wq[w][lane][2 * i] = v.x;
-        s_wq[w][lane][2 * i + 1] = v.y;
+        s_wq[g][lane][2 * i] = v.x;
+        s_wq[g][lane][2 * i + 1] = v.y;
       }
     }
   }
-  const double sv = wave_sum(qv);
-  const double sj = wave_sum(vj);
+  // sums over the node's W lanes (butterfly, the same bits in every lane)
+  double sv = qv, sj = vj;
+#pragma unroll
+  for (int m = W / 2; m >= 1; m >>= 1) {
+    sv += __shfl_xor(sv, m, 64);
+    sj += __shfl_xor(sj, m, 64);
+  }
   if (lane == 0) fb.jm1[a] = sv > 0.0 ? sj / sv : 0.0;
   if (!want_g) return;
   __builtin_amdgcn_wave_barrier();
@@ -119,13 +121,13 @@ __global__ __launch_bounds__(256) void fbar_node_kernel(DevPlan p, FbarDev fb, i
   // (q0 + q1) + (q2 + q3)
   const int part = lane & 3;
   const int k0 = (part * L) >> 2, k1 = ((part + 1) * L) >> 2;
-  for (int base = 0; base < deg; base += 16) {
+  for (int base = 0; base < deg; base += W / 4) {
     const int s = base + (lane >> 2);
     double g0 = 0.0, g1 = 0.0, g2 = 0.0;
     if (s < deg) {
       for (int k = k0; k < k1; ++k) {
-        const uint32_t r = s_rank[w][k];
-        const int lp = s_lp[w][k];
+        const uint32_t r = s_rank[g][k];
+        const int lp = s_lp[g][k];
         int loc = -1;
         if (s == self)
           loc = lp;
@@ -134,9 +136,9 @@ __global__ __launch_bounds__(256) void fbar_node_kernel(DevPlan p, FbarDev fb, i
           for (int q = 1; q < 4; ++q)
             if (int((r >> (8 * q)) & 0xffu) == s) loc = other_local(lp, q);
         if (loc >= 0) {
-          g0 += s_wq[w][k][3 * loc];
-          g1 += s_wq[w][k][3 * loc + 1];
-          g2 += s_wq[w][k][3 * loc + 2];
+          g0 += s_wq[g][k][3 * loc];
+          g1 += s_wq[g][k][3 * loc + 1];
+          g2 += s_wq[g][k][3 * loc + 2];
         }
       }
     }
@@ -265,7 +267,10 @@ int32_t launch_tet4_fbar(const DevPlan& p, const FbarDev& fb, const double* u, c
   // grids in multiples of 8 work-groups for the XCD-contiguous mapping
   auto grid = [](int64_t items, int per_group) { return dim3(unsigned(8 * ((items + 8 * per_group - 1) / (8 * per_group)))); };
   if (p.E > 0) hipLaunchKernelGGL(fbar_element_kernel, grid(p.E, 256), dim3(256), 0, s, p, fb, status);
-  hipLaunchKernelGGL(fbar_node_kernel, grid(p.N, 4), dim3(256), 0, s, p, fb, kval ? 1 : 0);
+  if (p.max_inc <= 32)
+    hipLaunchKernelGGL(fbar_node_kernel<2>, grid(p.N, 8), dim3(256), 0, s, p, fb, kval ? 1 : 0);
+  else
+    hipLaunchKernelGGL(fbar_node_kernel<1>, grid(p.N, 4), dim3(256), 0, s, p, fb, kval ? 1 : 0);
   if (kval && fb.max_deg2 > 0) {
     // up to four waves (two nodes each) per work-group within 64 KiB of LDS
     const size_t per_wave = size_t(2 * 9) * fb.max_deg2 * sizeof(double);
