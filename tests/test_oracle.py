@@ -426,3 +426,66 @@ def test_mooney_rivlin_and_yeoh_are_consistent():
     Cv, S = fem.mooney_rivlin_stress_and_tangent(np.identity(3), c10, c01, K)
     assert np.allclose(S, 0.0, atol=1e-12)
     assert np.allclose(Cv, fem.hookean_C(lam, 2 * (c10 + c01)), rtol=1e-12)
+
+
+def test_tet10_numbering_matches_the_reference_definition():
+    """tet4_to_tet10 against a literal restatement of
+    compute_tet10_connectivity_and_points (src/psierra/mesher/order.py:11-81):
+    edges (min, max) in the element order 01, 12, 20, 03, 13, 23, mid-edge id =
+    N + row of the edge in np.unique(edges, axis=0), coordinates = midpoints;
+    and the closed-form node count (2n + 1)^3 of an n^3 Kuhn box."""
+    from psierra_amd.mesh import box_tet4, tet4_to_tet10
+
+    m = box_tet4(4, 3, 2, 1.0, 0.6, 0.4)
+    nodes, conn = m.nodes, m.elements
+    rng = np.random.default_rng(5)
+    p = rng.permutation(len(nodes))           # arbitrary node ids
+    nodes_p = np.empty_like(nodes)
+    nodes_p[p] = nodes
+    conn_p = p[conn]
+    n10, c10 = tet4_to_tet10(nodes_p, conn_p)
+    pairs = [(0, 1), (1, 2), (2, 0), (0, 3), (1, 3), (2, 3)]
+    edges = np.stack([np.sort(conn_p[:, [a, b]], axis=1) for a, b in pairs], axis=1).reshape(-1, 2)
+    uniq, inv = np.unique(edges, axis=0, return_inverse=True)
+    assert np.array_equal(c10[:, :4], conn_p)
+    assert np.array_equal(c10[:, 4:], len(nodes_p) + inv.reshape(-1, 6))
+    assert np.array_equal(n10[: len(nodes_p)], nodes_p)
+    assert np.allclose(n10[len(nodes_p):], 0.5 * (nodes_p[uniq[:, 0]] + nodes_p[uniq[:, 1]]), rtol=0, atol=0)
+    for n in (2, 5):
+        b = box_tet4(n, n, n, 1.0, 1.0, 1.0)
+        assert tet4_to_tet10(b.nodes, b.elements)[0].shape[0] == (2 * n + 1) ** 3
+
+
+def test_dirichlet_bookkeeping_and_plane_selection():
+    """select_nodes_by_plane (selector.py:8-40: nodes within 1e-4 of the face)
+    and DirichletManager (dirichlet.py:126-134: sorted prescribed dofs, the
+    sorted complement as free dofs, values aligned with the sorted dofs) on the
+    20x4x4 cantilever: clamping x = 0 prescribes 25 nodes, 75 dofs, 1,500 free."""
+    from types import SimpleNamespace
+
+    from psierra_amd.boundary import DirichletConstraint, DirichletManager, DOFConstraint, select_nodes_by_plane
+    from psierra_amd.mesh import box_tet4
+
+    m = box_tet4(20, 4, 4, 1.0, 0.2, 0.2)
+    nodes = select_nodes_by_plane(m, "yz", "min")
+    assert len(nodes) == 25 and np.all(np.abs(m.nodes[nodes, 0]) <= 1e-4)
+    assert np.array_equal(nodes, np.nonzero(np.abs(m.nodes[:, 0]) <= 1e-4)[0])
+    far = select_nodes_by_plane(m, "xy", "max")
+    assert np.allclose(m.nodes[far, 2], 0.2) and len(far) == 21 * 5
+    # the tolerance is absolute: a node 5e-5 off the face is taken, 2e-4 is not
+    shifted = SimpleNamespace(nodes=m.nodes.copy())
+    shifted.nodes[nodes[0], 0] = 5e-5
+    shifted.nodes[nodes[1], 0] = 2e-4
+    got = select_nodes_by_plane(shifted, "yz", "min")
+    assert nodes[0] in got and nodes[1] not in got
+    mgr = DirichletManager(m.n_dofs)
+    mgr.add_constraint(DirichletConstraint(nodes, (0, 1, 2), 0.0))
+    assert len(mgr.prescribed_dofs) == 75 and len(mgr.free_dofs) == 1500
+    assert np.array_equal(np.sort(np.concatenate([mgr.prescribed_dofs, mgr.free_dofs])), np.arange(m.n_dofs))
+    # values follow the sorted dofs whatever order the constraints came in
+    mgr.add_constraint(DOFConstraint([3 * 100 + 2, 3 * 7], [0.5, -0.25]))
+    p, v = mgr.prescribed_dofs, mgr.prescribed_values
+    assert np.all(np.diff(p) > 0)
+    assert v[np.searchsorted(p, 3 * 100 + 2)] == 0.5 and v[np.searchsorted(p, 3 * 7)] == -0.25
+    with pytest.raises(ValueError, match="does not match"):
+        DOFConstraint([1, 2, 3], [0.0, 1.0])
