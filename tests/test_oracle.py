@@ -489,3 +489,58 @@ def test_dirichlet_bookkeeping_and_plane_selection():
     assert v[np.searchsorted(p, 3 * 100 + 2)] == 0.5 and v[np.searchsorted(p, 3 * 7)] == -0.25
     with pytest.raises(ValueError, match="does not match"):
         DOFConstraint([1, 2, 3], [0.0, 1.0])
+
+
+def test_line_search_rules():
+    """ContactAwareLineSearch (line_search.py:68-260 rules, as restated in
+    psierra_amd.line_search) on synthetic residual curves, and the oracle's
+    restatement (oracle/newton.py) taking the same steps: accept a decrease at
+    the full step; halve through element inversions (the GPU wrapper's message
+    is classified as one) up to the first admissible step; other errors
+    propagate; with no decrease, fall back to the best trial only below 5x the
+    current norm; all trials inverted is a failure with the reference's reason."""
+    from oracle import newton as ON
+    from psierra_amd.line_search import ContactAwareLineSearch, is_element_inversion_error
+
+    ls = ContactAwareLineSearch()
+    inv = ValueError("Element inversion detected at element 7: det(F) = -1.0e-03 <= 0.")
+    assert is_element_inversion_error(inv) and is_element_inversion_error(FloatingPointError())
+    assert not is_element_inversion_error(ValueError("shape mismatch"))
+
+    def curve(a):            # a decrease at every step length
+        return 1.0 - 0.5 * a
+
+    r = ls.search(curve, 1.0, 1e-8)
+    assert r.success and r.alpha == 1.0 and r.iterations == 1
+    assert ON._line_search(curve, 1.0, 1e-8) == (True, 1.0)
+
+    def inverting(a):        # inversion above a = 0.2, a decrease below
+        if a > 0.2:
+            raise inv
+        return 1.0 - a
+
+    r = ls.search(inverting, 1.0, 1e-8)
+    assert r.success and r.alpha == 0.125 and r.iterations == 4 and r.element_inversion_encountered
+    assert ON._line_search(inverting, 1.0, 1e-8) == (True, 0.125)
+
+    def broken(a):
+        raise ValueError("shape mismatch")
+
+    with pytest.raises(ValueError, match="shape mismatch"):
+        ls.search(broken, 1.0, 1e-8)
+
+    def rising(a):           # never a decrease; best trial 1 + 4a at the smallest a
+        return 1.0 + 4.0 * a
+
+    r = ls.search(rising, 1.0, 1e-8)
+    assert r.success and r.accepted_as_contact_fallback and r.alpha == 0.5 ** 15
+    assert ON._line_search(rising, 1.0, 1e-8) == (True, 0.5 ** 15)
+    r = ls.search(lambda a: 6.0, 1.0, 1e-8)
+    assert not r.success and "exceeded fallback tolerance" in r.rejection_reason
+
+    def always_inverted(a):
+        raise inv
+
+    r = ls.search(always_inverted, 1.0, 1e-8)
+    assert not r.success and r.rejection_reason == "Element inversion detected across all evaluated step lengths"
+    assert ON._line_search(always_inverted, 1.0, 1e-8)[0] is False
