@@ -7,12 +7,15 @@
 // m_e[a][b] = rho_e V_e c_ab I3, c the reference mass tensor of the element
 // (integral of N_a N_b over the element divided by its volume).
 //
-// One lane per CSR block (node a, slot s): it walks a's incidence run in
-// order and adds rho_e V_e c[p][q] for every element that contains the
-// slot's node (p = a's local index, q = the slot node's local index, found
-// from the incidence's rank bytes); the 3x3 block is written once with that
-// sum on its diagonal and explicit zeros elsewhere (the pattern keeps them,
-// as coo.tocsr does).  Deterministic, no atomics.
+// Sixteen lanes per node a (four nodes per wave): a's incidences (at most
+// 64) leave rho_e V_e, a's local index and the rank bytes in LDS (the mass
+// tensor is in LDS too); then
+// one lane per CSR block (a, slot s) walks the staged run in order and adds
+// rho_e V_e c[p][q] for every element that contains the slot's node (p = a's
+// local index, q = the slot node's local index, found from the rank bytes);
+// the 3x3 block is written once with that sum on its diagonal and explicit
+// zeros elsewhere (the pattern keeps them, as coo.tocsr does).
+// Deterministic, no atomics.
 #include "psx_device.h"
 #include "psx_internal.h"
 #include "psx_nh.h"
@@ -34,61 +37,79 @@ __device__ __forceinline__ double corner_volume(const DevPlan& p, int64_t e) {
   return det3(D) * (1.0 / 6.0);
 }
 
-template <int NPE>
-__global__ __launch_bounds__(256) void mass_rows_kernel(DevPlan p, const double* __restrict__ rho,
+template <int NPE, int NPW>
+__global__ __launch_bounds__(256) void mass_node_kernel(DevPlan p, const double* __restrict__ rho,
                                                         const double* __restrict__ coef,
                                                         double* __restrict__ kval) {
-  const int64_t u = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  const int64_t n_units = p.uoff[p.N];
-  if (u >= n_units) return;
-  // node owning unit u
-  int lo = 0, hi = p.N;
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (p.uoff[mid] <= u) lo = mid; else hi = mid;
-  }
-  const int a = lo;
+  constexpr int NRW = NPE == 4 ? 1 : 3;   // rank words per incidence
+  constexpr int W = 64 / NPW;             // lanes per node
+  constexpr int KMAX = 64;                // incidences per node at most
+  __shared__ double s_w[4 * NPW][KMAX];
+  __shared__ uint32_t s_rank[4 * NPW][KMAX][NRW];
+  __shared__ uint8_t s_lp[4 * NPW][KMAX];
+  __shared__ double s_coef[NPE * NPE];
+  const int w = int(threadIdx.x >> 6);
+  const int sub = int(threadIdx.x & 63) / W, lane = int(threadIdx.x & 63) % W;
+  const int g = w * NPW + sub;
+  for (int i = int(threadIdx.x); i < NPE * NPE; i += int(blockDim.x)) s_coef[i] = coef[i];
+  __syncthreads();
+  const int a = (xcd_tile() * 4 + w) * NPW + sub;
+  if (a >= p.N) return;
   const int4 rec = p.node_rec[a];
-  const int d = p.uoff[a + 1] - rec.y;
-  const int s = int(u - rec.y);
-  const bool self = s == rec.w;
-  const int k1 = p.node_rec[a + 1].x;
-  double sum = 0.0;
-  for (int k = rec.x; k < k1; ++k) {
-    const uint32_t pe = p.inc_elem[k];
+  const int4 rec1 = p.node_rec[a + 1];
+  const int kb = rec.x, L = rec1.x - rec.x;
+  const int d = rec1.y - rec.y;
+  for (int k = lane; k < L; k += W) {
+    const uint32_t pe = p.inc_elem[kb + k];
     const int64_t e = int64_t(pe & kIncElemMask);
-    const int lp = int(pe >> kIncElemBits);
-    int q = -1;
-    if (self) {
-      q = lp;
-    } else if (NPE == 4) {
-      // bytes 1..3: the other three nodes in increasing local index
-      const uint32_t r = p.inc_rank[k];
+    s_w[g][k] = rho[e] * corner_volume(p, e);
+    s_lp[g][k] = uint8_t(pe >> kIncElemBits);
 #pragma unroll
-      for (int j = 1; j < 4; ++j)
-        if (((r >> (8 * j)) & 0xffu) == uint32_t(s)) q = (j - 1 < lp) ? j - 1 : j;
-    } else {
-#pragma unroll
-      for (int j = 0; j < 10; ++j)
-        if (j != lp && ((p.inc_rank[3 * int64_t(k) + (j >> 2)] >> (8 * (j & 3))) & 0xffu) == uint32_t(s)) q = j;
-    }
-    if (q >= 0) sum += rho[e] * corner_volume(p, e) * coef[lp * NPE + q];
+    for (int i = 0; i < NRW; ++i) s_rank[g][k][i] = p.inc_rank[NRW * int64_t(kb + k) + i];
   }
-  double* dst = kval + rec.z + 3 * s;
+  // the run is this wave's own: its LDS writes land before its reads
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  for (int s = lane; s < d; s += W) {
+    const bool self = s == rec.w;
+    double sum = 0.0;
+#pragma unroll 4
+    for (int k = 0; k < L; ++k) {
+      const int lp = s_lp[g][k];
+      int q = -1;
+      if (self) {
+        q = lp;
+      } else if (NPE == 4) {
+        // bytes 1..3: the other three nodes in increasing local index
+        const uint32_t r = s_rank[g][k][0];
 #pragma unroll
-  for (int r = 0; r < 3; ++r)
+        for (int j = 1; j < 4; ++j)
+          if (((r >> (8 * j)) & 0xffu) == uint32_t(s)) q = (j - 1 < lp) ? j - 1 : j;
+      } else {
 #pragma unroll
-    for (int c = 0; c < 3; ++c) dst[r * 3 * d + c] = r == c ? sum : 0.0;
+        for (int j = 0; j < 10; ++j)
+          if (j != lp && ((s_rank[g][k][j >> 2] >> (8 * (j & 3))) & 0xffu) == uint32_t(s)) q = j;
+      }
+      if (q >= 0) sum += s_w[g][k] * s_coef[lp * NPE + q];
+    }
+    double* dst = kval + rec.z + 3 * s;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) dst[r * 3 * d + c] = r == c ? sum : 0.0;
+  }
 }
 
 int32_t launch_mass(const DevPlan& p, const double* rho, const double* coef, double* kval, hipStream_t s,
                     int64_t n_units) {
-  if (n_units <= 0) return PSX_OK;
-  const unsigned blocks = unsigned((n_units + 255) / 256);
+  if (n_units <= 0 || p.N <= 0) return PSX_OK;
+  // four nodes per wave (16 lanes each); groups in multiples of 8 for the
+  // XCD-contiguous mapping
+  const unsigned blocks = unsigned(8 * ((int64_t(p.N) + 127) / 128));
   if (p.npe == 4)
-    hipLaunchKernelGGL(mass_rows_kernel<4>, dim3(blocks), dim3(256), 0, s, p, rho, coef, kval);
+    hipLaunchKernelGGL((mass_node_kernel<4, 4>), dim3(blocks), dim3(256), 0, s, p, rho, coef, kval);
   else
-    hipLaunchKernelGGL(mass_rows_kernel<10>, dim3(blocks), dim3(256), 0, s, p, rho, coef, kval);
+    hipLaunchKernelGGL((mass_node_kernel<10, 4>), dim3(blocks), dim3(256), 0, s, p, rho, coef, kval);
   PSX_HIP_TRY(hipGetLastError());
   return PSX_OK;
 }

@@ -113,6 +113,25 @@ def test_free_csc_of_high_degree_node(n_elem):
         assert np.array_equal(got.data, ref.data)
 
 
+@pytest.mark.parametrize("n_elem", [33, 64])
+def test_mass_of_high_degree_node(n_elem):
+    """Consistent mass on a fan (one node with 33 or 64 incidences and up to
+    193 neighbours) against the oracle's coo.tocsr assembly."""
+    import torch
+
+    from psierra_amd.engine import AssemblyPlan
+
+    wl = _fan(n_elem, seed=14)
+    banner(f"edge: mass of a fan of {n_elem} elements")
+    plan = AssemblyPlan(wl.mesh.nodes, wl.mesh.elements)
+    rho = 1.0e3 * np.exp(0.2 * np.random.default_rng(15).standard_normal(plan.n_elements))
+    got = plan.assemble_mass(torch.as_tensor(rho, device=plan.device)).cpu().numpy()
+    ref = A.mass_matrix(wl.mesh.nodes, wl.mesh.elements, rho)
+    indptr, indices = plan.pattern()
+    assert np.array_equal(indptr, ref.indptr) and np.array_equal(indices, ref.indices)
+    assert rel_err(got, ref.data) < 1e-13
+
+
 def test_node_with_65_incidences_is_rejected():
     from psierra_amd.engine import AssemblyPlan
 
