@@ -191,8 +191,10 @@ int32_t psx_vjp(psx_plan* plan, const double* u, const double* mu, const double*
  * Replaces K_total = K_tan + K_contact; K_total[free][:, free].tocsc()
  * (src/psierra/analysis/nonlinear_static.py:354-355) and
  * r_free = f_step_free - f_int[free], ||r|| (nonlinear_static.py:322-323).
- * psx_csc_create (host inputs): free dofs strictly increasing; builds the
- * structural CSC of K[free][:, free] for a plan.  psx_csc_extract: from the
+ * psx_csc_create (host inputs): free dofs strictly increasing; records, per
+ * column node, where its 3x3 blocks sit in the plan's CSR values, and the
+ * structural entry count of K[free][:, free] (the object keeps its own copies:
+ * the plan may be destroyed first).  psx_csc_extract: from the
  * plan's CSR values kval, writes colptr (n_free + 1), rowind and vals
  * (capacity nnz_struct each) with row indices ascending; with drop_zeros,
  * entries whose value is exactly 0.0 are dropped, as scipy's CSR '+' does.
