@@ -134,7 +134,12 @@ __global__ __launch_bounds__(kCscBlock) void csc_scan_blocks_kernel(int nblocks,
 int32_t launch_csc_extract(const DevCsc& c, const double* kval, int drop, int* colptr, int* rowind,
                            double* vals, hipStream_t s) {
   const int nb = (c.n_free + kCscBlock - 1) / kCscBlock;
-  if (nb == 0) return PSX_OK;
+  if (nb == 0) {
+    // no free dof: only colptr[0] = 0 (the scan of zero blocks writes it)
+    hipLaunchKernelGGL(csc_scan_blocks_kernel, dim3(1), dim3(kCscBlock), 0, s, 0, c.scratch, colptr, 0);
+    PSX_HIP_TRY(hipGetLastError());
+    return PSX_OK;
+  }
   const int node_blocks = (c.N + kCscBlock / 64 - 1) / (kCscBlock / 64);
   hipLaunchKernelGGL(csc_node_kernel<false>, dim3(node_blocks), dim3(kCscBlock), 0, s, c, kval, drop,
                      static_cast<const int*>(nullptr), colptr, rowind, vals);
@@ -186,7 +191,12 @@ __global__ __launch_bounds__(kCscBlock) void sum_partials_kernel(int n, const do
 int32_t launch_free_residual(const DevCsc& c, const double* f_int, const double* f_ext_free, double* r,
                              double* sumsq, hipStream_t s) {
   const int nb = (c.n_free + kCscBlock - 1) / kCscBlock;
-  if (nb == 0) return PSX_OK;
+  if (nb == 0) {
+    // no free dof: sumsq = 0 (the sum of zero partials)
+    hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(kCscBlock), 0, s, 0, c.partial, sumsq);
+    PSX_HIP_TRY(hipGetLastError());
+    return PSX_OK;
+  }
   hipLaunchKernelGGL(residual_kernel, dim3(nb), dim3(kCscBlock), 0, s, c, f_int, f_ext_free, r,
                      c.partial);
   hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(kCscBlock), 0, s, nb, c.partial, sumsq);

@@ -132,6 +132,34 @@ def test_mass_of_high_degree_node(n_elem):
     assert rel_err(got, ref.data) < 1e-13
 
 
+def test_free_csc_degenerate_free_sets():
+    """The free-DOF CSC with no free dof, a single free dof and a single free
+    node, against scipy (shapes and every array)."""
+    import scipy.sparse as spsp
+    import torch
+
+    from psierra_amd.solver import FreeDofSystem
+
+    wl = _fan(5, seed=16)
+    banner("edge: free CSC of degenerate free sets")
+    plan, _, k, _ = _check(wl, "fp64")
+    kt = torch.as_tensor(k, device=plan.device)
+    indptr, indices = plan.pattern()
+    n = plan.n_dofs
+    K = spsp.csr_matrix((k, indices, indptr), shape=(n, n))
+    for free in (np.array([], dtype=np.int64), np.array([4]), np.array([0, 1, 2])):
+        ref = A.free_dof_csc(K, free)
+        got = FreeDofSystem(plan, free).csc(kt)
+        assert got.shape == ref.shape == (free.size, free.size)
+        assert np.array_equal(got.indptr, ref.indptr) and np.array_equal(got.indices, ref.indices)
+        assert np.array_equal(got.data, ref.data)
+    # the residual of an empty free set is empty with norm 0
+    sysm = FreeDofSystem(plan, np.array([], dtype=np.int64))
+    sysm.sumsq.fill_(123.0)
+    r, rn = sysm.residual(torch.zeros(n, dtype=torch.float64, device=plan.device))
+    assert r.numel() == 0 and rn == 0.0
+
+
 def test_node_with_65_incidences_is_rejected():
     from psierra_amd.engine import AssemblyPlan
 
