@@ -321,6 +321,7 @@ def bench_ecsw_train(args):
 
     from psierra_amd import _lib
     from psierra_amd import workloads as W
+    from psierra_amd.ecsw import train_work_elements
     from psierra_amd.engine import AssemblyPlan, current_stream_handle
 
     wl = W.vjp_box_tet4(n=48, seed=3)
@@ -336,12 +337,13 @@ def bench_ecsw_train(args):
     E = plan.n_elements
     rows = m * r + r * (r + 1) // 2
     A = torch.empty((rows, E), dtype=torch.float64, device=dev)
-    work = torch.empty((E, rows), dtype=torch.float64, device=dev)
+    chunk = train_work_elements(rows, E) if args.work_elements is None else min(args.work_elements, E)
+    work = torch.empty((chunk, rows), dtype=torch.float64, device=dev)
     status = torch.zeros(2, dtype=torch.int32, device=dev)
 
     def call():
         _lib.check(_lib.lib().psx_ecsw_training(plan._h, _lib.ptr(phi), r, _lib.ptr(snaps), m, _lib.ptr(mu),
-                                                _lib.ptr(lam), 1, _lib.ptr(A), _lib.ptr(work),
+                                                _lib.ptr(lam), 1, _lib.ptr(A), _lib.ptr(work), chunk,
                                                 current_stream_handle(), _lib.ptr(status)))
 
     ms = _event_time(call, args.steps, args.warmup)
@@ -354,7 +356,7 @@ def bench_ecsw_train(args):
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "fp64", "data": "synthetic",
         "config": {"workload": wl.name, "elements": E, "rank": r, "snapshots": m, "rows": rows,
-                   "A_bytes": 8 * rows * E},
+                   "A_bytes": 8 * rows * E, "work_elements": chunk, "work_bytes": 8 * rows * chunk},
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": ach / HBM_PEAK_GBPS, "traffic": None},
         "cpu_baseline": None if args.no_cpu else ecsw_train_cpu(r, m),
@@ -516,6 +518,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--tile", type=int, default=0)
+    ap.add_argument("--work-elements", type=int, default=None,
+                    help="ecsw_train: elements per chunk of the work block (default: 128 MB of rows)")
     ap.add_argument("--workload", choices=("tet4", "tet10", "vjp", "ecsw", "ecsw_train", "fbar", "csc", "mass"),
                     default="tet4")
     args = ap.parse_args()

@@ -242,13 +242,16 @@ int32_t psx_ecsw_evaluate(psx_ecsw* ecsw, const double* u_r, double* f_r, double
  *   A[k rank + i][e] = (Phi_e^T f_e(u_k))_i,
  *   A[n_snapshots rank + t][e] = (Phi_e^T K0_e Phi_e)[triu_t] (unscaled;
  *   t in numpy.triu_indices order, K0_e the element tangent at u = 0).
- * work: device scratch of rows x n_elements doubles.  The reference's
- * balancing scale of the stiffness rows (ecsw.py:350-357) and b = A 1 are
- * left to the caller.  status as psx_assemble (element inversion). */
+ * work: device scratch of rows x work_elements doubles (work_elements >= 1):
+ * the elements are evaluated in chunks of work_elements, each chunk's
+ * element-major block transposed into its columns of A; A is the same for
+ * every work_elements (bit for bit).  The reference's balancing scale of the stiffness rows
+ * (ecsw.py:350-357) and b = A 1 are left to the caller.  status as
+ * psx_assemble (element inversion). */
 int32_t psx_ecsw_training(psx_plan* plan, const double* phi, int32_t rank, const double* snapshots,
                           int32_t n_snapshots, const double* mu, const double* lam,
-                          int32_t include_stiffness, double* A, double* work, void* stream,
-                          int32_t* status);
+                          int32_t include_stiffness, double* A, double* work, int64_t work_elements,
+                          void* stream, int32_t* status);
 
 /* ---- Element batch (device) -------------------------------------------------
  * Replaces Tet4TotalLagrangian.compute_nonlinear_force_and_tangent_stiffness

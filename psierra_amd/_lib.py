@@ -84,8 +84,8 @@ SIGNATURES = {
     ),
     "psx_ecsw_training": (
         c_int32,
-        [c_void_p, c_void_p, c_int32, c_void_p, c_int32, c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_void_p,
-         c_void_p],
+        [c_void_p, c_void_p, c_int32, c_void_p, c_int32, c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_int64,
+         c_void_p, c_void_p],
     ),
     "psx_element_batch": (
         c_int32,

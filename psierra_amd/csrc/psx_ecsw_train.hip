@@ -41,12 +41,13 @@ __global__ __launch_bounds__(256) void ecsw_train_force_kernel(DevPlan p, const 
                                                                int r, const double* __restrict__ snaps, int m,
                                                                const double* __restrict__ mu,
                                                                const double* __restrict__ lam, int rows,
-                                                               double* __restrict__ At, int32_t* status) {
+                                                               int64_t e0, int64_t e1, double* __restrict__ At,
+                                                               int32_t* status) {
   __shared__ double s_f[4][kTrainSnapPass][13];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int64_t base = int64_t(blockIdx.x) * 4; base < p.E; base += int64_t(gridDim.x) * 4) {
+  for (int64_t base = e0 + int64_t(blockIdx.x) * 4; base < e1; base += int64_t(gridDim.x) * 4) {
     const int64_t e = base + wave;
-    const bool valid = e < p.E;
+    const bool valid = e < e1;
     int64_t un[4] = {0, 0, 0, 0};
     double4 X[4];
     double ph[12];
@@ -82,7 +83,7 @@ __global__ __launch_bounds__(256) void ecsw_train_force_kernel(DevPlan p, const 
       }
       __syncthreads();
       if (valid && lane < r) {
-        double* dst = At + e * rows + int64_t(k0) * r + lane;
+        double* dst = At + (e - e0) * rows + int64_t(k0) * r + lane;
         for (int kk = 0; kk < nk; ++kk) {
           double cv = 0.0;
 #pragma unroll
@@ -98,14 +99,14 @@ __global__ __launch_bounds__(256) void ecsw_train_force_kernel(DevPlan p, const 
 __global__ __launch_bounds__(256) void ecsw_train_stiff_kernel(DevPlan p, const double* __restrict__ phi,
                                                                int r, const double* __restrict__ mu,
                                                                const double* __restrict__ lam, int rows,
-                                                               int row0, double* __restrict__ At,
-                                                               int32_t* status) {
+                                                               int row0, int64_t e0, int64_t e1,
+                                                               double* __restrict__ At, int32_t* status) {
   __shared__ double s_phi[4][12][kTrainMaxRank];
   __shared__ double s_K[4][12][13];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int64_t base = int64_t(blockIdx.x) * 4; base < p.E; base += int64_t(gridDim.x) * 4) {
+  for (int64_t base = e0 + int64_t(blockIdx.x) * 4; base < e1; base += int64_t(gridDim.x) * 4) {
     const int64_t e = base + wave;
-    const bool valid = e < p.E;
+    const bool valid = e < e1;
     if (valid) {
       const int4 c = *reinterpret_cast<const int4*>(p.conn + 4 * e);
       const int ci[4] = {c.x, c.y, c.z, c.w};
@@ -146,7 +147,7 @@ __global__ __launch_bounds__(256) void ecsw_train_stiff_kernel(DevPlan p, const 
         for (int cc = 0; cc < 12; ++cc) s += s_K[wave][d][cc] * s_phi[wave][cc][j];
         T[d] = s;
       }
-      double* dst = At + e * rows + row0;
+      double* dst = At + (e - e0) * rows + row0;
       for (int i = 0; i <= j; ++i) {
         double s = 0.0;
 #pragma unroll
@@ -158,9 +159,10 @@ __global__ __launch_bounds__(256) void ecsw_train_stiff_kernel(DevPlan p, const 
   }
 }
 
-// out (cols x n) = in (n x cols)^T, 32 x 32 tiles through LDS.
+// out[c][x] = in[x][c] for x < n, c < cols (out rows of leading dimension
+// ldo), 32 x 32 tiles through LDS.
 __global__ __launch_bounds__(256) void transpose_f64_kernel(const double* __restrict__ in, int64_t n, int cols,
-                                                            double* __restrict__ out) {
+                                                            double* __restrict__ out, int64_t ldo) {
   __shared__ double tile[32][33];
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
   const int64_t r0 = int64_t(blockIdx.x) * 32;
@@ -174,7 +176,7 @@ __global__ __launch_bounds__(256) void transpose_f64_kernel(const double* __rest
   for (int y = ty; y < 32; y += 8) {
     const int cc = c0 + y;
     const int64_t rr = r0 + tx;
-    if (rr < n && cc < cols) out[int64_t(cc) * n + rr] = tile[tx][y];
+    if (rr < n && cc < cols) out[int64_t(cc) * ldo + rr] = tile[tx][y];
   }
 }
 
@@ -182,20 +184,29 @@ int32_t launch_status_reset(hipStream_t s, int32_t* status);
 
 int32_t launch_ecsw_training(const DevPlan& p, const double* phi, int r, const double* snaps, int m,
                              const double* mu, const double* lam, int stiff, double* A, double* work,
-                             hipStream_t s, int32_t* status) {
+                             int64_t work_elements, hipStream_t s, int32_t* status) {
   const int64_t rows = int64_t(m) * r + (stiff ? int64_t(r) * (r + 1) / 2 : 0);
   if (rows > INT_MAX / 2) return fail(PSX_EINVAL, "too many training rows");
+  if (work_elements < 1) return fail(PSX_EINVAL, "work_elements must be at least 1");
   int32_t rc = launch_status_reset(s, status);
   if (rc) return rc;
-  const int64_t groups = std::min<int64_t>((int64_t(p.E) + 3) / 4, 1 << 16);
-  hipLaunchKernelGGL(ecsw_train_force_kernel, dim3(unsigned(groups)), dim3(256), 0, s, p, phi, r, snaps, m, mu,
-                     lam, int(rows), work, status);
-  if (stiff)
-    hipLaunchKernelGGL(ecsw_train_stiff_kernel, dim3(unsigned(groups)), dim3(256), 0, s, p, phi, r, mu, lam,
-                       int(rows), m * r, work, status);
-  hipLaunchKernelGGL(transpose_f64_kernel, dim3(unsigned((int64_t(p.E) + 31) / 32), unsigned((rows + 31) / 32)),
-                     dim3(256), 0, s, work, int64_t(p.E), int(rows), A);
-  PSX_HIP_TRY(hipGetLastError());
+  // Elements in chunks of work_elements columns: each chunk's element-major
+  // block is transposed into its columns of A before the next chunk reuses
+  // it, so the scratch is rows x work_elements instead of rows x E.
+  const int64_t E = p.E;
+  const int64_t chunk = std::min<int64_t>(work_elements, std::max<int64_t>(E, 1));
+  for (int64_t e0 = 0; e0 < E; e0 += chunk) {
+    const int64_t e1 = std::min(E, e0 + chunk), n = e1 - e0;
+    const int64_t groups = std::min<int64_t>((n + 3) / 4, 1 << 16);
+    hipLaunchKernelGGL(ecsw_train_force_kernel, dim3(unsigned(groups)), dim3(256), 0, s, p, phi, r, snaps, m, mu,
+                       lam, int(rows), e0, e1, work, status);
+    if (stiff)
+      hipLaunchKernelGGL(ecsw_train_stiff_kernel, dim3(unsigned(groups)), dim3(256), 0, s, p, phi, r, mu, lam,
+                         int(rows), m * r, e0, e1, work, status);
+    hipLaunchKernelGGL(transpose_f64_kernel, dim3(unsigned((n + 31) / 32), unsigned((rows + 31) / 32)), dim3(256),
+                       0, s, work, n, int(rows), A + e0, E);
+    PSX_HIP_TRY(hipGetLastError());
+  }
   return PSX_OK;
 }
 

@@ -127,6 +127,17 @@ def _snapshot_matrix(snapshots) -> np.ndarray:
     raise ValueError(f"Snapshots must be a 2D array or sequence of 1D vectors, got type {type(snapshots)}")
 
 
+TRAIN_WORK_BYTES = 1 << 30
+
+
+def train_work_elements(rows: int, n_elements: int, work_bytes: int = TRAIN_WORK_BYTES) -> int:
+    """Elements per chunk of the training matrix's element-major work block:
+    ``work_bytes`` of FP64 rows.  On the 48^3 box (663k elements, 2,592
+    rows) a 1 GiB block is as fast as one block of all elements (13.8 GB)
+    on MI355X; much smaller blocks add launches (3,000 elements: +10%)."""
+    return int(max(1, min(n_elements, work_bytes // (8 * max(rows, 1)))))
+
+
 class ECSWTrainingMatrix:
     """Virtual-work training matrix for ECSW on the GPU.
 
@@ -148,7 +159,7 @@ class ECSWTrainingMatrix:
 
     @staticmethod
     def build_device(domain, basis, snapshots, include_stiffness: bool = False,
-                     stiffness_weight: float | None = None):
+                     stiffness_weight: float | None = None, work_elements: int | None = None):
         torch = _torch()
         snap = _snapshot_matrix(snapshots)
         n_dofs, m = snap.shape
@@ -173,11 +184,12 @@ class ECSWTrainingMatrix:
         n_triu = r * (r + 1) // 2 if include_stiffness else 0
         rows = m * r + n_triu
         A = torch.empty((rows, E), dtype=torch.float64, device=dev)
-        work = torch.empty((E, rows), dtype=torch.float64, device=dev)
+        chunk = train_work_elements(rows, E) if work_elements is None else int(work_elements)
+        work = torch.empty((max(min(chunk, E), 1), rows), dtype=torch.float64, device=dev)
         status = torch.zeros(2, dtype=torch.int32, device=dev)
         check(lib().psx_ecsw_training(plan._h, ptr(phi_t), r, ptr(snap_t), m, ptr(st["mu"]), ptr(st["lam"]),
-                                      int(bool(include_stiffness)), ptr(A), ptr(work), current_stream_handle(),
-                                      ptr(status)))
+                                      int(bool(include_stiffness)), ptr(A), ptr(work), chunk,
+                                      current_stream_handle(), ptr(status)))
         del work
         stv = status.cpu().numpy()
         if stv[0] != 0:

@@ -90,3 +90,42 @@ def test_snapshot_argument_checks():
         ECSWTrainingMatrix.build(dom, basis, [])
     with pytest.raises(ValueError, match="does not match domain DOF count"):
         ECSWTrainingMatrix.build(dom, basis, np.zeros(7))
+
+
+@pytest.mark.parametrize("chunk", [1, 37, 4096])
+def test_chunked_work_block_gives_identical_matrix(chunk):
+    """The elements in chunks of `chunk` columns (not dividing E = 10,368)
+    give the same bits as one chunk of all elements, and the oracle's A."""
+    import torch
+
+    from psierra_amd.ecsw import ECSWTrainingMatrix
+
+    wl = W.jittered_box_tet4(n=12, seed=18)
+    basis, snaps = _inputs(wl, 16, 3, seed=7)
+    dom = _domain(wl)
+    E = wl.mesh.n_elements
+    banner(f"ecsw training chunk={chunk} of E={E}")
+    A_one, b_one = ECSWTrainingMatrix.build_device(dom, basis, snaps, include_stiffness=True, work_elements=E)
+    A_chk, b_chk = ECSWTrainingMatrix.build_device(dom, basis, snaps, include_stiffness=True, work_elements=chunk)
+    torch.cuda.synchronize()
+    assert torch.equal(A_one, A_chk) and torch.equal(b_one, b_chk)
+    A_ref, _ = OE.training_matrix(wl.mesh.nodes, wl.mesh.elements, basis, snaps, wl.mu, wl.lam,
+                                  include_stiffness=True)
+    assert rel_err(A_chk.cpu().numpy(), A_ref) < 1e-12
+
+
+def test_inversion_in_a_later_chunk_raises():
+    from psierra_amd.ecsw import ECSWTrainingMatrix
+
+    wl = W.cantilever_tet4()
+    basis, snaps = _inputs(wl, 8, 2, seed=3)
+    e_bad = wl.mesh.n_elements - 3
+    conn = wl.mesh.elements[e_bad]
+    X = wl.mesh.nodes[conn]
+    bad = snaps[:, 1].reshape(-1, 3)
+    bad[conn[0]] = 2.5 * (X[1:].mean(axis=0) - X[0])
+    banner(f"ecsw training inversion of element {e_bad} in the last chunk of 64")
+    with pytest.raises(ValueError, match=r"Element inversion detected"):
+        ECSWTrainingMatrix.build_device(_domain(wl), basis, snaps, work_elements=64)
+    with pytest.raises(ValueError, match="work_elements"):
+        ECSWTrainingMatrix.build_device(_domain(wl), basis, snaps, work_elements=0)
