@@ -382,6 +382,106 @@ def ecsw_train_cpu(r, m, n=12):
                      f"FP64 oracle of ECSWTrainingMatrix.build took {t:.2f}s")
 
 
+def _newton_problem(n):
+    """Jittered n^3 Tet4 box, clamped at x = 0, a total load of -80 in z
+    spread over the x = max face (about 17% tip deflection: 6 Newton
+    iterations per load step in 2 steps)."""
+    from psierra_amd import workloads as W
+
+    wl = W.jittered_box_tet4(n=n, seed=9)
+    X = wl.mesh.nodes
+    clamp = np.nonzero(np.abs(X[:, 0] - X[:, 0].min()) < 1e-9)[0]
+    tip = np.nonzero(np.abs(X[:, 0] - X[:, 0].max()) < 1e-9)[0]
+    f_ext = np.zeros(3 * len(X))
+    f_ext[3 * tip + 2] = -80.0 / len(tip)
+    return wl, clamp, f_ext
+
+
+def bench_newton(args):
+    """Loaded Newton solve on the GPU path (the reference's
+    NonlinearStaticAnalysis.solve, analysis/nonlinear_static.py:250-474):
+    2 load steps, nr_tol 1e-8, HIP-graph assemblies, free-DOF CSC and
+    residual on the GPU, the sparse solve on the host (scipy spsolve, SuperLU,
+    as the reference).  A step is one whole solve from u = 0; the SuperLU
+    share of it is reported beside the total."""
+    import torch
+
+    from psierra_amd.analysis import NonlinearStaticAnalysis
+    from psierra_amd.boundary import DirichletConstraint, DirichletManager
+    from psierra_amd.domain import SolidDomain
+    from psierra_amd.element import GpuTet4TotalLagrangian
+    from psierra_amd.material import NeoHookeanField
+    from psierra_amd.solver import DirectSolver
+
+    n = args.n or 12
+    wl, clamp, f_ext = _newton_problem(n)
+    mgr = DirichletManager(wl.mesh.n_dofs)
+    mgr.add_constraint(DirichletConstraint(clamp, (0, 1, 2), 0.0))
+    mat = NeoHookeanField(wl.mu, wl.lam)
+    dom = SolidDomain(wl.mesh, mat, GpuTet4TotalLagrangian(mat))
+
+    class TimedSolver(DirectSolver):
+        seconds, calls = 0.0, 0
+
+        def solve(self, A, b):
+            t0 = time.perf_counter()
+            x = super().solve(A, b)
+            TimedSolver.seconds += time.perf_counter() - t0
+            TimedSolver.calls += 1
+            return x
+
+    ana = NonlinearStaticAnalysis(dom, linear_solver=TimedSolver())
+
+    def one():
+        dom.update_state(u=np.zeros(wl.mesh.n_dofs))
+        res = ana.solve(mgr, f_ext=f_ext, n_steps=2, nr_iterations=15, nr_tol=1e-8, use_graphs=True)
+        assert res.converged, "the benchmark solve did not converge"
+        return res
+
+    for _ in range(args.warmup):
+        one()
+    TimedSolver.seconds, TimedSolver.calls = 0.0, 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        res = one()
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / max(args.steps, 1)
+    slu = TimedSolver.seconds / max(args.steps, 1)
+    tip_uz = float(res.displacement[2::3][np.abs(wl.mesh.nodes[:, 0] - wl.mesh.nodes[:, 0].max()) < 1e-9].mean())
+    return {
+        "metric": "loaded Newton solve (2 load steps) wall time, GPU assembly + host SuperLU",
+        "value": t, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * t, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp64", "data": "synthetic (seeded jittered Kuhn box, clamped at x = 0, face load)",
+        "config": {"workload": f"newton_box_tet4_{n}^3", "elements": wl.mesh.n_elements,
+                   "free_dofs": int(mgr.free_dofs.size), "iterations": res.iteration_history,
+                   "superlu_s": slu, "superlu_calls_per_solve": TimedSolver.calls / max(args.steps, 1),
+                   "rest_s": t - slu, "tip_uz": tip_uz},
+        "roofline": None,
+        "cpu_baseline": None if args.no_cpu else newton_cpu(n),
+    }
+
+
+def newton_cpu(n):
+    """The oracle's Newton loop (vectorised FP64 numpy assembly, scipy CSC and
+    spsolve) on the same problem, one core."""
+    from oracle import newton as ON
+    from psierra_amd.boundary import DirichletConstraint, DirichletManager
+
+    wl, clamp, f_ext = _newton_problem(n)
+    mgr = DirichletManager(wl.mesh.n_dofs)
+    mgr.add_constraint(DirichletConstraint(clamp, (0, 1, 2), 0.0))
+    with single_thread():
+        t0 = time.perf_counter()
+        _, ok, iters = ON.solve(wl.mesh.nodes, wl.mesh.elements, wl.mu, wl.lam, mgr.free_dofs,
+                                mgr.prescribed_dofs, mgr.prescribed_values, f_ext, 2, 15, 1e-8)
+        t = time.perf_counter() - t0
+    assert ok
+    return cpu_entry(t, "s", f"the same solve ({iters} iterations) by the FP64 oracle's Newton loop "
+                             f"(vectorised numpy assembly, scipy free-DOF CSC, spsolve) took {t:.2f}s")
+
+
 def bench_mass(args):
     """Consistent mass matrix (FP64, constant per mesh, used by the dynamics
     drivers) on the 100^3 Tet4 box, per-element density, into the CSR pattern."""
@@ -519,8 +619,9 @@ def main():
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--work-elements", type=int, default=None,
-                    help="ecsw_train: elements per chunk of the work block (default: 128 MB of rows)")
-    ap.add_argument("--workload", choices=("tet4", "tet10", "vjp", "ecsw", "ecsw_train", "fbar", "csc", "mass"),
+                    help="ecsw_train: elements per chunk of the work block (default: 1 GiB of rows)")
+    ap.add_argument("--workload",
+                    choices=("tet4", "tet10", "vjp", "ecsw", "ecsw_train", "fbar", "csc", "mass", "newton"),
                     default="tet4")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -531,7 +632,7 @@ def main():
         sys.exit(2)
     if args.workload != "tet4":
         fn = {"tet10": bench_tet10, "vjp": bench_vjp, "ecsw": bench_ecsw, "ecsw_train": bench_ecsw_train,
-              "fbar": bench_fbar, "csc": bench_csc, "mass": bench_mass}[
+              "fbar": bench_fbar, "csc": bench_csc, "mass": bench_mass, "newton": bench_newton}[
             args.workload]
         print(json.dumps(fn(args)), flush=True)
         return

@@ -243,7 +243,8 @@ int32_t psx_ecsw_evaluate(psx_ecsw* ecsw, const double* u_r, double* f_r, double
  *   A[n_snapshots rank + t][e] = (Phi_e^T K0_e Phi_e)[triu_t] (unscaled;
  *   t in numpy.triu_indices order, K0_e the element tangent at u = 0).
  * work: device scratch of rows x work_elements doubles (work_elements >= 1):
- * the elements are evaluated in chunks of work_elements, each chunk's
+ * the elements are evaluated in chunks of work_elements (rounded down to a
+ * multiple of 64 when it is at least 64), each chunk's
  * element-major block transposed into its columns of A; A is the same for
  * every work_elements (bit for bit).  The reference's balancing scale of the stiffness rows
  * (ecsw.py:350-357) and b = A 1 are left to the caller.  status as

@@ -135,7 +135,8 @@ def train_work_elements(rows: int, n_elements: int, work_bytes: int = TRAIN_WORK
     ``work_bytes`` of FP64 rows.  On the 48^3 box (663k elements, 2,592
     rows) a 1 GiB block is as fast as one block of all elements (13.8 GB)
     on MI355X; much smaller blocks add launches (3,000 elements: +10%)."""
-    return int(max(1, min(n_elements, work_bytes // (8 * max(rows, 1)))))
+    per = work_bytes // (8 * max(rows, 1))
+    return int(max(1, min(n_elements, per - per % 64 if per >= 64 else per)))
 
 
 class ECSWTrainingMatrix:

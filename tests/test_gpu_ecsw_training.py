@@ -92,10 +92,11 @@ def test_snapshot_argument_checks():
         ECSWTrainingMatrix.build(dom, basis, np.zeros(7))
 
 
-@pytest.mark.parametrize("chunk", [1, 37, 4096])
+@pytest.mark.parametrize("chunk", [1, 37, 100, 4096])
 def test_chunked_work_block_gives_identical_matrix(chunk):
-    """The elements in chunks of `chunk` columns (not dividing E = 10,368)
-    give the same bits as one chunk of all elements, and the oracle's A."""
+    """The elements in chunks of `chunk` columns (not dividing E = 10,368;
+    100 is rounded down to 64) give the same bits as one chunk of all
+    elements, and the oracle's A."""
     import torch
 
     from psierra_amd.ecsw import ECSWTrainingMatrix
