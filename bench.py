@@ -506,7 +506,7 @@ def bench_mass(args):
         "metric": "Consistent mass matrix assembly (FP64) elements/s",
         "value": E / (ms * 1e-3), "unit": "elements/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "vs_baseline": None, "dtype": "fp64", "data": "synthetic",
         "config": {"workload": wl.name, "elements": E, "nodes": N, "nnz": nnz},
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": ach / HBM_PEAK_GBPS, "traffic": None},
@@ -576,7 +576,7 @@ def bench_csc(args):
         "metric": "Free-DOF CSC extraction K[free][:, free] (exact zeros dropped), nonzeros/s",
         "value": kept / (ms * 1e-3), "unit": "nonzeros/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "vs_baseline": None, "dtype": "fp64", "data": "synthetic",
         "config": {"workload": wl.name + "_clamped_x0", "n_free": int(fs.n_free), "structural_nnz": int(fs.nnz_struct),
                    "kept_nnz": kept},
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBPS, "unit": "GB/s",

@@ -61,6 +61,8 @@ enum {
   PSX_INFO_COUNT = 9
 };
 
+/* Version of this interface: 2 (1 had psx_ecsw_training without
+ * work_elements). */
 int32_t psx_abi_version(void);
 
 /* Message of the last failing call on this thread ("" if none). */

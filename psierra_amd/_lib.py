@@ -99,6 +99,9 @@ SIGNATURES = {
 
 _lib = None
 
+# The C ABI these signatures describe (psx_abi_version() of the library).
+ABI_VERSION = 2
+
 
 class PsxError(RuntimeError):
     def __init__(self, code: int, message: str):
@@ -119,6 +122,12 @@ def lib() -> ctypes.CDLL:
             fn = getattr(L, name)
             fn.restype = res
             fn.argtypes = args
+        got = L.psx_abi_version()
+        if got != ABI_VERSION:
+            raise RuntimeError(
+                f"{LIB_PATH} has C ABI version {got}, these bindings need {ABI_VERSION}; "
+                "rebuild it with `python -m psierra_amd.build`"
+            )
         _lib = L
     return _lib
 
