@@ -62,7 +62,7 @@ enum {
 };
 
 /* Version of this interface: 2 (1 had psx_ecsw_training without
- * work_elements). */
+ * work_elements and no psx_plan_bind_material). */
 int32_t psx_abi_version(void);
 
 /* Message of the last failing call on this thread ("" if none). */
@@ -108,12 +108,23 @@ int32_t psx_plan_set_tile(psx_plan* plan, int32_t tile);
  * u: N x 3 double (device), mu/lam: per-element Lame parameters (device,
  * n_elements each; the material is NeoHookean with bulk modulus
  * K = lam + 2 mu / 3, src/psierra/material/parameters.py:135).
+ * mu = lam = NULL: the material bound by psx_plan_bind_material (Tet4).
  * f: 3N doubles (device, overwritten); kval: nnz doubles (device,
  * overwritten) in the CSR order of psx_plan_pattern, or NULL for f only.
  * status: device int32[2], reset by the call. */
 int32_t psx_assemble(psx_plan* plan, const double* u, const double* mu,
                      const double* lam, double* f, double* kval,
                      int32_t precision, void* stream, int32_t* status);
+
+/* Binds per-element Lame parameters (device, n_elements each) to a Tet4
+ * plan, as pSierra's element holds its material (the assemble_* hooks of
+ * src/psierra/domain/solid.py:238-241, 288-291 take no parameters): the plan
+ * copies mu and lam, and its FP32 rows then read (V mu, V K) from their
+ * geometry records instead of gathering two parameters per element (one
+ * pass at bind time, 72 B per element of device memory).  Call it again when
+ * the parameters change; psx_assemble with mu = lam = NULL uses the last
+ * binding.  Stream-ordered on `stream`. */
+int32_t psx_plan_bind_material(psx_plan* plan, const double* mu, const double* lam, void* stream);
 
 /* ---- Other hyperelastic models (device) -----------------------------------
  * The psx_assemble path for the compressible models of

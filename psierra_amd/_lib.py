@@ -38,6 +38,7 @@ _f64p = POINTER(c_double)
 # name -> (restype, argtypes)
 SIGNATURES = {
     "psx_abi_version": (c_int32, []),
+    "psx_plan_bind_material": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "psx_last_error": (c_char_p, []),
     "psx_pattern_nnz": (c_int32, [c_void_p, c_int64, c_int32, c_int64, _i64p]),
     "psx_pattern_build": (c_int32, [c_void_p, c_int64, c_int32, c_int64, c_void_p, c_void_p]),

@@ -69,6 +69,7 @@ struct DevPlan {
   const int32_t* conn;            // internal node ids, AoS [E][conn_stride]
   const double4* Xi;              // reference coordinates, internal order
   const float4* geo;              // Tet4: per element and local node (h_b, V), FP32, 4 float4
+  const float4* geo_bound;        // Tet4 with bound material: (h_b, V mu) for b < 2, (h_b, V K) for b >= 2
   const int32_t* perm;            // internal -> user node id
   const int32_t* inc_ptr;         // N+1, internal order
   const uint32_t* inc_elem;       // packed (element | local << 28)
@@ -147,6 +148,9 @@ struct psx_plan {
   std::vector<int32_t> deg_h;      // internal order
   int32_t max_deg = 0;
   double* mass_coef = nullptr;     // device npe x npe reference mass tensor (lazy)
+  double* mu_bound = nullptr;      // device copies of the bound material (psx_plan_bind_material)
+  double* lam_bound = nullptr;
+  bool bound = false;
   int64_t n_inc = 0;
   int64_t device_bytes = 0;
   psx::DevPlan d{};
@@ -163,6 +167,9 @@ int32_t launch_t4_fbar_rows(const DevPlan& p, const double* mu, const double* la
                             const FbarView& fv, double* f, double* kval, int precision, hipStream_t s,
                             int32_t* status);
 int32_t launch_permute_u(const DevPlan& p, const double* u, hipStream_t s, int32_t* status);
+int32_t launch_tet4_assemble_bound(const DevPlan& p, const double* u, const double* mu, const double* lam,
+                                   double* f, double* kval, int precision, hipStream_t s, int32_t* status);
+int32_t launch_bind_material(const DevPlan& p, const double* mu, const double* lam, float4* out, hipStream_t s);
 int32_t launch_tet4_fbar(const DevPlan& p, const FbarDev& fb, const double* u, const double* mu,
                          const double* lam, double bulk, double* f, double* kval, int precision, hipStream_t s,
                          int32_t* status);

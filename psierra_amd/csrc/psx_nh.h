@@ -207,7 +207,9 @@ __device__ __forceinline__ void nh_point(const R A[3][3], R mu, R K, NHPoint<R>&
 // and the Mooney-Rivlin stress adds 2 c' C^-1 (2 dev E + 4 tr(E) E - 4 E^2
 // - 4/3 (tr(E)^2 - tr(E^2)) I), all formed from E so small strains keep
 // their relative accuracy.
-enum HyperModel { kNeoHookean = 0, kMooneyRivlin = 1, kYeoh = 2, kFbarNeoHookean = 3 };
+// kNeoHookeanBound: Neo-Hookean with (V mu, V K) carried in the FP32
+// geometry records of a plan whose material was bound beforehand.
+enum HyperModel { kNeoHookean = 0, kMooneyRivlin = 1, kYeoh = 2, kFbarNeoHookean = 3, kNeoHookeanBound = 4 };
 
 template <typename R>
 struct HyperExtra {
@@ -304,7 +306,9 @@ struct MatParams {
 // mu and K = lambda + 2 mu / 3).
 template <int M>
 __device__ __forceinline__ void load_params(const MatParams& mp, int64_t e, double pr[4]) {
-  if constexpr (M == kNeoHookean || M == kFbarNeoHookean) {
+  if constexpr (M == kNeoHookeanBound) {
+    pr[0] = pr[1] = pr[2] = pr[3] = 0.0;   // taken from the geometry records
+  } else if constexpr (M == kNeoHookean || M == kFbarNeoHookean) {
     pr[0] = mp.v[0][e];
     pr[1] = mp.v[1][e] + (2.0 / 3.0) * pr[0];
     pr[2] = pr[3] = 0.0;

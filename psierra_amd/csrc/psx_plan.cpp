@@ -712,11 +712,17 @@ int32_t psx_plan_set_tile(psx_plan* plan, int32_t tile) {
 int32_t psx_assemble(psx_plan* plan, const double* u, const double* mu, const double* lam, double* f,
                      double* kval, int32_t precision, void* stream, int32_t* status) {
   psx::clear_error();
-  if (!plan || !u || !mu || !lam || !f || !status)
+  if (!plan || !u || !f || !status || !mu != !lam)
     return psx::fail(PSX_EINVAL, "null argument to psx_assemble");
   if (precision != PSX_FP64 && precision != PSX_FP32)
     return psx::fail(PSX_EINVAL, "precision must be PSX_FP64 or PSX_FP32");
   hipStream_t s = static_cast<hipStream_t>(stream);
+  if (!mu) {
+    if (!plan->bound)
+      return psx::fail(PSX_EINVAL, "psx_assemble without mu and lam needs psx_plan_bind_material first");
+    return psx::launch_tet4_assemble_bound(plan->d, u, plan->mu_bound, plan->lam_bound, f, kval, precision, s,
+                                           status);
+  }
   if (plan->hp.npe == 4)
     return psx::launch_tet4_assemble(plan->d, u, mu, lam, f, kval, precision, s, status);
   return psx::launch_tet10_assemble(plan->d, u, mu, lam, f, kval, precision, s, status);
@@ -737,6 +743,38 @@ int32_t psx_assemble_hyperelastic(psx_plan* plan, int32_t model, const double* c
   const double* prm[4] = {params[0], params[1], n > 2 ? params[2] : nullptr, n > 3 ? params[3] : nullptr};
   if (plan->hp.npe != 4) return psx::launch_tet10_hyper(plan->d, model, prm, u, f, kval, precision, s, status);
   return psx::launch_tet4_hyper(plan->d, model, prm, u, f, kval, precision, s, status);
+}
+
+int32_t psx_plan_bind_material(psx_plan* plan, const double* mu, const double* lam, void* stream) {
+  psx::clear_error();
+  if (!plan || !mu || !lam) return psx::fail(PSX_EINVAL, "null argument to psx_plan_bind_material");
+  if (plan->hp.npe != 4) return psx::fail(PSX_EUNSUPPORTED, "psx_plan_bind_material binds Tet4 plans");
+  const size_t E = size_t(std::max<int64_t>(plan->d.E, 1));
+  if (!plan->d.geo_bound) {
+    void* g = nullptr;
+    void* m = nullptr;
+    void* l = nullptr;
+    PSX_HIP_TRY(hipMalloc(&g, 4 * E * sizeof(float4)));
+    plan->allocations.push_back(g);
+    PSX_HIP_TRY(hipMalloc(&m, E * sizeof(double)));
+    plan->allocations.push_back(m);
+    PSX_HIP_TRY(hipMalloc(&l, E * sizeof(double)));
+    plan->allocations.push_back(l);
+    plan->device_bytes += int64_t(E * (4 * sizeof(float4) + 2 * sizeof(double)));
+    plan->d.geo_bound = static_cast<const float4*>(g);
+    plan->mu_bound = static_cast<double*>(m);
+    plan->lam_bound = static_cast<double*>(l);
+  }
+  plan->bound = false;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (plan->d.E > 0) {
+    PSX_HIP_TRY(hipMemcpyAsync(plan->mu_bound, mu, size_t(plan->d.E) * sizeof(double), hipMemcpyDeviceToDevice, s));
+    PSX_HIP_TRY(hipMemcpyAsync(plan->lam_bound, lam, size_t(plan->d.E) * sizeof(double), hipMemcpyDeviceToDevice, s));
+  }
+  const int32_t rc = psx::launch_bind_material(plan->d, mu, lam, const_cast<float4*>(plan->d.geo_bound), s);
+  if (rc) return rc;
+  plan->bound = true;
+  return PSX_OK;
 }
 
 int32_t psx_assemble_mass(psx_plan* plan, const double* rho, double* kval, void* stream) {

@@ -83,10 +83,16 @@ __device__ __forceinline__ void load_incidence(const DevPlan& p, const MatParams
   const int i2 = lp <= 1 ? c.z : c.y;
   const int i3 = lp <= 2 ? c.w : c.z;
   if constexpr (std::is_same<R, float>::value) {
-    in.G[0] = p.geo[4 * e + lp];
-    in.G[1] = p.geo[4 * e + r1];
-    in.G[2] = p.geo[4 * e + r2];
-    in.G[3] = p.geo[4 * e + r3];
+    const float4* geo = M == kNeoHookeanBound ? p.geo_bound : p.geo;
+    in.G[0] = geo[4 * e + lp];
+    in.G[1] = geo[4 * e + r1];
+    in.G[2] = geo[4 * e + r2];
+    in.G[3] = geo[4 * e + r3];
+    if constexpr (M == kNeoHookeanBound) {
+      // local node r1 is 0 or 1 (V mu), r3 is 2 or 3 (V K) for every row
+      in.pr[0] = in.G[1].w;
+      in.pr[1] = in.G[3].w;
+    }
   } else {
     in.X[0] = p.Xi[i0];
     in.X[1] = p.Xi[i1];
@@ -460,6 +466,58 @@ int32_t launch_tet4_hyper(const DevPlan& p, int model, const double* const* para
     default: return fail(PSX_EINVAL, "unknown material model");
   }
   if (rc) return rc;
+  PSX_HIP_TRY(hipGetLastError());
+  return PSX_OK;
+}
+
+// Geometry records of a bound material (FP32 Neo-Hookean rows): record b of
+// element e is (h_b, V mu_e) for local nodes 0 and 1 and (h_b, V K_e) for 2
+// and 3, K = lambda + 2 mu / 3, each product formed in FP64 and rounded once.
+// Every row's node order (row node, then the others by local index) puts a
+// local node 0 or 1 second and a local node 2 or 3 last, so a row finds
+// V mu in G[1].w and V K in G[3].w.  The Neo-Hookean S, its tangent
+// coefficients and so f and K of a row are linear in (mu, K), so the row with
+// V = 1 and (V mu, V K) is the row with V and (mu, K).
+__global__ __launch_bounds__(256) void bind_material_kernel(const float4* __restrict__ geo,
+                                                            const double* __restrict__ mu,
+                                                            const double* __restrict__ lam, int64_t E,
+                                                            float4* __restrict__ out) {
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  const double m = mu[e];
+  const double K = lam[e] + (2.0 / 3.0) * m;
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    float4 g = geo[4 * e + b];
+    g.w = float(double(g.w) * (b < 2 ? m : K));
+    out[4 * e + b] = g;
+  }
+}
+
+int32_t launch_bind_material(const DevPlan& p, const double* mu, const double* lam, float4* out, hipStream_t s) {
+  if (p.E <= 0) return PSX_OK;
+  hipLaunchKernelGGL(bind_material_kernel, dim3(unsigned((int64_t(p.E) + 255) / 256)), dim3(256), 0, s, p.geo, mu,
+                     lam, int64_t(p.E), out);
+  PSX_HIP_TRY(hipGetLastError());
+  return PSX_OK;
+}
+
+// Neo-Hookean with the plan's bound material: FP32 rows read (V mu, V K) from
+// the bound geometry records and load no parameters; FP64 rows use the
+// plan's copies of mu and lambda as an ordinary call.
+int32_t launch_tet4_assemble_bound(const DevPlan& p, const double* u, const double* mu, const double* lam,
+                                   double* f, double* kval, int precision, hipStream_t s, int32_t* status) {
+  if (precision != PSX_FP32) return launch_tet4_assemble(p, u, mu, lam, f, kval, precision, s, status);
+  int32_t rc = launch_permute_u(p, u, s, status);
+  if (rc) return rc;
+  if (p.n_tiles <= 0) return PSX_OK;
+  const MatParams mp{{nullptr, nullptr, nullptr, nullptr}};
+  switch (p.tile) {
+    case 64: launch_t4<float, 64, kNeoHookeanBound>(p, mp, FbarView{}, f, kval, s, status); break;
+    case 128: launch_t4<float, 128, kNeoHookeanBound>(p, mp, FbarView{}, f, kval, s, status); break;
+    case 256: launch_t4<float, 256, kNeoHookeanBound>(p, mp, FbarView{}, f, kval, s, status); break;
+    default: return fail(PSX_EINVAL, "unsupported tile");
+  }
   PSX_HIP_TRY(hipGetLastError());
   return PSX_OK;
 }

@@ -293,7 +293,10 @@ __device__ __forceinline__ float tet4_row_grad_m(const float4 G[4], const double
 #pragma unroll
       for (int j = 0; j < 3; ++j) A[i][j] = dU[i][0] * h[1][j] + dU[i][1] * h[2][j] + dU[i][2] * h[3][j];
   }
-  return tet4_row_core_m<float, M, WANT_K>(h, G[0].w, A, prm, out);
+  if constexpr (M == kNeoHookeanBound)
+    return tet4_row_core_m<float, kNeoHookean, WANT_K>(h, 1.0f, A, prm, out);   // prm = (V mu, V K)
+  else
+    return tet4_row_core_m<float, M, WANT_K>(h, G[0].w, A, prm, out);
 }
 
 template <bool WANT_K>

@@ -164,17 +164,28 @@ class AssemblyPlan:
         check(lib().psx_assemble_mass(self._h, ptr(rho), ptr(kval), current_stream_handle()))
         return kval
 
+    def bind_material(self, mu, lam):
+        """Binds per-element (mu, lam) to the plan (Tet4): ``assemble`` with
+        ``mu = lam = None`` then uses them, the FP32 rows reading (V mu, V K)
+        from their geometry records (``psx_plan_bind_material``).  Bind again
+        after the parameters change."""
+        mu = self._device_f64(mu, self.n_elements, "mu")
+        lam = self._device_f64(lam, self.n_elements, "lam")
+        check(lib().psx_plan_bind_material(self._h, ptr(mu), ptr(lam), current_stream_handle()))
+
     def assemble(self, u, mu, lam, f=None, kval=None, precision="fp64", want_k=True, check_status=True):
         """f_int (3N) and K_T values (nnz, CSR order of ``pattern()``) on the device.
 
         u: (N,3) or (3N,) float64 device tensor; mu, lam: (E,) float64 device
-        tensors.  Launches on the current stream.  With ``check_status`` the
-        call synchronises and raises the reference's ValueError on inversion.
+        tensors, or both None for the material of ``bind_material``.  Launches
+        on the current stream.  With ``check_status`` the call synchronises
+        and raises the reference's ValueError on inversion.
         """
         torch = _torch()
         u = self._device_f64(u, self.n_dofs, "u")
-        mu = self._device_f64(mu, self.n_elements, "mu")
-        lam = self._device_f64(lam, self.n_elements, "lam")
+        if mu is not None or lam is not None:
+            mu = self._device_f64(mu, self.n_elements, "mu")
+            lam = self._device_f64(lam, self.n_elements, "lam")
         if f is None or (want_k and kval is None):
             f0, k0 = self.alloc_outputs(want_k)
             f = f if f is not None else f0

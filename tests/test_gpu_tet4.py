@@ -206,3 +206,68 @@ def test_solid_domain_hooks_match_oracle(cantilever):
     assert K.shape == K_ref.shape and K.nnz == K_ref.nnz
     assert rel_err(f, f_ref) < FP64_F_TOL
     assert rel_err(K.data, K_ref.data) < FP64_K_TOL
+
+
+@pytest.mark.parametrize("tile", [64, 256])
+def test_bound_material_matches_oracle_and_per_call_parameters(jittered, tile):
+    """psx_plan_bind_material: FP32 rows with (V mu, V K) from the bound
+    geometry records within the FP32 tolerance of the oracle and within 1e-6
+    of the per-call FP32 rows (V mu is rounded once instead of V and mu
+    separately); FP64 with the bound copies equals the per-call FP64 result
+    bit for bit; the f-only kernel takes the bound path too."""
+    import torch
+
+    from psierra_amd.engine import AssemblyPlan
+
+    wl, f_ref, K_ref = jittered
+    banner(f"tet4 bound material tile={tile}")
+    plan = AssemblyPlan(wl.mesh.nodes, wl.mesh.elements, tile=tile)
+    dev = plan.device
+    u = torch.as_tensor(wl.u.reshape(-1), device=dev)
+    mu = torch.as_tensor(wl.mu, device=dev)
+    lam = torch.as_tensor(wl.lam, device=dev)
+    plan.bind_material(mu, lam)
+    for prec in ("fp32", "fp64"):
+        f_b, k_b = plan.assemble(u, None, None, precision=prec)
+        f_c, k_c = plan.assemble(u, mu, lam, precision=prec)
+        f_o, _ = plan.assemble(u, None, None, precision=prec, want_k=False)
+        fb, kb = f_b.cpu().numpy(), k_b.cpu().numpy()
+        assert rel_err(kb, K_ref.data) < (FP32_TOL if prec == "fp32" else FP64_K_TOL)
+        assert rel_err(fb, f_ref) < (FP32_TOL if prec == "fp32" else FP64_F_TOL)
+        if prec == "fp64":
+            assert torch.equal(f_b, f_c) and torch.equal(k_b, k_c)
+        else:
+            assert rel_err(kb, k_c.cpu().numpy()) < 1e-6 and rel_err(fb, f_c.cpu().numpy()) < 1e-6
+        assert rel_err(f_o.cpu().numpy(), fb) < 1e-12
+
+
+def test_rebinding_and_binding_errors(jittered):
+    """A second binding replaces the first; an unbound plan refuses mu =
+    lam = None, and one of the two alone is refused."""
+    import torch
+
+    from psierra_amd.engine import AssemblyPlan
+
+    wl, _, _ = jittered
+    banner("tet4 rebinding")
+    plan = AssemblyPlan(wl.mesh.nodes, wl.mesh.elements)
+    dev = plan.device
+    u = torch.as_tensor(wl.u.reshape(-1), device=dev)
+    with pytest.raises(ValueError, match="psx_plan_bind_material first"):
+        plan.assemble(u, None, None, precision="fp32")
+    mu = torch.as_tensor(wl.mu, device=dev)
+    lam = torch.as_tensor(wl.lam, device=dev)
+    plan.bind_material(2.0 * mu, 2.0 * lam)
+    f2, _ = plan.assemble(u, None, None, precision="fp32", want_k=False)
+    plan.bind_material(mu, lam)
+    f1, _ = plan.assemble(u, None, None, precision="fp32", want_k=False)
+    f_c, _ = plan.assemble(u, mu, lam, precision="fp32", want_k=False)
+    assert rel_err(f1.cpu().numpy(), f_c.cpu().numpy()) < 1e-6
+    assert rel_err(f2.cpu().numpy(), 2.0 * f_c.cpu().numpy()) < 1e-6
+    with pytest.raises(ValueError, match="null argument"):
+        from psierra_amd._lib import check, lib, ptr
+        from psierra_amd.engine import current_stream_handle
+
+        f = torch.empty(plan.n_dofs, dtype=torch.float64, device=dev)
+        check(lib().psx_assemble(plan._h, ptr(u), ptr(mu), None, ptr(f), None, 1, current_stream_handle(),
+                                 ptr(plan.status)))
