@@ -61,8 +61,8 @@ enum {
   PSX_INFO_COUNT = 9
 };
 
-/* Version of this interface: 2 (1 had psx_ecsw_training without
- * work_elements and no psx_plan_bind_material). */
+/* Version of this interface: 3 (2: psx_ecsw_* took rank 64 only; 1 had
+ * psx_ecsw_training without work_elements and no psx_plan_bind_material). */
 int32_t psx_abi_version(void);
 
 /* Message of the last failing call on this thread ("" if none). */
@@ -224,15 +224,17 @@ int32_t psx_csc_extract(psx_csc* csc, const double* kval, int32_t drop_zeros,
 int32_t psx_free_residual(psx_csc* csc, const double* f_int, const double* f_ext_free,
                           double* r_free, double* sumsq, void* stream);
 
-/* ---- Batched ECSW hyper-reduced evaluation (device, Tet4, rank 64) -------
+/* ---- Batched ECSW hyper-reduced evaluation (device, Tet4, rank 1..64) ----
  * Replaces HyperReducedECSWModel.evaluate_forces_and_stiffness
  * (src/psierra/hyperreduction/model.py:254-318) for many independent
  * instances in one persistent launch; Phi_e is cached at creation as the
  * model does (model.py:185-213).  Host inputs to psx_ecsw_create, concatenated
  * over instances with rid_offsets (n_instances + 1): X_e (n x 4 x 3), phi_e
- * (n x 12 x 64, rows of the basis for the element's 12 DOFs), weights (> 0),
- * mu, lam (n each).  psx_ecsw_evaluate: u_r (n_instances x 64, device) ->
- * f_r (n_instances x 64) and K_r (n_instances x 64 x 64, symmetrised
+ * (n x 12 x rank, rows of the basis for the element's 12 DOFs; one rank for
+ * every instance, 1 <= rank <= 64, stored padded with zero columns to the
+ * 64-wide matrix-core tiles), weights (> 0), mu, lam (n each).
+ * psx_ecsw_evaluate: u_r (n_instances x rank, device) -> f_r
+ * (n_instances x rank) and K_r (n_instances x rank x rank, symmetrised
  * 0.5 (K + K^T) as model.py:317), FP64 outputs of FP32 element arithmetic
  * and FP32 matrix-core accumulation.
  * Info slots: n_instances, n_rid_elements, n_work_items, grid, device_bytes. */

@@ -47,7 +47,10 @@ def _line_search(eval_fn, r0, tol, c1=2.0e-4, max_it=16):
     return False, best_a
 
 
-def solve(nodes, elements, mu, lam, free, p_dofs, p_vals, f_ext, n_steps, nr_iterations=25, nr_tol=1e-4):
+def solve(nodes, elements, mu, lam, free, p_dofs, p_vals, f_ext, n_steps, nr_iterations=25, nr_tol=1e-4,
+          history=None):
+    """history: optional list; (load step, iteration, ||r||) is appended for
+    every residual evaluated at the top of a Newton iteration."""
     n = 3 * nodes.shape[0]
     u = np.zeros(n)
     iters = []
@@ -62,6 +65,8 @@ def solve(nodes, elements, mu, lam, free, p_dofs, p_vals, f_ext, n_steps, nr_ite
             f, _ = A.assemble(nodes, elements, u.reshape(-1, 3), mu, lam, want_k=False)
             r = fs - f[free]
             rn = np.linalg.norm(r)
+            if history is not None:
+                history.append((step, it, float(rn)))
             if f_ref is None:
                 f_ref = max(rn, np.linalg.norm(f[free]), 1.0)
             tol = nr_tol * f_ref

@@ -101,7 +101,7 @@ SIGNATURES = {
 _lib = None
 
 # The C ABI these signatures describe (psx_abi_version() of the library).
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 
 class PsxError(RuntimeError):
@@ -119,16 +119,19 @@ def lib() -> ctypes.CDLL:
                 f"{LIB_PATH} is not built; run `python -m psierra_amd.build` (the engine has no CPU fallback)"
             )
         L = ctypes.CDLL(str(LIB_PATH))
-        for name, (res, args) in SIGNATURES.items():
-            fn = getattr(L, name)
-            fn.restype = res
-            fn.argtypes = args
+        # the version first: a library of another version may lack symbols
+        L.psx_abi_version.restype = c_int32
+        L.psx_abi_version.argtypes = []
         got = L.psx_abi_version()
         if got != ABI_VERSION:
             raise RuntimeError(
                 f"{LIB_PATH} has C ABI version {got}, these bindings need {ABI_VERSION}; "
                 "rebuild it with `python -m psierra_amd.build`"
             )
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
         _lib = L
     return _lib
 

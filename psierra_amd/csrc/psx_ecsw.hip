@@ -1,6 +1,6 @@
 // Batched ECSW hyper-reduced force and tangent on FP32 MFMA.
 //
-// For each instance (a mesh, a reduced basis V of rank 64 and a reduced
+// For each instance (a mesh, a reduced basis V of rank r <= 64 and a reduced
 // integration domain RID with weights w_e) and reduced coordinates u_r:
 //   f_r = sum_{e in RID} w_e Phi_e^T f_e(Phi_e u_r)
 //   K_r = sym( sum_{e in RID} w_e Phi_e^T k_e(Phi_e u_r) Phi_e )
@@ -21,6 +21,9 @@
 // the matrix-core operand.  Each work item writes its 64x64 partial; a
 // second kernel adds the partials of an instance in a fixed order, 16x16
 // tile pairs per work-group, symmetrises (model.py:317) and writes FP64.
+// A basis of rank r < 64 is stored with its columns padded by zeros to the
+// 64 of the matrix-core tiles: the padded rows and columns of K_r and f_r
+// are exact zeros and are not written.
 #include <algorithm>
 #include <climits>
 #include <cmath>
@@ -42,6 +45,7 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 struct DevEcsw {
   int32_t n_inst, n_elem, n_items;
+  int32_t rank;              // basis rank r (u_r, f_r, K_r strides); phi is padded to 64
   const float4* geom;        // [n_elem][5]: (h_b, 0) for b = 0..3, then (V w, mu, K, 0)
   const float* phi;          // [n_elem][12][64]
   const int32_t* item_inst;  // [n_items]
@@ -78,7 +82,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
       for (int i = 0; i < kPre; ++i) pre[i] = t + 256 * i < n4 ? src[t + 256 * i] : make_float4(0.f, 0.f, 0.f, 0.f);
     }
     __syncthreads();   // the previous item's reads of s_ur are done
-    if (t < kRank) s_ur[t] = float(ur_all[int64_t(inst) * kRank + t]);
+    if (t < kRank) s_ur[t] = t < d.rank ? float(ur_all[int64_t(inst) * d.rank + t]) : 0.0f;
     floatx4 acc[4];
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt)
@@ -244,14 +248,17 @@ __global__ __launch_bounds__(256) void ecsw_reduce_kernel(DevEcsw d, double* __r
   __syncthreads();
   // element (16 I + i, 16 J + j): S_ij from a, S_ji = s_t[j][i]
   const double v = 0.5 * (a + s_t[j][i]);
-  double* K = K_r + int64_t(inst) * kRank * kRank;
-  K[(16 * I + i) * kRank + 16 * J + j] = v;
-  K[(16 * J + j) * kRank + 16 * I + i] = v;
-  if (I == 0 && J == 0 && t < kRank) {
+  const int r = d.rank;
+  double* K = K_r + int64_t(inst) * r * r;
+  if (16 * I + i < r && 16 * J + j < r) {
+    K[(16 * I + i) * r + 16 * J + j] = v;
+    K[(16 * J + j) * r + 16 * I + i] = v;
+  }
+  if (I == 0 && J == 0 && t < r) {
     double s = 0.0;
 #pragma unroll 8
     for (int it = i0; it < i1; ++it) s += double(d.partF[int64_t(it) * kRank + t]);
-    f_r[int64_t(inst) * kRank + t] = s;
+    f_r[int64_t(inst) * r + t] = s;
   }
 }
 
@@ -273,7 +280,7 @@ int32_t ecsw_create(int32_t n_inst, const int64_t* rid_offsets, const double* Xe
                     psx_ecsw** out) {
   if (!out) return fail(PSX_EINVAL, "out is null");
   *out = nullptr;
-  if (rank != kRank) return fail(PSX_EUNSUPPORTED, "the MFMA projection is built for rank 64");
+  if (rank < 1 || rank > kRank) return fail(PSX_EUNSUPPORTED, "the MFMA projection takes a basis rank of 1..64");
   if (n_inst <= 0 || !rid_offsets || !Xe || !phi || !weights || !mu || !lam)
     return fail(PSX_EINVAL, "null or empty ECSW input");
   if (rid_offsets[0] != 0) return fail(PSX_EINVAL, "rid_offsets[0] must be 0");
@@ -311,8 +318,12 @@ int32_t ecsw_create(int32_t n_inst, const int64_t* rid_offsets, const double* Xe
     for (int b = 0; b < 4; ++b) geom[5 * e + b] = make_float4(float(h[b][0]), float(h[b][1]), float(h[b][2]), 0.f);
     geom[5 * e + 4] = make_float4(float(det / 6.0 * weights[e]), float(mu[e]), float(K), 0.f);
   }
-  std::vector<float> phif(size_t(n_elem) * kPhi);
-  for (size_t i = 0; i < phif.size(); ++i) phif[i] = float(phi[i]);
+  // Phi_e (12 x rank per element) rounded to FP32 and padded to 64 columns
+  std::vector<float> phif(size_t(n_elem) * kPhi, 0.0f);
+  for (int64_t e = 0; e < n_elem; ++e)
+    for (int d = 0; d < 12; ++d)
+      for (int c = 0; c < rank; ++c)
+        phif[size_t(e) * kPhi + size_t(d) * kRank + c] = float(phi[(size_t(e) * 12 + d) * rank + c]);
   std::vector<int32_t> item_inst, item_e0, item_cnt, inst_item0{0};
   for (int i = 0; i < n_inst; ++i) {
     for (int64_t e = rid_offsets[i]; e < rid_offsets[i + 1]; e += kChunk) {
@@ -346,6 +357,7 @@ int32_t ecsw_create(int32_t n_inst, const int64_t* rid_offsets, const double* Xe
       (rc = up(nullptr, n_items * kRank * kRank * 4, &dk)) || (rc = up(nullptr, n_items * kRank * 4, &df)))
     return bail(rc);
   h->d.n_inst = n_inst;
+  h->d.rank = rank;
   h->d.n_elem = int32_t(n_elem);
   h->d.n_items = int32_t(n_items);
   h->d.geom = static_cast<const float4*>(dg);

@@ -472,32 +472,39 @@ int32_t launch_tet4_hyper(const DevPlan& p, int model, const double* const* para
 
 // Geometry records of a bound material (FP32 Neo-Hookean rows): record b of
 // element e is (h_b, V mu_e) for local nodes 0 and 1 and (h_b, V K_e) for 2
-// and 3, K = lambda + 2 mu / 3, each product formed in FP64 and rounded once.
+// and 3, K = lambda + 2 mu / 3.  V is recomputed here in FP64 from the
+// reference coordinates (the same determinant as the plan's), so each
+// product V mu and V K is formed in FP64 and rounded to FP32 once.
 // Every row's node order (row node, then the others by local index) puts a
 // local node 0 or 1 second and a local node 2 or 3 last, so a row finds
 // V mu in G[1].w and V K in G[3].w.  The Neo-Hookean S, its tangent
 // coefficients and so f and K of a row are linear in (mu, K), so the row with
 // V = 1 and (V mu, V K) is the row with V and (mu, K).
-__global__ __launch_bounds__(256) void bind_material_kernel(const float4* __restrict__ geo,
-                                                            const double* __restrict__ mu,
-                                                            const double* __restrict__ lam, int64_t E,
+__global__ __launch_bounds__(256) void bind_material_kernel(DevPlan p, const double* __restrict__ mu,
+                                                            const double* __restrict__ lam,
                                                             float4* __restrict__ out) {
   const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (e >= E) return;
+  if (e >= p.E) return;
+  const int4 c = *reinterpret_cast<const int4*>(p.conn + 4 * e);
+  const double4 x0 = p.Xi[c.x], x1 = p.Xi[c.y], x2 = p.Xi[c.z], x3 = p.Xi[c.w];
+  const double D[3][3] = {{x1.x - x0.x, x2.x - x0.x, x3.x - x0.x},
+                          {x1.y - x0.y, x2.y - x0.y, x3.y - x0.y},
+                          {x1.z - x0.z, x2.z - x0.z, x3.z - x0.z}};
+  const double V = det3(D) / 6.0;
   const double m = mu[e];
   const double K = lam[e] + (2.0 / 3.0) * m;
 #pragma unroll
   for (int b = 0; b < 4; ++b) {
-    float4 g = geo[4 * e + b];
-    g.w = float(double(g.w) * (b < 2 ? m : K));
+    float4 g = p.geo[4 * e + b];
+    g.w = float(V * (b < 2 ? m : K));
     out[4 * e + b] = g;
   }
 }
 
 int32_t launch_bind_material(const DevPlan& p, const double* mu, const double* lam, float4* out, hipStream_t s) {
   if (p.E <= 0) return PSX_OK;
-  hipLaunchKernelGGL(bind_material_kernel, dim3(unsigned((int64_t(p.E) + 255) / 256)), dim3(256), 0, s, p.geo, mu,
-                     lam, int64_t(p.E), out);
+  hipLaunchKernelGGL(bind_material_kernel, dim3(unsigned((int64_t(p.E) + 255) / 256)), dim3(256), 0, s, p, mu,
+                     lam, out);
   PSX_HIP_TRY(hipGetLastError());
   return PSX_OK;
 }

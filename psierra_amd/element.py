@@ -15,6 +15,18 @@ Tet4Fbar keys its geometry cache, tet4_fbar.py:54-57), so an unmodified
 The per-element ``compute_nonlinear_force_and_tangent_stiffness`` runs the
 device element-batch kernel on one element (used by code such as the ECSW
 model that calls elements one at a time).
+
+Two things the hooks do beyond a plain call:
+* a Neo-Hookean material is bound to a Tet4 plan whenever it changes
+  (``psx_plan_bind_material``), so every assembly reads the per-element
+  parameters once per bind instead of once per call;
+* the hooks keep the last displacement's f (and K when it was formed) on
+  the device: the Newton loop asks for f and then K at the same u
+  (src/psierra/analysis/nonlinear_static.py:296-297), and its line search
+  evaluates f at the step it then accepts (:366-395), so each iteration's
+  f request is served from the cache and its K request is one fused f + K
+  launch.  The cache is keyed on the displacement's values, the material
+  version and the precision.
 """
 from __future__ import annotations
 
@@ -71,7 +83,7 @@ class _GpuTotalLagrangian:
                 raise ValueError(f"{type(self).__name__} needs {self._npe}-node elements")
             tile = 0 if self.tile == "auto" else int(self.tile)
             st = {"mesh": mesh, "plan": engine.AssemblyPlan(mesh.nodes, elements, tile), "params_key": None,
-                  "tuned": self.tile != "auto"}
+                  "tuned": self.tile != "auto", "bound": False, "hook": None}
             self._cache = {key: st}
         pkey = (id(self._material), getattr(self._material, "version", 0))
         if st["params_key"] != pkey:
@@ -81,9 +93,14 @@ class _GpuTotalLagrangian:
             dev = st["plan"].device
             st["model"] = model
             st["params"] = [torch.as_tensor(a, dtype=torch.float64, device=dev) for a in arrays]
+            st["bound"] = False
             if model == "neohookean":
                 st["mu"], st["lam"] = st["params"]
+                if st["plan"].npe == 4 and self._binds_material:
+                    st["plan"].bind_material(st["mu"], st["lam"])
+                    st["bound"] = True
             st["params_key"] = pkey
+            st["hook"] = None
         return st
 
     def plan(self, mesh) -> engine.AssemblyPlan:
@@ -98,6 +115,9 @@ class _GpuTotalLagrangian:
     # Element name carried by the inversion message (None: the Total
     # Lagrangian text "Element inversion detected: det(F) = ...").
     _inversion_name: str | None = None
+    # Bind a Neo-Hookean material to Tet4 plans (the F-bar element runs its
+    # own kernels with per-call parameters).
+    _binds_material: bool = True
 
     def _relabel(self, err: ValueError) -> ValueError:
         msg = str(err)
@@ -139,9 +159,9 @@ class _GpuTotalLagrangian:
         prec = precision or self.precision
         try:
             if st["model"] == "neohookean":
+                mu, lam = (None, None) if st["bound"] else (st["mu"], st["lam"])
                 return st["plan"].assemble(
-                    u, st["mu"], st["lam"], f=f, kval=kval, precision=prec, want_k=want_k,
-                    check_status=check_status,
+                    u, mu, lam, f=f, kval=kval, precision=prec, want_k=want_k, check_status=check_status,
                 )
             return st["plan"].assemble_hyperelastic(
                 u, st["model"], st["params"], f=f, kval=kval, precision=prec, want_k=want_k,
@@ -181,16 +201,32 @@ class _GpuTotalLagrangian:
         return GraphedCall(launch, u, (f, k), check=lambda uu: self._raise_status(mesh, uu))
 
     # -- SolidDomain hooks (src/psierra/domain/solid.py:238-241, 288-291) ---
+    def _hook_eval(self, mesh, displacement, want_k: bool):
+        """(f host array, K values device tensor or None) at ``displacement``,
+        from the cache when the last hook call had the same displacement,
+        material version and precision (and K when K is asked for)."""
+        st = self._state(mesh)
+        u = np.ascontiguousarray(displacement, dtype=np.float64).reshape(-1)
+        key = (self.precision, self._hook_extra())
+        c = st["hook"]
+        if c is not None and c["key"] == key and (c["k"] is not None or not want_k) and np.array_equal(c["u"], u):
+            return c["f"], c["k"]
+        f, k = self.assemble_device(mesh, u, want_k=want_k)
+        fh = f.cpu().numpy()
+        st["hook"] = {"u": u.copy(), "f": fh, "k": k, "key": key}
+        return fh, k
+
+    def _hook_extra(self):
+        """Settings besides precision and material that change the result."""
+        return None
+
     def assemble_internal_forces(self, mesh, displacement) -> np.ndarray:
-        f, _ = self.assemble_device(mesh, np.asarray(displacement, dtype=np.float64).reshape(-1), want_k=False)
-        return f.cpu().numpy()
+        f, _ = self._hook_eval(mesh, displacement, want_k=False)
+        return f.copy()
 
     def assemble_tangent_stiffness_matrix(self, mesh, displacement) -> spsp.csr_matrix:
-        plan = self._state(mesh)["plan"]
-        _, k = self.assemble_device(mesh, np.asarray(displacement, dtype=np.float64).reshape(-1), want_k=True)
-        indptr, indices = plan.pattern()
-        n = plan.n_dofs
-        return spsp.csr_matrix((k.cpu().numpy(), indices, indptr), shape=(n, n))
+        _, k = self._hook_eval(mesh, displacement, want_k=True)
+        return self._csr(mesh, k)
 
     def _csr(self, mesh, kval) -> spsp.csr_matrix:
         plan = self._state(mesh)["plan"]
@@ -342,6 +378,7 @@ class GpuTet4Fbar(GpuTet4TotalLagrangian):
     """
 
     supports_element_indexing = True
+    _binds_material = False
 
     def __init__(self, material, precision: str = "fp64", tile=0, bulk_modulus: float | None = None) -> None:
         super().__init__(material, precision=precision, tile=tile)
@@ -380,6 +417,9 @@ class GpuTet4Fbar(GpuTet4TotalLagrangian):
     def _nnz(self, mesh) -> int:
         return self._fbar(mesh)["fbar"].nnz
 
+    def _hook_extra(self):
+        return self.bulk_modulus
+
     def _raise_status(self, mesh, u) -> None:
         self._fbar(mesh)["fbar"].raise_on_status(u)
 
@@ -412,10 +452,6 @@ class GpuTet4Fbar(GpuTet4TotalLagrangian):
         # the reference returns the CSR sum K_dev + K_vol, which drops exact zeros
         K.eliminate_zeros()
         return K
-
-    def assemble_tangent_stiffness_matrix(self, mesh, displacement) -> spsp.csr_matrix:
-        _, k = self.assemble_device(mesh, np.asarray(displacement, dtype=np.float64).reshape(-1), want_k=True)
-        return self._csr(mesh, k)
 
     def assemble_mass_matrix(self, mesh) -> spsp.csr_matrix:
         return GpuTet4TotalLagrangian._csr(self, mesh, self._state(mesh)["plan"].assemble_mass(

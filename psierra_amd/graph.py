@@ -53,40 +53,39 @@ class GraphedCall:
         return self.outputs
 
 
-class GraphedAssembly:
+class GraphedAssembly(GraphedCall):
+    """``psx_assemble`` of ``plan`` with fixed (mu, lam) captured in a graph;
+    ``replay(u)`` returns the device (f, K values) buffers."""
+
     def __init__(self, plan: AssemblyPlan, mu, lam, precision: str = "fp32", want_k: bool = True, warmup: int = 2):
         torch = _torch()
-        self.plan = plan
         dev = plan.device
-        self.u = torch.zeros(plan.n_dofs, dtype=torch.float64, device=dev)
+        self.plan = plan
         self.mu = torch.as_tensor(mu, dtype=torch.float64, device=dev).contiguous()
         self.lam = torch.as_tensor(lam, dtype=torch.float64, device=dev).contiguous()
-        self.f, self.k = plan.alloc_outputs(want_k)
         self.precision = precision
         self.want_k = want_k
-        side = torch.cuda.Stream()
-        side.wait_stream(torch.cuda.current_stream())
-        with torch.cuda.stream(side):
-            for _ in range(warmup):
-                self._launch()
-        torch.cuda.current_stream().wait_stream(side)
-        torch.cuda.synchronize()
-        self.graph = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(self.graph):
-            self._launch()
-        torch.cuda.synchronize()
+        u = torch.zeros(plan.n_dofs, dtype=torch.float64, device=dev)
+        f, k = plan.alloc_outputs(want_k)
 
-    def _launch(self):
-        self.plan.assemble(self.u, self.mu, self.lam, f=self.f, kval=self.k, precision=self.precision,
-                           want_k=self.want_k, check_status=False)
+        def launch():
+            plan.assemble(u, self.mu, self.lam, f=f, kval=k, precision=precision, want_k=want_k,
+                          check_status=False)
+
+        super().__init__(launch, u, (f, k), check=plan.raise_on_status, warmup=warmup)
+
+    @property
+    def f(self):
+        return self.outputs[0]
+
+    @property
+    def k(self):
+        return self.outputs[1]
 
     def replay(self, u=None, check_status: bool = False):
         if u is not None:
-            self.u.copy_(self.plan._device_f64(u, self.plan.n_dofs, "u"))
-        self.graph.replay()
-        if check_status:
-            self.plan.raise_on_status(self.u)
-        return self.f, self.k
+            u = self.plan._device_f64(u, self.plan.n_dofs, "u")
+        return super().replay(u, check_status)
 
 
 def autotune_tile(plan: AssemblyPlan, u, mu, lam, precision: str = "fp32", reps: int = 5, tiles=None):

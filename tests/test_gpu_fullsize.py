@@ -5,7 +5,8 @@ A full scipy assembly at this size needs > 35 GB, so parity is checked by
 evaluated in FP64), (b) size-independent properties: nnz closed form,
 sum of f_int = 0 (each element's forces cancel), K_T t = 0 for the three
 rigid translations t (f is translation invariant), (c) FP32 vs FP64 on the
-whole matrix within the 1e-5 relative tolerance.
+whole matrix within the 1e-5 relative tolerance (the FP64 result itself
+being pinned to the oracle at 1e-12 on the sampled rows).
 """
 from __future__ import annotations
 
@@ -68,7 +69,10 @@ def test_fullsize_sampled_rows_match_oracle(big):
         assert rel_err(got64, ref) < 1e-12
         assert rel_err(got32, ref) < 1e-5
         assert rel_err(f64[3 * a : 3 * a + 3], f_ref[i]) < 1e-10
-        assert rel_err(f32[3 * a : 3 * a + 3], f_ref[i]) < 2e-4
+    # f over the sampled rows at the stated 1e-5 (a single node's f is a sum
+    # of ~24 element forces that nearly cancel, so it is compared as a whole)
+    got32 = np.stack([f32[3 * int(a) : 3 * int(a) + 3] for a in sample])
+    assert rel_err(got32, f_ref) < 1e-5
 
 
 def test_fullsize_invariants(big):

@@ -271,3 +271,62 @@ def test_rebinding_and_binding_errors(jittered):
         f = torch.empty(plan.n_dofs, dtype=torch.float64, device=dev)
         check(lib().psx_assemble(plan._h, ptr(u), ptr(mu), None, ptr(f), None, 1, current_stream_handle(),
                                  ptr(plan.status)))
+
+
+def test_element_hooks_run_the_bound_path_and_fuse_f_then_k(jittered):
+    """The drop-in element binds its Neo-Hookean material to the plan, so the
+    SolidDomain hooks and the element's graph replay give the bits of
+    ``assemble(u, None, None)`` after ``bind_material``; f then K at one
+    displacement is one f-only launch then one fused launch at a fresh u, and
+    the K request alone at the u whose f the last call formed; repeated
+    requests are served from the cache; a material change rebinds."""
+    import torch
+
+    from psierra_amd.domain import SolidDomain
+    from psierra_amd.element import GpuTet4TotalLagrangian
+    from psierra_amd.engine import AssemblyPlan
+    from psierra_amd.material import NeoHookeanField
+
+    wl, f_ref, K_ref = jittered
+    banner("tet4 element hooks: bound path and f/K cache")
+    mat = NeoHookeanField(wl.mu, wl.lam)
+    el = GpuTet4TotalLagrangian(mat, precision="fp32")
+    dom = SolidDomain(wl.mesh, mat, el)
+    calls = []
+    orig = el.assemble_device
+
+    def counting(mesh, u, want_k=True, **kw):
+        calls.append(want_k)
+        return orig(mesh, u, want_k=want_k, **kw)
+
+    el.assemble_device = counting
+    dom.update_state(u=wl.u.ravel())
+    f = dom.assemble_internal_forces()
+    K = dom.assemble_tangent_stiffness_matrix()
+    assert calls == [False, True]
+    f_again = dom.assemble_internal_forces()
+    K_again = dom.assemble_tangent_stiffness_matrix()
+    assert calls == [False, True]
+    assert np.array_equal(K_again.data, K.data)
+    plan = AssemblyPlan(wl.mesh.nodes, wl.mesh.elements)
+    dev = plan.device
+    u = torch.as_tensor(wl.u.reshape(-1), device=dev)
+    plan.bind_material(torch.as_tensor(wl.mu, device=dev), torch.as_tensor(wl.lam, device=dev))
+    f_b, k_b = plan.assemble(u, None, None, precision="fp32")
+    assert np.array_equal(K.data, k_b.cpu().numpy()) and np.array_equal(f_again, f_b.cpu().numpy())
+    assert rel_err(K.data, K_ref.data) < FP32_TOL and rel_err(f, f_ref) < FP32_TOL
+    # a new displacement: f-only, then the K request recomputes f + K
+    dom.update_state(u=0.5 * wl.u.ravel())
+    dom.assemble_internal_forces()
+    dom.assemble_tangent_stiffness_matrix()
+    assert calls == [False, True, False, True]
+    # the graph replay of the element takes the bound path too
+    el.assemble_device = orig
+    g = el.graph_assembly(wl.mesh, want_k=True)
+    f_g, k_g = g.replay(u, check_status=True)
+    assert torch.equal(f_g, f_b) and torch.equal(k_g, k_b)
+    # a changed material rebinds and invalidates the cache
+    mat.set_parameters(2.0 * wl.mu, 2.0 * wl.lam)
+    dom.update_state(u=wl.u.ravel())
+    f2 = dom.assemble_internal_forces()
+    assert rel_err(f2, 2.0 * f_b.cpu().numpy()) < 1e-6
