@@ -61,8 +61,9 @@ enum {
   PSX_INFO_COUNT = 9
 };
 
-/* Version of this interface: 3 (2: psx_ecsw_* took rank 64 only; 1 had
- * psx_ecsw_training without work_elements and no psx_plan_bind_material). */
+/* Version of this interface: 4 (3 had no psx_ecsw_training_noalloc; 2:
+ * psx_ecsw_* took rank 64 only; 1 had psx_ecsw_training without
+ * work_elements and no psx_plan_bind_material). */
 int32_t psx_abi_version(void);
 
 /* Message of the last failing call on this thread ("" if none). */
@@ -268,6 +269,21 @@ int32_t psx_ecsw_training(psx_plan* plan, const double* phi, int32_t rank, const
                           int32_t n_snapshots, const double* mu, const double* lam,
                           int32_t include_stiffness, double* A, double* work, int64_t work_elements,
                           void* stream, int32_t* status);
+
+/* psx_ecsw_training without the work argument, allocating nothing.  No
+ * counterpart in the reference: it is for callers that cannot allocate device
+ * memory (code under hipGraph capture, serving loops, harnesses that forbid
+ * hipMalloc).  The element-major work block lives in a 32 MiB device array
+ * of the library (one per GPU, held until the process ends); the elements go
+ * through it in chunks of floor(4 Mi / rows) (rounded down to a multiple of
+ * 64 when at least 64 fit), so A is the same as psx_ecsw_training's.  Rows =
+ * n_snapshots * rank (+ rank (rank + 1) / 2) above 4 Mi (4,194,304) are
+ * refused with PSX_EUNSUPPORTED.  Every call on a GPU shares that GPU's
+ * array: calls that overlap on one GPU (different streams or threads) race.
+ * The plan's device must be the current device. */
+int32_t psx_ecsw_training_noalloc(psx_plan* plan, const double* phi, int32_t rank, const double* snapshots,
+                                  int32_t n_snapshots, const double* mu, const double* lam,
+                                  int32_t include_stiffness, double* A, void* stream, int32_t* status);
 
 /* ---- Element batch (device) -------------------------------------------------
  * Replaces Tet4TotalLagrangian.compute_nonlinear_force_and_tangent_stiffness

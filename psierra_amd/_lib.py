@@ -88,6 +88,10 @@ SIGNATURES = {
         [c_void_p, c_void_p, c_int32, c_void_p, c_int32, c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_int64,
          c_void_p, c_void_p],
     ),
+    "psx_ecsw_training_noalloc": (
+        c_int32,
+        [c_void_p, c_void_p, c_int32, c_void_p, c_int32, c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_void_p],
+    ),
     "psx_element_batch": (
         c_int32,
         [c_int32, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_void_p],
@@ -101,7 +105,7 @@ SIGNATURES = {
 _lib = None
 
 # The C ABI these signatures describe (psx_abi_version() of the library).
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 
 class PsxError(RuntimeError):

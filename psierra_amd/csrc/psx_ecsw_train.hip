@@ -21,6 +21,9 @@
 // reference's (rows x E) layout.
 #include <algorithm>
 #include <climits>
+#include <mutex>
+#include <string>
+#include <vector>
 
 #include "psx_device.h"
 #include "psx_internal.h"
@@ -215,3 +218,64 @@ int32_t launch_ecsw_training(const DevPlan& p, const double* phi, int r, const d
 }
 
 }  // namespace psx
+
+namespace psx {
+
+// Scratch of psx_ecsw_training_noalloc: a file-scope device array (one copy
+// per GPU, held until the process ends) in place of the caller's work block.
+// Host code never passes `scratch` itself to a kernel (on the host it names
+// a stand-in, not device memory): each device's address comes from
+// hipGetSymbolAddress on that device's first call and is kept here.
+constexpr int64_t kNoAllocScratchDoubles = int64_t(4) << 20;   // 32 MiB
+static __device__ __align__(256) double scratch[kNoAllocScratchDoubles];
+
+namespace {
+std::mutex g_scratch_mutex;
+std::vector<double*> g_scratch_ptr;   // per device id, nullptr until looked up
+
+int32_t scratch_on_current_device(double** out) {
+  int dev = 0;
+  PSX_HIP_TRY(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (size_t(dev) >= g_scratch_ptr.size()) g_scratch_ptr.resize(size_t(dev) + 1, nullptr);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    PSX_HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return fail(PSX_ENOTREADY, "hipGetSymbolAddress returned no address for the scratch array");
+    g_scratch_ptr[dev] = static_cast<double*>(ptr);
+  }
+  *out = g_scratch_ptr[dev];
+  return PSX_OK;
+}
+}  // namespace
+
+}  // namespace psx
+
+extern "C" int32_t psx_ecsw_training_noalloc(psx_plan* plan, const double* phi, int32_t rank,
+                                             const double* snapshots, int32_t n_snapshots, const double* mu,
+                                             const double* lam, int32_t include_stiffness, double* A,
+                                             void* stream, int32_t* status) {
+  psx::clear_error();
+  if (!plan || !phi || !snapshots || !mu || !lam || !A || !status)
+    return psx::fail(PSX_EINVAL, "null argument to psx_ecsw_training_noalloc");
+  if (plan->hp.npe != 4) return psx::fail(PSX_EUNSUPPORTED, "psx_ecsw_training_noalloc supports Tet4 plans");
+  if (rank < 1 || rank > psx::kTrainMaxRank)
+    return psx::fail(PSX_EUNSUPPORTED, "psx_ecsw_training_noalloc supports ranks 1..64");
+  if (n_snapshots < 1) return psx::fail(PSX_EINVAL, "need at least one snapshot");
+  const int64_t rows = int64_t(n_snapshots) * rank + (include_stiffness ? int64_t(rank) * (rank + 1) / 2 : 0);
+  if (rows > psx::kNoAllocScratchDoubles)
+    return psx::fail(PSX_EUNSUPPORTED, "psx_ecsw_training_noalloc: " + std::to_string(rows) +
+                                           " training rows exceed its 32 MiB scratch (at most " +
+                                           std::to_string(psx::kNoAllocScratchDoubles) + " rows)");
+  int dev = 0;
+  PSX_HIP_TRY(hipGetDevice(&dev));
+  if (dev != plan->device)
+    return psx::fail(PSX_EINVAL, "psx_ecsw_training_noalloc: the plan's device is not the current device");
+  double* work = nullptr;
+  const int32_t rc = psx::scratch_on_current_device(&work);
+  if (rc) return rc;
+  // whole elements of the scratch, in multiples of 64 when at least 64 fit
+  const int64_t work_elements = psx::kNoAllocScratchDoubles / rows;
+  return psx::launch_ecsw_training(plan->d, phi, rank, snapshots, n_snapshots, mu, lam, include_stiffness != 0, A,
+                                   work, work_elements, static_cast<hipStream_t>(stream), status);
+}
