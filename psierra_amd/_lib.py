@@ -29,6 +29,8 @@ INFO_KEYS = (
     "max_degree",
     "device_bytes",
     "n_incidences",
+    "max_incidences",
+    "wide",
 )
 
 _i64p = POINTER(c_int64)

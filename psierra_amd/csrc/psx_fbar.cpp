@@ -210,6 +210,9 @@ int32_t psx_fbar_create(psx_plan* plan, psx_fbar** out) {
   psx::clear_error();
   if (!plan || !out) return psx::fail(PSX_EINVAL, "plan/out is null");
   if (plan->hp.npe != 4) return psx::fail(PSX_EUNSUPPORTED, "F-bar needs a Tet4 plan");
+  if (plan->d.wide)
+    return psx::fail(PSX_EUNSUPPORTED, "F-bar needs every node to have at most 64 incident elements and "
+                                       "255 neighbours");
   psx_fbar* fb = new psx_fbar();
   fb->plan = plan;
   int dev = 0;

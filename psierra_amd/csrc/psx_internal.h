@@ -30,11 +30,21 @@ constexpr uint32_t kIncElemMask = (1u << kIncElemBits) - 1u;
 // A tile of TILE lanes holds at most TILE incidences and at most TILE
 // reduction lanes (Tet4; Tet10: kTet10LanePasses * TILE): kTet4DiagLanes
 // (Tet10: kTet10DiagLanes) per node for the diagonal block and f, plus one
-// per (node, slot) unit.  Every node has at most kMaxNodeIncidences
-// incidences (feed bit masks are 64-bit).
+// per (node, slot) unit.  The kernels with feed bit masks or slot-rank bytes
+// (Tet10, mass, F-bar) need every node to have at most kMaxNodeIncidences
+// incidences and at most kMaxRankDegree neighbours; a plan outside those
+// bounds ("wide") is tiled for the Tet4 kernels only (a node's run at most
+// TILE incidences, any degree) and the other kernels refuse it.
 constexpr int kTet4DiagLanes = 8;
 constexpr int kTet10DiagLanes = 4;
 constexpr int kMaxNodeIncidences = 64;
+constexpr int kMaxRankDegree = 255;
+// Tet4 row kernel (tet4_rows): per incidence the 10-bit staging positions of
+// its three off-diagonal blocks (one in area A, two in area B, each area
+// unit-major); per unit (node, slot) its first A and first B positions
+// (10 bits each) and its tile-local node (bits 20..28).
+constexpr int kPosBits = 10;
+constexpr uint32_t kPosMask = (1u << kPosBits) - 1u;
 constexpr int kTet10LanePasses = 4;
 
 // Host-side structural data (all int32 unless noted), built once per mesh.
@@ -81,6 +91,9 @@ struct DevPlan {
   const int32_t* tile_node;       // n_tiles+1
   const int4* tile_info;          // n_tiles: first node, nodes, first incidence, incidences
   const int4* node_rec;           // N+1: first incidence, first unit, kofs, self slot rank
+  const uint32_t* inc_pos;        // Tet4, per incidence: staging positions of its 3 off-diagonal blocks
+  const uint32_t* unit_rec;       // Tet4, per unit: first A | first B << 10 | tile-local node << 20
+  int32_t wide;                   // 1: a node exceeds the mask/rank-byte bounds (Tet4 row kernels only)
   double4* ui;                    // scratch: displacement in internal order
   double4* wi;                    // scratch: adjoint cotangent in internal order
 };
@@ -146,7 +159,9 @@ struct psx_plan {
   std::vector<int32_t> perm_h;     // internal -> user
   std::vector<int32_t> inc_ptr_h;  // internal order
   std::vector<int32_t> deg_h;      // internal order
+  std::vector<uint16_t> inc_slot_h;  // Tet4: per incidence, the slot of each other node (local order) in the row
   int32_t max_deg = 0;
+  int32_t max_run = 0;             // most incidences of any node
   double* mass_coef = nullptr;     // device npe x npe reference mass tensor (lazy)
   double* mu_bound = nullptr;      // device copies of the bound material (psx_plan_bind_material)
   double* lam_bound = nullptr;

@@ -10,6 +10,7 @@
 //   value(3a + r, 3 b_s + c) = kofs(a) + 3 r deg(a) + 3 s + c,
 // with b_s the s-th neighbour of a ("slot rank" s).
 #include <algorithm>
+#include <atomic>
 #include <array>
 #include <cmath>
 #include <cstdio>
@@ -299,32 +300,97 @@ int32_t upload(psx_plan* plan, const T* host, size_t count, T** dev) {
   return PSX_OK;
 }
 
+// Tet4 row-kernel records of one tiling (tet4_rows): an incidence stages
+// its first off-diagonal block (the other node of lowest local index) in
+// area A and its other two in area B; each area is unit-major, a (node,
+// slot) unit's feeds contiguous and in run order (increasing element id).
+// Per incidence: the A position and the two B positions (10 bits each).
+// Per unit: its first A and first B positions and its tile-local node; a
+// unit's feed counts are the differences to the next unit's (the tile's last
+// unit ends at nk in A and 2 nk in B).  A unit sums its A feeds, then its B
+// feeds: an order fixed by the mesh alone.
+int32_t make_tet4_positions(psx_plan* plan, const std::vector<int32_t>& tiles) {
+  const int64_t N = plan->hp.N;
+  std::vector<int32_t> uoff(N + 1, 0);
+  for (int64_t i = 0; i < N; ++i) uoff[i + 1] = uoff[i] + plan->deg_h[i];
+  std::vector<uint32_t> unit_rec(size_t(std::max<int64_t>(uoff[N], 1)), 0u);
+  std::vector<uint32_t> inc_pos(size_t(std::max<int64_t>(plan->n_inc, 1)), 0u);
+  const int64_t n_tiles = int64_t(tiles.size()) - 1;
+  std::atomic<int> bad{0};
+  parallel_for(n_tiles, [&](int64_t tb, int64_t te) {
+    std::vector<int32_t> fa, fb, ca, cb;
+    for (int64_t t = tb; t < te; ++t) {
+      const int64_t n0 = tiles[t], n1 = tiles[t + 1];
+      int32_t pa = 0, pb = 0;
+      for (int64_t i = n0; i < n1; ++i) {
+        const int32_t d = plan->deg_h[i];
+        fa.assign(size_t(d), 0);
+        fb.assign(size_t(d), 0);
+        ca.assign(size_t(d), 0);
+        cb.assign(size_t(d), 0);
+        const int64_t kb = plan->inc_ptr_h[i], ke = plan->inc_ptr_h[i + 1];
+        for (int64_t k = kb; k < ke; ++k) {
+          ++ca[plan->inc_slot_h[3 * k]];
+          ++cb[plan->inc_slot_h[3 * k + 1]];
+          ++cb[plan->inc_slot_h[3 * k + 2]];
+        }
+        for (int32_t sl = 0; sl < d; ++sl) {
+          fa[sl] = pa;
+          fb[sl] = pb;
+          pa += ca[sl];
+          pb += cb[sl];
+          if (pa > int32_t(kPosMask) + 1 || pb > int32_t(kPosMask) + 1 || i - n0 > 511) bad = 1;
+          unit_rec[uoff[i] + sl] = uint32_t(fa[sl]) | (uint32_t(fb[sl]) << 10) | (uint32_t(i - n0) << 20);
+          ca[sl] = cb[sl] = 0;
+        }
+        for (int64_t k = kb; k < ke; ++k) {
+          const int s0 = plan->inc_slot_h[3 * k], s1 = plan->inc_slot_h[3 * k + 1], s2 = plan->inc_slot_h[3 * k + 2];
+          const uint32_t a0 = uint32_t(fa[s0] + ca[s0]++);
+          const uint32_t b1 = uint32_t(fb[s1] + cb[s1]++);
+          const uint32_t b2 = uint32_t(fb[s2] + cb[s2]++);
+          inc_pos[k] = a0 | (b1 << kPosBits) | (b2 << (2 * kPosBits));
+        }
+      }
+    }
+  });
+  if (bad) return fail(PSX_EUNSUPPORTED, "a tile's Tet4 staging positions exceed their 10-bit fields");
+  uint32_t* dpos = nullptr;
+  uint32_t* drec = nullptr;
+  int32_t rc = upload(plan, inc_pos.data(), inc_pos.size(), &dpos);
+  if (rc) return rc;
+  if ((rc = upload(plan, unit_rec.data(), unit_rec.size(), &drec))) return rc;
+  plan->d.inc_pos = dpos;
+  plan->d.unit_rec = drec;
+  return PSX_OK;
+}
+
 int32_t make_tiles(psx_plan* plan, int32_t tile) {
   if (tile == 0) tile = plan->hp.npe == 4 ? 256 : 64;
   if (tile != 64 && tile != 128 && tile != 256)
     return fail(PSX_EINVAL, "tile must be 64, 128 or 256");
   const int64_t N = plan->hp.N;
+  const bool wide = plan->d.wide != 0;
+  if (wide && plan->hp.npe != 4)
+    return fail(PSX_EUNSUPPORTED, "Tet10 plans need every node to have at most " +
+                                      std::to_string(kMaxNodeIncidences) + " incident elements and " +
+                                      std::to_string(kMaxRankDegree) + " neighbours");
+  if (plan->max_run > tile)
+    return fail(PSX_EUNSUPPORTED, "node with " + std::to_string(plan->max_run) +
+                                      " incident elements exceeds the tile of " + std::to_string(tile));
   // Greedy packing of consecutive nodes: at most `tile` incidences and at
   // most `cap` reduction lanes (the diagonal lanes of each node plus one
-  // per (node, slot) unit) per tile.
-  // Tet10 tiles also hold at most 64 nodes (their node lanes are one wave).
+  // per (node, slot) unit) per tile.  Tet10 tiles also hold at most 64
+  // nodes (their node lanes are one wave).  A wide plan packs by incidences
+  // only (its Tet4 row kernel loops its reduction lanes).
   std::vector<int32_t> tiles{0};
-  int64_t start_inc = 0, lanes = 0, max_inc = 0;
-  const int64_t max_nodes = plan->hp.npe == 4 ? int64_t(tile) : 64;
+  int64_t start_inc = 0, lanes = 0;
+  const int64_t max_nodes = plan->hp.npe == 4 ? std::min<int64_t>(tile, 255) : 64;
   for (int64_t i = 0; i < N; ++i) {
-    int64_t cnt = plan->inc_ptr_h[i + 1] - plan->inc_ptr_h[i];
-    max_inc = std::max(max_inc, cnt);
-    if (cnt > std::min<int64_t>(tile, kMaxNodeIncidences))
-      return fail(PSX_EUNSUPPORTED, "node with " + std::to_string(cnt) +
-                                        " incident elements exceeds the supported " +
-                                        std::to_string(std::min<int64_t>(tile, kMaxNodeIncidences)));
-    // Reduction lanes: the node's diagonal lanes and one per (node, slot)
-    // unit.  Tet4 has one pass of TILE lanes; the Tet10 kernel loops its
-    // lanes over at most kTet10LanePasses passes.
     const int64_t need = (plan->hp.npe == 4 ? kTet4DiagLanes : kTet10DiagLanes) + plan->deg_h[i];
     const int64_t cap = plan->hp.npe == 4 ? tile : int64_t(kTet10LanePasses) * tile;
-    if (need > cap) return fail(PSX_EUNSUPPORTED, "node degree exceeds the tile's reduction lanes");
-    if (plan->inc_ptr_h[i + 1] - start_inc > tile || lanes + need > cap || i - tiles.back() >= max_nodes) {
+    if (!wide && need > cap) return fail(PSX_EUNSUPPORTED, "node degree exceeds the tile's reduction lanes");
+    if (plan->inc_ptr_h[i + 1] - start_inc > tile || (!wide && lanes + need > cap) ||
+        i - tiles.back() >= max_nodes) {
       tiles.push_back(int32_t(i));
       start_inc = plan->inc_ptr_h[i];
       lanes = 0;
@@ -344,11 +410,12 @@ int32_t make_tiles(psx_plan* plan, int32_t tile) {
   int32_t rc = upload(plan, tiles.data(), tiles.size(), &dtiles);
   if (rc) return rc;
   if ((rc = upload(plan, info.data(), info.size(), &dinfo))) return rc;
+  if (plan->hp.npe == 4 && wide && (rc = make_tet4_positions(plan, tiles))) return rc;
   plan->d.tile_node = dtiles;
   plan->d.tile_info = dinfo;
   plan->d.n_tiles = int32_t(tiles.size() - 1);
   plan->d.tile = tile;
-  plan->d.max_inc = int32_t(max_inc);
+  plan->d.max_inc = plan->max_run;
   return PSX_OK;
 }
 
@@ -434,10 +501,15 @@ int32_t create_plan(const int64_t* conn, int64_t E, int npe, const double* X, in
     kofs[i] = 9 * hp.adj_ptr[a];
     uoff[i + 1] = uoff[i] + d;
   }
-  if (plan->max_deg > 255)
+  for (int64_t i = 0; i < N; ++i)
+    plan->max_run = std::max(plan->max_run, plan->inc_ptr_h[i + 1] - plan->inc_ptr_h[i]);
+  if (plan->max_deg > 65535)
     return bail(fail(PSX_EUNSUPPORTED, "node degree " + std::to_string(plan->max_deg) +
-                                           " above 255 is not supported"));
+                                           " above 65535 is not supported"));
+  // beyond the bounds of the feed masks and slot-rank bytes: Tet4 row kernels only
+  plan->d.wide = plan->max_deg > kMaxRankDegree || plan->max_run > kMaxNodeIncidences ? 1 : 0;
   plan->n_inc = plan->inc_ptr_h[N];
+  if (npe == 4 && plan->d.wide) plan->inc_slot_h.assign(size_t(plan->n_inc) * 3, 0);
   std::vector<uint32_t> inc_elem(plan->n_inc), inc_rank(plan->n_inc * rw, 0);
   parallel_for(N, [&](int64_t b, int64_t e) {
     for (int64_t i = b; i < e; ++i) {
@@ -456,12 +528,15 @@ int32_t create_plan(const int64_t* conn, int64_t E, int npe, const double* X, in
           // Tet4 incidences are evaluated with the row node first, then the
           // other three in increasing local order (see tet4 kernel).  Byte 0
           // is the incidence's position in its node's run (< 64); bytes 1..3
-          // are the slot ranks of the other three nodes in a's row.
-          uint32_t r = uint32_t(k - plan->inc_ptr_h[i]);
+          // are the slot ranks of the other three nodes in a's row (bytes
+          // only meaningful for a plan that is not wide).
+          uint32_t r = uint32_t(k - plan->inc_ptr_h[i]) & 0xffu;
           int slot = 1;
           for (int q = 0; q < 4; ++q) {
             if (q == p) continue;
-            r |= rank_of(conn[el * 4 + q]) << (8 * slot);
+            const uint32_t rk = rank_of(conn[el * 4 + q]);
+            if (plan->d.wide) plan->inc_slot_h[3 * k + slot - 1] = uint16_t(rk);
+            r |= (rk & 0xffu) << (8 * slot);
             ++slot;
           }
           inc_rank[k] = r;
@@ -469,7 +544,7 @@ int32_t create_plan(const int64_t* conn, int64_t E, int npe, const double* X, in
           // bytes 0..9: slot ranks of the element's nodes in a's row (byte p
           // is a's own); byte 10: the incidence's position in a's run
           for (int q = 0; q < 10; ++q)
-            inc_rank[k * rw + q / 4] |= rank_of(conn[el * 10 + q]) << (8 * (q % 4));
+            inc_rank[k * rw + q / 4] |= (rank_of(conn[el * 10 + q]) & 0xffu) << (8 * (q % 4));
           inc_rank[k * rw + 2] |= uint32_t(k - plan->inc_ptr_h[i]) << 16;
         }
       }
@@ -691,7 +766,8 @@ int32_t psx_plan_info(const psx_plan* plan, int64_t* info, int32_t n_info) {
   if (!plan || !info) return psx::fail(PSX_EINVAL, "plan/info is null");
   int64_t v[PSX_INFO_COUNT] = {plan->hp.E,     plan->hp.N,      plan->hp.npe,
                                plan->hp.nnz(), plan->d.n_tiles, plan->d.tile,
-                               plan->max_deg,  plan->device_bytes, plan->n_inc};
+                               plan->max_deg,  plan->device_bytes, plan->n_inc,
+                               plan->max_run,  plan->d.wide};
   for (int32_t i = 0; i < n_info && i < PSX_INFO_COUNT; ++i) info[i] = v[i];
   return PSX_OK;
 }
@@ -780,6 +856,9 @@ int32_t psx_plan_bind_material(psx_plan* plan, const double* mu, const double* l
 int32_t psx_assemble_mass(psx_plan* plan, const double* rho, double* kval, void* stream) {
   psx::clear_error();
   if (!plan || !rho || !kval) return psx::fail(PSX_EINVAL, "null argument to psx_assemble_mass");
+  if (plan->d.wide)
+    return psx::fail(PSX_EUNSUPPORTED, "psx_assemble_mass needs every node to have at most 64 incident elements "
+                                       "and 255 neighbours");
   if (!plan->mass_coef) {
     const std::vector<double> c = psx::reference_mass_tensor(plan->hp.npe);
     double* d = nullptr;

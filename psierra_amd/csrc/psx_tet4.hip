@@ -398,10 +398,154 @@ __global__ __launch_bounds__(TILE) void tet4_assemble_rows(DevPlan p, MatParams 
 
 }
 
+// f_int + K_T for a wide plan (a node with more than 64 incident elements or
+// more than 255 neighbours, beyond the feed masks and slot-rank bytes of
+// tet4_assemble_rows): one tile of node rows per work-group, the reduction
+// through unit-major staging areas laid out by the plan.
+//
+// Incidence t (element e seen from node a) evaluates its row in registers
+// and stages four 9-value blocks: its diagonal block (the upper triangle of
+// K_aa) with f_a at block t -- so a node's diagonal feeds are its incidence
+// run, contiguous -- and its three off-diagonal blocks K_ab at the positions
+// the plan gave it (inc_pos): the first (the other node of lowest local
+// index) in area A, the other two in area B.  In each area every (node,
+// slot) unit finds its feeds contiguous and in run order (unit_rec: first A
+// and B positions, tile-local node).  Then the lanes loop over the reduction
+// items, each summing one list in FP64: eight items per node take an eighth
+// of the node's run each (combined across the aligned group of eight lanes
+// in a fixed order, the same bits in all eight; three write the rows of
+// K_aa, one writes f_a), one item per unit sums its A feeds and then its B
+// feeds.  Every CSR value and f entry is written once, every sum has an
+// order fixed by the mesh alone: deterministic, the same for every tile
+// size, no masks, no atomics; runs of up to TILE incidences and any number
+// of neighbours.
+template <typename R, int TILE, int M>
+__global__ __launch_bounds__(TILE) void tet4_rows(DevPlan p, MatParams mp, double* __restrict__ f,
+                                                  double* __restrict__ kval, int32_t* status) {
+  constexpr int NB = 4 * TILE;       // staged blocks, value-major: value v of block b at v NB + b
+  constexpr int A0 = TILE;           // first block of area A
+  constexpr int B0 = 2 * TILE;       // first block of area B
+  constexpr int ND = kTet4DiagLanes; // items per node for K_aa and f_a
+  __shared__ R s_blk[9 * NB];
+  __shared__ int32_t s_kb[TILE + 1];    // first incidence of each node (tile-relative)
+  __shared__ int32_t s_ub[TILE + 1];    // first unit of each node (tile-relative)
+  __shared__ int32_t s_kofs[TILE];      // first CSR value of the node's three rows
+  __shared__ int32_t s_self[TILE];      // slot of the node in its own row
+  __shared__ int32_t s_user[TILE];      // user id of the node
+  const int tile = xcd_tile();
+  if (tile >= p.n_tiles) return;
+  const int4 ti = load_uniform(p.tile_info, tile);
+  const int n0 = ti.x, nn = ti.y, k0 = ti.z, nk = ti.w;
+  const int t = threadIdx.x;
+  const int u0 = load_uniform(p.node_rec, n0).y;
+  const int units = load_uniform(p.node_rec, n0 + nn).y - u0;
+  const int nd = ND * nn;
+  IncInputs<R> in;
+  uint32_t pos = 0;
+  if (t < nk) {
+    load_incidence<R, M>(p, mp, k0 + t, in);
+    pos = p.inc_pos[k0 + t];
+  }
+  for (int i = t; i <= nn; i += TILE) {
+    const int4 r = p.node_rec[n0 + i];
+    s_kb[i] = r.x - k0;
+    s_ub[i] = r.y - u0;
+    if (i < nn) {
+      s_kofs[i] = r.z;
+      s_self[i] = r.w;
+      s_user[i] = p.perm[n0 + i];
+    }
+  }
+  if (t < nk) {
+    R out[39];
+    const R J = incidence_row<R, M, true>(in, out);
+    if (J <= R(0)) flag_inversion(status, int(in.pe & kIncElemMask));
+    // diagonal block: K_aa upper triangle (xx xy xz yy yz zz) and f_a
+    const R dv[9] = {out[0], out[1], out[2], out[4], out[5], out[8], out[36], out[37], out[38]};
+#pragma unroll
+    for (int c = 0; c < 9; ++c) s_blk[c * NB + t] = dv[c];
+#pragma unroll
+    for (int q = 1; q < 4; ++q) {
+      const int b = (q == 1 ? A0 : B0) + int((pos >> (kPosBits * (q - 1))) & kPosMask);
+#pragma unroll
+      for (int c = 0; c < 9; ++c) s_blk[c * NB + b] = out[9 * q + c];
+    }
+  }
+  __syncthreads();
+  // Sum of `cnt` staged blocks from block `b`, in order.
+  auto sum_blocks = [&](double* acc, int b, int cnt) {
+    for (int k = 0; k < cnt; ++k) {
+      R v[9];
+#pragma unroll
+      for (int c = 0; c < 9; ++c) v[c] = s_blk[c * NB + b + k];
+#pragma unroll
+      for (int c = 0; c < 9; ++c) acc[c] += double(v[c]);
+    }
+  };
+  // the lanes loop over the items; a node's eight diagonal items stay in one
+  // aligned group of eight lanes (nd and TILE are multiples of 8)
+  for (int w = t; w < nd + units; w += TILE) {
+    double acc[9];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) acc[c] = 0.0;
+    if (w < nd) {
+      // (node, eighth of its run): K_aa upper triangle and f_a
+      const int j = w / ND, part = w % ND;
+      const int kb = s_kb[j], L = s_kb[j + 1] - kb;
+      const int b = kb + (part * L) / ND, e = kb + ((part + 1) * L) / ND;
+      sum_blocks(acc, b, e - b);
+#pragma unroll
+      for (int c = 0; c < 9; ++c) acc[c] = oct_sum(acc[c], (w & 4) != 0);
+      if (part == 3) {
+        const int64_t a = s_user[j];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) f[3 * a + c] = acc[6 + c];
+      } else if (part < 3) {
+        const int d = s_ub[j + 1] - s_ub[j];
+        if (d > 0) {
+          // row `part` of K_aa from its upper triangle
+          const double e0 = part == 0 ? acc[0] : part == 1 ? acc[1] : acc[2];
+          const double e1 = part == 0 ? acc[1] : part == 1 ? acc[3] : acc[4];
+          const double e2 = part == 0 ? acc[2] : part == 1 ? acc[4] : acc[5];
+          double* dst = kval + s_kofs[j] + 3 * s_self[j] + part * 3 * d;
+          dst[0] = e0;
+          dst[1] = e1;
+          dst[2] = e2;
+        }
+      }
+    } else {
+      // (node, off-diagonal slot): first A and B positions from the unit's
+      // record, counts from the next unit's (the tile's ends for the last)
+      const int u = w - nd;
+      const uint32_t r = p.unit_rec[u0 + u];
+      const uint32_t r2 = u + 1 < units ? p.unit_rec[u0 + u + 1] : uint32_t(nk) | (uint32_t(2 * nk) << 10);
+      const int fa = int(r & kPosMask), fb = int((r >> 10) & kPosMask);
+      const int ca = int(r2 & kPosMask) - fa, cb = int((r2 >> 10) & kPosMask) - fb;
+      if (ca + cb > 0) {
+        sum_blocks(acc, A0 + fa, ca);
+        sum_blocks(acc, B0 + fb, cb);
+        const int j = int(r >> 20);
+        const int d = s_ub[j + 1] - s_ub[j];
+        double* dst = kval + s_kofs[j] + 3 * (u - s_ub[j]);
+#pragma unroll
+        for (int rr = 0; rr < 3; ++rr)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) dst[rr * 3 * d + c] = acc[3 * rr + c];
+      }
+    }
+  }
+}
+
 template <typename R, int TILE, int M>
 static void launch_t4(const DevPlan& p, const MatParams& mp, const FbarView& fv, double* f, double* kval,
                       hipStream_t s, int32_t* status) {
   const int blocks = 8 * ((p.n_tiles + 7) / 8);
+  if constexpr (M != kFbarNeoHookean) {
+    if (kval && p.wide) {
+      hipLaunchKernelGGL((tet4_rows<R, TILE, M>), dim3(blocks), dim3(TILE), 0, s, p, mp, f, kval, status);
+      return;
+    }
+  }
   if (kval && p.max_inc <= 32)
     hipLaunchKernelGGL((tet4_assemble_rows<R, TILE, M, uint32_t>), dim3(blocks), dim3(TILE), 0, s, p, mp, fv, f,
                        kval, status);

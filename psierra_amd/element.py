@@ -134,7 +134,8 @@ class _GpuTotalLagrangian:
         st = self._state(mesh)
         st["tuned"] = True
         plan = st["plan"]
-        tiles = (64, 128, 256) if plan.npe == 4 else (64, 128)
+        # a tile holds every node's whole incidence run
+        tiles = [t for t in ((64, 128, 256) if plan.npe == 4 else (64, 128)) if t >= plan.info["max_incidences"]]
         timings = {}
         for tile in tiles:
             plan.set_tile(tile)

@@ -92,7 +92,8 @@ def autotune_tile(plan: AssemblyPlan, u, mu, lam, precision: str = "fp32", reps:
     """Pick the fastest work-group tile for this plan; returns {tile: ms}."""
     torch = _torch()
     if tiles is None:
-        tiles = (64, 128, 256) if plan.npe == 4 else (64, 128)
+        # a tile holds every node's whole incidence run
+        tiles = [t for t in ((64, 128, 256) if plan.npe == 4 else (64, 128)) if t >= plan.info["max_incidences"]]
     f, k = plan.alloc_outputs(True)
     timings = {}
     for tile in tiles:

@@ -3,8 +3,9 @@
 * nodes that no element references (empty CSR rows, f = 0 exactly) and a
   one-element mesh;
 * a node shared by 32, 33 and 64 elements (the kernel's feed masks are
-  32-bit up to 32 incidences per node, 64-bit above; 64 is the most a node
-  may have), and the plan's rejection of a 65th;
+  32-bit up to 32 incidences per node, 64-bit above), and by 65, 128 and 200
+  (a wide plan: the mask-free row kernel, f-only, adjoint and free CSC;
+  the kernels that need the masks or rank bytes refuse it);
 * zero displacement: f = 0 bit-exactly and K equal to the small-strain
   stiffness of the oracle (S = 0, C = Hookean);
 * every supported tile size gives the same bits on a mesh whose node runs
@@ -160,13 +161,88 @@ def test_free_csc_degenerate_free_sets():
     assert r.numel() == 0 and rn == 0.0
 
 
-def test_node_with_65_incidences_is_rejected():
+def _fan_and_box(n_elem: int, seed: int) -> Workload:
+    """A fan of n_elem elements around one node (3 n_elem neighbours) next to
+    a jittered 5^3 box (750 elements in many ordinary tiles), one mesh."""
+    from psierra_amd import workloads as W
+
+    fan = _fan(n_elem, seed)
+    box = W.jittered_box_tet4(n=5, seed=seed)
+    off = box.mesh.nodes.shape[0]
+    nodes = np.vstack([box.mesh.nodes, fan.mesh.nodes + np.array([3.0, 0.0, 0.0])])
+    conn = np.vstack([box.mesh.elements, fan.mesh.elements + off])
+    u = np.vstack([box.u, fan.u])
+    mu = np.concatenate([box.mu, fan.mu])
+    return Workload(f"fan{n_elem}_box5", TetMesh(nodes, conn), u, mu, 5.0 * mu)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("n_elem", [65, 128, 200])
+def test_wide_node_matches_oracle(precision, n_elem):
+    """A node shared by 65, 128 or 200 elements (beyond the 64-bit feed masks;
+    200 also beyond the 255-neighbour slot-rank bytes): a wide plan, whose
+    Tet4 row kernel stages feeds at positions from the plan.  Pattern
+    bit-exact, K and f against the oracle, the f-only kernel and the adjoint
+    K_T w too, and the same bits for every tile that holds the run."""
+    import torch
+
     from psierra_amd.engine import AssemblyPlan
 
-    wl = _fan(65, seed=9)
-    banner("edge: fan of 65 elements (rejected)")
-    with pytest.raises(Exception, match="incident elements exceeds the supported 64"):
-        AssemblyPlan(wl.mesh.nodes, wl.mesh.elements)
+    wl = _fan_and_box(n_elem, seed=9)
+    banner(f"edge: wide plan, fan of {n_elem} elements {precision}")
+    plan, f, k, f_ref = _check(wl, precision)
+    assert plan.info["wide"] == 1 and plan.info["max_incidences"] == n_elem
+    dev = plan.device
+    u = torch.as_tensor(wl.u.reshape(-1), device=dev)
+    mu = torch.as_tensor(wl.mu, device=dev)
+    lam = torch.as_tensor(wl.lam, device=dev)
+    f_only, _ = plan.assemble(u, mu, lam, precision=precision, want_k=False)
+    assert rel_err(f_only.cpu().numpy(), f) < (1e-13 if precision == "fp64" else 1e-6)
+    _, K_ref = A.assemble(wl.mesh.nodes, wl.mesh.elements, wl.u, wl.mu, wl.lam)
+    w = np.random.default_rng(n_elem).standard_normal(plan.n_dofs)
+    gu, _, _ = plan.vjp(u, mu, lam, torch.as_tensor(w, device=dev), precision=precision, want_params=False)
+    assert rel_err(gu.cpu().numpy(), K_ref @ w) < (1e-12 if precision == "fp64" else 1e-5)
+    for tile in (64, 128, 256):
+        if tile < n_elem:
+            with pytest.raises(Exception, match="exceeds the tile"):
+                AssemblyPlan(wl.mesh.nodes, wl.mesh.elements, tile=tile)
+            continue
+        p2 = AssemblyPlan(wl.mesh.nodes, wl.mesh.elements, tile=tile)
+        f2, k2 = p2.assemble(u, mu, lam, precision=precision)
+        assert np.array_equal(f2.cpu().numpy(), f) and np.array_equal(k2.cpu().numpy(), k), tile
+
+
+def test_wide_plan_free_csc_and_refusals():
+    """The free-DOF CSC of a wide plan (a column node with 601 neighbours) is
+    scipy's bit for bit; the mass and F-bar kernels refuse a wide plan with a
+    clear message, and so does a Tet10 plan of such a mesh."""
+    import torch
+
+    from psierra_amd.engine import AssemblyPlan, FbarAssembly
+    from psierra_amd.mesh import tet4_to_tet10
+    from psierra_amd.solver import FreeDofSystem
+
+    wl = _fan_and_box(200, seed=10)
+    banner("edge: wide plan free CSC and refusals")
+    plan, _, k, _ = _check(wl, "fp64")
+    indptr, indices = plan.pattern()
+    import scipy.sparse as spsp
+
+    n = plan.n_dofs
+    K = spsp.csr_matrix((k, indices, indptr), shape=(n, n))
+    free = np.sort(np.random.default_rng(3).choice(n, 2 * n // 3, replace=False))
+    ref = (K + spsp.csr_matrix((n, n)))[free][:, free].tocsc()
+    sysm = FreeDofSystem(plan, free)
+    got = sysm.csc(torch.as_tensor(k, device=plan.device))
+    assert np.array_equal(got.indptr, ref.indptr) and np.array_equal(got.indices, ref.indices)
+    assert np.array_equal(got.data, ref.data)
+    with pytest.raises(Exception, match="at most 64 incident elements"):
+        plan.assemble_mass(1.0)
+    with pytest.raises(Exception, match="at most 64 incident elements"):
+        FbarAssembly(plan)
+    X10, c10 = tet4_to_tet10(wl.mesh.nodes, wl.mesh.elements)
+    with pytest.raises(Exception, match="Tet10 plans need"):
+        AssemblyPlan(X10, c10)
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])

@@ -238,7 +238,9 @@ def test_bound_material_matches_oracle_and_per_call_parameters(jittered, tile):
             assert torch.equal(f_b, f_c) and torch.equal(k_b, k_c)
         else:
             assert rel_err(kb, k_c.cpu().numpy()) < 1e-6 and rel_err(fb, f_c.cpu().numpy()) < 1e-6
-        assert rel_err(f_o.cpu().numpy(), fb) < 1e-12
+        # the f-only kernel may contract FP32 FMAs differently (see
+        # test_force_only_matches_fused): FP32 rounding, not bits
+        assert rel_err(f_o.cpu().numpy(), fb) < (1e-6 if prec == "fp32" else 1e-12)
 
 
 def test_rebinding_and_binding_errors(jittered):
