@@ -54,6 +54,91 @@ def cpu_baseline(wl, sample_elems, seed=0):
     return sample_elems / t_loop, sample_elems / t_vec, t_loop, t_vec
 
 
+def host_info():
+    """CPU model (/proc/cpuinfo), usable cores, numpy/scipy versions and the
+    BLAS libraries with their thread counts."""
+    model = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    import scipy
+    from threadpoolctl import threadpool_info
+
+    blas = [{"api": d.get("internal_api"), "threads": d.get("num_threads")} for d in threadpool_info()
+            if d.get("user_api") == "blas"]
+    return {"cpu_model": model, "host_cores": len(os.sched_getaffinity(0)), "numpy": np.__version__,
+            "scipy": scipy.__version__, "blas": blas}
+
+
+_FORK = None   # (workload, sample indices, shared k, shared f) inherited by the forked baseline workers
+
+
+def _vec_chunk(rng_):
+    """Worker of the all-cores baseline: the oracle's vectorised element
+    routine on sample elements [a, b), numpy/BLAS on one thread, written into
+    the shared element arrays."""
+    from threadpoolctl import threadpool_limits
+
+    from oracle import fem
+
+    wl, idx, k_all, f_all = _FORK
+    a, b = rng_
+    sel = idx[a:b]
+    conn = wl.mesh.elements[sel]
+    with threadpool_limits(1):
+        k, f = fem.tet4_batch(wl.mesh.nodes[conn], wl.u[conn], wl.mu[sel], wl.lam[sel])
+    k_all[a:b] = k
+    f_all[a:b] = f
+    return b - a
+
+
+def cpu_vectorised_all_cores(wl, sample_elems, workers, seed=0):
+    """The vectorised FP64 oracle on the same seeded sample with all cores:
+    the element routine over `workers` forked processes (one core each, the
+    workload inherited, the element results in shared anonymous memory),
+    then the global scatter as the reference does it, np.add.at for f and
+    coo_matrix(...).tocsr() for K, in the parent (scipy is single-threaded);
+    returns (elements/s, seconds)."""
+    import mmap
+    import multiprocessing as mp
+
+    from oracle import assembly as A
+
+    global _FORK
+    rng = np.random.default_rng(seed)
+    idx = np.sort(rng.choice(wl.mesh.n_elements, sample_elems, replace=False))
+    S = idx.size
+    kbuf = mmap.mmap(-1, S * 144 * 8)
+    fbuf = mmap.mmap(-1, S * 12 * 8)
+    k_all = np.frombuffer(kbuf, dtype=np.float64).reshape(S, 12, 12)
+    f_all = np.frombuffer(fbuf, dtype=np.float64).reshape(S, 12)
+    bounds = np.linspace(0, S, workers + 1).astype(int)
+    ranges = [(int(bounds[i]), int(bounds[i + 1])) for i in range(workers)]
+    _FORK = (wl, idx, k_all, f_all)
+    try:
+        with mp.get_context("fork").Pool(workers) as pool:
+            pool.map(_vec_chunk, [(a, min(a + 8, b)) for a, b in ranges])   # workers started and warm
+            t0 = time.perf_counter()
+            pool.map(_vec_chunk, ranges)
+            with single_thread():
+                dofs = A.compute_element_dofs(wl.mesh.elements[idx])
+                rows, cols = A.create_coo_sparsity_indices(dofs)
+                A.assemble_vector(f_all, dofs, 3 * wl.mesh.n_nodes)
+                A.assemble_sparse_matrix(k_all, rows, cols, 3 * wl.mesh.n_nodes)
+            t = time.perf_counter() - t0
+    finally:
+        _FORK = None
+        del k_all, f_all
+        kbuf.close()
+        fbuf.close()
+    return S / t, t
+
+
 def single_thread():
     """numpy and its BLAS limited to one thread, so a CPU baseline's core
     count is 1."""
@@ -65,8 +150,8 @@ def single_thread():
 def cpu_entry(value, unit, sample, **extra):
     """The ``cpu_baseline`` object of a bench line: the oracle (the FP64
     restatement of the reference, "port") timed on one host core."""
-    out = {"value": value, "unit": unit, "cores": 1, "kind": "port", "sample": sample,
-           "cpu_model": platform.processor() or platform.machine(), "numpy": np.__version__}
+    out = {"value": value, "unit": unit, "cores": 1, "kind": "port", "sample": sample}
+    out.update(host_info())
     out.update(extra)
     return out
 
@@ -650,6 +735,9 @@ def main():
     if not args.no_cpu:
         with single_thread():
             loop_eps, vec_eps, t_loop, t_vec = cpu_baseline(wl, args.cpu_sample)
+        # all cores of this process's CPU share (at most 16: the box's share of one GPU)
+        workers = max(1, min(16, len(os.sched_getaffinity(0))))
+        par_eps, t_par = cpu_vectorised_all_cores(wl, args.cpu_sample, workers)
         cpu = {
             "value": loop_eps,
             "unit": "elements/s",
@@ -657,15 +745,17 @@ def main():
             "kind": "port",
             "sample": (
                 f"{args.cpu_sample} seeded elements of the same mesh; FP64 oracle per-element loop "
-                f"(element evaluated twice, add.at, coo.tocsr) took {t_loop:.2f}s; "
-                f"vectorised FP64 numpy on the same sample: {vec_eps:.4g} elements/s ({t_vec:.2f}s)"
+                f"(element evaluated twice, add.at, coo.tocsr) took {t_loop:.2f}s on one core; "
+                f"vectorised FP64 numpy on the same sample: {vec_eps:.4g} elements/s ({t_vec:.2f}s) on one core, "
+                f"{par_eps:.4g} elements/s ({t_par:.2f}s) with the element routine over {workers} processes "
+                f"(one BLAS thread each) and the add.at / coo.tocsr scatter on one"
             ),
             "vectorised_value": vec_eps,
-            "cpu_model": platform.processor() or platform.machine(),
-            "numpy": np.__version__,
+            "vectorised_all_cores": {"value": par_eps, "cores": workers, "seconds": t_par},
         }
-        print(f"[bench] cpu baseline loop {loop_eps:.4g} el/s, vectorised {vec_eps:.4g} el/s",
-              file=sys.stderr, flush=True)
+        cpu.update(host_info())
+        print(f"[bench] cpu baseline loop {loop_eps:.4g} el/s, vectorised {vec_eps:.4g} el/s, "
+              f"{workers} cores {par_eps:.4g} el/s", file=sys.stderr, flush=True)
 
     import torch
 
@@ -712,6 +802,8 @@ def main():
     f64 = f.clone()
     k64 = k.clone()
     print(f"[bench] fp64 plain {ms64_ev:.3f} ms/assembly", file=sys.stderr, flush=True)
+    _, ms64_bound_ev, _ = timed("fp64", args.steps, args.warmup)
+    print(f"[bench] fp64 with the bound material {ms64_bound_ev:.3f} ms/assembly", file=sys.stderr, flush=True)
     _, ms32_call_ev, _ = timed("fp32", args.steps, args.warmup, bound=False)
     print(f"[bench] fp32 with per-call parameters {ms32_call_ev:.3f} ms/assembly", file=sys.stderr, flush=True)
     ms32_wall, ms32_ev, ms32_med = timed("fp32", args.steps, args.warmup)
@@ -789,12 +881,15 @@ def main():
             "dofs": 3 * N,
             "nnz": nnz,
             "mode": "fp32 element arithmetic, fp64 scatter/f accumulation",
-            "material": "per-element (mu, lam) bound to the plan once at setup (psx_plan_bind_material); "
-                        "the fp64 plain form takes them per call",
+            "material": "per-element (mu, lam) bound to the plan once at setup (psx_plan_bind_material), "
+                        "as the drop-in element hooks do; the fp64 plain form takes them per call",
             "fp32_per_call_params_ms": ms32_call_ev,
             "fp64_plain_ms": ms64_ev,
+            "fp64_bound_material_ms": ms64_bound_ev,
             "fp64_plain_elements_per_s": E / (ms64_wall * 1e-3),
             "speedup_fp32_vs_fp64_plain": ms64_ev / ms32_ev,
+            "speedup_fp32_vs_fp64_same_path": {"per_call_params": ms64_ev / ms32_call_ev,
+                                               "bound_material": ms64_bound_ev / ms32_ev},
             "event_ms_mean": ms32_ev,
             "event_ms_median": ms32_med,
             "setup_s": {"workload": t_build, "plan": t_plan, "bind_material": t_bind},
@@ -814,6 +909,18 @@ def main():
             "kernel": f"psx_assemble = permute_u_kernel + tet4_assemble_rows<float,{plan.info['tile']},"
                       "kNeoHookeanBound>",
             "fp64_plain_frac": b_alg64 / (ms64_ev * 1e-3) / 1e9 / HBM_PEAK_GBPS,
+        },
+        "roofline_fp64_plain": {
+            "bound": "hbm",
+            "achieved": b_alg64 / (ms64_ev * 1e-3) / 1e9,
+            "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": b_alg64 / (ms64_ev * 1e-3) / 1e9 / HBM_PEAK_GBPS,
+            "traffic": traffic.get("traffic_bytes_fp64") if traffic else None,
+            "traffic_source": TRAFFIC_FILE if traffic else None,
+            "kernel": f"psx_assemble = permute_u_kernel + tet4_assemble_rows<double,{plan.info['tile']},"
+                      "kNeoHookean>",
+            "elements_per_s": E / (ms64_ev * 1e-3),
         },
         "cpu_baseline": cpu,
     }
