@@ -401,12 +401,12 @@ def ecsw_cpu(insts, min_s=4.0):
 
 def bench_ecsw_train(args):
     """ECSW training matrix (FP64) on the 48^3 Tet4 box: rank 64, 8 snapshots,
-    with the linear-stiffness block (rows = 8*64 + 64*65/2 = 2592)."""
+    with the linear-stiffness block (rows = 8*64 + 64*65/2 = 2592), and the
+    sums of squares of both blocks."""
     import torch
 
     from psierra_amd import _lib
     from psierra_amd import workloads as W
-    from psierra_amd.ecsw import train_work_elements
     from psierra_amd.engine import AssemblyPlan, current_stream_handle
 
     wl = W.vjp_box_tet4(n=48, seed=3)
@@ -422,14 +422,15 @@ def bench_ecsw_train(args):
     E = plan.n_elements
     rows = m * r + r * (r + 1) // 2
     A = torch.empty((rows, E), dtype=torch.float64, device=dev)
-    chunk = train_work_elements(rows, E) if args.work_elements is None else min(args.work_elements, E)
-    work = torch.empty((chunk, rows), dtype=torch.float64, device=dev)
+    nwork = int(_lib.lib().psx_ecsw_training_work_doubles(plan._h))
+    work = torch.empty(nwork, dtype=torch.float64, device=dev)
+    sumsq = torch.zeros(2, dtype=torch.float64, device=dev)
     status = torch.zeros(2, dtype=torch.int32, device=dev)
 
     def call():
         _lib.check(_lib.lib().psx_ecsw_training(plan._h, _lib.ptr(phi), r, _lib.ptr(snaps), m, _lib.ptr(mu),
-                                                _lib.ptr(lam), 1, _lib.ptr(A), _lib.ptr(work), chunk,
-                                                current_stream_handle(), _lib.ptr(status)))
+                                                _lib.ptr(lam), 1, _lib.ptr(A), _lib.ptr(work), nwork,
+                                                _lib.ptr(sumsq), current_stream_handle(), _lib.ptr(status)))
 
     ms = _event_time(call, args.steps, args.warmup)
     assert int(status[0]) == 0
@@ -441,7 +442,8 @@ def bench_ecsw_train(args):
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "fp64", "data": "synthetic",
         "config": {"workload": wl.name, "elements": E, "rank": r, "snapshots": m, "rows": rows,
-                   "A_bytes": 8 * rows * E, "work_elements": chunk, "work_bytes": 8 * rows * chunk},
+                   "A_bytes": 8 * rows * E, "layout": "A rows written directly in runs of 16 elements; "
+                   "sums of squares of both blocks from per-group partials"},
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": ach / HBM_PEAK_GBPS, "traffic": None},
         "cpu_baseline": None if args.no_cpu else ecsw_train_cpu(r, m),
@@ -703,8 +705,6 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--tile", type=int, default=0)
-    ap.add_argument("--work-elements", type=int, default=None,
-                    help="ecsw_train: elements per chunk of the work block (default: 1 GiB of rows)")
     ap.add_argument("--workload",
                     choices=("tet4", "tet10", "vjp", "ecsw", "ecsw_train", "fbar", "csc", "mass", "newton"),
                     default="tet4")

@@ -66,8 +66,9 @@ enum {
   PSX_INFO_COUNT = 11
 };
 
-/* Version of this interface: 4 (3 had no psx_ecsw_training_noalloc; 2:
- * psx_ecsw_* took rank 64 only; 1 had psx_ecsw_training without
+/* Version of this interface: 5 (4: psx_ecsw_training took an element-major
+ * work block and gave no sums of squares; 3 had no psx_ecsw_training_noalloc;
+ * 2: psx_ecsw_* took rank 64 only; 1 had psx_ecsw_training without
  * work_elements and no psx_plan_bind_material). */
 int32_t psx_abi_version(void);
 
@@ -266,33 +267,36 @@ int32_t psx_ecsw_evaluate(psx_ecsw* ecsw, const double* u_r, double* f_r, double
  * (+ rank (rank + 1) / 2 when include_stiffness):
  *   A[k rank + i][e] = (Phi_e^T f_e(u_k))_i,
  *   A[n_snapshots rank + t][e] = (Phi_e^T K0_e Phi_e)[triu_t] (unscaled;
- *   t in numpy.triu_indices order, K0_e the element tangent at u = 0).
- * work: device scratch of rows x work_elements doubles (work_elements >= 1):
- * the elements are evaluated in chunks of work_elements (rounded down to a
- * multiple of 64 when it is at least 64), each chunk's
- * element-major block transposed into its columns of A; A is the same for
- * every work_elements (bit for bit).  The reference's balancing scale of the stiffness rows
- * (ecsw.py:350-357) and b = A 1 are left to the caller.  status as
- * psx_assemble (element inversion). */
+ *   t in numpy.triu_indices order, K0_e the element tangent at u = 0),
+ * written directly in that layout (no element-major copy, no transpose).
+ * sumsq (device, 2 doubles, or NULL): the sums of squares of the force rows
+ * and of the stiffness rows, in a fixed order, from which the caller forms
+ * the reference's balancing scale (ecsw.py:350-357) without another pass
+ * over A; it needs work (device) of at least
+ * psx_ecsw_training_work_doubles(plan) doubles for the per-group partial
+ * sums (work may be NULL when sumsq is).  b = A 1 is left to the caller.
+ * status as psx_assemble (element inversion). */
 int32_t psx_ecsw_training(psx_plan* plan, const double* phi, int32_t rank, const double* snapshots,
                           int32_t n_snapshots, const double* mu, const double* lam,
-                          int32_t include_stiffness, double* A, double* work, int64_t work_elements,
-                          void* stream, int32_t* status);
+                          int32_t include_stiffness, double* A, double* work, int64_t work_doubles,
+                          double* sumsq, void* stream, int32_t* status);
+/* Doubles of psx_ecsw_training's work array: 2 per group of 16 elements. */
+int64_t psx_ecsw_training_work_doubles(const psx_plan* plan);
 
 /* psx_ecsw_training without the work argument, allocating nothing.  No
  * counterpart in the reference: it is for callers that cannot allocate device
  * memory (code under hipGraph capture, serving loops, harnesses that forbid
- * hipMalloc).  The element-major work block lives in a 32 MiB device array
- * of the library (one per GPU, held until the process ends); the elements go
- * through it in chunks of floor(4 Mi / rows) (rounded down to a multiple of
- * 64 when at least 64 fit), so A is the same as psx_ecsw_training's.  Rows =
- * n_snapshots * rank (+ rank (rank + 1) / 2) above 4 Mi (4,194,304) are
- * refused with PSX_EUNSUPPORTED.  Every call on a GPU shares that GPU's
- * array: calls that overlap on one GPU (different streams or threads) race.
- * The plan's device must be the current device. */
+ * hipMalloc).  The per-group partial sums of squares live in a 32 MiB device
+ * array of the library (one per GPU, held until the process ends), so A and
+ * sumsq are the same as psx_ecsw_training's; plans of more than 33,554,432
+ * elements (2 doubles per group of 16 beyond 4 Mi doubles) are refused with
+ * PSX_EUNSUPPORTED.  Every call on a GPU shares that GPU's array: calls that
+ * overlap on one GPU (different streams or threads) race.  The plan's
+ * device must be the current device. */
 int32_t psx_ecsw_training_noalloc(psx_plan* plan, const double* phi, int32_t rank, const double* snapshots,
                                   int32_t n_snapshots, const double* mu, const double* lam,
-                                  int32_t include_stiffness, double* A, void* stream, int32_t* status);
+                                  int32_t include_stiffness, double* A, double* sumsq, void* stream,
+                                  int32_t* status);
 
 /* ---- Element batch (device) -------------------------------------------------
  * Replaces Tet4TotalLagrangian.compute_nonlinear_force_and_tangent_stiffness

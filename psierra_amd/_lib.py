@@ -88,11 +88,13 @@ SIGNATURES = {
     "psx_ecsw_training": (
         c_int32,
         [c_void_p, c_void_p, c_int32, c_void_p, c_int32, c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_int64,
-         c_void_p, c_void_p],
+         c_void_p, c_void_p, c_void_p],
     ),
+    "psx_ecsw_training_work_doubles": (c_int64, [c_void_p]),
     "psx_ecsw_training_noalloc": (
         c_int32,
-        [c_void_p, c_void_p, c_int32, c_void_p, c_int32, c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_void_p],
+        [c_void_p, c_void_p, c_int32, c_void_p, c_int32, c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_void_p,
+         c_void_p],
     ),
     "psx_element_batch": (
         c_int32,
@@ -107,7 +109,7 @@ SIGNATURES = {
 _lib = None
 
 # The C ABI these signatures describe (psx_abi_version() of the library).
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 
 class PsxError(RuntimeError):

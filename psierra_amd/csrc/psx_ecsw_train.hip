@@ -7,18 +7,29 @@
 // Phi_e^T K0_e Phi_e, K0_e the element tangent at u = 0, in rows m r .. .
 // Phi_e = Phi[dofs of e, :] with Phi (3N x r, user DOF order, row-major) and
 // snapshots (3N x m, row-major) as the reference's arrays.  The stiffness
-// block is returned unscaled (the caller applies the reference's norm
-// balancing, :350-357, which needs global norms of both blocks).
+// block is returned unscaled; the reference balances it by the norms of
+// both blocks (:350-357), so the kernel also leaves the sum of squares of
+// each block, and the caller scales.
 //
-// Layout of the work: one wave per element.  Lanes = snapshots evaluate the
-// element force in FP64 and leave it in LDS; lanes = modes then hold the
-// element's 12 basis rows (one coalesced 8-byte load per row) and form
-// Phi_e^T f for every snapshot, writing an element-major row of the work
-// array (consecutive lanes, consecutive values).  The stiffness block: four
-// lanes evaluate the rows of K0_e, the wave stages Phi_e in LDS, lane j forms
-// T = K0_e Phi_e[:, j] and the column j of Phi_e^T T above the diagonal.  A
-// tiled transpose through LDS turns the element-major work array into the
-// reference's (rows x E) layout.
+// Layout of the work: a work-group of 256 threads takes 16 consecutive
+// elements; thread t works for element t % 16, so the 16 lanes of each
+// quarter-wave hold 16 consecutive elements and every store instruction
+// writes four rows of A as four 128-byte runs.  Forces: (element, snapshot)
+// pairs evaluate f_e(u_k) in FP64 into LDS; each thread then takes four
+// modes i of its element, reads Phi_e[:, i] and writes (Phi_e^T f_k)_i for
+// every snapshot.  Stiffness: K0_e, the tangent at u = 0, is Hooke's law
+// with the element's (lambda, mu) and S = 0, so with G_i = sum_b phi_{i,b}
+// h_b^T (the gradient of mode i over the element) and eps_i = sym(G_i),
+//   (Phi_e^T K0_e Phi_e)_ij = V (lambda tr(G_i) tr(G_j) + 2 mu eps_i : eps_j),
+// a short inner product: each thread forms eps of its four modes (6
+// components) into LDS, then the work-group writes the upper triangle row by
+// row, element fastest, each thread keeping eps_i while its walk stays on
+// triangle row i.  Measured on the 48^3 box (rank 64, 8 snapshots): 9.4 ms
+// for A's 13.8 GB, against 11.5 ms for an element-major work block and a
+// transpose; the writes, 128-byte runs 5.3 MB apart, set the pace.  Each thread keeps the sums of squares of its force and stiffness
+// values; the work-group adds them in a fixed tree into two partials per
+// group, and a second kernel adds the partials in a fixed order, so the
+// norms are deterministic.
 #include <algorithm>
 #include <climits>
 #include <mutex>
@@ -33,187 +44,227 @@
 namespace psx {
 
 constexpr int kTrainMaxRank = 64;
-constexpr int kTrainSnapPass = 64;
+constexpr int kTrainGroup = 16;       // elements per work-group
+constexpr int kTrainThreads = 256;
+constexpr int kTrainSnapPass = 16;    // snapshots whose forces are staged together
 
 __device__ __forceinline__ void train_flag(int32_t* status, int e) {
   atomicMin(&status[1], e);
   status[0] = 1;
 }
 
-__global__ __launch_bounds__(256) void ecsw_train_force_kernel(DevPlan p, const double* __restrict__ phi,
-                                                               int r, const double* __restrict__ snaps, int m,
-                                                               const double* __restrict__ mu,
-                                                               const double* __restrict__ lam, int rows,
-                                                               int64_t e0, int64_t e1, double* __restrict__ At,
-                                                               int32_t* status) {
-  __shared__ double s_f[4][kTrainSnapPass][13];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int64_t base = e0 + int64_t(blockIdx.x) * 4; base < e1; base += int64_t(gridDim.x) * 4) {
-    const int64_t e = base + wave;
-    const bool valid = e < e1;
-    int64_t un[4] = {0, 0, 0, 0};
-    double4 X[4];
-    double ph[12];
-    double mue = 0.0, Ke = 0.0;
-    if (valid) {
-      const int4 c = *reinterpret_cast<const int4*>(p.conn + 4 * e);
-      const int ci[4] = {c.x, c.y, c.z, c.w};
+// Element groups of the training matrix (and partial-sum pairs per call).
+__host__ __device__ constexpr int64_t train_groups(int64_t E) { return (E + kTrainGroup - 1) / kTrainGroup; }
+
+__global__ __launch_bounds__(kTrainThreads) void ecsw_train_kernel(DevPlan p, const double* __restrict__ phi, int r,
+                                                                   const double* __restrict__ snaps, int m,
+                                                                   const double* __restrict__ mu,
+                                                                   const double* __restrict__ lam, int stiff,
+                                                                   double* __restrict__ A, int64_t groups,
+                                                                   double* __restrict__ partial,
+                                                                   int32_t* status) {
+  // forces of a snapshot pass [element][snapshot][dof]; afterwards the
+  // strain vectors of every mode [mode][component][element]
+  constexpr int kForce = kTrainGroup * kTrainSnapPass * 13;
+  constexpr int kStrain = kTrainMaxRank * 6 * kTrainGroup;
+  __shared__ double s_buf[kForce > kStrain ? kForce : kStrain];
+  __shared__ double s_red[2][kTrainThreads];
+  const int t = threadIdx.x;
+  const int el = t % kTrainGroup, sub = t / kTrainGroup;   // element, and which of the 16 sub-lanes
+  // XCD-contiguous groups: the groups of one XCD are consecutive
+  const int64_t per = int64_t(gridDim.x) >> 3;
+  const int64_t g = (int64_t(blockIdx.x) & 7) * per + (int64_t(blockIdx.x) >> 3);
+  const int64_t E = p.E;
+  const int64_t e0 = g * kTrainGroup;
+  const int64_t e = e0 + el;
+  const bool valid = g < groups && e < E;
+  const int n_tri = r * (r + 1) / 2;
+  int64_t un[4] = {0, 0, 0, 0};
+  double4 X[4];
+  double mue = 0.0, Ke = 0.0, lame = 0.0;
+  if (valid) {
+    const int4 c = *reinterpret_cast<const int4*>(p.conn + 4 * e);
+    const int ci[4] = {c.x, c.y, c.z, c.w};
 #pragma unroll
-      for (int b = 0; b < 4; ++b) {
-        un[b] = p.perm[ci[b]];
-        X[b] = p.Xi[ci[b]];
-      }
-      if (lane < r)
-#pragma unroll
-        for (int d = 0; d < 12; ++d) ph[d] = phi[(3 * un[d / 3] + d % 3) * r + lane];
-      mue = mu[e];
-      Ke = lam[e] + (2.0 / 3.0) * mue;
+    for (int b = 0; b < 4; ++b) {
+      un[b] = p.perm[ci[b]];
+      X[b] = p.Xi[ci[b]];
     }
-    for (int k0 = 0; k0 < m; k0 += kTrainSnapPass) {
-      const int nk = min(kTrainSnapPass, m - k0);
-      if (valid && lane < nk) {
-        const int64_t k = k0 + lane;
-        double4 U[4];
+    mue = mu[e];
+    lame = lam[e];
+    Ke = lame + (2.0 / 3.0) * mue;
+  }
+  double sq_f = 0.0, sq_s = 0.0;
+  // ---- force rows: A[k r + i][e] = (Phi_e^T f_e(u_k))_i
+  for (int k0 = 0; k0 < m; k0 += kTrainSnapPass) {
+    const int nk = min(kTrainSnapPass, m - k0);
+    __syncthreads();   // the previous pass's forces are read
+    for (int kk = sub; kk < nk; kk += kTrainThreads / kTrainGroup) {
+      if (!valid) continue;
+      const int64_t k = k0 + kk;
+      double4 U[4];
 #pragma unroll
-        for (int b = 0; b < 4; ++b)
-          U[b] = make_double4(snaps[(3 * un[b]) * m + k], snaps[(3 * un[b] + 1) * m + k],
-                              snaps[(3 * un[b] + 2) * m + k], 0.0);
-        double fe[12];
-        const double J = tet4_element_force<double>(X, U, mue, Ke, fe);
-        if (J <= 0.0) train_flag(status, int(e));
+      for (int b = 0; b < 4; ++b)
+        U[b] = make_double4(snaps[(3 * un[b]) * m + k], snaps[(3 * un[b] + 1) * m + k],
+                            snaps[(3 * un[b] + 2) * m + k], 0.0);
+      double fe[12];
+      const double J = tet4_element_force<double>(X, U, mue, Ke, fe);
+      if (J <= 0.0) train_flag(status, int(e));
 #pragma unroll
-        for (int d = 0; d < 12; ++d) s_f[wave][lane][d] = fe[d];
-      }
-      __syncthreads();
-      if (valid && lane < r) {
-        double* dst = At + (e - e0) * rows + int64_t(k0) * r + lane;
-        for (int kk = 0; kk < nk; ++kk) {
+      for (int d = 0; d < 12; ++d) s_buf[(el * kTrainSnapPass + kk) * 13 + d] = fe[d];
+    }
+    __syncthreads();
+    if (valid) {
+      for (int i = sub; i < r; i += kTrainThreads / kTrainGroup) {
+        double ph[12];
+#pragma unroll
+        for (int d = 0; d < 12; ++d) ph[d] = phi[(3 * un[d / 3] + d % 3) * r + i];
+        double* dst = A + (int64_t(k0) * r + i) * E + e;
+        for (int kk = 0; kk < nk; ++kk, dst += int64_t(r) * E) {
+          const double* fk = s_buf + (el * kTrainSnapPass + kk) * 13;
           double cv = 0.0;
 #pragma unroll
-          for (int d = 0; d < 12; ++d) cv += ph[d] * s_f[wave][kk][d];
-          dst[int64_t(kk) * r] = cv;
+          for (int d = 0; d < 12; ++d) cv += ph[d] * fk[d];
+          *dst = cv;
+          sq_f += cv * cv;
         }
+      }
+    }
+  }
+  // ---- stiffness rows: upper triangle of Phi_e^T K0_e Phi_e, triu order
+  if (stiff) {
+    __syncthreads();   // the forces are read; s_buf takes the strain vectors
+    double V = 0.0;
+    if (valid) {
+      const double4 Z[4] = {make_double4(0.0, 0.0, 0.0, 0.0), make_double4(0.0, 0.0, 0.0, 0.0),
+                            make_double4(0.0, 0.0, 0.0, 0.0), make_double4(0.0, 0.0, 0.0, 0.0)};
+      double h[4][3], A0[3][3];
+      tet4_geometry<double>(X, Z, h, V, A0);
+      for (int i = sub; i < r; i += kTrainThreads / kTrainGroup) {
+        double ph[12];
+#pragma unroll
+        for (int d = 0; d < 12; ++d) ph[d] = phi[(3 * un[d / 3] + d % 3) * r + i];
+        double G[3][3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+          for (int k = 0; k < 3; ++k)
+            G[a][k] = ph[a] * h[0][k] + ph[3 + a] * h[1][k] + ph[6 + a] * h[2][k] + ph[9 + a] * h[3][k];
+        // (eps_xx, eps_yy, eps_zz, eps_xy, eps_yz, eps_zx)
+        const double v[6] = {G[0][0], G[1][1], G[2][2], 0.5 * (G[0][1] + G[1][0]), 0.5 * (G[1][2] + G[2][1]),
+                             0.5 * (G[2][0] + G[0][2])};
+#pragma unroll
+        for (int q = 0; q < 6; ++q) s_buf[(i * 6 + q) * kTrainGroup + el] = v[q];
+      }
+    }
+    __syncthreads();
+    const double wl = V * lame, wm = 2.0 * V * mue;
+    if (valid) {
+      constexpr int kStep = kTrainThreads / kTrainGroup;
+      double* dst = A + (int64_t(m) * r + sub) * E + e;
+      // (i, j) of triangle row tt (row-major upper triangle): row i holds
+      // j = i .. r - 1, and the entry after (i, r - 1) is (i + 1, i + 1)
+      int i = 0, j = sub;
+      while (j >= r && i < r) {
+        j = j - r + i + 1;
+        ++i;
+      }
+      // eps of mode i, kept while the walk stays on triangle row i
+      int ci = -1;
+      double a[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, ta = 0.0;
+      for (int tt = sub; tt < n_tri; tt += kStep, dst += int64_t(kStep) * E) {
+        if (i != ci) {
+          const double* pi = s_buf + i * 6 * kTrainGroup + el;
+#pragma unroll
+          for (int q = 0; q < 6; ++q) a[q] = pi[q * kTrainGroup];
+          ta = (a[0] + a[1]) + a[2];
+          ci = i;
+        }
+        const double* pj = s_buf + j * 6 * kTrainGroup + el;
+        double b[6];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) b[q] = pj[q * kTrainGroup];
+        double ee = a[0] * b[0];
+        ee += a[1] * b[1];
+        ee += a[2] * b[2];
+        double oo = a[3] * b[3];
+        oo += a[4] * b[4];
+        oo += a[5] * b[5];
+        const double sv = wl * ta * ((b[0] + b[1]) + b[2]) + wm * (ee + 2.0 * oo);
+        *dst = sv;
+        sq_s += sv * sv;
+        j += kStep;
+        while (j >= r && i < r) {
+          j = j - r + i + 1;
+          ++i;
+        }
+      }
+    }
+  }
+  // ---- sums of squares of the group, fixed tree over the threads
+  if (partial) {
+    s_red[0][t] = sq_f;
+    s_red[1][t] = sq_s;
+    __syncthreads();
+    for (int stride = kTrainThreads / 2; stride > 0; stride >>= 1) {
+      if (t < stride) {
+        s_red[0][t] += s_red[0][t + stride];
+        s_red[1][t] += s_red[1][t + stride];
       }
       __syncthreads();
     }
+    if (t == 0 && g < groups) {
+      partial[2 * g] = s_red[0][0];
+      partial[2 * g + 1] = s_red[1][0];
+    }
   }
 }
 
-__global__ __launch_bounds__(256) void ecsw_train_stiff_kernel(DevPlan p, const double* __restrict__ phi,
-                                                               int r, const double* __restrict__ mu,
-                                                               const double* __restrict__ lam, int rows,
-                                                               int row0, int64_t e0, int64_t e1,
-                                                               double* __restrict__ At, int32_t* status) {
-  __shared__ double s_phi[4][12][kTrainMaxRank];
-  __shared__ double s_K[4][12][13];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int64_t base = e0 + int64_t(blockIdx.x) * 4; base < e1; base += int64_t(gridDim.x) * 4) {
-    const int64_t e = base + wave;
-    const bool valid = e < e1;
-    if (valid) {
-      const int4 c = *reinterpret_cast<const int4*>(p.conn + 4 * e);
-      const int ci[4] = {c.x, c.y, c.z, c.w};
-      if (lane < r)
-#pragma unroll
-        for (int d = 0; d < 12; ++d) s_phi[wave][d][lane] = phi[(3 * int64_t(p.perm[ci[d / 3]]) + d % 3) * r + lane];
-      if (lane < 4) {
-        // row `a` of K0_e: blocks in the order [a, others ascending]
-        const int a = lane;
-        const int rot[4] = {a, a == 0 ? 1 : 0, a <= 1 ? 2 : 1, a <= 2 ? 3 : 2};
-        double4 Xr[4], Ur[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int cq = rot[q] == 0 ? ci[0] : rot[q] == 1 ? ci[1] : rot[q] == 2 ? ci[2] : ci[3];
-          Xr[q] = p.Xi[cq];
-          Ur[q] = make_double4(0.0, 0.0, 0.0, 0.0);
-        }
-        const double mue = mu[e];
-        double out[39];
-        const double J = tet4_row<double, true>(Xr, Ur, mue, lam[e] + (2.0 / 3.0) * mue, out);
-        if (J <= 0.0) train_flag(status, int(e));
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-#pragma unroll
-          for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int cc = 0; cc < 3; ++cc) s_K[wave][3 * a + i][3 * rot[q] + cc] = out[9 * q + 3 * i + cc];
-      }
-    }
-    __syncthreads();
-    if (valid && lane < r) {
-      const int j = lane;
-      double T[12];
-#pragma unroll
-      for (int d = 0; d < 12; ++d) {
-        double s = 0.0;
-#pragma unroll
-        for (int cc = 0; cc < 12; ++cc) s += s_K[wave][d][cc] * s_phi[wave][cc][j];
-        T[d] = s;
-      }
-      double* dst = At + (e - e0) * rows + row0;
-      for (int i = 0; i <= j; ++i) {
-        double s = 0.0;
-#pragma unroll
-        for (int d = 0; d < 12; ++d) s += s_phi[wave][d][i] * T[d];
-        dst[i * r - (i * (i - 1)) / 2 + (j - i)] = s;
-      }
-    }
-    __syncthreads();
+// sumsq[0..1] = sums of the groups' partials, each lane over a strided
+// subset in order, then a fixed tree.
+__global__ __launch_bounds__(1024) void ecsw_train_norms_kernel(const double* __restrict__ partial, int64_t groups,
+                                                                 double* __restrict__ sumsq) {
+  __shared__ double red[2][1024];
+  const int t = threadIdx.x;
+  double a = 0.0, b = 0.0;
+  for (int64_t g = t; g < groups; g += 1024) {
+    a += partial[2 * g];
+    b += partial[2 * g + 1];
   }
-}
-
-// out[c][x] = in[x][c] for x < n, c < cols (out rows of leading dimension
-// ldo), 32 x 32 tiles through LDS.
-__global__ __launch_bounds__(256) void transpose_f64_kernel(const double* __restrict__ in, int64_t n, int cols,
-                                                            double* __restrict__ out, int64_t ldo) {
-  __shared__ double tile[32][33];
-  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
-  const int64_t r0 = int64_t(blockIdx.x) * 32;
-  const int c0 = blockIdx.y * 32;
-  for (int y = ty; y < 32; y += 8) {
-    const int64_t rr = r0 + y;
-    const int cc = c0 + tx;
-    if (rr < n && cc < cols) tile[y][tx] = in[rr * cols + cc];
-  }
+  red[0][t] = a;
+  red[1][t] = b;
   __syncthreads();
-  for (int y = ty; y < 32; y += 8) {
-    const int cc = c0 + y;
-    const int64_t rr = r0 + tx;
-    if (rr < n && cc < cols) out[int64_t(cc) * ldo + rr] = tile[tx][y];
+  for (int stride = 512; stride > 0; stride >>= 1) {
+    if (t < stride) {
+      red[0][t] += red[0][t + stride];
+      red[1][t] += red[1][t + stride];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    sumsq[0] = red[0][0];
+    sumsq[1] = red[1][0];
   }
 }
 
 int32_t launch_status_reset(hipStream_t s, int32_t* status);
 
 int32_t launch_ecsw_training(const DevPlan& p, const double* phi, int r, const double* snaps, int m,
-                             const double* mu, const double* lam, int stiff, double* A, double* work,
-                             int64_t work_elements, hipStream_t s, int32_t* status) {
+                             const double* mu, const double* lam, int stiff, double* A, double* partial,
+                             double* sumsq, hipStream_t s, int32_t* status) {
   const int64_t rows = int64_t(m) * r + (stiff ? int64_t(r) * (r + 1) / 2 : 0);
   if (rows > INT_MAX / 2) return fail(PSX_EINVAL, "too many training rows");
-  if (work_elements < 1) return fail(PSX_EINVAL, "work_elements must be at least 1");
   int32_t rc = launch_status_reset(s, status);
   if (rc) return rc;
-  // Elements in chunks of work_elements columns: each chunk's element-major
-  // block is transposed into its columns of A before the next chunk reuses
-  // it, so the scratch is rows x work_elements instead of rows x E.
-  // Chunks of a multiple of 64 elements start A's columns on a 512-byte
-  // boundary; a chunk that starts inside a cache line makes every row
-  // segment of the transpose straddle one more line (3.3 instead of 5 TB/s).
-  const int64_t E = p.E;
-  const int64_t aligned = work_elements >= 64 ? work_elements & ~int64_t(63) : work_elements;
-  const int64_t chunk = std::min<int64_t>(aligned, std::max<int64_t>(E, 1));
-  for (int64_t e0 = 0; e0 < E; e0 += chunk) {
-    const int64_t e1 = std::min(E, e0 + chunk), n = e1 - e0;
-    const int64_t groups = std::min<int64_t>((n + 3) / 4, 1 << 16);
-    hipLaunchKernelGGL(ecsw_train_force_kernel, dim3(unsigned(groups)), dim3(256), 0, s, p, phi, r, snaps, m, mu,
-                       lam, int(rows), e0, e1, work, status);
-    if (stiff)
-      hipLaunchKernelGGL(ecsw_train_stiff_kernel, dim3(unsigned(groups)), dim3(256), 0, s, p, phi, r, mu, lam,
-                         int(rows), m * r, e0, e1, work, status);
-    hipLaunchKernelGGL(transpose_f64_kernel, dim3(unsigned((n + 31) / 32), unsigned((rows + 31) / 32)), dim3(256),
-                       0, s, work, n, int(rows), A + e0, E);
-    PSX_HIP_TRY(hipGetLastError());
-  }
+  const int64_t groups = train_groups(p.E);
+  if (groups > INT_MAX) return fail(PSX_EINVAL, "too many elements");
+  // a multiple of 8 work-groups for the XCD-contiguous group map
+  const int64_t blocks = 8 * ((groups + 7) / 8);
+  if (groups > 0)
+    hipLaunchKernelGGL(ecsw_train_kernel, dim3(unsigned(blocks)), dim3(kTrainThreads), 0, s, p, phi, r, snaps, m,
+                       mu, lam, stiff, A, groups, sumsq ? partial : nullptr, status);
+  if (sumsq && groups > 0) hipLaunchKernelGGL(ecsw_train_norms_kernel, dim3(1), dim3(1024), 0, s, partial, groups, sumsq);
+  PSX_HIP_TRY(hipGetLastError());
   return PSX_OK;
 }
 
@@ -222,7 +273,8 @@ int32_t launch_ecsw_training(const DevPlan& p, const double* phi, int r, const d
 namespace psx {
 
 // Scratch of psx_ecsw_training_noalloc: a file-scope device array (one copy
-// per GPU, held until the process ends) in place of the caller's work block.
+// per GPU, held until the process ends) in place of the caller's work array
+// (the per-group partial sums of squares).
 // Host code never passes `scratch` itself to a kernel (on the host it names
 // a stand-in, not device memory): each device's address comes from
 // hipGetSymbolAddress on that device's first call and is kept here.
@@ -251,10 +303,14 @@ int32_t scratch_on_current_device(double** out) {
 
 }  // namespace psx
 
+extern "C" int64_t psx_ecsw_training_work_doubles(const psx_plan* plan) {
+  return plan ? 2 * psx::train_groups(plan->d.E) : 0;
+}
+
 extern "C" int32_t psx_ecsw_training_noalloc(psx_plan* plan, const double* phi, int32_t rank,
                                              const double* snapshots, int32_t n_snapshots, const double* mu,
                                              const double* lam, int32_t include_stiffness, double* A,
-                                             void* stream, int32_t* status) {
+                                             double* sumsq, void* stream, int32_t* status) {
   psx::clear_error();
   if (!plan || !phi || !snapshots || !mu || !lam || !A || !status)
     return psx::fail(PSX_EINVAL, "null argument to psx_ecsw_training_noalloc");
@@ -262,20 +318,19 @@ extern "C" int32_t psx_ecsw_training_noalloc(psx_plan* plan, const double* phi, 
   if (rank < 1 || rank > psx::kTrainMaxRank)
     return psx::fail(PSX_EUNSUPPORTED, "psx_ecsw_training_noalloc supports ranks 1..64");
   if (n_snapshots < 1) return psx::fail(PSX_EINVAL, "need at least one snapshot");
-  const int64_t rows = int64_t(n_snapshots) * rank + (include_stiffness ? int64_t(rank) * (rank + 1) / 2 : 0);
-  if (rows > psx::kNoAllocScratchDoubles)
-    return psx::fail(PSX_EUNSUPPORTED, "psx_ecsw_training_noalloc: " + std::to_string(rows) +
-                                           " training rows exceed its 32 MiB scratch (at most " +
-                                           std::to_string(psx::kNoAllocScratchDoubles) + " rows)");
+  const int64_t need = 2 * psx::train_groups(plan->d.E);
+  if (need > psx::kNoAllocScratchDoubles)
+    return psx::fail(PSX_EUNSUPPORTED, "psx_ecsw_training_noalloc: " + std::to_string(plan->d.E) +
+                                           " elements exceed its 32 MiB scratch of partial sums (at most " +
+                                           std::to_string(psx::kNoAllocScratchDoubles / 2 * psx::kTrainGroup) +
+                                           " elements)");
   int dev = 0;
   PSX_HIP_TRY(hipGetDevice(&dev));
   if (dev != plan->device)
     return psx::fail(PSX_EINVAL, "psx_ecsw_training_noalloc: the plan's device is not the current device");
-  double* work = nullptr;
-  const int32_t rc = psx::scratch_on_current_device(&work);
+  double* partial = nullptr;
+  const int32_t rc = psx::scratch_on_current_device(&partial);
   if (rc) return rc;
-  // whole elements of the scratch, in multiples of 64 when at least 64 fit
-  const int64_t work_elements = psx::kNoAllocScratchDoubles / rows;
   return psx::launch_ecsw_training(plan->d, phi, rank, snapshots, n_snapshots, mu, lam, include_stiffness != 0, A,
-                                   work, work_elements, static_cast<hipStream_t>(stream), status);
+                                   partial, sumsq, static_cast<hipStream_t>(stream), status);
 }

@@ -196,8 +196,8 @@ int32_t launch_tet4_vjp(const DevPlan& p, const double* u, const double* mu, con
                         const double* w, double* gu, double* gmu, double* glam, int precision,
                         hipStream_t s, int32_t* status);
 int32_t launch_ecsw_training(const DevPlan& p, const double* phi, int r, const double* snaps, int m,
-                             const double* mu, const double* lam, int stiff, double* A, double* work,
-                             int64_t work_elements, hipStream_t s, int32_t* status);
+                             const double* mu, const double* lam, int stiff, double* A, double* partial,
+                             double* sumsq, hipStream_t s, int32_t* status);
 int32_t launch_mass(const DevPlan& p, const double* rho, const double* coef, double* kval, hipStream_t s,
                     int64_t n_units);
 int32_t launch_csc_extract(const DevCsc& c, const double* kval, int drop, int* colptr, int* rowind,

@@ -717,7 +717,7 @@ int32_t retile(psx_plan* plan, int32_t tile) {
 // ---------------------------------------------------------------------------
 extern "C" {
 
-int32_t psx_abi_version(void) { return 4; }
+int32_t psx_abi_version(void) { return 5; }
 
 const char* psx_last_error(void) { return psx::g_error.c_str(); }
 
@@ -887,16 +887,22 @@ int32_t psx_vjp(psx_plan* plan, const double* u, const double* mu, const double*
 
 int32_t psx_ecsw_training(psx_plan* plan, const double* phi, int32_t rank, const double* snapshots,
                           int32_t n_snapshots, const double* mu, const double* lam, int32_t include_stiffness,
-                          double* A, double* work, int64_t work_elements, void* stream, int32_t* status) {
+                          double* A, double* work, int64_t work_doubles, double* sumsq, void* stream,
+                          int32_t* status) {
   psx::clear_error();
-  if (!plan || !phi || !snapshots || !mu || !lam || !A || !work || !status)
+  if (!plan || !phi || !snapshots || !mu || !lam || !A || !status)
     return psx::fail(PSX_EINVAL, "null argument to psx_ecsw_training");
-  if (work_elements < 1) return psx::fail(PSX_EINVAL, "psx_ecsw_training needs work_elements >= 1");
   if (plan->hp.npe != 4) return psx::fail(PSX_EUNSUPPORTED, "psx_ecsw_training supports Tet4 plans");
   if (rank < 1 || rank > 64) return psx::fail(PSX_EUNSUPPORTED, "psx_ecsw_training supports ranks 1..64");
   if (n_snapshots < 1) return psx::fail(PSX_EINVAL, "need at least one snapshot");
+  if (sumsq) {
+    const int64_t need = psx_ecsw_training_work_doubles(plan);
+    if (!work || work_doubles < need)
+      return psx::fail(PSX_EINVAL, "psx_ecsw_training with sumsq needs a work array of " + std::to_string(need) +
+                                       " doubles");
+  }
   return psx::launch_ecsw_training(plan->d, phi, rank, snapshots, n_snapshots, mu, lam, include_stiffness != 0,
-                                   A, work, work_elements, static_cast<hipStream_t>(stream), status);
+                                   A, work, sumsq, static_cast<hipStream_t>(stream), status);
 }
 
 int32_t psx_csc_create(const psx_plan* plan, const int64_t* free_dofs, int64_t n_free, psx_csc** out) {

@@ -255,18 +255,6 @@ def _snapshot_matrix(snapshots) -> np.ndarray:
     raise ValueError(f"Snapshots must be a 2D array or sequence of 1D vectors, got type {type(snapshots)}")
 
 
-TRAIN_WORK_BYTES = 1 << 30
-
-
-def train_work_elements(rows: int, n_elements: int, work_bytes: int = TRAIN_WORK_BYTES) -> int:
-    """Elements per chunk of the training matrix's element-major work block:
-    ``work_bytes`` of FP64 rows.  On the 48^3 box (663k elements, 2,592
-    rows) a 1 GiB block is as fast as one block of all elements (13.8 GB)
-    on MI355X; much smaller blocks add launches (3,000 elements: +10%)."""
-    per = work_bytes // (8 * max(rows, 1))
-    return int(max(1, min(n_elements, per - per % 64 if per >= 64 else per)))
-
-
 class ECSWTrainingMatrix:
     """Virtual-work training matrix for ECSW on the GPU.
 
@@ -288,7 +276,7 @@ class ECSWTrainingMatrix:
 
     @staticmethod
     def build_device(domain, basis, snapshots, include_stiffness: bool = False,
-                     stiffness_weight: float | None = None, work_elements: int | None = None):
+                     stiffness_weight: float | None = None):
         torch = _torch()
         snap = _snapshot_matrix(snapshots)
         n_dofs, m = snap.shape
@@ -313,13 +301,13 @@ class ECSWTrainingMatrix:
         n_triu = r * (r + 1) // 2 if include_stiffness else 0
         rows = m * r + n_triu
         A = torch.empty((rows, E), dtype=torch.float64, device=dev)
-        chunk = train_work_elements(rows, E) if work_elements is None else int(work_elements)
-        work = torch.empty((max(min(chunk, E), 1), rows), dtype=torch.float64, device=dev)
+        nwork = int(lib().psx_ecsw_training_work_doubles(plan._h))
+        work = torch.empty(max(nwork, 1), dtype=torch.float64, device=dev)
+        sumsq = torch.zeros(2, dtype=torch.float64, device=dev)
         status = torch.zeros(2, dtype=torch.int32, device=dev)
         check(lib().psx_ecsw_training(plan._h, ptr(phi_t), r, ptr(snap_t), m, ptr(st["mu"]), ptr(st["lam"]),
-                                      int(bool(include_stiffness)), ptr(A), ptr(work), chunk,
+                                      int(bool(include_stiffness)), ptr(A), ptr(work), nwork, ptr(sumsq),
                                       current_stream_handle(), ptr(status)))
-        del work
         stv = status.cpu().numpy()
         if stv[0] != 0:
             # the reference's exception for the lowest inverted element, with
@@ -333,8 +321,10 @@ class ECSWTrainingMatrix:
             dets = [_det3((X + snap[:, k].reshape(-1, 3)[conn]).T @ H) for k in range(m)]
             raise ValueError(INVERSION_MSG.format(next((d for d in dets if d <= 0.0), min(dets))))
         if include_stiffness:
-            s_force = float(torch.linalg.vector_norm(A[: m * r])) / np.sqrt(max(m * r, 1))
-            s_stiff = float(torch.linalg.vector_norm(A[m * r:])) / np.sqrt(max(n_triu, 1))
+            # the reference's balancing (ecsw.py:350-357) from the kernel's sums of squares
+            sq = sumsq.cpu().numpy()
+            s_force = float(np.sqrt(sq[0])) / np.sqrt(max(m * r, 1))
+            s_stiff = float(np.sqrt(sq[1])) / np.sqrt(max(n_triu, 1))
             if s_force > 1e-12 and s_stiff > 1e-12:
                 weight = float(stiffness_weight) if stiffness_weight is not None else 1.0
                 A[m * r:] *= (s_force / s_stiff) * weight

@@ -5,8 +5,10 @@ oracle.ecsw.training_matrix, the restatement of ECSWTrainingMatrix.build
 (src/psierra/hyperreduction/ecsw.py:171-365), which test_oracle_ecsw_training
 pins to the assembled force and stiffness.  Tolerance 1e-12 relative
 (Frobenius) for A and b; rank 8 and 64 (the kernel's lane-per-mode limit),
-70 snapshots to cross the 64-snapshot pass, the stiffness block with and
-without a weight, and the reference's exception for an inverting snapshot.
+70 snapshots to cross the 16-snapshot passes, the stiffness block with and
+without a weight, a ragged last group of elements, the sums of squares the
+norm balancing uses, and the reference's exception for an inverting
+snapshot.
 """
 from __future__ import annotations
 
@@ -92,30 +94,66 @@ def test_snapshot_argument_checks():
         ECSWTrainingMatrix.build(dom, basis, np.zeros(7))
 
 
-@pytest.mark.parametrize("chunk", [1, 37, 100, 4096])
-def test_chunked_work_block_gives_identical_matrix(chunk):
-    """The elements in chunks of `chunk` columns (not dividing E = 10,368;
-    100 is rounded down to 64) give the same bits as one chunk of all
-    elements, and the oracle's A."""
+def _raw_training(plan, st, basis, snaps, stiff):
+    """psx_ecsw_training through the C ABI: unscaled A and the sums of squares."""
     import torch
 
-    from psierra_amd.ecsw import ECSWTrainingMatrix
+    from psierra_amd._lib import check, lib, ptr
+    from psierra_amd.engine import current_stream_handle
 
-    wl = W.jittered_box_tet4(n=12, seed=18)
-    basis, snaps = _inputs(wl, 16, 3, seed=7)
-    dom = _domain(wl)
-    E = wl.mesh.n_elements
-    banner(f"ecsw training chunk={chunk} of E={E}")
-    A_one, b_one = ECSWTrainingMatrix.build_device(dom, basis, snaps, include_stiffness=True, work_elements=E)
-    A_chk, b_chk = ECSWTrainingMatrix.build_device(dom, basis, snaps, include_stiffness=True, work_elements=chunk)
+    dev = plan.device
+    r, m = basis.shape[1], snaps.shape[1]
+    rows = m * r + (r * (r + 1) // 2 if stiff else 0)
+    A = torch.full((rows, plan.n_elements), float("nan"), dtype=torch.float64, device=dev)
+    nwork = int(lib().psx_ecsw_training_work_doubles(plan._h))
+    work = torch.empty(nwork, dtype=torch.float64, device=dev)
+    sumsq = torch.zeros(2, dtype=torch.float64, device=dev)
+    status = torch.zeros(2, dtype=torch.int32, device=dev)
+    phi_t = torch.as_tensor(np.ascontiguousarray(basis), device=dev)
+    snap_t = torch.as_tensor(np.ascontiguousarray(snaps), device=dev)
+    check(lib().psx_ecsw_training(plan._h, ptr(phi_t), r, ptr(snap_t), m, ptr(st["mu"]), ptr(st["lam"]), int(stiff),
+                                  ptr(A), ptr(work), nwork, ptr(sumsq), current_stream_handle(), ptr(status)))
     torch.cuda.synchronize()
-    assert torch.equal(A_one, A_chk) and torch.equal(b_one, b_chk)
-    A_ref, _ = OE.training_matrix(wl.mesh.nodes, wl.mesh.elements, basis, snaps, wl.mu, wl.lam,
-                                  include_stiffness=True)
-    assert rel_err(A_chk.cpu().numpy(), A_ref) < 1e-12
+    assert status.cpu().tolist()[0] == 0
+    return A, sumsq
 
 
-def test_inversion_in_a_later_chunk_raises():
+def test_ragged_last_group_sums_of_squares_and_determinism():
+    """E = 2,058 (the last work-group holds 10 of its 16 elements): A equals
+    the oracle's unscaled blocks, every entry written; the kernel's sums of
+    squares equal the squared Frobenius norms of the two blocks; a second
+    call gives the same bits."""
+    import torch
+
+    wl = W.jittered_box_tet4(n=7, seed=18)
+    E = wl.mesh.n_elements
+    assert E % 16 == 10
+    r, m = 24, 5
+    basis, snaps = _inputs(wl, r, m, seed=7)
+    dom = _domain(wl)
+    plan = dom.element.plan(wl.mesh)
+    st = dom.element._state(wl.mesh)
+    banner(f"ecsw training ragged group E={E} r={r} m={m}")
+    A, sumsq = _raw_training(plan, st, basis, snaps, True)
+    assert not bool(torch.isnan(A).any())
+    A_ref, _ = OE.training_matrix(wl.mesh.nodes, wl.mesh.elements, basis, snaps, wl.mu, wl.lam)
+    Ah = A.cpu().numpy()
+    assert rel_err(Ah[: m * r], A_ref) < 1e-12
+    # unscaled stiffness rows: the oracle's blocks before its norm balancing
+    Kst = OE.training_matrix(wl.mesh.nodes, wl.mesh.elements, basis, snaps[:, :1], wl.mu, wl.lam,
+                             include_stiffness=True, stiffness_weight=None)[0][r:]
+    s_f1 = np.linalg.norm(OE.training_matrix(wl.mesh.nodes, wl.mesh.elements, basis, snaps[:, :1], wl.mu,
+                                             wl.lam)[0]) / np.sqrt(r)
+    s_s = np.linalg.norm(Ah[m * r:]) / np.sqrt(r * (r + 1) // 2)
+    assert rel_err(Ah[m * r:] * (s_f1 / s_s), Kst) < 1e-12
+    sq = sumsq.cpu().numpy()
+    assert abs(sq[0] - np.sum(Ah[: m * r] ** 2)) <= 1e-13 * sq[0]
+    assert abs(sq[1] - np.sum(Ah[m * r:] ** 2)) <= 1e-13 * sq[1]
+    A2, sumsq2 = _raw_training(plan, st, basis, snaps, True)
+    assert torch.equal(A, A2) and torch.equal(sumsq, sumsq2)
+
+
+def test_inversion_in_the_last_group_raises():
     from psierra_amd.ecsw import ECSWTrainingMatrix
 
     wl = W.cantilever_tet4()
@@ -125,8 +163,6 @@ def test_inversion_in_a_later_chunk_raises():
     X = wl.mesh.nodes[conn]
     bad = snaps[:, 1].reshape(-1, 3)
     bad[conn[0]] = 2.5 * (X[1:].mean(axis=0) - X[0])
-    banner(f"ecsw training inversion of element {e_bad} in the last chunk of 64")
+    banner(f"ecsw training inversion of element {e_bad} in the last group")
     with pytest.raises(ValueError, match=r"Element inversion detected"):
-        ECSWTrainingMatrix.build_device(_domain(wl), basis, snaps, work_elements=64)
-    with pytest.raises(ValueError, match="work_elements"):
-        ECSWTrainingMatrix.build_device(_domain(wl), basis, snaps, work_elements=0)
+        ECSWTrainingMatrix.build_device(_domain(wl), basis, snaps)

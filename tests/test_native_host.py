@@ -30,21 +30,7 @@ def test_library_exports_header_symbols():
     for s in syms:
         assert hasattr(L, s), s
         assert s in _lib.SIGNATURES, f"{s} has no ctypes signature"
-    assert L.psx_abi_version() == _lib.ABI_VERSION == 4
-
-
-def test_training_work_chunk_sizes():
-    """Elements per training-matrix work chunk: a multiple of 64 within the
-    byte budget, all elements when they fit, at least one."""
-    from psierra_amd.ecsw import train_work_elements
-
-    assert train_work_elements(2592, 663_552) == 51_776           # 1 GiB of 2592-row columns
-    assert train_work_elements(2592, 663_552) % 64 == 0
-    assert 8 * 2592 * train_work_elements(2592, 663_552) <= 1 << 30
-    assert train_work_elements(2592, 1_000) == 1_000              # fits: one chunk
-    assert train_work_elements(10**9, 5_000) == 1                 # one column over budget
-    assert train_work_elements(512, 10**7, work_bytes=8 * 512 * 100) == 64
-    assert train_work_elements(512, 10**7, work_bytes=8 * 512 * 40) == 40
+    assert L.psx_abi_version() == _lib.ABI_VERSION == 5
 
 
 def _random_mesh(seed, npe):
