@@ -60,9 +60,10 @@ enum {
   PSX_INFO_INCIDENCES = 8,
   PSX_INFO_MAX_INCIDENCES = 9, /* most incident elements of any node */
   PSX_INFO_WIDE = 10,          /* 1: a node has more than 64 incident elements or more than 255
-                                  neighbours; such a plan runs the Tet4 assembly, f-only and
-                                  adjoint kernels (runs of up to `tile` incidences) and the CSC,
-                                  and the mass, F-bar and Tet10 kernels refuse it */
+                                  neighbours; such a Tet4 plan runs its mask-free row kernel
+                                  (runs of up to `tile` incidences), a Tet10 plan its general
+                                  path (element rows summed per CSR value in element order);
+                                  the mass and F-bar kernels refuse it */
   PSX_INFO_COUNT = 11
 };
 
@@ -94,11 +95,14 @@ int32_t psx_pattern_build(const int64_t* conn, int64_t n_elements, int32_t npe,
  * Rejects elements with det([1 X]) <= 0 (Tet4, src/psierra/element/
  * tet4_base.py:56-61) or det J <= 0 at a quadrature point (Tet10,
  * src/psierra/element/tet10_total_lagrangian.py:104-108) with PSX_EVOLUME.
- * tile: incidences per work-group (0 = default; 64, 128 or 256).  Every
- * node's incident elements must fit one tile (at most `tile`); a Tet4 mesh
+ * tile: incidences per work-group (0 = default; 64, 128 or 256).  A mesh
  * with a node of more than 64 incident elements or more than 255 neighbours
- * gives a "wide" plan (PSX_INFO_WIDE), which the Tet10, mass and F-bar
- * kernels refuse with PSX_EUNSUPPORTED. */
+ * gives a "wide" plan (PSX_INFO_WIDE): Tet4 then needs every node's incident
+ * elements to fit one tile (at most `tile`) and assembles through a
+ * mask-free row kernel; Tet10 assembles through a general path (per
+ * element row into scratch, then each CSR value and f entry summed over its
+ * contributions in element order, up to 2,386,092 elements); the mass and
+ * F-bar kernels refuse a wide plan with PSX_EUNSUPPORTED. */
 int32_t psx_plan_create(const int64_t* conn, int64_t n_elements, int32_t npe,
                         const double* X /* N x 3 */, int64_t n_nodes,
                         int32_t tile, psx_plan** out);

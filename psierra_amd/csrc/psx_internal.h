@@ -94,6 +94,16 @@ struct DevPlan {
   const uint32_t* inc_pos;        // Tet4, per incidence: staging positions of its 3 off-diagonal blocks
   const uint32_t* unit_rec;       // Tet4, per unit: first A | first B << 10 | tile-local node << 20
   int32_t wide;                   // 1: a node exceeds the mask/rank-byte bounds (Tet4 row kernels only)
+  // General path of a wide Tet10 plan: per element and row node the 90 K
+  // values and 3 f values into ke / fe, then every CSR value (and f entry)
+  // sums its contributions in increasing element order.
+  int64_t gen_nnz;                // CSR values of the plan
+  const int32_t* gen_kptr;        // gen_nnz + 1: contributions of each CSR value
+  const int32_t* gen_kidx;        // index into ke (e 900 + 30 row + column)
+  const int32_t* gen_fptr;        // 3N + 1: contributions of each f entry
+  const int32_t* gen_fidx;        // index into fe (e 30 + 3 p + c)
+  double* gen_ke;                 // E x 900 element tangents (scratch)
+  double* gen_fe;                 // E x 30 element forces (scratch)
   double4* ui;                    // scratch: displacement in internal order
   double4* wi;                    // scratch: adjoint cotangent in internal order
 };

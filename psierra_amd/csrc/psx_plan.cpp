@@ -295,7 +295,7 @@ int32_t upload(psx_plan* plan, const T* host, size_t count, T** dev) {
   PSX_HIP_TRY(hipMalloc(&p, bytes));
   plan->allocations.push_back(p);
   plan->device_bytes += int64_t(bytes);
-  if (count) PSX_HIP_TRY(hipMemcpy(p, host, count * sizeof(T), hipMemcpyHostToDevice));
+  if (count && host) PSX_HIP_TRY(hipMemcpy(p, host, count * sizeof(T), hipMemcpyHostToDevice));
   *dev = static_cast<T*>(p);
   return PSX_OK;
 }
@@ -364,16 +364,79 @@ int32_t make_tet4_positions(psx_plan* plan, const std::vector<int32_t>& tiles) {
   return PSX_OK;
 }
 
+// Contribution lists of a wide Tet10 plan's general path: for every CSR
+// value the element-tangent entries (e 900 + 30 row + column) that sum into
+// it, and for every f entry its element-force entries (e 30 + 3 p + c), each
+// in increasing element order; plus the element scratch.
+int32_t make_general_maps(psx_plan* plan, const int64_t* conn, int64_t E) {
+  const int npe = plan->hp.npe, nd = 3 * npe;
+  const int64_t N = plan->hp.N, nnz = plan->hp.nnz();
+  if (E * nd * nd >= (int64_t(1) << 31))
+    return fail(PSX_EUNSUPPORTED, "the general path of a wide Tet10 plan takes at most " +
+                                      std::to_string(((int64_t(1) << 31) - 1) / (nd * nd)) + " elements");
+  const auto& ap = plan->hp.adj_ptr;
+  const auto& adj = plan->hp.adj;
+  auto pos_of = [&](int64_t a, int i, int64_t b, int c) -> int64_t {
+    const int32_t d = ap[a + 1] - ap[a];
+    const int32_t s = int32_t(std::lower_bound(adj.begin() + ap[a], adj.begin() + ap[a + 1], int32_t(b)) -
+                              (adj.begin() + ap[a]));
+    return 9 * int64_t(ap[a]) + 3 * int64_t(i) * d + 3 * s + c;
+  };
+  std::vector<int32_t> kptr(nnz + 1, 0), fptr(3 * N + 1, 0);
+  for (int64_t e = 0; e < E; ++e)
+    for (int p = 0; p < npe; ++p) {
+      const int64_t a = conn[e * npe + p];
+      for (int i = 0; i < 3; ++i) {
+        ++fptr[3 * a + i + 1];
+        for (int q = 0; q < npe; ++q)
+          for (int c = 0; c < 3; ++c) ++kptr[pos_of(a, i, conn[e * npe + q], c) + 1];
+      }
+    }
+  for (int64_t v = 0; v < nnz; ++v) kptr[v + 1] += kptr[v];
+  for (int64_t v = 0; v < 3 * N; ++v) fptr[v + 1] += fptr[v];
+  std::vector<int32_t> kidx(size_t(std::max<int64_t>(E * nd * nd, 1))), fidx(size_t(std::max<int64_t>(E * nd, 1)));
+  std::vector<int32_t> kfill(kptr.begin(), kptr.end() - 1), ffill(fptr.begin(), fptr.end() - 1);
+  for (int64_t e = 0; e < E; ++e)
+    for (int p = 0; p < npe; ++p) {
+      const int64_t a = conn[e * npe + p];
+      for (int i = 0; i < 3; ++i) {
+        fidx[ffill[3 * a + i]++] = int32_t(e * nd + 3 * p + i);
+        for (int q = 0; q < npe; ++q)
+          for (int c = 0; c < 3; ++c)
+            kidx[kfill[pos_of(a, i, conn[e * npe + q], c)]++] = int32_t(e * nd * nd + nd * (3 * p + i) + 3 * q + c);
+      }
+    }
+  int32_t *dkp, *dki, *dfp, *dfi;
+  double *dke, *dfe;
+  int32_t rc;
+  if ((rc = upload(plan, kptr.data(), kptr.size(), &dkp)) || (rc = upload(plan, kidx.data(), kidx.size(), &dki)) ||
+      (rc = upload(plan, fptr.data(), fptr.size(), &dfp)) || (rc = upload(plan, fidx.data(), fidx.size(), &dfi)) ||
+      (rc = upload<double>(plan, nullptr, size_t(std::max<int64_t>(E * nd * nd, 1)), &dke)) ||
+      (rc = upload<double>(plan, nullptr, size_t(std::max<int64_t>(E * nd, 1)), &dfe)))
+    return rc;
+  plan->d.gen_nnz = nnz;
+  plan->d.gen_kptr = dkp;
+  plan->d.gen_kidx = dki;
+  plan->d.gen_fptr = dfp;
+  plan->d.gen_fidx = dfi;
+  plan->d.gen_ke = dke;
+  plan->d.gen_fe = dfe;
+  return PSX_OK;
+}
+
 int32_t make_tiles(psx_plan* plan, int32_t tile) {
   if (tile == 0) tile = plan->hp.npe == 4 ? 256 : 64;
   if (tile != 64 && tile != 128 && tile != 256)
     return fail(PSX_EINVAL, "tile must be 64, 128 or 256");
   const int64_t N = plan->hp.N;
   const bool wide = plan->d.wide != 0;
-  if (wide && plan->hp.npe != 4)
-    return fail(PSX_EUNSUPPORTED, "Tet10 plans need every node to have at most " +
-                                      std::to_string(kMaxNodeIncidences) + " incident elements and " +
-                                      std::to_string(kMaxRankDegree) + " neighbours");
+  if (wide && plan->hp.npe != 4) {
+    // a wide Tet10 plan assembles through its general path: no tiles
+    plan->d.n_tiles = 0;
+    plan->d.tile = tile;
+    plan->d.max_inc = plan->max_run;
+    return PSX_OK;
+  }
   if (plan->max_run > tile)
     return fail(PSX_EUNSUPPORTED, "node with " + std::to_string(plan->max_run) +
                                       " incident elements exceeds the tile of " + std::to_string(tile));
@@ -616,6 +679,7 @@ int32_t create_plan(const int64_t* conn, int64_t E, int npe, const double* X, in
   plan->device_bytes += int64_t(N) * int64_t(sizeof(double4));
   d.wi = static_cast<double4*>(wi);
   if ((rc = make_tiles(plan, tile))) return bail(rc);
+  if (npe == 10 && plan->d.wide && (rc = make_general_maps(plan, conn, E))) return bail(rc);
   *out = plan;
   return PSX_OK;
 }

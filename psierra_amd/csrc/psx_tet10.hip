@@ -576,10 +576,101 @@ static int32_t launch_t10_tiles(const DevPlan& p, const MatParams& mp, double* f
   return PSX_OK;
 }
 
+// ---------------------------------------------------------------------------
+// General path of a wide plan (a node with more than 64 incident elements or
+// more than 255 neighbours, beyond the feed masks and rank bytes of
+// tet10_assemble_rows): one lane per (element, row node) evaluates the row
+// over the four quadrature points (FP64 sums of the points) into the plan's
+// element scratch, then one lane per CSR value (and per f entry) adds its
+// contributions in increasing element order, from a list the plan built.
+// Deterministic, no atomics, no limit on incidences or neighbours.
+// ---------------------------------------------------------------------------
+template <typename R, int M, bool WANT_K>
+__global__ __launch_bounds__(64) void tet10_elements_kernel(DevPlan p, MatParams mp, int32_t* status) {
+  const int64_t w = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (w >= int64_t(p.E) * 10) return;
+  const int e = int(w / 10), pr = int(w % 10);
+  R dX[10][3], dU[10][3];
+  tet10_gather<R>(p, e, dX, dU);
+  double pd[4];
+  load_params<M>(mp, e, pd);
+  const R prm[4] = {R(pd[0]), R(pd[1]), R(pd[2]), R(pd[3])};
+  double acc[93];
+#pragma unroll
+  for (int v = 0; v < 93; ++v) acc[v] = 0.0;
+  const R J = tet10_row<R, M, WANT_K>(dX, dU, prm, pr, [&](int v, R val) { acc[v] += double(val); });
+  if (J <= R(0)) {
+    atomicMin(&status[1], e);
+    status[0] = 1;
+  }
+  if (WANT_K) {
+    double* ko = p.gen_ke + 900 * int64_t(e) + 90 * pr;   // rows 3 pr .. 3 pr + 2 of the 30 x 30 tangent
+#pragma unroll
+    for (int b = 0; b < 10; ++b)
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) ko[30 * i + 3 * b + c] = acc[9 * b + 3 * i + c];
+  }
+  double* fo = p.gen_fe + 30 * int64_t(e) + 3 * pr;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) fo[i] = acc[90 + i];
+}
+
+// out[v] = sum of src[idx[k]] for k in ptr[v] .. ptr[v + 1], in order.
+__global__ __launch_bounds__(256) void segment_sum_kernel(const int32_t* __restrict__ ptr,
+                                                          const int32_t* __restrict__ idx,
+                                                          const double* __restrict__ src, int64_t n,
+                                                          double* __restrict__ out) {
+  const int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  double s = 0.0;
+  for (int k = ptr[v]; k < ptr[v + 1]; ++k) s += src[idx[k]];
+  out[v] = s;
+}
+
+template <int M>
+static int32_t launch_tet10_general(const DevPlan& p, const MatParams& mp, double* f, double* kval, bool fp32,
+                                    hipStream_t s, int32_t* status) {
+  const int64_t lanes = int64_t(p.E) * 10;
+  const dim3 g(unsigned((lanes + 63) / 64));
+  if (kval) {
+    if (fp32)
+      hipLaunchKernelGGL((tet10_elements_kernel<float, M, true>), g, dim3(64), 0, s, p, mp, status);
+    else
+      hipLaunchKernelGGL((tet10_elements_kernel<double, M, true>), g, dim3(64), 0, s, p, mp, status);
+    if (p.gen_nnz > 0)
+      hipLaunchKernelGGL(segment_sum_kernel, dim3(unsigned((p.gen_nnz + 255) / 256)), dim3(256), 0, s, p.gen_kptr,
+                         p.gen_kidx, p.gen_ke, p.gen_nnz, kval);
+  } else {
+    if (fp32)
+      hipLaunchKernelGGL((tet10_elements_kernel<float, M, false>), g, dim3(64), 0, s, p, mp, status);
+    else
+      hipLaunchKernelGGL((tet10_elements_kernel<double, M, false>), g, dim3(64), 0, s, p, mp, status);
+  }
+  const int64_t n = 3 * int64_t(p.N);
+  hipLaunchKernelGGL(segment_sum_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, p.gen_fptr, p.gen_fidx,
+                     p.gen_fe, n, f);
+  return PSX_OK;
+}
+
 int32_t launch_tet10_hyper(const DevPlan& p, int model, const double* const* params, const double* u, double* f,
                            double* kval, int precision, hipStream_t s, int32_t* status) {
   int32_t rc = launch_permute_u(p, u, s, status);
   if (rc) return rc;
+  if (p.wide) {
+    const MatParams mp{{params[0], params[1], params[2], params[3]}};
+    const bool fp32 = precision == PSX_FP32;
+    switch (model) {
+      case kNeoHookean: rc = launch_tet10_general<kNeoHookean>(p, mp, f, kval, fp32, s, status); break;
+      case kMooneyRivlin: rc = launch_tet10_general<kMooneyRivlin>(p, mp, f, kval, fp32, s, status); break;
+      case kYeoh: rc = launch_tet10_general<kYeoh>(p, mp, f, kval, fp32, s, status); break;
+      default: return fail(PSX_EINVAL, "unknown material model");
+    }
+    if (rc) return rc;
+    PSX_HIP_TRY(hipGetLastError());
+    return PSX_OK;
+  }
   if (p.n_tiles <= 0) return PSX_OK;
   const bool fp32 = precision == PSX_FP32;
   const MatParams mp{{params[0], params[1], params[2], params[3]}};

@@ -215,11 +215,10 @@ def test_wide_node_matches_oracle(precision, n_elem):
 def test_wide_plan_free_csc_and_refusals():
     """The free-DOF CSC of a wide plan (a column node with 601 neighbours) is
     scipy's bit for bit; the mass and F-bar kernels refuse a wide plan with a
-    clear message, and so does a Tet10 plan of such a mesh."""
+    clear message."""
     import torch
 
-    from psierra_amd.engine import AssemblyPlan, FbarAssembly
-    from psierra_amd.mesh import tet4_to_tet10
+    from psierra_amd.engine import FbarAssembly
     from psierra_amd.solver import FreeDofSystem
 
     wl = _fan_and_box(200, seed=10)
@@ -240,9 +239,80 @@ def test_wide_plan_free_csc_and_refusals():
         plan.assemble_mass(1.0)
     with pytest.raises(Exception, match="at most 64 incident elements"):
         FbarAssembly(plan)
+
+
+def _tet10_wide(n_elem: int, seed: int) -> Workload:
+    wl = _fan_and_box(n_elem, seed)
+    from psierra_amd.mesh import tet4_to_tet10
+
     X10, c10 = tet4_to_tet10(wl.mesh.nodes, wl.mesh.elements)
-    with pytest.raises(Exception, match="Tet10 plans need"):
-        AssemblyPlan(X10, c10)
+    u = 0.005 * np.random.default_rng(seed).standard_normal(X10.shape)
+    return Workload(f"tet10_fan{n_elem}_box5", TetMesh(X10, c10), u, wl.mu, wl.lam)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("n_elem", [65, 128, 200])
+def test_wide_tet10_matches_oracle(precision, n_elem):
+    """Tet10: a corner node shared by 65, 128 or 200 elements makes the plan
+    wide; its general path (element rows into scratch, every CSR value and f
+    entry summed over its contributions in element order) against the
+    oracle, the f-only path, and the same bits on a second call."""
+    import torch
+
+    wl = _tet10_wide(n_elem, seed=19)
+    banner(f"edge: wide Tet10 plan, fan of {n_elem} elements {precision}")
+    plan, f, k, f_ref = _check(wl, precision)
+    assert plan.info["wide"] == 1 and plan.info["max_incidences"] == n_elem and plan.npe == 10
+    dev = plan.device
+    u = torch.as_tensor(wl.u.reshape(-1), device=dev)
+    mu = torch.as_tensor(wl.mu, device=dev)
+    lam = torch.as_tensor(wl.lam, device=dev)
+    f2, k2 = plan.assemble(u, mu, lam, precision=precision)
+    assert np.array_equal(f2.cpu().numpy(), f) and np.array_equal(k2.cpu().numpy(), k)
+    f_only, _ = plan.assemble(u, mu, lam, precision=precision, want_k=False)
+    assert np.array_equal(f_only.cpu().numpy(), f)
+
+
+def test_wide_tet10_mooney_rivlin_matches_oracle():
+    """Another model through the same general path: Mooney-Rivlin (c10, c01,
+    K per element) against the oracle's Tet10 element summed by the
+    reference's scatter."""
+    import scipy.sparse as spsp
+    import torch
+
+    from oracle import fem
+    from psierra_amd.engine import AssemblyPlan
+
+    wl = _tet10_wide(80, seed=20)
+    m = wl.mesh
+    E = m.n_elements
+    c10 = 0.4 * wl.mu
+    c01 = 0.1 * wl.mu
+    K = wl.lam + (2.0 / 3.0) * wl.mu
+    banner("edge: wide Tet10 plan, Mooney-Rivlin")
+    plan = AssemblyPlan(m.nodes, m.elements)
+    assert plan.info["wide"] == 1
+    dev = plan.device
+    prm = [torch.as_tensor(a, device=dev) for a in (c10, c01, K)]
+    f, k = plan.assemble_hyperelastic(torch.as_tensor(wl.u.reshape(-1), device=dev), "mooney_rivlin", prm,
+                                      precision="fp64")
+    n = 3 * m.n_nodes
+    f_ref = np.zeros(n)
+    rows, cols, vals = [], [], []
+    for e in range(E):
+        conn = m.elements[e]
+        ke, fe = fem.tet10_force_and_tangent(m.nodes[conn], wl.u[conn],
+                                             fem.material_fn(1, (c10[e], c01[e], K[e])))
+        dofs = (3 * conn[:, None] + np.arange(3)).ravel()
+        np.add.at(f_ref, dofs, fe)
+        rows.append(np.repeat(dofs, 30))
+        cols.append(np.tile(dofs, 30))
+        vals.append(ke.ravel())
+    K_ref = spsp.coo_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(n, n)).tocsr()
+    indptr, indices = plan.pattern()
+    assert np.array_equal(indptr, K_ref.indptr) and np.array_equal(indices, K_ref.indices)
+    assert rel_err(k.cpu().numpy(), K_ref.data) < 1e-12
+    assert rel_err(f.cpu().numpy(), f_ref) < 1e-11
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
