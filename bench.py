@@ -694,19 +694,107 @@ def csc_cpu(n=48):
                      f"scipy CSR '+' and fancy slicing to CSC took {t:.2f}s")
 
 
+def bench_batch(args):
+    """Batched assembly: 128 Tet4 boxes of uneven size (sides 8..48 hexes per
+    axis, 3k-663k tets, 25M in all), per-instance parameters, f_int + K_T of
+    every instance by one launch, against a loop of one launch per mesh."""
+    import torch
+
+    from psierra_amd import workloads as W
+    from psierra_amd.engine import AssemblyBatch, AssemblyPlan
+
+    n_inst = args.n or 128
+    t0 = time.perf_counter()
+    wls = W.batch_boxes_tet4(n_instances=n_inst, seed=5)
+    t_wl = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    plans = [AssemblyPlan(w.mesh.nodes, w.mesh.elements) for w in wls]
+    t_plan = time.perf_counter() - t0
+    E = sum(p.n_elements for p in plans)
+    print(f"[bench] {n_inst} instances, {E} elements (workload {t_wl:.1f}s, plans {t_plan:.1f}s)",
+          file=sys.stderr, flush=True)
+    dev = plans[0].device
+    us = [torch.as_tensor(w.u.reshape(-1), device=dev) for w in wls]
+    mus = [torch.as_tensor(w.mu, device=dev) for w in wls]
+    lams = [torch.as_tensor(w.lam, device=dev) for w in wls]
+    batch = AssemblyBatch(plans)
+    fs, ks = batch.alloc_outputs(True)
+    launch = batch.launcher(us, mus, lams, fs, ks)
+    res = {}
+    for prec in ("fp64", "fp32"):
+        for mode in ("persistent", "tile_per_group"):
+            batch.set_persistent(mode == "persistent")
+            res[f"batch_{mode}_{prec}"] = _event_time(lambda: launch(prec), args.steps, args.warmup)
+            batch.raise_on_status(us)
+            print(f"[bench] batched {mode} {prec}: {res[f'batch_{mode}_{prec}']:.3f} ms", file=sys.stderr,
+                  flush=True)
+
+        def loop():
+            for p, u, m, l, f, k in zip(plans, us, mus, lams, fs, ks):
+                p.assemble(u, m, l, f=f, kval=k, precision=prec, check_status=False)
+
+        res[f"loop_{prec}"] = _event_time(loop, args.steps, args.warmup)
+        print(f"[bench] loop of {n_inst} launches {prec}: {res[f'loop_{prec}']:.3f} ms", file=sys.stderr,
+              flush=True)
+    best_mode = min(("persistent", "tile_per_group"), key=lambda m: res[f"batch_{m}_fp32"])
+    batch.set_persistent(best_mode == "persistent")
+    ms = res[f"batch_{best_mode}_fp32"]
+    # the batched results are the single-plan results, bit for bit
+    identical = True
+    if not args.no_check:
+        launch("fp32")
+        for i in sorted({0, int(np.argmax([p.n_elements for p in plans])), n_inst - 1}):
+            f1, k1 = plans[i].assemble(us[i], mus[i], lams[i], precision="fp32")
+            identical = identical and bool(torch.equal(fs[i], f1)) and bool(torch.equal(ks[i], k1))
+        if not identical:
+            raise RuntimeError("batched assembly differs from the single-plan assembly")
+    nnz = sum(p.nnz for p in plans)
+    b_alg = sum(algorithmic_bytes(p.n_elements, p.n_nodes, 4, p.nnz, coord_bytes=4, param_bytes=4)
+                for p in plans)
+    ach = b_alg / (ms * 1e-3) / 1e9
+    cpu = None
+    if not args.no_cpu:
+        big = wls[int(np.argmax([p.n_elements for p in plans]))]
+        with single_thread():
+            loop_eps, vec_eps, t_loop, t_vec = cpu_baseline(big, 20000)
+        cpu = cpu_entry(loop_eps, "elements/s",
+                        f"20000 seeded elements of the largest instance ({big.name}); FP64 oracle per-element "
+                        f"loop (element evaluated twice, add.at, coo.tocsr) took {t_loop:.2f}s; vectorised FP64 "
+                        f"numpy on the same sample: {vec_eps:.4g} elements/s ({t_vec:.2f}s)",
+                        vectorised_value=vec_eps)
+    return {
+        "metric": "batched Tet4 NeoHookean f_int+K_T assembly, 128 uneven meshes in one launch, elements/s",
+        "value": E / (ms * 1e-3), "unit": "elements/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic (seeded jittered Kuhn boxes, permuted node ids, one mu per instance, lam = 5 mu)",
+        "config": {"workload": f"batch_boxes_tet4_{n_inst}x(8..48)^3", "instances": n_inst, "elements": E,
+                   "nnz": nnz, "mode": "fp32 element arithmetic, fp64 scatter/f accumulation, per-call parameters",
+                   "launch": best_mode, "ms": res, "speedup_vs_loop_fp32": res["loop_fp32"] / ms,
+                   "speedup_vs_loop_fp64": res["loop_fp64"] / min(res["batch_persistent_fp64"],
+                                                                    res["batch_tile_per_group_fp64"]),
+                   "bit_identical_to_single_plans": identical if not args.no_check else None,
+                   "setup_s": {"workload": t_wl, "plans": t_plan}},
+        "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": ach / HBM_PEAK_GBPS, "traffic": None,
+                     "kernel": f"batch_permute_kernel + tet4_batch_rows<float,256,kNeoHookean> ({best_mode})"},
+        "cpu_baseline": cpu,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=None,
-                    help="hexes per box side (default: tet4 100 -> 6.0M tets, tet10 40, fbar 48)")
+                    help="hexes per box side (default: tet4 100 -> 6.0M tets, tet10 40, fbar 48); "
+                         "batch: number of instances (default 128)")
     ap.add_argument("--cpu-sample", type=int, default=80000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--workload",
-                    choices=("tet4", "tet10", "vjp", "ecsw", "ecsw_train", "fbar", "csc", "mass", "newton"),
+                    choices=("tet4", "tet10", "vjp", "ecsw", "ecsw_train", "fbar", "csc", "mass", "newton", "batch"),
                     default="tet4")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -717,7 +805,8 @@ def main():
         sys.exit(2)
     if args.workload != "tet4":
         fn = {"tet10": bench_tet10, "vjp": bench_vjp, "ecsw": bench_ecsw, "ecsw_train": bench_ecsw_train,
-              "fbar": bench_fbar, "csc": bench_csc, "mass": bench_mass, "newton": bench_newton}[
+              "fbar": bench_fbar, "csc": bench_csc, "mass": bench_mass, "newton": bench_newton,
+              "batch": bench_batch}[
             args.workload]
         print(json.dumps(fn(args)), flush=True)
         return

@@ -67,7 +67,8 @@ enum {
   PSX_INFO_COUNT = 11
 };
 
-/* Version of this interface: 5 (4: psx_ecsw_training took an element-major
+/* Version of this interface: 6 (5 had no psx_batch_* / psx_assemble_batch;
+ * 4: psx_ecsw_training took an element-major
  * work block and gave no sums of squares; 3 had no psx_ecsw_training_noalloc;
  * 2: psx_ecsw_* took rank 64 only; 1 had psx_ecsw_training without
  * work_elements and no psx_plan_bind_material). */
@@ -140,6 +141,52 @@ int32_t psx_assemble(psx_plan* plan, const double* u, const double* mu,
  * the parameters change; psx_assemble with mu = lam = NULL uses the last
  * binding.  Stream-ordered on `stream`. */
 int32_t psx_plan_bind_material(psx_plan* plan, const double* mu, const double* lam, void* stream);
+
+/* ---- Batched assembly (device) -------------------------------------------
+ * Many Tet4 plans -- meshes of uneven size, or one plan repeated for
+ * parameter samples -- assembled by one launch over the union of their
+ * tiles, each instance's f and K bit-identical to its own psx_assemble.  No
+ * single reference function: it replaces a Python loop of
+ * SolidDomain.assemble_internal_forces / assemble_tangent_stiffness_matrix
+ * (src/psierra/domain/solid.py:230-325) over many domains, e.g. the forward
+ * evaluations least_squares makes for one 2-point Jacobian
+ * (src/psierra/inverse/parameter_identification.py:370-379).
+ *
+ * psx_batch_create takes n_plans Tet4 plans (host array of handles; one plan
+ * may appear several times) that share a device and a tile size and are not
+ * wide (PSX_EUNSUPPORTED otherwise; assemble those with psx_assemble).  The
+ * plans must outlive the batch.  The batch holds its own displacement
+ * buffer per instance (32 B per node) and a table of the plans' views,
+ * refreshed by the next psx_assemble_batch after psx_plan_set_tile or
+ * psx_plan_bind_material on one of its plans (that call synchronises the
+ * device once; do not make it under graph capture).
+ *
+ * psx_assemble_batch: u, mu, lam, f, kval are DEVICE arrays of n_plans
+ * device pointers, instance i's arrays as psx_assemble takes them (u 3N_i,
+ * mu/lam n_elements_i, f 3N_i, kval nnz_i).  mu = lam = NULL: every plan's
+ * bound material (psx_plan_bind_material on each).  kval = NULL: f only.
+ * status: device int32[2 n_plans], instance i's inversion flag and lowest
+ * inverted element at 2i, 2i+1, reset by the call.  Two launches on
+ * `stream`: the displacement permutation of all instances, then the tile
+ * kernel.
+ *
+ * psx_batch_set_persistent: 0 (default) launches one work-group per tile of
+ * the global list (XCD-contiguous); 1 launches a persistent grid of as many
+ * work-groups as stay resident, each walking the list (XCD x takes the x-th
+ * eighth).  Same results either way; at 128 meshes of 3k-663k elements the
+ * first is faster (2.53 against 2.98 ms in FP32: the hardware dispatcher
+ * already balances the list, and the loop keeps more registers live).
+ *
+ * psx_batch_info fills up to n_info of {instances, tiles, nodes, tile,
+ * persistent, total nnz}. */
+typedef struct psx_batch psx_batch;
+int32_t psx_batch_create(psx_plan* const* plans, int32_t n_plans, psx_batch** out);
+int32_t psx_batch_destroy(psx_batch* batch);
+int32_t psx_batch_set_persistent(psx_batch* batch, int32_t persistent);
+int32_t psx_batch_info(const psx_batch* batch, int64_t* info, int32_t n_info);
+int32_t psx_assemble_batch(psx_batch* batch, const double* const* u, const double* const* mu,
+                           const double* const* lam, double* const* f, double* const* kval,
+                           int32_t precision, void* stream, int32_t* status);
 
 /* ---- Other hyperelastic models (device) -----------------------------------
  * The psx_assemble path for the compressible models of

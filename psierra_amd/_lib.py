@@ -91,6 +91,14 @@ SIGNATURES = {
          c_void_p, c_void_p, c_void_p],
     ),
     "psx_ecsw_training_work_doubles": (c_int64, [c_void_p]),
+    "psx_batch_create": (c_int32, [c_void_p, c_int32, POINTER(c_void_p)]),
+    "psx_batch_destroy": (c_int32, [c_void_p]),
+    "psx_batch_set_persistent": (c_int32, [c_void_p, c_int32]),
+    "psx_batch_info": (c_int32, [c_void_p, _i64p, c_int32]),
+    "psx_assemble_batch": (
+        c_int32,
+        [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_void_p],
+    ),
     "psx_ecsw_training_noalloc": (
         c_int32,
         [c_void_p, c_void_p, c_int32, c_void_p, c_int32, c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_void_p,
@@ -109,7 +117,7 @@ SIGNATURES = {
 _lib = None
 
 # The C ABI these signatures describe (psx_abi_version() of the library).
-ABI_VERSION = 5
+ABI_VERSION = 6
 
 
 class PsxError(RuntimeError):

@@ -108,6 +108,20 @@ struct DevPlan {
   double4* wi;                    // scratch: adjoint cotangent in internal order
 };
 
+// Device view of a batch of Tet4 plans assembled by one persistent launch:
+// the tiles of all plans form one global list (plan i owns tiles
+// tile_start[i] .. tile_start[i+1]), as do their nodes for the displacement
+// permutation.  Each plan's view is copied with ui pointing at the batch's
+// own buffer, so one plan may appear several times (parameter samples).
+struct BatchDev {
+  const DevPlan* plans = nullptr;         // n
+  const int32_t* tile_start = nullptr;    // n + 1
+  const int32_t* node_start = nullptr;    // n + 1
+  const double* const* bound_mu = nullptr;   // n: the plans' bound (mu, lambda) (FP64 rows)
+  const double* const* bound_lam = nullptr;
+  int32_t n = 0, n_tiles = 0, n_nodes = 0, tile = 0, max_inc = 0;
+};
+
 // F-bar store layout in the distance-2 pattern of K_dev + K_vol (device view):
 // per internal node {first value, row length in blocks, diagonal slot, 0},
 // per (node, slot) unit of the element pattern its slot in the wider row.
@@ -176,6 +190,7 @@ struct psx_plan {
   double* mu_bound = nullptr;      // device copies of the bound material (psx_plan_bind_material)
   double* lam_bound = nullptr;
   bool bound = false;
+  int64_t version = 0;             // bumped by psx_plan_set_tile and psx_plan_bind_material
   int64_t n_inc = 0;
   int64_t device_bytes = 0;
   psx::DevPlan d{};
@@ -192,6 +207,9 @@ int32_t launch_t4_fbar_rows(const DevPlan& p, const double* mu, const double* la
                             const FbarView& fv, double* f, double* kval, int precision, hipStream_t s,
                             int32_t* status);
 int32_t launch_permute_u(const DevPlan& p, const double* u, hipStream_t s, int32_t* status);
+int32_t launch_tet4_batch(const BatchDev& b, const double* const* u, const double* const* mu,
+                          const double* const* lam, double* const* f, double* const* kval, int precision,
+                          int cus, bool persistent, hipStream_t s, int32_t* status);
 int32_t launch_tet4_assemble_bound(const DevPlan& p, const double* u, const double* mu, const double* lam,
                                    double* f, double* kval, int precision, hipStream_t s, int32_t* status);
 int32_t launch_bind_material(const DevPlan& p, const double* mu, const double* lam, float4* out, hipStream_t s);

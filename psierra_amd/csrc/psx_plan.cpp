@@ -781,7 +781,7 @@ int32_t retile(psx_plan* plan, int32_t tile) {
 // ---------------------------------------------------------------------------
 extern "C" {
 
-int32_t psx_abi_version(void) { return 5; }
+int32_t psx_abi_version(void) { return 6; }
 
 const char* psx_last_error(void) { return psx::g_error.c_str(); }
 
@@ -846,7 +846,9 @@ int32_t psx_plan_pattern(const psx_plan* plan, int32_t* indptr, int32_t* indices
 int32_t psx_plan_set_tile(psx_plan* plan, int32_t tile) {
   psx::clear_error();
   if (!plan) return psx::fail(PSX_EINVAL, "plan is null");
-  return psx::retile(plan, tile);
+  const int32_t rc = psx::retile(plan, tile);
+  ++plan->version;
+  return rc;
 }
 
 int32_t psx_assemble(psx_plan* plan, const double* u, const double* mu, const double* lam, double* f,
@@ -914,6 +916,7 @@ int32_t psx_plan_bind_material(psx_plan* plan, const double* mu, const double* l
   const int32_t rc = psx::launch_bind_material(plan->d, mu, lam, const_cast<float4*>(plan->d.geo_bound), s);
   if (rc) return rc;
   plan->bound = true;
+  ++plan->version;
   return PSX_OK;
 }
 

@@ -63,8 +63,11 @@ struct IncInputs {
   double4 X[4];              // FP64 path: reference coordinates
 };
 
-template <typename R, int M>
-__device__ __forceinline__ void load_incidence(const DevPlan& p, const MatParams& mp, int k, IncInputs<R>& in) {
+// P is DevPlan (a kernel argument) or a constant-address-space DevPlan (a
+// batch's plan table): either way each field is read through the scalar unit
+// where it is used.
+template <typename R, int M, typename P>
+__device__ __forceinline__ void load_incidence(const P& p, const MatParams& mp, int k, IncInputs<R>& in) {
   in.pe = p.inc_elem[k];
   in.rank = p.inc_rank[k];
   const int64_t e = int64_t(in.pe & kIncElemMask);
@@ -118,16 +121,13 @@ __device__ __forceinline__ R incidence_row(const IncInputs<R>& in, R* out) {
 
 // f_int only (line-search trials): one lane per incidence, f_a summed in FP64
 // over the node's incidence run by one lane per (node, component).
-template <typename R, int TILE, int M>
-__global__ __launch_bounds__(TILE) void tet4_force_kernel(DevPlan p, MatParams mp, double* __restrict__ f,
-                                                          int32_t* status) {
+template <typename R, int TILE, int M, typename P>
+__device__ __forceinline__ void force_tile(const P& p, const MatParams& mp, double* __restrict__ f,
+                                           int32_t* status, int tile, int t) {
   __shared__ R contrib[3 * TILE];
   __shared__ int32_t s_incp[TILE + 1];
-  const int tile = xcd_tile();
-  if (tile >= p.n_tiles) return;
   const int4 ti = load_uniform(p.tile_info, tile);
   const int n0 = ti.x, nn = ti.y, k0 = ti.z, nk = ti.w;
-  const int t = threadIdx.x;
   IncInputs<R> in;
   if (t < nk) load_incidence<R, M>(p, mp, k0 + t, in);
   for (int i = t; i <= nn; i += TILE) s_incp[i] = p.inc_ptr[n0 + i] - k0;
@@ -146,6 +146,14 @@ __global__ __launch_bounds__(TILE) void tet4_force_kernel(DevPlan p, MatParams m
     for (int kk = s_incp[j]; kk < s_incp[j + 1]; ++kk) fa += double(contrib[c * TILE + kk]);
     f[3 * int64_t(p.perm[n0 + j]) + c] = fa;
   }
+}
+
+template <typename R, int TILE, int M>
+__global__ __launch_bounds__(TILE) void tet4_force_kernel(DevPlan p, MatParams mp, double* __restrict__ f,
+                                                          int32_t* status) {
+  const int tile = xcd_tile();
+  if (tile >= p.n_tiles) return;
+  force_tile<R, TILE, M>(p, mp, f, status, tile, int(threadIdx.x));
 }
 
 // Position in incidence_row's output of LDS value v: 0..5 the upper
@@ -181,10 +189,10 @@ __device__ __forceinline__ double oct_sum(double x, bool upper_quad) {
 // fv gives each node's first value, row length and diagonal slot there, and
 // each (node, slot) unit its slot in that row; the blocks are added to the
 // K_vol values already stored.
-template <typename R, int TILE, int M, typename MaskT>
-__global__ __launch_bounds__(TILE) void tet4_assemble_rows(DevPlan p, MatParams mp, FbarView fv,
-                                                           double* __restrict__ f, double* __restrict__ kval,
-                                                           int32_t* status) {
+template <typename R, int TILE, int M, typename MaskT, typename P>
+__device__ __forceinline__ void rows_tile(const P& p, const MatParams& mp, const FbarView& fv,
+                                          double* __restrict__ f, double* __restrict__ kval, int32_t* status,
+                                          int tile, int t) {
   constexpr bool FBAR = M == kFbarNeoHookean;
   constexpr int NV = 18;
   constexpr int ND = kTet4DiagLanes;
@@ -208,11 +216,8 @@ __global__ __launch_bounds__(TILE) void tet4_assemble_rows(DevPlan p, MatParams 
   __shared__ uint8_t s_first[TILE];
   __shared__ int32_t s_f2[FBAR ? 2 * NN : 1];   // F-bar: diagonal block offset, row length
 
-  const int tile = xcd_tile();
-  if (tile >= p.n_tiles) return;
   const int4 ti = load_uniform(p.tile_info, tile);
   const int n0 = ti.x, nn = ti.y, k0 = ti.z, nk = ti.w;
-  const int t = threadIdx.x;
   const int lane = t & 63, wave = t >> 6;
 
   IncInputs<R> in;
@@ -395,7 +400,15 @@ __global__ __launch_bounds__(TILE) void tet4_assemble_rows(DevPlan p, MatParams 
         for (int c = 0; c < 3; ++c) put(dst + r * 3 * d + c, acc[3 * r + c]);
     }
   }
+}
 
+template <typename R, int TILE, int M, typename MaskT>
+__global__ __launch_bounds__(TILE) void tet4_assemble_rows(DevPlan p, MatParams mp, FbarView fv,
+                                                           double* __restrict__ f, double* __restrict__ kval,
+                                                           int32_t* status) {
+  const int tile = xcd_tile();
+  if (tile >= p.n_tiles) return;
+  rows_tile<R, TILE, M, MaskT>(p, mp, fv, f, kval, status, tile, int(threadIdx.x));
 }
 
 // f_int + K_T for a wide plan (a node with more than 64 incident elements or
@@ -677,6 +690,199 @@ int32_t launch_tet4_assemble(const DevPlan& p, const double* u, const double* mu
                              double* f, double* kval, int precision, hipStream_t s, int32_t* status) {
   const double* params[4] = {mu, lam, nullptr, nullptr};
   return launch_tet4_hyper(p, kNeoHookean, params, u, f, kval, precision, s, status);
+}
+
+// ---------------------------------------------------------------------------
+// Batched assembly: many Tet4 plans (meshes of uneven size, or one mesh
+// repeated for parameter samples) in one persistent launch.  The tiles of
+// all plans form one list; a fixed grid of work-groups (as many as stay
+// resident) walks it, XCD x taking the x-th eighth of the list so
+// neighbouring tiles of one mesh share an L2.  Each tile is assembled by the
+// same tile routine as a single-plan launch, so every instance's f and K
+// are bit-identical to psx_assemble's.  Tiles hold at most TILE incidences
+// whatever the mesh size, so uneven instances balance by construction.
+
+// A work-group-uniform value (a plan view, a pointer, an index) read through
+// the scalar unit, dword by dword.
+template <typename T>
+__device__ __forceinline__ T load_uniform_val(const T* p) {
+  static_assert(sizeof(T) % 4 == 0 && std::is_trivially_copyable<T>::value, "dword-copyable type");
+  T out;
+  const auto* q = (const __attribute__((address_space(4))) uint32_t*)p;
+  uint32_t* o = reinterpret_cast<uint32_t*>(&out);
+#pragma unroll
+  for (int i = 0; i < int(sizeof(T) / 4); ++i) o[i] = q[i];
+  return out;
+}
+
+// A plan view in the constant address space: its fields are read through
+// the scalar unit where they are used, as a kernel argument's are.
+using ConstDevPlan = __attribute__((address_space(4))) DevPlan;
+
+// Instance owning global item g: the largest i with start[i] <= g
+// (start[0] = 0; an empty instance has the next one's start and is skipped).
+template <bool UNIFORM>
+__device__ __forceinline__ int batch_owner(const int32_t* start, int n, int g) {
+  int lo = 0, hi = n;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    const int v = UNIFORM ? load_uniform_val(start + mid) : start[mid];
+    if (v <= g)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// Displacements of every instance into its plan's internal node order, and
+// the per-instance status words reset.
+__global__ __launch_bounds__(256) void batch_permute_kernel(BatchDev b, const double* const* __restrict__ u,
+                                                            int32_t* status) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < b.n) {
+    status[2 * i] = 0;
+    status[2 * i + 1] = INT_MAX;
+  }
+  if (i >= b.n_nodes) return;
+  const int inst = batch_owner<false>(b.node_start, b.n, i);
+  const int k = i - b.node_start[inst];
+  const DevPlan& p = b.plans[inst];
+  const double* ui = u[inst];
+  const int64_t a = p.perm[k];
+  p.ui[k] = make_double4(ui[3 * a], ui[3 * a + 1], ui[3 * a + 2], 0.0);
+}
+
+// Tiles of a batched launch.  PERSISTENT: work-group w of XCD x takes tiles
+// x * per + w, x * per + w + P, ... of the global list (P work-groups per
+// XCD, as many as stay resident); the thread index is passed through an
+// empty asm so that the compiler recomputes what derives from it in each
+// tile instead of keeping it live across the loop (registers, occupancy).
+// Otherwise one tile per work-group, XCD-contiguous as in a single-plan
+// launch.
+template <bool PERSISTENT, typename Body>
+__device__ __forceinline__ void batch_tiles(const BatchDev& b, Body body) {
+  if constexpr (PERSISTENT) {
+    const int per = (b.n_tiles + 7) >> 3;
+    const int x = int(blockIdx.x & 7u), w0 = int(blockIdx.x >> 3), P = int(gridDim.x >> 3);
+    for (int w = w0; w < per; w += P) {
+      const int g = x * per + w;
+      if (g >= b.n_tiles) break;
+      const int inst = batch_owner<true>(b.tile_start, b.n, g);
+      if (w != w0) __syncthreads();   // the previous tile's LDS reads are done
+      int t = int(threadIdx.x);
+      __asm__ volatile("" : "+v"(t));
+      body(inst, g - load_uniform_val(b.tile_start + inst), t);
+    }
+  } else {
+    const int g = xcd_tile();
+    if (g >= b.n_tiles) return;
+    const int inst = batch_owner<true>(b.tile_start, b.n, g);
+    body(inst, g - load_uniform_val(b.tile_start + inst), int(threadIdx.x));
+  }
+}
+
+template <typename R, int TILE, int M, typename MaskT, bool PERSISTENT>
+__global__ __launch_bounds__(TILE) void tet4_batch_rows(BatchDev b, const double* const* mu,
+                                                        const double* const* lam, double* const* f,
+                                                        double* const* kval, int32_t* status) {
+  batch_tiles<PERSISTENT>(b, [&](int inst, int tile, int t) {
+    const ConstDevPlan& p = *(const ConstDevPlan*)(b.plans + inst);
+    MatParams mp{{nullptr, nullptr, nullptr, nullptr}};
+    if constexpr (M != kNeoHookeanBound) {
+      mp.v[0] = load_uniform_val(mu + inst);
+      mp.v[1] = load_uniform_val(lam + inst);
+    }
+    rows_tile<R, TILE, M, MaskT>(p, mp, FbarView{}, load_uniform_val(f + inst), load_uniform_val(kval + inst),
+                                 status + 2 * inst, tile, t);
+  });
+}
+
+template <typename R, int TILE, int M, bool PERSISTENT>
+__global__ __launch_bounds__(TILE) void tet4_batch_force(BatchDev b, const double* const* mu,
+                                                         const double* const* lam, double* const* f,
+                                                         int32_t* status) {
+  batch_tiles<PERSISTENT>(b, [&](int inst, int tile, int t) {
+    const ConstDevPlan& p = *(const ConstDevPlan*)(b.plans + inst);
+    MatParams mp{{nullptr, nullptr, nullptr, nullptr}};
+    if constexpr (M != kNeoHookeanBound) {
+      mp.v[0] = load_uniform_val(mu + inst);
+      mp.v[1] = load_uniform_val(lam + inst);
+    }
+    force_tile<R, TILE, M>(p, mp, load_uniform_val(f + inst), status + 2 * inst, tile, t);
+  });
+}
+
+// Grid of a batched launch: persistent, as many work-groups as stay
+// resident on the device (never more than there are tiles); otherwise one per
+// tile.  A multiple of 8 either way (one share per XCD).
+static int batch_grid(const void* kernel, int threads, int cus, int n_tiles, bool persistent) {
+  int64_t g = n_tiles;
+  if (persistent) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess || per_cu < 1)
+      per_cu = 1;
+    g = std::min<int64_t>(int64_t(cus) * per_cu, n_tiles);
+  }
+  return int(8 * ((g + 7) / 8));
+}
+
+template <typename R, int TILE, int M, bool PERSISTENT>
+static void launch_batch_t4(const BatchDev& b, const double* const* mu, const double* const* lam,
+                            double* const* f, double* const* kval, int cus, hipStream_t s, int32_t* status) {
+  if (kval && b.max_inc <= 32) {
+    const auto k = tet4_batch_rows<R, TILE, M, uint32_t, PERSISTENT>;
+    const int grid = batch_grid(reinterpret_cast<const void*>(k), TILE, cus, b.n_tiles, PERSISTENT);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(TILE), 0, s, b, mu, lam, f, kval, status);
+  } else if (kval) {
+    const auto k = tet4_batch_rows<R, TILE, M, unsigned long long, PERSISTENT>;
+    const int grid = batch_grid(reinterpret_cast<const void*>(k), TILE, cus, b.n_tiles, PERSISTENT);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(TILE), 0, s, b, mu, lam, f, kval, status);
+  } else {
+    const auto k = tet4_batch_force<R, TILE, M, PERSISTENT>;
+    const int grid = batch_grid(reinterpret_cast<const void*>(k), TILE, cus, b.n_tiles, PERSISTENT);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(TILE), 0, s, b, mu, lam, f, status);
+  }
+}
+
+template <typename R, int M>
+static int32_t launch_batch_tiles(const BatchDev& b, const double* const* mu, const double* const* lam,
+                                  double* const* f, double* const* kval, int cus, bool persistent,
+                                  hipStream_t s, int32_t* status) {
+  switch (b.tile * 2 + int(persistent)) {
+    case 128: launch_batch_t4<R, 64, M, false>(b, mu, lam, f, kval, cus, s, status); break;
+    case 129: launch_batch_t4<R, 64, M, true>(b, mu, lam, f, kval, cus, s, status); break;
+    case 256: launch_batch_t4<R, 128, M, false>(b, mu, lam, f, kval, cus, s, status); break;
+    case 257: launch_batch_t4<R, 128, M, true>(b, mu, lam, f, kval, cus, s, status); break;
+    case 512: launch_batch_t4<R, 256, M, false>(b, mu, lam, f, kval, cus, s, status); break;
+    case 513: launch_batch_t4<R, 256, M, true>(b, mu, lam, f, kval, cus, s, status); break;
+    default: return fail(PSX_EINVAL, "unsupported tile");
+  }
+  return PSX_OK;
+}
+
+// u, mu, lam, f, kval: device arrays of n device pointers; mu = lam = NULL
+// for the plans' bound material (FP32: the bound geometry records; FP64:
+// the plans' copies of mu and lambda); kval = NULL for f only.
+int32_t launch_tet4_batch(const BatchDev& b, const double* const* u, const double* const* mu,
+                          const double* const* lam, double* const* f, double* const* kval, int precision,
+                          int cus, bool persistent, hipStream_t s, int32_t* status) {
+  const int total = std::max(b.n_nodes, b.n);
+  hipLaunchKernelGGL(batch_permute_kernel, dim3((total + 255) / 256), dim3(256), 0, s, b, u, status);
+  PSX_HIP_TRY(hipGetLastError());
+  if (b.n_tiles <= 0) return PSX_OK;
+  int32_t rc;
+  if (!mu && precision == PSX_FP32)
+    rc = launch_batch_tiles<float, kNeoHookeanBound>(b, nullptr, nullptr, f, kval, cus, persistent, s, status);
+  else if (!mu)
+    rc = launch_batch_tiles<double, kNeoHookean>(b, b.bound_mu, b.bound_lam, f, kval, cus, persistent, s, status);
+  else if (precision == PSX_FP32)
+    rc = launch_batch_tiles<float, kNeoHookean>(b, mu, lam, f, kval, cus, persistent, s, status);
+  else
+    rc = launch_batch_tiles<double, kNeoHookean>(b, mu, lam, f, kval, cus, persistent, s, status);
+  if (rc) return rc;
+  PSX_HIP_TRY(hipGetLastError());
+  return PSX_OK;
 }
 
 // ---------------------------------------------------------------------------

@@ -288,6 +288,127 @@ class AssemblyPlan:
             raise ValueError(f"{name} must have {n} entries on {self.device}")
 
 
+BATCH_INFO_KEYS = ("n_instances", "n_tiles", "n_nodes", "tile", "persistent", "nnz")
+
+
+class AssemblyBatch:
+    """Many Tet4 plans assembled by one launch (``psx_assemble_batch``).
+
+    ``plans``: ``AssemblyPlan`` objects of one device and tile size, meshes
+    of uneven size or one plan repeated for parameter samples (each instance
+    has its own displacement buffer).  The batch keeps the plans alive.
+    ``assemble`` takes one displacement (and, unless the plans' bound
+    materials are used, one mu and lambda) per instance and returns the
+    per-instance f and K values, each bit-identical to that plan's own
+    ``assemble``.  ``persistent`` selects a persistent grid walking the
+    global tile list instead of one work-group per tile (the default, faster
+    as measured).  No reference counterpart: it stands for a loop
+    of ``SolidDomain`` assemblies over many domains or parameter samples
+    (src/psierra/domain/solid.py:230-325).
+    """
+
+    def __init__(self, plans, persistent: bool = False):
+        torch = _torch()
+        self.plans = list(plans)
+        if not self.plans:
+            raise ValueError("AssemblyBatch needs at least one plan")
+        self.device = self.plans[0].device
+        handles = (ctypes.c_void_p * len(self.plans))(*[p._h.value for p in self.plans])
+        h = ctypes.c_void_p()
+        check(lib().psx_batch_create(handles, len(self.plans), ctypes.byref(h)))
+        self._h = h
+        self.status = torch.zeros(2 * len(self.plans), dtype=torch.int32, device=self.device)
+        self.set_persistent(persistent)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                lib().psx_batch_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    def __len__(self):
+        return len(self.plans)
+
+    def set_persistent(self, persistent: bool) -> None:
+        check(lib().psx_batch_set_persistent(self._h, int(bool(persistent))))
+        self.refresh_info()
+
+    def refresh_info(self):
+        buf = (ctypes.c_int64 * len(BATCH_INFO_KEYS))()
+        check(lib().psx_batch_info(self._h, buf, len(BATCH_INFO_KEYS)))
+        self.info = dict(zip(BATCH_INFO_KEYS, (int(v) for v in buf)))
+
+    def alloc_outputs(self, want_k: bool = True):
+        outs = [p.alloc_outputs(want_k) for p in self.plans]
+        return [o[0] for o in outs], [o[1] for o in outs]
+
+    def _table(self, tensors):
+        torch = _torch()
+        return torch.tensor([t.data_ptr() for t in tensors], dtype=torch.int64, device=self.device)
+
+    def assemble(self, us, mus=None, lams=None, fs=None, ks=None, precision="fp64", want_k=True,
+                 check_status=True):
+        """Per-instance lists (f, K values); ``mus = lams = None`` for the
+        plans' bound materials.  Launches on the current stream; with
+        ``check_status`` synchronises and raises the reference's ValueError
+        for the first instance with an inverted element."""
+        n = len(self.plans)
+        if len(us) != n:
+            raise ValueError(f"{len(us)} displacements for {n} instances")
+        if (mus is None) != (lams is None):
+            raise ValueError("give both mus and lams, or neither (bound materials)")
+        us = [p._device_f64(u, p.n_dofs, "u") for p, u in zip(self.plans, us)]
+        if mus is not None:
+            if len(mus) != n or len(lams) != n:
+                raise ValueError(f"mus and lams need {n} entries")
+            mus = [p._device_f64(m, p.n_elements, "mu") for p, m in zip(self.plans, mus)]
+            lams = [p._device_f64(m, p.n_elements, "lam") for p, m in zip(self.plans, lams)]
+        if fs is None or (want_k and ks is None):
+            f0, k0 = self.alloc_outputs(want_k)
+            fs = fs if fs is not None else f0
+            ks = ks if ks is not None else k0
+        for p, f in zip(self.plans, fs):
+            p._check_out(f, p.n_dofs, "f")
+        if want_k:
+            for p, k in zip(self.plans, ks):
+                p._check_out(k, p.nnz, "kval")
+        else:
+            ks = None
+        self.launcher(us, mus, lams, fs, ks)(precision)
+        if check_status:
+            self.raise_on_status(us)
+        return fs, (ks if want_k else [None] * n)
+
+    def launcher(self, us, mus, lams, fs, ks):
+        """A launch of ``psx_assemble_batch`` over fixed device buffers (the
+        arguments as ``assemble`` checks them, already on the device): the
+        pointer tables are built now, so ``launch(precision)`` makes no copy
+        and no allocation and can be captured in a HIP graph."""
+        tables = [self._table(us), self._table(mus) if mus is not None else None,
+                  self._table(lams) if lams is not None else None, self._table(fs),
+                  self._table(ks) if ks is not None else None]
+        args = [ptr(t) for t in tables]
+
+        def launch(precision="fp64"):
+            # ``tables`` stays referenced by this closure while it can run
+            check(lib().psx_assemble_batch(self._h, *args, precision_code(precision), current_stream_handle(),
+                                           ptr(self.status)))
+            return tables
+
+        return launch
+
+    def raise_on_status(self, us):
+        st = self.status.cpu().numpy().reshape(-1, 2)
+        for i, (flag, e) in enumerate(st):
+            if flag != 0:
+                p = self.plans[i]
+                uh = us[i].reshape(-1, 3)
+                raise inversion_error(p.nodes, p.elements, lambda conn: uh[conn].cpu().numpy(), int(e))
+
+
 FBAR_INVERSION_MSG = "Element inversion detected at element {}: det(F) = {:.4e} <= 0."
 
 

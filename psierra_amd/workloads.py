@@ -211,6 +211,36 @@ def vjp_box_tet4(n: int = 48, seed: int = 3) -> Workload:
     return wl
 
 
+def batch_boxes_tet4(
+    n_instances: int = 128, seed: int = 5, side_range: tuple[int, int] = (8, 48), grad_target: float = 0.1
+) -> list[Workload]:
+    """Seeded meshes of uneven size for batched assembly: box i has sides
+    s ~ U{lo..hi} per axis (h = 1/max side), Kuhn Tet4 with jittered interior
+    nodes and permuted node ids as in ``jittered_box_tet4``, one mu ~
+    logU[3e2, 3e3] per instance with lam = 5 mu, and the smooth displacement
+    field scaled so that max ||grad u|| = grad_target."""
+    out = []
+    for i, child in enumerate(np.random.SeedSequence(seed).spawn(n_instances)):
+        rng = np.random.default_rng(child)
+        sx, sy, sz = (int(v) for v in rng.integers(side_range[0], side_range[1] + 1, size=3))
+        h = 1.0 / max(sx, sy, sz)
+        X0 = box_grid_nodes(sx, sy, sz, sx * h, sy * h, sz * h)
+        conn0 = box_tet4_connectivity(sx, sy, sz)
+        X, p = _jitter_and_permute(rng, X0, conn0, h)
+        conn = p[conn0]
+        E = conn.shape[0]
+        mu = float(np.exp(rng.uniform(np.log(3e2), np.log(3e3))))
+        u = _smooth_field(rng, X, sx * h)
+        u *= grad_target / tet4_max_grad_norm(X, conn, u)
+        out.append(
+            Workload(
+                f"batch_box_tet4_{sx}x{sy}x{sz}", TetMesh(X, conn), u, np.full(E, mu), np.full(E, 5.0 * mu),
+                {"sides": (sx, sy, sz), "instance": i},
+            )
+        )
+    return out
+
+
 def ecsw_instances(
     n_instances: int = 128,
     seed: int = 4,
