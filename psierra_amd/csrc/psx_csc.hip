@@ -9,11 +9,14 @@
 // of b's s-th neighbour a (ascending), whose three values K[3a+r][3b..3b+2]
 // are consecutive in the CSR values (one 24-byte read serves the three
 // columns).  A ballot per column marks the kept entries (free row, and
-// nonzero when exact zeros are dropped, as the CSR '+' does); a first pass
-// counts them per column, a scan gives the column pointers, and a second
-// pass writes each column's rows in ascending order at the positions the
-// ballots give.  Everything is in a fixed order: the output is the same on
-// every run.
+// nonzero when exact zeros are dropped, as the CSR '+' does).  The first
+// pass writes every structural entry at its structural position (column
+// starts fixed at creation) and counts the kept entries per column; a scan
+// gives the column pointers; only when the scanned total is below the
+// structural count (an exact zero was dropped) does a second pass rewrite
+// each column's kept rows in ascending order at the positions the ballots
+// give.  Everything is in a fixed order: the output is the same on every
+// run.
 //
 // psx_free_residual forms r = f_ext_free - f_int[free]
 // (nonlinear_static.py:322-323) and sum(r^2) in FP64 with a fixed-order
@@ -40,32 +43,29 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int* sh) {
   return incl - v;
 }
 
-// One wave per column node b, taken in locality order.  FILL = false: the kept entries of each free
-// column of b into colptr[column].  FILL = true: colptr[column] holds the
-// column's offset within its 256-column block and blockoff the blocks'
-// offsets; the column's rows and values are written and colptr[column] is
-// set to the final start.
-template <bool FILL>
-__global__ __launch_bounds__(kCscBlock) void csc_node_kernel(DevCsc c, const double* __restrict__ kval, int drop,
-                                                             const int* __restrict__ blockoff,
-                                                             int* __restrict__ colptr, int* __restrict__ rowind,
-                                                             double* __restrict__ vals) {
-  const int lane = threadIdx.x & 63;
-  const int bi = blockIdx.x * (kCscBlock / 64) + int(threadIdx.x >> 6);
-  if (bi >= c.N) return;
-  const int b = c.order[bi];
+// Column node b's free columns (one wave): kStructFill writes every
+// structural entry (free row) at its structural position (c.sstart: the
+// column starts with no entry dropped) and colptr[column] receives the count
+// of kept entries (nonzero, when exact zeros are dropped); when nothing is
+// dropped -- the scan of the counts then ends at the structural count -- this
+// is the whole output.  kCompactFill (only when something was dropped)
+// rewrites the kept rows and values from colptr[column], the final start.
+enum CscPass { kStructFill = 0, kCompactFill = 1 };
+
+template <int PASS>
+__device__ __forceinline__ void csc_node(const DevCsc& c, int b, int lane, const double* __restrict__ kval,
+                                         int drop, int* __restrict__ colptr, int* __restrict__ rowind,
+                                         double* __restrict__ vals) {
   int jc[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) jc[k] = c.newidx[3 * b + k];
   if (jc[0] < 0 && jc[1] < 0 && jc[2] < 0) return;
   const int u0 = c.adj_ptr[b], items = 3 * (c.adj_ptr[b + 1] - u0);
   int start[3] = {0, 0, 0};
-  if (FILL) {
 #pragma unroll
-    for (int k = 0; k < 3; ++k)
-      if (jc[k] >= 0) start[k] = colptr[jc[k]] + blockoff[jc[k] / kCscBlock];
-  }
-  int kept[3] = {0, 0, 0};
+  for (int k = 0; k < 3; ++k)
+    if (jc[k] >= 0) start[k] = PASS == kStructFill ? c.sstart[jc[k]] : colptr[jc[k]];
+  int kept[3] = {0, 0, 0}, placed[3] = {0, 0, 0};
   const unsigned long long below = (1ull << lane) - 1ull;
   for (int base = 0; base < items; base += 64) {
     const int l = base + lane;
@@ -84,9 +84,18 @@ __global__ __launch_bounds__(kCscBlock) void csc_node_kernel(DevCsc c, const dou
     }
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      const bool keep = ri >= 0 && jc[k] >= 0 && (!drop || v[k] != 0.0);
+      const bool structural = ri >= 0 && jc[k] >= 0;
+      const bool keep = structural && (!drop || v[k] != 0.0);
       const unsigned long long m = __ballot(keep);
-      if (FILL && keep) {
+      if (PASS == kStructFill) {
+        const unsigned long long ms = __ballot(structural);
+        if (structural) {
+          const int o = start[k] + placed[k] + __popcll(ms & below);
+          rowind[o] = ri;
+          vals[o] = v[k];
+        }
+        placed[k] += __popcll(ms);
+      } else if (keep) {
         const int o = start[k] + kept[k] + __popcll(m & below);
         rowind[o] = ri;
         vals[o] = v[k];
@@ -94,11 +103,40 @@ __global__ __launch_bounds__(kCscBlock) void csc_node_kernel(DevCsc c, const dou
       kept[k] += __popcll(m);
     }
   }
-  if (lane == 0) {
+  if (PASS == kStructFill && lane == 0) {
 #pragma unroll
     for (int k = 0; k < 3; ++k)
-      if (jc[k] >= 0) colptr[jc[k]] = FILL ? start[k] : kept[k];
+      if (jc[k] >= 0) colptr[jc[k]] = kept[k];
   }
+}
+
+// One wave per column node, taken in locality order.
+__global__ __launch_bounds__(kCscBlock) void csc_struct_kernel(DevCsc c, const double* __restrict__ kval, int drop,
+                                                               int* __restrict__ colptr, int* __restrict__ rowind,
+                                                               double* __restrict__ vals) {
+  const int bi = blockIdx.x * (kCscBlock / 64) + int(threadIdx.x >> 6);
+  if (bi >= c.N) return;
+  csc_node<kStructFill>(c, c.order[bi], threadIdx.x & 63, kval, drop, colptr, rowind, vals);
+}
+
+// kCscCompactNodes column nodes per wave; every wave returns at once when
+// nothing was dropped.
+constexpr int kCscCompactNodes = 16;
+__global__ __launch_bounds__(kCscBlock) void csc_compact_kernel(DevCsc c, const double* __restrict__ kval, int drop,
+                                                                int* __restrict__ colptr, int* __restrict__ rowind,
+                                                                double* __restrict__ vals) {
+  if (colptr[c.n_free] == c.nnz_struct) return;
+  const int w = blockIdx.x * (kCscBlock / 64) + int(threadIdx.x >> 6);
+  const int b0 = w * kCscCompactNodes, b1 = min(c.N, b0 + kCscCompactNodes);
+  for (int bi = b0; bi < b1; ++bi) csc_node<kCompactFill>(c, c.order[bi], threadIdx.x & 63, kval, drop, colptr, rowind, vals);
+}
+
+// colptr[j] += the offset of its 256-column block (the block-local prefix
+// becomes the column's start).
+__global__ __launch_bounds__(kCscBlock) void csc_scan_add_kernel(int n_free, const int* __restrict__ blockoff,
+                                                                 int* __restrict__ colptr) {
+  const int j = blockIdx.x * kCscBlock + threadIdx.x;
+  if (j < n_free) colptr[j] += blockoff[j / kCscBlock];
 }
 
 // colptr[j] (the column counts) -> the exclusive prefix within each block of
@@ -140,13 +178,18 @@ int32_t launch_csc_extract(const DevCsc& c, const double* kval, int drop, int* c
     PSX_HIP_TRY(hipGetLastError());
     return PSX_OK;
   }
-  const int node_blocks = (c.N + kCscBlock / 64 - 1) / (kCscBlock / 64);
-  hipLaunchKernelGGL(csc_node_kernel<false>, dim3(node_blocks), dim3(kCscBlock), 0, s, c, kval, drop,
-                     static_cast<const int*>(nullptr), colptr, rowind, vals);
+  // structural fill and kept counts; scan of the counts; the compacting fill
+  // only if the scanned total shows that entries were dropped
+  constexpr int waves = kCscBlock / 64;
+  const int node_blocks = (c.N + waves - 1) / waves;
+  const int compact_blocks = (c.N + waves * kCscCompactNodes - 1) / (waves * kCscCompactNodes);
+  hipLaunchKernelGGL(csc_struct_kernel, dim3(node_blocks), dim3(kCscBlock), 0, s, c, kval, drop, colptr, rowind,
+                     vals);
   hipLaunchKernelGGL(csc_scan_local_kernel, dim3(nb), dim3(kCscBlock), 0, s, c.n_free, colptr, c.scratch);
   hipLaunchKernelGGL(csc_scan_blocks_kernel, dim3(1), dim3(kCscBlock), 0, s, nb, c.scratch, colptr, c.n_free);
-  hipLaunchKernelGGL(csc_node_kernel<true>, dim3(node_blocks), dim3(kCscBlock), 0, s, c, kval, drop, c.scratch,
-                     colptr, rowind, vals);
+  hipLaunchKernelGGL(csc_scan_add_kernel, dim3(nb), dim3(kCscBlock), 0, s, c.n_free, c.scratch, colptr);
+  hipLaunchKernelGGL(csc_compact_kernel, dim3(compact_blocks), dim3(kCscBlock), 0, s, c, kval, drop, colptr,
+                     rowind, vals);
   PSX_HIP_TRY(hipGetLastError());
   return PSX_OK;
 }

@@ -154,6 +154,8 @@ struct DevCsc {
   const int32_t* adj_ptr;   // N + 1: node b's neighbours a (ascending) are units adj_ptr[b]..
   const int4* unit;         // per unit (b, a): {K position of K[3a][3b], row stride 3 deg(a), a, 0}
   const int32_t* newidx;    // 3N: free index of each dof, -1 if prescribed
+  const int32_t* sstart;    // n_free: structural start of each column (no entry dropped)
+  int32_t nnz_struct;       // structural entries of K[free][:, free]
   const int32_t* free;      // free dof ids (ascending)
   int32_t* scratch;         // per-block counts / offsets
   double* partial;          // per-block residual partial sums

@@ -728,6 +728,23 @@ int32_t create_csc_adj(const std::vector<int32_t>& adj_ptr, const std::vector<in
     }
   }
   if (nnz_struct >= (int64_t(1) << 31)) return fail(PSX_EINVAL, "free-dof matrix has 2^31 or more entries");
+  // structural column starts: free columns in dof order, each with the free
+  // rows of its node's neighbours
+  std::vector<int32_t> sstart(size_t(std::max<int64_t>(n_free, 1)));
+  {
+    int64_t pos = 0;
+    for (int64_t b = 0; b < N; ++b) {
+      int64_t rows = 0;
+      for (int32_t t = hp.adj_ptr[b]; t < hp.adj_ptr[b + 1]; ++t)
+        for (int r = 0; r < 3; ++r) rows += newidx[3 * int64_t(hp.adj[t]) + r] >= 0;
+      for (int c = 0; c < 3; ++c) {
+        const int32_t k = newidx[3 * b + c];
+        if (k < 0) continue;
+        sstart[k] = int32_t(pos);
+        pos += rows;
+      }
+    }
+  }
   auto* csc = new psx_csc();
   auto bail = [&](int32_t rc) {
     for (void* p : csc->allocations) (void)hipFree(p);
@@ -744,13 +761,14 @@ int32_t create_csc_adj(const std::vector<int32_t>& adj_ptr, const std::vector<in
     return PSX_OK;
   };
   if (int64_t(order.size()) != N) return bail(fail(PSX_EINVAL, "node order has the wrong length"));
-  void *dord, *dptr, *dunit, *dnew, *dfree, *dscr, *dpart;
+  void *dord, *dptr, *dunit, *dnew, *dfree, *dscr, *dpart, *dss;
   const size_t nb = size_t((n_free + 255) / 256) + 1;
   int32_t rc;
   if ((rc = up(order.data(), size_t(N) * 4, &dord)) || (rc = up(hp.adj_ptr.data(), size_t(N + 1) * 4, &dptr)) ||
       (rc = up(unit.data(), size_t(n_units) * sizeof(int4), &dunit)) ||
       (rc = up(newidx.data(), newidx.size() * 4, &dnew)) || (rc = up(freev.data(), freev.size() * 4, &dfree)) ||
-      (rc = up(nullptr, nb * 4, &dscr)) || (rc = up(nullptr, nb * 8, &dpart)))
+      (rc = up(nullptr, nb * 4, &dscr)) || (rc = up(nullptr, nb * 8, &dpart)) ||
+      (rc = up(sstart.data(), sstart.size() * 4, &dss)))
     return bail(rc);
   csc->d.n_free = int32_t(n_free);
   csc->d.N = int32_t(N);
@@ -761,6 +779,8 @@ int32_t create_csc_adj(const std::vector<int32_t>& adj_ptr, const std::vector<in
   csc->d.free = static_cast<int32_t*>(dfree);
   csc->d.scratch = static_cast<int32_t*>(dscr);
   csc->d.partial = static_cast<double*>(dpart);
+  csc->d.sstart = static_cast<int32_t*>(dss);
+  csc->d.nnz_struct = int32_t(nnz_struct);
   *out = csc;
   return PSX_OK;
 }

@@ -96,6 +96,44 @@ def test_csc_drops_exact_zeros_like_scipy():
     assert kept.nnz > got.nnz
 
 
+def test_csc_structural_pass_and_compaction_in_turn():
+    """The extraction writes every structural entry in one pass and rewrites
+    only when an exact zero was dropped: with no zeros, with zeros (dropped,
+    and kept with drop_zeros=False: scipy's indexing keeps explicit zeros),
+    and with no zeros again on the same object, each output equals scipy's."""
+    import scipy.sparse as spsp
+    import torch
+
+    from psierra_amd.engine import AssemblyPlan
+    from psierra_amd.solver import FreeDofSystem
+
+    wl = W.jittered_box_tet4(n=5, seed=8)
+    plan = AssemblyPlan(wl.mesh.nodes, wl.mesh.elements)
+    dev = plan.device
+    n = plan.n_dofs
+    free = _clamped(wl).free_dofs
+    u, mu, lam = (torch.as_tensor(x, device=dev) for x in (wl.u.ravel(), wl.mu, wl.lam))
+    banner("csc structural pass and compaction")
+    _, k = plan.assemble(u, mu, lam)
+    indptr, indices = plan.pattern()
+    sysm = FreeDofSystem(plan, free)
+
+    def host(kv):
+        return spsp.csr_matrix((kv.cpu().numpy(), indices, indptr), shape=(n, n))
+
+    def same(got, ref):
+        return (np.array_equal(got.indptr, ref.indptr) and np.array_equal(got.indices, ref.indices)
+                and np.array_equal(got.data, ref.data))
+
+    assert int((k == 0).sum()) == 0
+    assert same(sysm.csc(k), A.free_dof_csc(host(k), free))
+    kz = k.clone()
+    kz[torch.as_tensor(np.random.default_rng(4).choice(plan.nnz, 300, replace=False), device=dev)] = 0.0
+    assert same(sysm.csc(kz), A.free_dof_csc(host(kz), free))
+    assert same(sysm.csc(kz, drop_zeros=False), host(kz)[free, :][:, free].tocsc())
+    assert same(sysm.csc(k), A.free_dof_csc(host(k), free))
+
+
 def test_loaded_cantilever_solve_matches_oracle():
     from psierra_amd.analysis import NonlinearStaticAnalysis
     from psierra_amd.boundary import select_nodes_by_plane
