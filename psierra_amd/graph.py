@@ -11,7 +11,9 @@ new iterate into ``u`` and replay.
 ``psierra_amd.analysis``).
 
 ``autotune_tile`` times the work-group tile sizes the kernels are built for
-on the caller's own plan and inputs and keeps the fastest.
+on the caller's own plan and inputs and keeps the fastest; ``autotune_batch``
+does the same for a batch's launch mode (persistent grid or one work-group
+per tile).
 """
 from __future__ import annotations
 
@@ -110,4 +112,33 @@ def autotune_tile(plan: AssemblyPlan, u, mu, lam, precision: str = "fp32", reps:
     best = min(timings, key=timings.get)
     plan.set_tile(best)
     plan.raise_on_status(plan._device_f64(u, plan.n_dofs, "u"))
+    return best, timings
+
+
+def autotune_batch(batch, us, mus=None, lams=None, precision: str = "fp32", reps: int = 5):
+    """Pick the faster launch mode of an ``AssemblyBatch`` on the caller's
+    inputs (the results are the same bits either way); returns
+    (persistent, {mode: ms})."""
+    torch = _torch()
+    fs, ks = batch.assemble(us, mus, lams, precision=precision)   # checks the inputs and the status
+    us = [p._device_f64(u, p.n_dofs, "u") for p, u in zip(batch.plans, us)]
+    if mus is not None:
+        mus = [p._device_f64(m, p.n_elements, "mu") for p, m in zip(batch.plans, mus)]
+        lams = [p._device_f64(m, p.n_elements, "lam") for p, m in zip(batch.plans, lams)]
+    launch = batch.launcher(us, mus, lams, fs, ks)
+    timings = {}
+    for persistent in (False, True):
+        batch.set_persistent(persistent)
+        launch(precision)
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(reps):
+            launch(precision)
+        b.record()
+        torch.cuda.synchronize()
+        timings["persistent" if persistent else "tile_per_group"] = a.elapsed_time(b) / reps
+    best = timings["persistent"] < timings["tile_per_group"]
+    batch.set_persistent(best)
+    batch.raise_on_status(us)
     return best, timings

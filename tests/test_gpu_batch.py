@@ -216,3 +216,24 @@ def test_batch_graph_replay(meshes):
         torch.cuda.synchronize()
         for i in range(len(plans)):
             assert torch.equal(fs[i], eager_f[i]) and torch.equal(ks[i], eager_k[i]), (rep, i)
+
+
+def test_batch_autotune_keeps_results(meshes):
+    import torch
+
+    from psierra_amd.engine import AssemblyBatch
+    from psierra_amd.graph import autotune_batch
+
+    wls, plans = meshes
+    us, mus, lams = _inputs(wls, plans, seed=6)
+    batch = AssemblyBatch(plans)
+    banner("batch: autotune of the launch mode")
+    f0, k0 = batch.assemble(us, mus, lams, precision="fp32")
+    f0 = [f.clone() for f in f0]
+    k0 = [k.clone() for k in k0]
+    best, timings = autotune_batch(batch, us, mus, lams, precision="fp32", reps=2)
+    assert set(timings) == {"persistent", "tile_per_group"} and batch.info["persistent"] == int(best)
+    fs, ks = batch.assemble(us, mus, lams, precision="fp32")
+    torch.cuda.synchronize()
+    for i in range(len(plans)):
+        assert torch.equal(fs[i], f0[i]) and torch.equal(ks[i], k0[i]), i
