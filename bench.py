@@ -781,6 +781,47 @@ def bench_batch(args):
     }
 
 
+def bench_hooks(args):
+    """The drop-in element hooks at the headline size, host arrays in and out
+    (PCIe-inclusive): pSierra's SolidDomain calls assemble_internal_forces and
+    then assemble_tangent_stiffness_matrix at each Newton iterate
+    (src/psierra/analysis/nonlinear_static.py:296-297); each step here is that
+    pair at a new displacement: u to the device, the f-only launch, f back,
+    the fused f + K launch, f and the 1.1 GB of K values back into a scipy
+    CSR matrix."""
+    import torch
+
+    from psierra_amd import workloads as W
+    from psierra_amd.element import GpuTet4TotalLagrangian
+    from psierra_amd.material import NeoHookeanField
+
+    wl = W.jittered_box_tet4(n=args.n or 100, seed=1)
+    el = GpuTet4TotalLagrangian(NeoHookeanField(wl.mu, wl.lam), precision="fp32")
+    el.assemble_internal_forces(wl.mesh, wl.u)   # plan, material binding, tile choice
+    times = []
+    for step in range(args.warmup + args.steps):
+        u = wl.u * (1.0 + 1e-3 * (step + 1))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        f = el.assemble_internal_forces(wl.mesh, u)
+        K = el.assemble_tangent_stiffness_matrix(wl.mesh, u)
+        t1 = time.perf_counter()
+        if step >= args.warmup:
+            times.append(t1 - t0)
+    ms = 1e3 * float(np.mean(times))
+    E = wl.mesh.n_elements
+    return {
+        "metric": "drop-in element hooks f_int then K_T (host in/out, PCIe-inclusive), elements/s",
+        "value": E / (ms * 1e-3), "unit": "elements/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic (seeded jittered Kuhn box, permuted node ids, lognormal mu_e)",
+        "config": {"workload": wl.name, "elements": E, "nnz": int(K.nnz), "f_dofs": int(f.size),
+                   "ms_min": 1e3 * min(times), "ms_max": 1e3 * max(times),
+                   "host_bytes_per_step": int(2 * 8 * f.size + 8 * K.nnz + 8 * f.size)},
+        "roofline": None, "cpu_baseline": None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -794,7 +835,8 @@ def main():
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--workload",
-                    choices=("tet4", "tet10", "vjp", "ecsw", "ecsw_train", "fbar", "csc", "mass", "newton", "batch"),
+                    choices=("tet4", "tet10", "vjp", "ecsw", "ecsw_train", "fbar", "csc", "mass", "newton", "batch",
+                             "hooks"),
                     default="tet4")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -806,7 +848,7 @@ def main():
     if args.workload != "tet4":
         fn = {"tet10": bench_tet10, "vjp": bench_vjp, "ecsw": bench_ecsw, "ecsw_train": bench_ecsw_train,
               "fbar": bench_fbar, "csc": bench_csc, "mass": bench_mass, "newton": bench_newton,
-              "batch": bench_batch}[
+              "batch": bench_batch, "hooks": bench_hooks}[
             args.workload]
         print(json.dumps(fn(args)), flush=True)
         return

@@ -27,6 +27,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from .boundary import DirichletManager
+from .engine import device_f64, host_array
 from .line_search import ContactAwareLineSearch
 from .solver import DirectSolver, FreeDofSystem
 
@@ -110,7 +111,7 @@ class NonlinearStaticAnalysis:
                 try:
                     f_int = force(u)
                 except ValueError:
-                    dom.update_state(u=accepted.cpu().numpy())
+                    dom.update_state(u=host_array(accepted))
                     raise
                 r, rnorm = system.residual(f_int, f_step_free)
                 if not np.isfinite(rnorm):
@@ -126,12 +127,12 @@ class NonlinearStaticAnalysis:
                     break
                 K_ff = system.csc(tangent(u))
                 try:
-                    du = self.linear_solver.solve(K_ff, r.cpu().numpy())
+                    du = self.linear_solver.solve(K_ff, host_array(r))
                 except (RuntimeError, np.linalg.LinAlgError):
                     break
                 if not np.all(np.isfinite(du)):
                     break
-                du_t = torch.as_tensor(du, device=dev)
+                du_t = device_f64(du, dev)
 
                 def trial(alpha):
                     uc = u.clone()
@@ -146,15 +147,15 @@ class NonlinearStaticAnalysis:
                 u[free_t] += res.alpha * du_t
             if not converged:
                 u = accepted
-                dom.update_state(u=u.cpu().numpy())
-                result.displacement = u.cpu().numpy()
-                result.reactions = force(u).cpu().numpy() - ((step - 1) / n_steps) * f_ext
+                dom.update_state(u=host_array(u))
+                result.displacement = host_array(u)
+                result.reactions = host_array(force(u)) - ((step - 1) / n_steps) * f_ext
                 result.steps_completed = step - 1
                 return result
             result.iteration_history.append(iters)
             result.steps_completed = step
-        dom.update_state(u=u.cpu().numpy())
-        result.displacement = u.cpu().numpy()
-        result.reactions = force(u).cpu().numpy() - f_ext
+        dom.update_state(u=host_array(u))
+        result.displacement = host_array(u)
+        result.reactions = host_array(force(u)) - f_ext
         result.converged = True
         return result

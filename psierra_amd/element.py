@@ -213,7 +213,7 @@ class _GpuTotalLagrangian:
         if c is not None and c["key"] == key and (c["k"] is not None or not want_k) and np.array_equal(c["u"], u):
             return c["f"], c["k"]
         f, k = self.assemble_device(mesh, u, want_k=want_k)
-        fh = f.cpu().numpy()
+        fh = engine.host_array(f)
         st["hook"] = {"u": u.copy(), "f": fh, "k": k, "key": key}
         return fh, k
 
@@ -233,7 +233,7 @@ class _GpuTotalLagrangian:
         plan = self._state(mesh)["plan"]
         indptr, indices = plan.pattern()
         n = plan.n_dofs
-        return spsp.csr_matrix((kval.cpu().numpy(), indices, indptr), shape=(n, n))
+        return spsp.csr_matrix((engine.host_array(kval), indices, indptr), shape=(n, n))
 
     # SolidDomain.assemble_stiffness_matrix delegates here when the element
     # defines it (src/psierra/domain/solid.py:180-181): the linear stiffness
@@ -449,7 +449,7 @@ class GpuTet4Fbar(GpuTet4TotalLagrangian):
         fb = self._fbar(mesh)["fbar"]
         indptr, indices = fb.pattern()
         n = fb.n_dofs
-        K = spsp.csr_matrix((kval.cpu().numpy(), indices.copy(), indptr.copy()), shape=(n, n))
+        K = spsp.csr_matrix((engine.host_array(kval), indices.copy(), indptr.copy()), shape=(n, n))
         # the reference returns the CSR sum K_dev + K_vol, which drops exact zeros
         K.eliminate_zeros()
         return K

@@ -34,6 +34,35 @@ def current_stream_handle():
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
+# Host copies above this many bytes go through page-locked memory: a pageable
+# device-to-host copy runs at ~4 GB/s on the MI355X boxes, a page-locked one
+# at ~56 GB/s (1.1 GB of K values: 253 ms against 20 ms).
+PINNED_MIN_BYTES = 1 << 20
+
+
+def host_array(t) -> np.ndarray:
+    """numpy copy of a device tensor, through page-locked memory for large
+    ones; the page-locked block returns to torch's host cache when the array
+    (which holds the tensor) is freed."""
+    torch = _torch()
+    if t.numel() * t.element_size() < PINNED_MIN_BYTES:
+        return t.cpu().numpy()
+    h = torch.empty(t.shape, dtype=t.dtype, pin_memory=True)
+    h.copy_(t, non_blocking=True)
+    torch.cuda.current_stream().synchronize()
+    return h.numpy()
+
+
+def device_f64(x, device):
+    """float64 device copy of a host array, staged through page-locked memory
+    when large (the copy is ordered on the current stream)."""
+    torch = _torch()
+    a = np.ascontiguousarray(x, dtype=np.float64)
+    if a.nbytes < PINNED_MIN_BYTES:
+        return torch.as_tensor(a, device=device)
+    return torch.from_numpy(a).pin_memory().to(device, non_blocking=True)
+
+
 def precision_code(precision: str) -> int:
     try:
         return PRECISIONS[precision]
@@ -272,7 +301,7 @@ class AssemblyPlan:
     def _device_f64(self, x, n, name):
         torch = _torch()
         if not isinstance(x, torch.Tensor):
-            x = torch.as_tensor(np.asarray(x, dtype=np.float64), device=self.device)
+            x = device_f64(x, self.device)
         if x.device != self.device or x.dtype != torch.float64:
             x = x.to(device=self.device, dtype=torch.float64)
         x = x.contiguous()

@@ -18,7 +18,7 @@ import scipy.sparse as spsp
 import scipy.sparse.linalg as spla
 
 from ._lib import check, lib, ptr
-from .engine import AssemblyPlan, _torch, current_stream_handle
+from .engine import AssemblyPlan, _torch, current_stream_handle, host_array
 
 
 class DirectSolver:
@@ -70,10 +70,10 @@ class FreeDofSystem:
                 current_stream_handle(),
             )
         )
-        colptr = self.colptr.cpu().numpy()
+        colptr = host_array(self.colptr)
         nnz = int(colptr[-1])
         return spsp.csc_matrix(
-            (self.vals[:nnz].cpu().numpy(), self.rowind[:nnz].cpu().numpy(), colptr),
+            (host_array(self.vals[:nnz]), host_array(self.rowind[:nnz]), colptr),
             shape=(self.n_free, self.n_free),
         )
 
