@@ -1,8 +1,11 @@
 """Solid domain with pSierra's SolidDomain interface, assembled on the GPU.
 
-Mirrors the parts of ``SolidDomain`` (src/psierra/domain/solid.py:20-325)
-that the nonlinear analyses use: state vectors, ``update_state``,
-``assemble_internal_forces`` and ``assemble_tangent_stiffness_matrix``.
+Mirrors the parts of ``SolidDomain`` (src/psierra/domain/solid.py:20-339)
+that the analyses use: state vectors, ``update_state``, the history-state
+pass-throughs to the material, ``assemble_internal_forces``,
+``assemble_tangent_stiffness_matrix``, the linear stiffness and consistent
+mass and the two energies -- without the reference's per-element host
+buffers (E x nde^2 for K and M, the COO indices: ~28 GB at 6.0M Tet4).
 Assembly is delegated to the element's hooks, exactly as the reference does
 when the element provides them (solid.py:238-241, 288-291).
 """
@@ -56,6 +59,41 @@ class SolidDomain:
         self.u.fill(0.0)
         self.v.fill(0.0)
         self.a.fill(0.0)
+        if hasattr(self.material, "reset_state"):
+            self.material.reset_state()
+
+    # History-dependent materials: the analyses call these around steps and
+    # cutbacks (solid.py:142-166); they pass through to the material.
+    def set_time_step(self, dt: float) -> None:
+        if hasattr(self.material, "set_time_step"):
+            self.material.set_time_step(dt)
+
+    def commit_state(self) -> None:
+        if hasattr(self.material, "commit_state"):
+            self.material.commit_state()
+
+    def revert_state(self) -> None:
+        if hasattr(self.material, "revert_state"):
+            self.material.revert_state()
+
+    def get_history_variables(self) -> dict:
+        if hasattr(self.material, "get_history_variables"):
+            return self.material.get_history_variables()
+        return {}
+
+    def set_history_variables(self, vars: dict) -> None:
+        if hasattr(self.material, "set_history_variables"):
+            self.material.set_history_variables(vars)
+
+    def compute_kinetic_energy(self) -> float:
+        """0.5 v^T M v with the consistent mass (solid.py:331-334)."""
+        M = self.assemble_mass_matrix()
+        return 0.5 * float(self.v @ (M @ self.v))
+
+    def compute_strain_energy(self) -> float:
+        """0.5 u^T K u with the linear stiffness (solid.py:336-339)."""
+        K = self.assemble_stiffness_matrix()
+        return 0.5 * float(self.u @ (K @ self.u))
 
     def assemble_internal_forces(self) -> np.ndarray:
         return self.element.assemble_internal_forces(self.mesh, self.displacement)
