@@ -143,8 +143,8 @@ int32_t psx_assemble(psx_plan* plan, const double* u, const double* mu,
 int32_t psx_plan_bind_material(psx_plan* plan, const double* mu, const double* lam, void* stream);
 
 /* ---- Batched assembly (device) -------------------------------------------
- * Many Tet4 plans -- meshes of uneven size, or one plan repeated for
- * parameter samples -- assembled by one launch over the union of their
+ * Many Tet4 (or many Tet10) plans -- meshes of uneven size, or one plan
+ * repeated for parameter samples -- assembled by one launch over the union of their
  * tiles, each instance's f and K bit-identical to its own psx_assemble.  No
  * single reference function: it replaces a Python loop of
  * SolidDomain.assemble_internal_forces / assemble_tangent_stiffness_matrix
@@ -152,9 +152,10 @@ int32_t psx_plan_bind_material(psx_plan* plan, const double* mu, const double* l
  * evaluations least_squares makes for one 2-point Jacobian
  * (src/psierra/inverse/parameter_identification.py:370-379).
  *
- * psx_batch_create takes n_plans Tet4 plans (host array of handles; one plan
- * may appear several times) that share a device and a tile size and are not
- * wide (PSX_EUNSUPPORTED otherwise; assemble those with psx_assemble).  The
+ * psx_batch_create takes n_plans plans of one element type (host array of
+ * handles; one plan may appear several times) that share a device and a
+ * tile size and are not wide (PSX_EUNSUPPORTED for a wide plan: assemble it
+ * with psx_assemble; PSX_EINVAL for mixed types, devices or tiles).  The
  * plans must outlive the batch.  The batch holds its own displacement
  * buffer per instance (32 B per node) and a table of the plans' views,
  * refreshed by the next psx_assemble_batch after psx_plan_set_tile or
@@ -164,7 +165,8 @@ int32_t psx_plan_bind_material(psx_plan* plan, const double* mu, const double* l
  * psx_assemble_batch: u, mu, lam, f, kval are DEVICE arrays of n_plans
  * device pointers, instance i's arrays as psx_assemble takes them (u 3N_i,
  * mu/lam n_elements_i, f 3N_i, kval nnz_i).  mu = lam = NULL: every plan's
- * bound material (psx_plan_bind_material on each).  kval = NULL: f only.
+ * bound material (psx_plan_bind_material on each; Tet4 only).  kval = NULL:
+ * f only.
  * status: device int32[2 n_plans], instance i's inversion flag and lowest
  * inverted element at 2i, 2i+1, reset by the call.  Two launches on
  * `stream`: the displacement permutation of all instances, then the tile

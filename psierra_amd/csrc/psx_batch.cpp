@@ -40,8 +40,8 @@ int32_t batch_alloc(psx_batch* b, size_t bytes, void** out) {
 }
 
 // Rebuild the device table from the plans' current views.  Refuses what the
-// batched kernels do not take: Tet10 or wide plans, plans of another device
-// or tile size.
+// batched kernels do not take: wide plans, plans of another element type,
+// device or tile size.
 int32_t batch_refresh(psx_batch* b, bool wait_for_device) {
   const int n = int(b->plans.size());
   std::vector<DevPlan> views(n);
@@ -50,10 +50,12 @@ int32_t batch_refresh(psx_batch* b, bool wait_for_device) {
   int64_t tiles = 0, nodes = 0;
   int32_t tile = 0, max_inc = 0;
   bool all_bound = true;
+  const int npe = b->plans[0]->hp.npe;
   for (int i = 0; i < n; ++i) {
     const psx_plan* p = b->plans[i];
     const std::string which = "plan " + std::to_string(i) + " of the batch";
-    if (p->hp.npe != 4) return fail(PSX_EUNSUPPORTED, which + " is not Tet4: batched assembly takes Tet4 plans");
+    if (p->hp.npe != npe)
+      return fail(PSX_EINVAL, which + " has another element type than plan 0 (Tet4 and Tet10 plans batch apart)");
     if (p->d.wide)
       return fail(PSX_EUNSUPPORTED, which + " is wide (a node beyond 64 incidences or 255 neighbours): "
                                             "assemble it with psx_assemble");
@@ -90,6 +92,7 @@ int32_t batch_refresh(psx_batch* b, bool wait_for_device) {
   b->d.n_nodes = int32_t(nodes);
   b->d.tile = tile;
   b->d.max_inc = max_inc;
+  b->d.npe = npe;
   b->all_bound = all_bound;
   for (int i = 0; i < n; ++i) b->versions[i] = b->plans[i]->version;
   return PSX_OK;
@@ -193,10 +196,13 @@ int32_t psx_assemble_batch(psx_batch* batch, const double* const* u, const doubl
     const int32_t rc = psx::batch_refresh(batch, true);
     if (rc) return rc;
   }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (batch->d.npe == 10)
+    return psx::launch_tet10_batch(batch->d, u, mu, lam, f, kval, precision, batch->cus, batch->persistent, s,
+                                   status);
   if (!mu && !batch->all_bound)
     return psx::fail(PSX_EINVAL, "psx_assemble_batch without mu and lam needs psx_plan_bind_material on every plan");
-  return psx::launch_tet4_batch(batch->d, u, mu, lam, f, kval, precision, batch->cus, batch->persistent,
-                                static_cast<hipStream_t>(stream), status);
+  return psx::launch_tet4_batch(batch->d, u, mu, lam, f, kval, precision, batch->cus, batch->persistent, s, status);
 }
 
 }  // extern "C"

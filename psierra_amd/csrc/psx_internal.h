@@ -120,6 +120,7 @@ struct BatchDev {
   const double* const* bound_mu = nullptr;   // n: the plans' bound (mu, lambda) (FP64 rows)
   const double* const* bound_lam = nullptr;
   int32_t n = 0, n_tiles = 0, n_nodes = 0, tile = 0, max_inc = 0;
+  int32_t npe = 4;                        // nodes per element of every plan (4 or 10)
 };
 
 // F-bar store layout in the distance-2 pattern of K_dev + K_vol (device view):
@@ -209,6 +210,10 @@ int32_t launch_t4_fbar_rows(const DevPlan& p, const double* mu, const double* la
                             const FbarView& fv, double* f, double* kval, int precision, hipStream_t s,
                             int32_t* status);
 int32_t launch_permute_u(const DevPlan& p, const double* u, hipStream_t s, int32_t* status);
+int32_t launch_batch_permute(const BatchDev& b, const double* const* u, hipStream_t s, int32_t* status);
+int32_t launch_tet10_batch(const BatchDev& b, const double* const* u, const double* const* mu,
+                           const double* const* lam, double* const* f, double* const* kval, int precision,
+                           int cus, bool persistent, hipStream_t s, int32_t* status);
 int32_t launch_tet4_batch(const BatchDev& b, const double* const* u, const double* const* mu,
                           const double* const* lam, double* const* f, double* const* kval, int precision,
                           int cus, bool persistent, hipStream_t s, int32_t* status);

@@ -15,6 +15,7 @@
 #include "psx_device.h"
 #include "psx_internal.h"
 #include "psx_nh.h"
+#include "psx_batch.h"
 
 namespace psx {
 
@@ -165,8 +166,8 @@ __device__ __forceinline__ R tet10_row(const R dX[10][3], const R dU[10][3], con
 
 // Gathered inputs of one Tet10 incidence: node coordinates and displacements
 // relative to node 0 (differences in FP64, then rounded to R).
-template <typename R>
-__device__ __forceinline__ void tet10_gather(const DevPlan& p, int e, R dX[10][3], R dU[10][3]) {
+template <typename R, typename P>
+__device__ __forceinline__ void tet10_gather(const P& p, int e, R dX[10][3], R dU[10][3]) {
   const int32_t* c = p.conn + 12 * int64_t(e);
   const int4 c0 = *reinterpret_cast<const int4*>(c);
   const int4 c1 = *reinterpret_cast<const int4*>(c + 4);
@@ -208,8 +209,8 @@ __device__ __forceinline__ double quad_bcast(double x) {
 // tet10_gather for the quad of lanes that shares one incidence (all four
 // lanes active): lane qp loads nodes qp, qp + 4, qp + 8 (besides node 0),
 // forms their differences, and the quad exchanges them.
-template <typename R>
-__device__ __forceinline__ void tet10_gather_quad(const DevPlan& p, int e, int qp, R dX[10][3], R dU[10][3]) {
+template <typename R, typename P>
+__device__ __forceinline__ void tet10_gather_quad(const P& p, int e, int qp, R dX[10][3], R dU[10][3]) {
   const int32_t* c = p.conn + 12 * int64_t(e);
   const int4 c0 = *reinterpret_cast<const int4*>(c);
   const int4 c1 = *reinterpret_cast<const int4*>(c + 4);
@@ -267,16 +268,13 @@ __device__ __forceinline__ void tet10_gather_quad(const DevPlan& p, int e, int q
 // f_int only, one lane per incidence over the four quadrature points (the
 // FP32 form: its registers and LDS leave room for enough waves), f_a summed
 // in FP64 over the node's run.
-template <typename R, int TILE, int M>
-__global__ __launch_bounds__(TILE) void tet10_force_kernel(DevPlan p, MatParams mp, double* __restrict__ f,
-                                                           int32_t* status) {
+template <typename R, int TILE, int M, typename P>
+__device__ __forceinline__ void tet10_force_tile(const P& p, const MatParams& mp, double* __restrict__ f,
+                                                 int32_t* status, int tile, int t) {
   __shared__ R contrib[3 * TILE];
   __shared__ int32_t s_incp[TILE + 1];
-  const int tile = xcd_tile();
-  if (tile >= p.n_tiles) return;
   const int4 ti = load_uniform(p.tile_info, tile);
   const int n0 = ti.x, nn = ti.y, k0 = ti.z, nk = ti.w;
-  const int t = threadIdx.x;
   for (int i = t; i <= nn; i += TILE) s_incp[i] = p.node_rec[n0 + i].x - k0;
   if (t < nk) {
     const uint32_t pe = p.inc_elem[k0 + t];
@@ -308,21 +306,26 @@ __global__ __launch_bounds__(TILE) void tet10_force_kernel(DevPlan p, MatParams 
   }
 }
 
+template <typename R, int TILE, int M>
+__global__ __launch_bounds__(TILE) void tet10_force_kernel(DevPlan p, MatParams mp, double* __restrict__ f,
+                                                           int32_t* status) {
+  const int tile = xcd_tile();
+  if (tile >= p.n_tiles) return;
+  tet10_force_tile<R, TILE, M>(p, mp, f, status, tile, int(threadIdx.x));
+}
+
 // f_int only, four lanes per incidence (one per quadrature point, the quad
 // summing f_a with DPP; the FP64 form, whose one-lane version is held to few
 // waves by its registers), f_a summed in FP64 over the node's run by one lane
 // per (node, component).
-template <typename R, int TILE, int M>
-__global__ __launch_bounds__(4 * TILE) void tet10_force_quad_kernel(DevPlan p, MatParams mp, double* __restrict__ f,
-                                                               int32_t* status) {
+template <typename R, int TILE, int M, typename P>
+__device__ __forceinline__ void tet10_force_quad_tile(const P& p, const MatParams& mp, double* __restrict__ f,
+                                                      int32_t* status, int tile, int t) {
   constexpr int NT = 4 * TILE;
   __shared__ R contrib[3 * TILE];
   __shared__ int32_t s_incp[TILE + 1];
-  const int tile = xcd_tile();
-  if (tile >= p.n_tiles) return;
   const int4 ti = load_uniform(p.tile_info, tile);
   const int n0 = ti.x, nn = ti.y, k0 = ti.z, nk = ti.w;
-  const int t = threadIdx.x;
   for (int i = t; i <= nn; i += NT) s_incp[i] = p.node_rec[n0 + i].x - k0;
   const int it = t >> 2, qp = t & 3;
   if (it < nk) {   // the same for the four lanes of a quad
@@ -354,6 +357,14 @@ __global__ __launch_bounds__(4 * TILE) void tet10_force_quad_kernel(DevPlan p, M
   }
 }
 
+template <typename R, int TILE, int M>
+__global__ __launch_bounds__(4 * TILE) void tet10_force_quad_kernel(DevPlan p, MatParams mp, double* __restrict__ f,
+                                                                    int32_t* status) {
+  const int tile = xcd_tile();
+  if (tile >= p.n_tiles) return;
+  tet10_force_quad_tile<R, TILE, M>(p, mp, f, status, tile, int(threadIdx.x));
+}
+
 // Element position q (0..9) of the node whose slot rank in the row is s,
 // from the three rank words of an incidence (bytes 0..9).
 __device__ __forceinline__ int tet10_rank_pos(uint32_t w0, uint32_t w1, uint32_t w2, uint32_t s) {
@@ -380,9 +391,9 @@ __device__ __forceinline__ int tet10_rank_pos(uint32_t w0, uint32_t w1, uint32_t
 // point; each value of the row is summed over the four points across the
 // quad ((q0 + q1) + (q2 + q3), DPP) and written to LDS once, so the LDS per
 // incidence (93 values) is shared by four waves' worth of threads.
-template <typename R, int TILE, int M>
-__global__ __launch_bounds__(4 * TILE) void tet10_assemble_rows(DevPlan p, MatParams mp, double* __restrict__ f,
-                                                                double* __restrict__ kval, int32_t* status) {
+template <typename R, int TILE, int M, typename P>
+__device__ __forceinline__ void tet10_rows_tile(const P& p, const MatParams& mp, double* __restrict__ f,
+                                                double* __restrict__ kval, int32_t* status, int tile, int t) {
   constexpr int NV = 93;
   constexpr int NT = 4 * TILE;                     // threads
   constexpr int UMAX = kTet10LanePasses * TILE;   // reduction lanes per tile
@@ -399,11 +410,8 @@ __global__ __launch_bounds__(4 * TILE) void tet10_assemble_rows(DevPlan p, MatPa
   __shared__ uint8_t s_nz[NN];
   __shared__ uint8_t s_first[TILE];
 
-  const int tile = xcd_tile();
-  if (tile >= p.n_tiles) return;
   const int4 ti = load_uniform(p.tile_info, tile);
   const int n0 = ti.x, nn = ti.y, k0 = ti.z, nk = ti.w;
-  const int t = threadIdx.x;
   const int lane = t & 63;
 
   int ub = 0, ue = 0;
@@ -544,6 +552,14 @@ __global__ __launch_bounds__(4 * TILE) void tet10_assemble_rows(DevPlan p, MatPa
   }
 }
 
+template <typename R, int TILE, int M>
+__global__ __launch_bounds__(4 * TILE) void tet10_assemble_rows(DevPlan p, MatParams mp, double* __restrict__ f,
+                                                                double* __restrict__ kval, int32_t* status) {
+  const int tile = xcd_tile();
+  if (tile >= p.n_tiles) return;
+  tet10_rows_tile<R, TILE, M>(p, mp, f, kval, status, tile, int(threadIdx.x));
+}
+
 
 template <typename R, int TILE, int M>
 static void launch_t10(const DevPlan& p, const MatParams& mp, double* f, double* kval, hipStream_t s,
@@ -573,6 +589,77 @@ static int32_t launch_t10_tiles(const DevPlan& p, const MatParams& mp, double* f
     default:
       return fail(PSX_EINVAL, "Tet10 supports tiles of 64 or 128 incidences");
   }
+  return PSX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batched Tet10 assembly: the tiles of many plans in one launch (the scheme
+// of the Tet4 batch, psx_batch.h), each tile by the same tile routine as a
+// single-plan launch, so every instance's f and K are the single-plan bits.
+// ---------------------------------------------------------------------------
+template <typename R, int TILE, int M, bool PERSISTENT>
+__global__ __launch_bounds__(4 * TILE) void tet10_batch_rows(BatchDev b, const double* const* mu,
+                                                             const double* const* lam, double* const* f,
+                                                             double* const* kval, int32_t* status) {
+  batch_tiles<PERSISTENT>(b, [&](int inst, int tile, int t) {
+    const ConstDevPlan& p = *(const ConstDevPlan*)(b.plans + inst);
+    const MatParams mp{{load_uniform_val(mu + inst), load_uniform_val(lam + inst), nullptr, nullptr}};
+    tet10_rows_tile<R, TILE, M>(p, mp, load_uniform_val(f + inst), load_uniform_val(kval + inst),
+                                status + 2 * inst, tile, t);
+  });
+}
+
+// f only: one lane per incidence in FP32, a quad per incidence in FP64 (as
+// the single-plan launches).
+template <typename R, int TILE, int M, bool PERSISTENT>
+__global__ __launch_bounds__((std::is_same<R, double>::value ? 4 * TILE : TILE)) void tet10_batch_force(
+    BatchDev b, const double* const* mu, const double* const* lam, double* const* f, int32_t* status) {
+  batch_tiles<PERSISTENT>(b, [&](int inst, int tile, int t) {
+    const ConstDevPlan& p = *(const ConstDevPlan*)(b.plans + inst);
+    const MatParams mp{{load_uniform_val(mu + inst), load_uniform_val(lam + inst), nullptr, nullptr}};
+    if constexpr (std::is_same<R, double>::value)
+      tet10_force_quad_tile<R, TILE, M>(p, mp, load_uniform_val(f + inst), status + 2 * inst, tile, t);
+    else
+      tet10_force_tile<R, TILE, M>(p, mp, load_uniform_val(f + inst), status + 2 * inst, tile, t);
+  });
+}
+
+template <typename R, int TILE, bool PERSISTENT>
+static void launch_batch_t10(const BatchDev& b, const double* const* mu, const double* const* lam,
+                             double* const* f, double* const* kval, int cus, hipStream_t s, int32_t* status) {
+  if (kval) {
+    const auto k = tet10_batch_rows<R, TILE, kNeoHookean, PERSISTENT>;
+    const int grid = batch_grid(reinterpret_cast<const void*>(k), 4 * TILE, cus, b.n_tiles, PERSISTENT);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(4 * TILE), 0, s, b, mu, lam, f, kval, status);
+  } else {
+    constexpr int threads = std::is_same<R, double>::value ? 4 * TILE : TILE;
+    const auto k = tet10_batch_force<R, TILE, kNeoHookean, PERSISTENT>;
+    const int grid = batch_grid(reinterpret_cast<const void*>(k), threads, cus, b.n_tiles, PERSISTENT);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(threads), 0, s, b, mu, lam, f, status);
+  }
+}
+
+// Neo-Hookean with per-call (mu, lambda) (Tet10 plans bind no material).
+int32_t launch_tet10_batch(const BatchDev& b, const double* const* u, const double* const* mu,
+                           const double* const* lam, double* const* f, double* const* kval, int precision,
+                           int cus, bool persistent, hipStream_t s, int32_t* status) {
+  if (!mu || !lam) return fail(PSX_EINVAL, "a batch of Tet10 plans takes mu and lam (Tet10 plans bind no material)");
+  int32_t rc = launch_batch_permute(b, u, s, status);
+  if (rc) return rc;
+  if (b.n_tiles <= 0) return PSX_OK;
+  const bool fp32 = precision == PSX_FP32;
+  switch (b.tile * 4 + 2 * int(fp32) + int(persistent)) {
+    case 256: launch_batch_t10<double, 64, false>(b, mu, lam, f, kval, cus, s, status); break;
+    case 257: launch_batch_t10<double, 64, true>(b, mu, lam, f, kval, cus, s, status); break;
+    case 258: launch_batch_t10<float, 64, false>(b, mu, lam, f, kval, cus, s, status); break;
+    case 259: launch_batch_t10<float, 64, true>(b, mu, lam, f, kval, cus, s, status); break;
+    case 512: launch_batch_t10<double, 128, false>(b, mu, lam, f, kval, cus, s, status); break;
+    case 513: launch_batch_t10<double, 128, true>(b, mu, lam, f, kval, cus, s, status); break;
+    case 514: launch_batch_t10<float, 128, false>(b, mu, lam, f, kval, cus, s, status); break;
+    case 515: launch_batch_t10<float, 128, true>(b, mu, lam, f, kval, cus, s, status); break;
+    default: return fail(PSX_EINVAL, "Tet10 supports tiles of 64 or 128 incidences");
+  }
+  PSX_HIP_TRY(hipGetLastError());
   return PSX_OK;
 }
 

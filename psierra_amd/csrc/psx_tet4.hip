@@ -32,6 +32,7 @@
 #include "psx_internal.h"
 #include "psx_nh.h"
 #include "psx_tet4_elem.h"
+#include "psx_batch.h"
 
 namespace psx {
 
@@ -702,39 +703,6 @@ int32_t launch_tet4_assemble(const DevPlan& p, const double* u, const double* mu
 // are bit-identical to psx_assemble's.  Tiles hold at most TILE incidences
 // whatever the mesh size, so uneven instances balance by construction.
 
-// A work-group-uniform value (a plan view, a pointer, an index) read through
-// the scalar unit, dword by dword.
-template <typename T>
-__device__ __forceinline__ T load_uniform_val(const T* p) {
-  static_assert(sizeof(T) % 4 == 0 && std::is_trivially_copyable<T>::value, "dword-copyable type");
-  T out;
-  const auto* q = (const __attribute__((address_space(4))) uint32_t*)p;
-  uint32_t* o = reinterpret_cast<uint32_t*>(&out);
-#pragma unroll
-  for (int i = 0; i < int(sizeof(T) / 4); ++i) o[i] = q[i];
-  return out;
-}
-
-// A plan view in the constant address space: its fields are read through
-// the scalar unit where they are used, as a kernel argument's are.
-using ConstDevPlan = __attribute__((address_space(4))) DevPlan;
-
-// Instance owning global item g: the largest i with start[i] <= g
-// (start[0] = 0; an empty instance has the next one's start and is skipped).
-template <bool UNIFORM>
-__device__ __forceinline__ int batch_owner(const int32_t* start, int n, int g) {
-  int lo = 0, hi = n;
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    const int v = UNIFORM ? load_uniform_val(start + mid) : start[mid];
-    if (v <= g)
-      lo = mid;
-    else
-      hi = mid;
-  }
-  return lo;
-}
-
 // Displacements of every instance into its plan's internal node order, and
 // the per-instance status words reset.
 __global__ __launch_bounds__(256) void batch_permute_kernel(BatchDev b, const double* const* __restrict__ u,
@@ -751,35 +719,6 @@ __global__ __launch_bounds__(256) void batch_permute_kernel(BatchDev b, const do
   const double* ui = u[inst];
   const int64_t a = p.perm[k];
   p.ui[k] = make_double4(ui[3 * a], ui[3 * a + 1], ui[3 * a + 2], 0.0);
-}
-
-// Tiles of a batched launch.  PERSISTENT: work-group w of XCD x takes tiles
-// x * per + w, x * per + w + P, ... of the global list (P work-groups per
-// XCD, as many as stay resident); the thread index is passed through an
-// empty asm so that the compiler recomputes what derives from it in each
-// tile instead of keeping it live across the loop (registers, occupancy).
-// Otherwise one tile per work-group, XCD-contiguous as in a single-plan
-// launch.
-template <bool PERSISTENT, typename Body>
-__device__ __forceinline__ void batch_tiles(const BatchDev& b, Body body) {
-  if constexpr (PERSISTENT) {
-    const int per = (b.n_tiles + 7) >> 3;
-    const int x = int(blockIdx.x & 7u), w0 = int(blockIdx.x >> 3), P = int(gridDim.x >> 3);
-    for (int w = w0; w < per; w += P) {
-      const int g = x * per + w;
-      if (g >= b.n_tiles) break;
-      const int inst = batch_owner<true>(b.tile_start, b.n, g);
-      if (w != w0) __syncthreads();   // the previous tile's LDS reads are done
-      int t = int(threadIdx.x);
-      __asm__ volatile("" : "+v"(t));
-      body(inst, g - load_uniform_val(b.tile_start + inst), t);
-    }
-  } else {
-    const int g = xcd_tile();
-    if (g >= b.n_tiles) return;
-    const int inst = batch_owner<true>(b.tile_start, b.n, g);
-    body(inst, g - load_uniform_val(b.tile_start + inst), int(threadIdx.x));
-  }
 }
 
 template <typename R, int TILE, int M, typename MaskT, bool PERSISTENT>
@@ -811,20 +750,6 @@ __global__ __launch_bounds__(TILE) void tet4_batch_force(BatchDev b, const doubl
     }
     force_tile<R, TILE, M>(p, mp, load_uniform_val(f + inst), status + 2 * inst, tile, t);
   });
-}
-
-// Grid of a batched launch: persistent, as many work-groups as stay
-// resident on the device (never more than there are tiles); otherwise one per
-// tile.  A multiple of 8 either way (one share per XCD).
-static int batch_grid(const void* kernel, int threads, int cus, int n_tiles, bool persistent) {
-  int64_t g = n_tiles;
-  if (persistent) {
-    int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess || per_cu < 1)
-      per_cu = 1;
-    g = std::min<int64_t>(int64_t(cus) * per_cu, n_tiles);
-  }
-  return int(8 * ((g + 7) / 8));
 }
 
 template <typename R, int TILE, int M, bool PERSISTENT>
@@ -864,14 +789,19 @@ static int32_t launch_batch_tiles(const BatchDev& b, const double* const* mu, co
 // u, mu, lam, f, kval: device arrays of n device pointers; mu = lam = NULL
 // for the plans' bound material (FP32: the bound geometry records; FP64:
 // the plans' copies of mu and lambda); kval = NULL for f only.
-int32_t launch_tet4_batch(const BatchDev& b, const double* const* u, const double* const* mu,
-                          const double* const* lam, double* const* f, double* const* kval, int precision,
-                          int cus, bool persistent, hipStream_t s, int32_t* status) {
+int32_t launch_batch_permute(const BatchDev& b, const double* const* u, hipStream_t s, int32_t* status) {
   const int total = std::max(b.n_nodes, b.n);
   hipLaunchKernelGGL(batch_permute_kernel, dim3((total + 255) / 256), dim3(256), 0, s, b, u, status);
   PSX_HIP_TRY(hipGetLastError());
+  return PSX_OK;
+}
+
+int32_t launch_tet4_batch(const BatchDev& b, const double* const* u, const double* const* mu,
+                          const double* const* lam, double* const* f, double* const* kval, int precision,
+                          int cus, bool persistent, hipStream_t s, int32_t* status) {
+  int32_t rc = launch_batch_permute(b, u, s, status);
+  if (rc) return rc;
   if (b.n_tiles <= 0) return PSX_OK;
-  int32_t rc;
   if (!mu && precision == PSX_FP32)
     rc = launch_batch_tiles<float, kNeoHookeanBound>(b, nullptr, nullptr, f, kval, cus, persistent, s, status);
   else if (!mu)

@@ -321,11 +321,12 @@ BATCH_INFO_KEYS = ("n_instances", "n_tiles", "n_nodes", "tile", "persistent", "n
 
 
 class AssemblyBatch:
-    """Many Tet4 plans assembled by one launch (``psx_assemble_batch``).
+    """Many Tet4 (or many Tet10) plans assembled by one launch
+    (``psx_assemble_batch``).
 
-    ``plans``: ``AssemblyPlan`` objects of one device and tile size, meshes
-    of uneven size or one plan repeated for parameter samples (each instance
-    has its own displacement buffer).  The batch keeps the plans alive.
+    ``plans``: ``AssemblyPlan`` objects of one element type, device and tile
+    size, meshes of uneven size or one plan repeated for parameter samples
+    (each instance has its own displacement buffer).  The batch keeps the plans alive.
     ``assemble`` takes one displacement (and, unless the plans' bound
     materials are used, one mu and lambda) per instance and returns the
     per-instance f and K values, each bit-identical to that plan's own

@@ -1,5 +1,5 @@
-"""Batched Tet4 assembly (psx_assemble_batch): many meshes of uneven size,
-or one mesh repeated for parameter samples, in one launch.
+"""Batched assembly (psx_assemble_batch): many Tet4 (or many Tet10) meshes
+of uneven size, or one mesh repeated for parameter samples, in one launch.
 
 Every instance's f and K must be bit-identical to its own psx_assemble (the
 same tile routine runs on the same tiles), in both launch modes (persistent
@@ -154,11 +154,11 @@ def test_batch_retile_refresh_and_refusals(meshes):
     sub[0].set_tile(256)
     with pytest.raises(ValueError, match="another tile size"):
         batch.assemble(us, mus, lams, precision="fp64")
-    # Tet10 and wide plans are refused at creation
+    # mixed element types and wide plans are refused at creation
     from psierra_amd.mesh import tet4_to_tet10
 
     X10, c10 = tet4_to_tet10(wls[0].mesh.nodes, wls[0].mesh.elements)
-    with pytest.raises(PsxError, match="not Tet4"):
+    with pytest.raises(ValueError, match="another element type"):
         AssemblyBatch([plans[0], AssemblyPlan(X10, c10)])
     from tests.test_gpu_edge_cases import _fan_and_box
 
@@ -237,3 +237,52 @@ def test_batch_autotune_keeps_results(meshes):
     torch.cuda.synchronize()
     for i in range(len(plans)):
         assert torch.equal(fs[i], f0[i]) and torch.equal(ks[i], k0[i]), i
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_tet10_batch_bit_identical_to_single_plans(precision):
+    """Tet10 meshes of uneven size (the Tet4 boxes with mid-edge nodes), one
+    of them twice: f + K and f only equal each plan's own assembly bit for
+    bit; instance 0 against the oracle's scipy assembly."""
+    import torch
+
+    from psierra_amd.engine import AssemblyBatch, AssemblyPlan
+    from psierra_amd.mesh import tet4_to_tet10
+
+    wls = W.batch_boxes_tet4(n_instances=4, seed=23, side_range=(2, 5))
+    dev = None
+    meshes, plans, us, mus, lams = [], [], [], [], []
+    rng = np.random.default_rng(7)
+    for w in wls:
+        X10, c10 = tet4_to_tet10(w.mesh.nodes, w.mesh.elements)
+        u = 0.02 * np.sin(3.0 * X10) * rng.uniform(0.5, 1.0)
+        plan = AssemblyPlan(X10, c10)
+        dev = plan.device
+        mu = w.mu[0] * np.exp(0.2 * rng.standard_normal(c10.shape[0]))
+        meshes.append((X10, c10, u, mu))
+        plans.append(plan)
+        us.append(torch.as_tensor(u.reshape(-1), device=dev))
+        mus.append(torch.as_tensor(mu, device=dev))
+        lams.append(torch.as_tensor(5.0 * mu, device=dev))
+    plans.append(plans[1])
+    us.append(0.5 * us[1])
+    mus.append(mus[1])
+    lams.append(lams[1])
+    banner(f"batch: {len(plans)} Tet10 meshes {precision}")
+    batch = AssemblyBatch(plans)
+    assert batch.info["n_tiles"] == sum(p.info["n_tiles"] for p in plans)
+    fs, ks = batch.assemble(us, mus, lams, precision=precision)
+    fo, _ = batch.assemble(us, mus, lams, precision=precision, want_k=False)
+    torch.cuda.synchronize()
+    for i, p in enumerate(plans):
+        f1, k1 = p.assemble(us[i], mus[i], lams[i], precision=precision)
+        f2, _ = p.assemble(us[i], mus[i], lams[i], precision=precision, want_k=False)
+        assert torch.equal(fs[i], f1) and torch.equal(ks[i], k1), i
+        assert torch.equal(fo[i], f2), i
+    X10, c10, u, mu = meshes[0]
+    f_ref, K_ref = A.assemble(X10, c10, u, mu, 5.0 * mu)
+    tol_k, tol_f = (1e-12, 1e-11) if precision == "fp64" else (1e-5, 1e-5)
+    assert rel_err(ks[0].cpu().numpy(), K_ref.data) < tol_k
+    assert rel_err(fs[0].cpu().numpy(), f_ref) < tol_f
+    with pytest.raises(ValueError, match="Tet10 plans bind no material"):
+        batch.assemble(us, precision=precision)
