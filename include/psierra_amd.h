@@ -67,7 +67,8 @@ enum {
   PSX_INFO_COUNT = 11
 };
 
-/* Version of this interface: 6 (5 had no psx_batch_* / psx_assemble_batch;
+/* Version of this interface: 7 (6 had no psx_vjp_batch; 5 had no psx_batch_* /
+ * psx_assemble_batch;
  * 4: psx_ecsw_training took an element-major
  * work block and gave no sums of squares; 3 had no psx_ecsw_training_noalloc;
  * 2: psx_ecsw_* took rank 64 only; 1 had psx_ecsw_training without
@@ -189,6 +190,18 @@ int32_t psx_batch_info(const psx_batch* batch, int64_t* info, int32_t n_info);
 int32_t psx_assemble_batch(psx_batch* batch, const double* const* u, const double* const* mu,
                            const double* const* lam, double* const* f, double* const* kval,
                            int32_t precision, void* stream, int32_t* status);
+/* The adjoint psx_vjp of every instance of a Tet4 batch in one launch: u, mu,
+ * lam, w, g_u (and g_mu, g_lam, both or neither) are DEVICE arrays of
+ * n_plans device pointers laid out as psx_vjp takes them; each instance's
+ * outputs are the bits of its own psx_vjp.  It stands for a loop of the
+ * forward evaluations that pSierra's parameter identification runs per
+ * parameter sample or load case
+ * (src/psierra/inverse/parameter_identification.py:300-379).  The first call
+ * allocates the batch's cotangent buffer (32 B per node). */
+int32_t psx_vjp_batch(psx_batch* batch, const double* const* u, const double* const* mu,
+                      const double* const* lam, const double* const* w, double* const* g_u,
+                      double* const* g_mu, double* const* g_lam, int32_t precision, void* stream,
+                      int32_t* status);
 
 /* ---- Other hyperelastic models (device) -----------------------------------
  * The psx_assemble path for the compressible models of

@@ -92,6 +92,10 @@ SIGNATURES = {
     ),
     "psx_ecsw_training_work_doubles": (c_int64, [c_void_p]),
     "psx_batch_create": (c_int32, [c_void_p, c_int32, POINTER(c_void_p)]),
+    "psx_vjp_batch": (
+        c_int32,
+        [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_void_p],
+    ),
     "psx_batch_destroy": (c_int32, [c_void_p]),
     "psx_batch_set_persistent": (c_int32, [c_void_p, c_int32]),
     "psx_batch_info": (c_int32, [c_void_p, _i64p, c_int32]),
@@ -117,7 +121,7 @@ SIGNATURES = {
 _lib = None
 
 # The C ABI these signatures describe (psx_abi_version() of the library).
-ABI_VERSION = 6
+ABI_VERSION = 7
 
 
 class PsxError(RuntimeError):

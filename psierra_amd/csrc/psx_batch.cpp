@@ -27,13 +27,13 @@ struct psx_batch {
   int32_t* d_node_start = nullptr;
   const double** d_bound = nullptr;   // 2n: bound mu of each plan, then bound lambda
   double4* ui = nullptr;              // the instances' displacements, internal node order
+  double4* wi = nullptr;              // the instances' adjoint cotangents, internal node order (lazy)
   std::vector<void*> allocations;
 };
 
 namespace psx {
-namespace {
 
-int32_t batch_alloc(psx_batch* b, size_t bytes, void** out) {
+static int32_t batch_alloc(psx_batch* b, size_t bytes, void** out) {
   PSX_HIP_TRY(hipMalloc(out, std::max<size_t>(bytes, 16)));
   b->allocations.push_back(*out);
   return PSX_OK;
@@ -42,7 +42,7 @@ int32_t batch_alloc(psx_batch* b, size_t bytes, void** out) {
 // Rebuild the device table from the plans' current views.  Refuses what the
 // batched kernels do not take: wide plans, plans of another element type,
 // device or tile size.
-int32_t batch_refresh(psx_batch* b, bool wait_for_device) {
+static int32_t batch_refresh(psx_batch* b, bool wait_for_device) {
   const int n = int(b->plans.size());
   std::vector<DevPlan> views(n);
   std::vector<int32_t> tile_start(n + 1), node_start(n + 1);
@@ -64,6 +64,7 @@ int32_t batch_refresh(psx_batch* b, bool wait_for_device) {
     if (p->d.tile != tile) return fail(PSX_EINVAL, which + " has another tile size than plan 0");
     views[i] = p->d;
     views[i].ui = b->ui + nodes;
+    views[i].wi = b->wi ? b->wi + nodes : nullptr;
     tile_start[i] = int32_t(tiles);
     node_start[i] = int32_t(nodes);
     tiles += p->d.n_tiles;
@@ -98,7 +99,7 @@ int32_t batch_refresh(psx_batch* b, bool wait_for_device) {
   return PSX_OK;
 }
 
-int32_t batch_create(psx_plan* const* plans, int32_t n, psx_batch** out) {
+static int32_t batch_create(psx_plan* const* plans, int32_t n, psx_batch** out) {
   if (!plans || !out || n < 1) return fail(PSX_EINVAL, "psx_batch_create needs n >= 1 plans and an output");
   *out = nullptr;
   for (int i = 0; i < n; ++i)
@@ -141,7 +142,6 @@ int32_t batch_create(psx_plan* const* plans, int32_t n, psx_batch** out) {
   return PSX_OK;
 }
 
-}  // namespace
 }  // namespace psx
 
 extern "C" {
@@ -161,6 +161,37 @@ int32_t psx_batch_destroy(psx_batch* batch) {
   (void)hipSetDevice(dev);
   delete batch;
   return PSX_OK;
+}
+
+int32_t psx_vjp_batch(psx_batch* batch, const double* const* u, const double* const* mu, const double* const* lam,
+                      const double* const* w, double* const* g_u, double* const* g_mu, double* const* g_lam,
+                      int32_t precision, void* stream, int32_t* status) {
+  psx::clear_error();
+  if (!batch || !u || !mu || !lam || !w || !g_u || !status)
+    return psx::fail(PSX_EINVAL, "null argument to psx_vjp_batch");
+  if ((g_mu == nullptr) != (g_lam == nullptr))
+    return psx::fail(PSX_EINVAL, "g_mu and g_lam must both be given or both be null");
+  if (precision != PSX_FP64 && precision != PSX_FP32)
+    return psx::fail(PSX_EINVAL, "precision must be PSX_FP64 or PSX_FP32");
+  if (batch->d.npe != 4) return psx::fail(PSX_EUNSUPPORTED, "psx_vjp_batch supports batches of Tet4 plans");
+  bool stale = !batch->wi;
+  if (!batch->wi) {
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipSetDevice(batch->device);
+    void* w_buf = nullptr;
+    const int32_t rc = psx::batch_alloc(batch, size_t(batch->d.n_nodes) * sizeof(double4), &w_buf);
+    (void)hipSetDevice(dev);
+    if (rc) return rc;
+    batch->wi = static_cast<double4*>(w_buf);
+  }
+  for (size_t i = 0; i < batch->plans.size(); ++i) stale = stale || batch->plans[i]->version != batch->versions[i];
+  if (stale) {
+    const int32_t rc = psx::batch_refresh(batch, true);
+    if (rc) return rc;
+  }
+  return psx::launch_tet4_batch_vjp(batch->d, u, mu, lam, w, g_u, g_mu, g_lam, precision,
+                                    static_cast<hipStream_t>(stream), status);
 }
 
 int32_t psx_batch_set_persistent(psx_batch* batch, int32_t persistent) {

@@ -211,6 +211,10 @@ int32_t launch_t4_fbar_rows(const DevPlan& p, const double* mu, const double* la
                             int32_t* status);
 int32_t launch_permute_u(const DevPlan& p, const double* u, hipStream_t s, int32_t* status);
 int32_t launch_batch_permute(const BatchDev& b, const double* const* u, hipStream_t s, int32_t* status);
+int32_t launch_tet4_batch_vjp(const BatchDev& b, const double* const* u, const double* const* mu,
+                              const double* const* lam, const double* const* w, double* const* gu,
+                              double* const* gmu, double* const* glam, int precision, hipStream_t s,
+                              int32_t* status);
 int32_t launch_tet10_batch(const BatchDev& b, const double* const* u, const double* const* mu,
                            const double* const* lam, double* const* f, double* const* kval, int precision,
                            int cus, bool persistent, hipStream_t s, int32_t* status);

@@ -801,7 +801,7 @@ int32_t retile(psx_plan* plan, int32_t tile) {
 // ---------------------------------------------------------------------------
 extern "C" {
 
-int32_t psx_abi_version(void) { return 6; }
+int32_t psx_abi_version(void) { return 7; }
 
 const char* psx_last_error(void) { return psx::g_error.c_str(); }
 

@@ -18,6 +18,7 @@
 
 #include "psx_internal.h"
 #include "psx_tet4_elem.h"
+#include "psx_batch.h"
 
 namespace psx {
 
@@ -29,22 +30,18 @@ __global__ void permute_w_kernel(DevPlan p, const double* __restrict__ w) {
   }
 }
 
-template <typename R, int TILE>
-__global__ __launch_bounds__(TILE) void tet4_vjp_kernel(DevPlan p, const double* __restrict__ mu,
-                                                        const double* __restrict__ lam,
-                                                        double* __restrict__ gu,
-                                                        double* __restrict__ gmu,
-                                                        double* __restrict__ glam, int32_t* status) {
+// One tile of node rows (P: DevPlan, or a batch's plan view in the constant
+// address space).
+template <typename R, int TILE, typename P>
+__device__ __forceinline__ void vjp_tile(const P& p, const double* __restrict__ mu, const double* __restrict__ lam,
+                                         double* __restrict__ gu, double* __restrict__ gmu,
+                                         double* __restrict__ glam, int32_t* status, int tile, int t) {
   __shared__ R contrib[3 * TILE];
   __shared__ int32_t s_incp[TILE + 1];
-  const int per = gridDim.x >> 3;
-  const int tile = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
-  if (tile >= p.n_tiles) return;
   const int n0 = p.tile_node[tile], n1 = p.tile_node[tile + 1];
   const int nn = n1 - n0;
   const int k0 = p.inc_ptr[n0];
   const int nk = p.inc_ptr[n1] - k0;
-  const int t = threadIdx.x;
   for (int i = t; i <= nn; i += TILE) s_incp[i] = p.inc_ptr[n0 + i] - k0;
   if (t < nk) {
     const uint32_t pe = p.inc_elem[k0 + t];
@@ -84,6 +81,82 @@ __global__ __launch_bounds__(TILE) void tet4_vjp_kernel(DevPlan p, const double*
 #pragma unroll
     for (int v = 0; v < 3; ++v) gu[3 * a + v] = acc[v];
   }
+}
+
+template <typename R, int TILE>
+__global__ __launch_bounds__(TILE) void tet4_vjp_kernel(DevPlan p, const double* __restrict__ mu,
+                                                        const double* __restrict__ lam,
+                                                        double* __restrict__ gu,
+                                                        double* __restrict__ gmu,
+                                                        double* __restrict__ glam, int32_t* status) {
+  const int tile = xcd_tile();
+  if (tile >= p.n_tiles) return;
+  vjp_tile<R, TILE>(p, mu, lam, gu, gmu, glam, status, tile, int(threadIdx.x));
+}
+
+// The adjoint of many plans in one launch (psx_batch.h): instance i's
+// cotangent in its plan's internal order, then every tile of every plan.
+__global__ __launch_bounds__(256) void batch_permute_w_kernel(BatchDev b, const double* const* __restrict__ w) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= b.n_nodes) return;
+  const int inst = batch_owner<false>(b.node_start, b.n, i);
+  const int k = i - b.node_start[inst];
+  const DevPlan& p = b.plans[inst];
+  const double* wi = w[inst];
+  const int64_t a = p.perm[k];
+  p.wi[k] = make_double4(wi[3 * a], wi[3 * a + 1], wi[3 * a + 2], 0.0);
+}
+
+template <typename R, int TILE>
+__global__ __launch_bounds__(TILE) void tet4_batch_vjp(BatchDev b, const double* const* mu, const double* const* lam,
+                                                       double* const* gu, double* const* gmu,
+                                                       double* const* glam, int32_t* status) {
+  batch_tiles<false>(b, [&](int inst, int tile, int t) {
+    const ConstDevPlan& p = *(const ConstDevPlan*)(b.plans + inst);
+    double* gm = gmu ? load_uniform_val(gmu + inst) : nullptr;
+    double* gl = glam ? load_uniform_val(glam + inst) : nullptr;
+    vjp_tile<R, TILE>(p, load_uniform_val(mu + inst), load_uniform_val(lam + inst), load_uniform_val(gu + inst), gm,
+                      gl, status + 2 * inst, tile, t);
+  });
+}
+
+template <typename R, int TILE>
+static void launch_batch_vjp_t(const BatchDev& b, const double* const* mu, const double* const* lam,
+                               double* const* gu, double* const* gmu, double* const* glam, hipStream_t s,
+                               int32_t* status) {
+  const dim3 grid(unsigned(8 * ((int64_t(b.n_tiles) + 7) / 8)));
+  hipLaunchKernelGGL((tet4_batch_vjp<R, TILE>), grid, dim3(TILE), 0, s, b, mu, lam, gu, gmu, glam, status);
+}
+
+int32_t launch_tet4_batch_vjp(const BatchDev& b, const double* const* u, const double* const* mu,
+                              const double* const* lam, const double* const* w, double* const* gu,
+                              double* const* gmu, double* const* glam, int precision, hipStream_t s,
+                              int32_t* status) {
+  int32_t rc = launch_batch_permute(b, u, s, status);
+  if (rc) return rc;
+  if (b.n_nodes > 0)
+    hipLaunchKernelGGL(batch_permute_w_kernel, dim3((b.n_nodes + 255) / 256), dim3(256), 0, s, b, w);
+  PSX_HIP_TRY(hipGetLastError());
+  if (b.n_tiles <= 0) return PSX_OK;
+  const bool fp32 = precision == PSX_FP32;
+  switch (b.tile) {
+    case 64:
+      fp32 ? launch_batch_vjp_t<float, 64>(b, mu, lam, gu, gmu, glam, s, status)
+           : launch_batch_vjp_t<double, 64>(b, mu, lam, gu, gmu, glam, s, status);
+      break;
+    case 128:
+      fp32 ? launch_batch_vjp_t<float, 128>(b, mu, lam, gu, gmu, glam, s, status)
+           : launch_batch_vjp_t<double, 128>(b, mu, lam, gu, gmu, glam, s, status);
+      break;
+    case 256:
+      fp32 ? launch_batch_vjp_t<float, 256>(b, mu, lam, gu, gmu, glam, s, status)
+           : launch_batch_vjp_t<double, 256>(b, mu, lam, gu, gmu, glam, s, status);
+      break;
+    default:
+      return fail(PSX_EINVAL, "unsupported tile");
+  }
+  PSX_HIP_TRY(hipGetLastError());
+  return PSX_OK;
 }
 
 

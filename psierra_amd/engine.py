@@ -430,6 +430,31 @@ class AssemblyBatch:
 
         return launch
 
+    def vjp(self, us, mus, lams, ws, precision="fp64", want_params=True, check_status=True):
+        """Per-instance lists (K_T w, d(w^T f)/dmu, d(w^T f)/dlambda) of a Tet4
+        batch in one launch (``psx_vjp_batch``), each the bits of its plan's
+        own ``vjp``."""
+        torch = _torch()
+        n = len(self.plans)
+        if not (len(us) == len(mus) == len(lams) == len(ws) == n):
+            raise ValueError(f"vjp needs {n} displacements, parameter pairs and cotangents")
+        us = [p._device_f64(u, p.n_dofs, "u") for p, u in zip(self.plans, us)]
+        ws = [p._device_f64(w, p.n_dofs, "w") for p, w in zip(self.plans, ws)]
+        mus = [p._device_f64(m, p.n_elements, "mu") for p, m in zip(self.plans, mus)]
+        lams = [p._device_f64(m, p.n_elements, "lam") for p, m in zip(self.plans, lams)]
+        gus = [torch.empty(p.n_dofs, dtype=torch.float64, device=self.device) for p in self.plans]
+        gmus = glams = None
+        if want_params:
+            gmus = [torch.empty(p.n_elements, dtype=torch.float64, device=self.device) for p in self.plans]
+            glams = [torch.empty(p.n_elements, dtype=torch.float64, device=self.device) for p in self.plans]
+        tables = [self._table(x) for x in (us, mus, lams, ws, gus)]
+        tables += [self._table(gmus) if gmus else None, self._table(glams) if glams else None]
+        check(lib().psx_vjp_batch(self._h, *[ptr(t) for t in tables], precision_code(precision),
+                                  current_stream_handle(), ptr(self.status)))
+        if check_status:
+            self.raise_on_status(us)
+        return gus, gmus, glams
+
     def raise_on_status(self, us):
         st = self.status.cpu().numpy().reshape(-1, 2)
         for i, (flag, e) in enumerate(st):

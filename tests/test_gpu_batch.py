@@ -286,3 +286,32 @@ def test_tet10_batch_bit_identical_to_single_plans(precision):
     assert rel_err(fs[0].cpu().numpy(), f_ref) < tol_f
     with pytest.raises(ValueError, match="Tet10 plans bind no material"):
         batch.assemble(us, precision=precision)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_batch_vjp_bit_identical_to_single_plans(meshes, precision):
+    """The adjoint of every instance in one launch: the bits of each plan's
+    own vjp (one plan twice, with different inputs); instance 0's K_T w
+    against the oracle's assembled K_T times w."""
+    import torch
+
+    from psierra_amd.engine import AssemblyBatch
+
+    wls, plans = meshes
+    us, mus, lams = _inputs(wls, plans, seed=8)
+    rng = np.random.default_rng(12)
+    ws = [torch.as_tensor(rng.standard_normal(p.n_dofs), device=p.device) for p in plans]
+    banner(f"batch: adjoint of {len(plans)} instances {precision}")
+    batch = AssemblyBatch(plans)
+    gus, gmus, glams = batch.vjp(us, mus, lams, ws, precision=precision)
+    gus2, none_mu, none_lam = batch.vjp(us, mus, lams, ws, precision=precision, want_params=False)
+    assert none_mu is None and none_lam is None
+    torch.cuda.synchronize()
+    for i, p in enumerate(plans):
+        gu, gmu, glam = p.vjp(us[i], mus[i], lams[i], ws[i], precision=precision)
+        assert torch.equal(gus[i], gu) and torch.equal(gmus[i], gmu) and torch.equal(glams[i], glam), i
+        assert torch.equal(gus2[i], gu), i
+    w = wls[0]
+    _, K_ref = A.assemble(w.mesh.nodes, w.mesh.elements, w.u, mus[0].cpu().numpy(), lams[0].cpu().numpy())
+    ref = K_ref @ ws[0].cpu().numpy()
+    assert rel_err(gus[0].cpu().numpy(), ref) < (1e-12 if precision == "fp64" else 1e-5)

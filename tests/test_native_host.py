@@ -30,7 +30,7 @@ def test_library_exports_header_symbols():
     for s in syms:
         assert hasattr(L, s), s
         assert s in _lib.SIGNATURES, f"{s} has no ctypes signature"
-    assert L.psx_abi_version() == _lib.ABI_VERSION == 6
+    assert L.psx_abi_version() == _lib.ABI_VERSION == 7
 
 
 def _random_mesh(seed, npe):
