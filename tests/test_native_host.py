@@ -98,3 +98,25 @@ def test_fbar_pattern_bit_exact_vs_oracle(seed):
     ip1, ix1 = pattern_host(conn, n_nodes)
     for r in range(0, 3 * n_nodes, 7):
         assert set(ix1[ip1[r]:ip1[r + 1]]) <= set(ix[ip[r]:ip[r + 1]])
+
+
+def test_batch_entry_points_reject_bad_arguments_before_any_device_call():
+    """The batch entry points check their arguments first (no GPU needed):
+    null or empty plan lists, null handles and pointer tables, mismatched
+    parameter tables, an unknown precision."""
+    L = _lib.lib()
+    out = ctypes.c_void_p()
+    assert L.psx_batch_create(None, 0, ctypes.byref(out)) == _lib.PSX_EINVAL
+    assert "n >= 1" in _lib.last_error()
+    plans = (ctypes.c_void_p * 2)(None, None)
+    assert L.psx_batch_create(plans, 2, ctypes.byref(out)) == _lib.PSX_EINVAL
+    assert "null plan" in _lib.last_error()
+    assert L.psx_batch_destroy(None) == _lib.PSX_OK
+    assert L.psx_batch_set_persistent(None, 1) == _lib.PSX_EINVAL
+    info = (ctypes.c_int64 * 6)()
+    assert L.psx_batch_info(None, info, 6) == _lib.PSX_EINVAL
+    tab = (ctypes.c_void_p * 1)(None)
+    assert L.psx_assemble_batch(None, tab, None, None, tab, None, 0, None, tab) == _lib.PSX_EINVAL
+    assert "null argument to psx_assemble_batch" in _lib.last_error()
+    assert L.psx_vjp_batch(None, tab, tab, tab, tab, tab, None, None, 0, None, tab) == _lib.PSX_EINVAL
+    assert "null argument to psx_vjp_batch" in _lib.last_error()
