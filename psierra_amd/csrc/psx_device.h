@@ -21,6 +21,11 @@ __device__ __forceinline__ int4 load_uniform(const int4* p, int i) {
   return make_int4(q[0], q[1], q[2], q[3]);
 }
 
+__device__ __forceinline__ int2 load_uniform(const int2* p, int i) {
+  const auto* q = (const __attribute__((address_space(4))) int*)(p + i);
+  return make_int2(q[0], q[1]);
+}
+
 // Lane exchange inside groups of four lanes (DPP quad permutation; 0xB1
 // swaps neighbours, 0x4E swaps pairs), for a double.
 // Lanes whose source falls outside the row read 0 (bound_ctrl).

@@ -106,7 +106,18 @@ struct DevPlan {
   double* gen_fe;                 // E x 30 element forces (scratch)
   double4* ui;                    // scratch: displacement in internal order
   double4* wi;                    // scratch: adjoint cotangent in internal order
+  // Tet4 tiles of a plan that is not wide: each tile's halo (the nodes of its
+  // elements outside the tile) so the FP32 row kernels stage the tile's
+  // displacements in LDS once per node instead of gathering four per
+  // incidence.
+  const int2* tile_halo;          // n_tiles: first halo node, halo nodes (-1: too many nodes to stage)
+  const int32_t* halo;            // the tiles' halo nodes (internal ids, ascending per tile)
+  const uint32_t* inc_loc;        // per incidence: tile-local index (own nodes first, then the halo)
+                                  //   of the element's nodes in the row's order, byte q
 };
+
+// Own + halo nodes whose displacements a Tet4 row tile stages in LDS.
+constexpr int kTileLdsNodes = 128;
 
 // Device view of a batch of Tet4 plans assembled by one persistent launch:
 // the tiles of all plans form one global list (plan i owns tiles

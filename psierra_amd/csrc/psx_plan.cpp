@@ -11,6 +11,7 @@
 // with b_s the s-th neighbour of a ("slot rank" s).
 #include <algorithm>
 #include <atomic>
+#include <climits>
 #include <array>
 #include <cmath>
 #include <cstdio>
@@ -424,6 +425,89 @@ int32_t make_general_maps(psx_plan* plan, const int64_t* conn, int64_t E) {
   return PSX_OK;
 }
 
+// Halo of each Tet4 tile (a plan that is not wide): the nodes of the tile's
+// elements outside the tile, ascending, and per incidence the tile-local
+// index (the tile's own nodes 0 .. nodes-1, then its halo) of the element's
+// four nodes in the row's order -- the row node, then the others by local
+// index -- one byte each.  A tile with more than kTileLdsNodes own + halo
+// nodes gets halo count -1 (its rows read the device copy).
+int32_t make_tile_halos(psx_plan* plan, const std::vector<int32_t>& tiles) {
+  const HostPattern& hp = plan->hp;
+  const int64_t N = hp.N, E = hp.E;
+  std::vector<int32_t> inv(static_cast<size_t>(N));
+  for (int64_t i = 0; i < N; ++i) inv[plan->perm_h[i]] = int32_t(i);
+  std::vector<int32_t> conn_i(static_cast<size_t>(E) * 4);
+  for (int64_t a = 0; a < N; ++a)
+    for (int32_t t = hp.n2e_ptr[a]; t < hp.n2e_ptr[a + 1]; ++t) conn_i[hp.n2e[t]] = inv[a];
+  const int64_t T = int64_t(tiles.size()) - 1;
+  std::vector<std::vector<int32_t>> halos(static_cast<size_t>(T));
+  std::vector<uint32_t> loc(size_t(std::max<int64_t>(plan->n_inc, 1)), 0u);
+  std::vector<int2> th(static_cast<size_t>(std::max<int64_t>(T, 1)));
+  parallel_for(T, [&](int64_t tb, int64_t te) {
+    for (int64_t t = tb; t < te; ++t) {
+      const int32_t n0 = tiles[t], n1 = tiles[t + 1];
+      std::vector<int32_t>& h = halos[t];
+      for (int32_t i = n0; i < n1; ++i) {
+        const int32_t a = plan->perm_h[i];
+        for (int32_t r = hp.n2e_ptr[a]; r < hp.n2e_ptr[a + 1]; ++r) {
+          const int64_t el = hp.n2e[r] / 4;
+          for (int q = 0; q < 4; ++q) {
+            const int32_t n = conn_i[4 * el + q];
+            if (n < n0 || n >= n1) h.push_back(n);
+          }
+        }
+      }
+      std::sort(h.begin(), h.end());
+      h.erase(std::unique(h.begin(), h.end()), h.end());
+      if (int64_t(n1 - n0) + int64_t(h.size()) > kTileLdsNodes) {
+        h.clear();
+        th[t] = make_int2(0, -1);
+        continue;
+      }
+      th[t] = make_int2(0, int32_t(h.size()));
+      auto local = [&](int32_t n) -> uint32_t {
+        if (n >= n0 && n < n1) return uint32_t(n - n0);
+        return uint32_t((n1 - n0) + (std::lower_bound(h.begin(), h.end(), n) - h.begin()));
+      };
+      for (int32_t i = n0; i < n1; ++i) {
+        const int32_t a = plan->perm_h[i];
+        int64_t k = plan->inc_ptr_h[i];
+        for (int32_t r = hp.n2e_ptr[a]; r < hp.n2e_ptr[a + 1]; ++r, ++k) {
+          const int64_t el = hp.n2e[r] / 4;
+          const int p = hp.n2e[r] % 4;
+          uint32_t w = local(conn_i[4 * el + p]);
+          int q = 1;
+          for (int b = 0; b < 4; ++b) {
+            if (b == p) continue;
+            w |= local(conn_i[4 * el + b]) << (8 * q);
+            ++q;
+          }
+          loc[k] = w;
+        }
+      }
+    }
+  });
+  int64_t nh = 0;
+  for (int64_t t = 0; t < T; ++t) {
+    th[t].x = int32_t(nh);
+    nh += int64_t(halos[t].size());
+  }
+  if (nh > INT32_MAX) return fail(PSX_EUNSUPPORTED, "tile halos exceed 32-bit offsets");
+  std::vector<int32_t> hall(size_t(std::max<int64_t>(nh, 1)));
+  for (int64_t t = 0; t < T; ++t) std::copy(halos[t].begin(), halos[t].end(), hall.begin() + th[t].x);
+  int2* dth = nullptr;
+  int32_t* dh = nullptr;
+  uint32_t* dloc = nullptr;
+  int32_t rc;
+  if ((rc = upload(plan, th.data(), th.size(), &dth)) || (rc = upload(plan, hall.data(), hall.size(), &dh)) ||
+      (rc = upload(plan, loc.data(), loc.size(), &dloc)))
+    return rc;
+  plan->d.tile_halo = dth;
+  plan->d.halo = dh;
+  plan->d.inc_loc = dloc;
+  return PSX_OK;
+}
+
 int32_t make_tiles(psx_plan* plan, int32_t tile) {
   if (tile == 0) tile = plan->hp.npe == 4 ? 256 : 64;
   if (tile != 64 && tile != 128 && tile != 256)
@@ -474,6 +558,7 @@ int32_t make_tiles(psx_plan* plan, int32_t tile) {
   if (rc) return rc;
   if ((rc = upload(plan, info.data(), info.size(), &dinfo))) return rc;
   if (plan->hp.npe == 4 && wide && (rc = make_tet4_positions(plan, tiles))) return rc;
+  if (plan->hp.npe == 4 && !wide && (rc = make_tile_halos(plan, tiles))) return rc;
   plan->d.tile_node = dtiles;
   plan->d.tile_info = dinfo;
   plan->d.n_tiles = int32_t(tiles.size() - 1);

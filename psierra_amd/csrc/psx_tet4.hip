@@ -67,11 +67,28 @@ struct IncInputs {
 // P is DevPlan (a kernel argument) or a constant-address-space DevPlan (a
 // batch's plan table): either way each field is read through the scalar unit
 // where it is used.
-template <typename R, int M, typename P>
+// WITH_U = false (FP32 rows of a tile that stages its displacements in
+// LDS): no connectivity and no displacement loads; the caller fills in.U.
+template <typename R, int M, typename P, bool WITH_U = true>
 __device__ __forceinline__ void load_incidence(const P& p, const MatParams& mp, int k, IncInputs<R>& in) {
   in.pe = p.inc_elem[k];
   in.rank = p.inc_rank[k];
   const int64_t e = int64_t(in.pe & kIncElemMask);
+  if constexpr (!WITH_U && std::is_same<R, float>::value && M != kFbarNeoHookean) {
+    load_params<M>(mp, e, in.pr);
+    const int lp = int(in.pe >> kIncElemBits);
+    const int r1 = lp == 0 ? 1 : 0, r2 = lp <= 1 ? 2 : 1, r3 = lp <= 2 ? 3 : 2;
+    const float4* geo = M == kNeoHookeanBound ? p.geo_bound : p.geo;
+    in.G[0] = geo[4 * e + lp];
+    in.G[1] = geo[4 * e + r1];
+    in.G[2] = geo[4 * e + r2];
+    in.G[3] = geo[4 * e + r3];
+    if constexpr (M == kNeoHookeanBound) {
+      in.pr[0] = in.G[1].w;
+      in.pr[1] = in.G[3].w;
+    }
+    return;
+  }
   const int4 c = *reinterpret_cast<const int4*>(p.conn + 4 * e);
   load_params<M>(mp, e, in.pr);
   if constexpr (M == kFbarNeoHookean) {
@@ -216,13 +233,41 @@ __device__ __forceinline__ void rows_tile(const P& p, const MatParams& mp, const
   __shared__ uint8_t s_nz[NN];
   __shared__ uint8_t s_first[TILE];
   __shared__ int32_t s_f2[FBAR ? 2 * NN : 1];   // F-bar: diagonal block offset, row length
+  // FP32 rows stage the displacements of the tile's nodes and its halo in
+  // LDS (one gather per node instead of four per incidence), unless the
+  // tile has more than kTileLdsNodes of them
+  constexpr bool LDSU = std::is_same<R, float>::value && !FBAR;
+  constexpr int NLU = kTileLdsNodes;
+  __shared__ double s_u[LDSU ? 3 * NLU : 1];
 
   const int4 ti = load_uniform(p.tile_info, tile);
   const int n0 = ti.x, nn = ti.y, k0 = ti.z, nk = ti.w;
   const int lane = t & 63, wave = t >> 6;
+  int2 th = make_int2(0, -1);
+  if constexpr (LDSU) th = load_uniform(p.tile_halo, tile);
+  const bool lds_u = LDSU && th.y >= 0;
 
   IncInputs<R> in;
-  if (t < nk) load_incidence<R, M>(p, mp, k0 + t, in);
+  uint32_t loc = 0u;
+  if (t < nk) {
+    if (lds_u) {
+      load_incidence<R, M, P, false>(p, mp, k0 + t, in);
+      loc = p.inc_loc[k0 + t];
+    } else {
+      load_incidence<R, M>(p, mp, k0 + t, in);
+    }
+  }
+  if constexpr (LDSU) {
+    if (lds_u) {
+      for (int i = t; i < nn + th.y; i += TILE) {
+        const int node = i < nn ? n0 + i : p.halo[th.x + i - nn];
+        const double4 u = p.ui[node];
+        s_u[i] = u.x;
+        s_u[NLU + i] = u.y;
+        s_u[2 * NLU + i] = u.z;
+      }
+    }
+  }
   // node table
   int ub = 0, ue = 0;
   if (t < nn) {
@@ -260,6 +305,15 @@ __device__ __forceinline__ void rows_tile(const P& p, const MatParams& mp, const
   // ---- phase 1: incidence t
   const int u0 = s_uoff[0];
   if (t < nn && ue > ub) atomicOr(&s_ustart[(ub - u0) >> 6], 1ull << ((ub - u0) & 63));
+  if constexpr (LDSU) {
+    if (lds_u && t < nk) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int l = int((loc >> (8 * q)) & 0xffu);
+        in.U[q] = make_double4(s_u[l], s_u[NLU + l], s_u[2 * NLU + l], 0.0);
+      }
+    }
+  }
   R keep[NV];   // pass B values: the blocks at q = 2 and 3
   if (t < nk) {
     R out[39];
