@@ -144,11 +144,48 @@ __device__ __forceinline__ void force_tile(const P& p, const MatParams& mp, doub
                                            int32_t* status, int tile, int t) {
   __shared__ R contrib[3 * TILE];
   __shared__ int32_t s_incp[TILE + 1];
+  // FP32: the displacements of the tile's nodes and halo staged in LDS, as
+  // in rows_tile (plans that are not wide have the tile halos)
+  constexpr bool LDSU = std::is_same<R, float>::value && M != kFbarNeoHookean;
+  constexpr int NLU = kTileLdsNodes;
+  __shared__ double s_u[LDSU ? 3 * NLU : 1];
   const int4 ti = load_uniform(p.tile_info, tile);
   const int n0 = ti.x, nn = ti.y, k0 = ti.z, nk = ti.w;
+  int2 th = make_int2(0, -1);
+  if constexpr (LDSU) {
+    if (p.tile_halo) th = load_uniform(p.tile_halo, tile);
+  }
+  const bool lds_u = LDSU && th.y >= 0;
   IncInputs<R> in;
-  if (t < nk) load_incidence<R, M>(p, mp, k0 + t, in);
+  uint32_t loc = 0u;
+  if (t < nk) {
+    if (lds_u) {
+      load_incidence<R, M, P, false>(p, mp, k0 + t, in);
+      loc = p.inc_loc[k0 + t];
+    } else {
+      load_incidence<R, M>(p, mp, k0 + t, in);
+    }
+  }
   for (int i = t; i <= nn; i += TILE) s_incp[i] = p.inc_ptr[n0 + i] - k0;
+  if constexpr (LDSU) {
+    if (lds_u) {
+      for (int i = t; i < nn + th.y; i += TILE) {
+        const int node = i < nn ? n0 + i : p.halo[th.x + i - nn];
+        const double4 u = p.ui[node];
+        s_u[i] = u.x;
+        s_u[NLU + i] = u.y;
+        s_u[2 * NLU + i] = u.z;
+      }
+      __syncthreads();
+      if (t < nk) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int l = int((loc >> (8 * q)) & 0xffu);
+          in.U[q] = make_double4(s_u[l], s_u[NLU + l], s_u[2 * NLU + l], 0.0);
+        }
+      }
+    }
+  }
   if (t < nk) {
     R out[3];
     const R J = incidence_row<R, M, false>(in, out);
