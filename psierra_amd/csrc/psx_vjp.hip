@@ -16,6 +16,7 @@
 // no atomics.
 #include <climits>
 
+#include "psx_device.h"
 #include "psx_internal.h"
 #include "psx_tet4_elem.h"
 #include "psx_batch.h"
@@ -38,24 +39,62 @@ __device__ __forceinline__ void vjp_tile(const P& p, const double* __restrict__ 
                                          double* __restrict__ glam, int32_t* status, int tile, int t) {
   __shared__ R contrib[3 * TILE];
   __shared__ int32_t s_incp[TILE + 1];
+  // the coordinates, displacements and cotangents of the tile's nodes and
+  // halo staged in LDS (plans that are not wide, tiles of at most
+  // kTileLdsNodes nodes): one gather per node instead of twelve per incidence
+  constexpr int NLU = kTileLdsNodes;
+  __shared__ double s_n[9 * NLU];
   const int n0 = p.tile_node[tile], n1 = p.tile_node[tile + 1];
   const int nn = n1 - n0;
   const int k0 = p.inc_ptr[n0];
   const int nk = p.inc_ptr[n1] - k0;
+  int2 th = make_int2(0, -1);
+  if (p.tile_halo) th = load_uniform(p.tile_halo, tile);
+  const bool lds = th.y >= 0;
   for (int i = t; i <= nn; i += TILE) s_incp[i] = p.inc_ptr[n0 + i] - k0;
+  uint32_t pe = 0u, loc = 0u;
   if (t < nk) {
-    const uint32_t pe = p.inc_elem[k0 + t];
+    pe = p.inc_elem[k0 + t];
+    if (lds) loc = p.inc_loc[k0 + t];
+  }
+  if (lds) {
+    for (int i = t; i < nn + th.y; i += TILE) {
+      const int node = i < nn ? n0 + i : p.halo[th.x + i - nn];
+      const double4 x = p.Xi[node], u = p.ui[node], w = p.wi[node];
+      s_n[i] = x.x;
+      s_n[NLU + i] = x.y;
+      s_n[2 * NLU + i] = x.z;
+      s_n[3 * NLU + i] = u.x;
+      s_n[4 * NLU + i] = u.y;
+      s_n[5 * NLU + i] = u.z;
+      s_n[6 * NLU + i] = w.x;
+      s_n[7 * NLU + i] = w.y;
+      s_n[8 * NLU + i] = w.z;
+    }
+    __syncthreads();
+  }
+  if (t < nk) {
     const int e = int(pe & kIncElemMask);
     const int lp = int(pe >> kIncElemBits);
-    const int4 c = *reinterpret_cast<const int4*>(p.conn + 4 * int64_t(e));
-    const int ids[4] = {lp == 0 ? c.x : lp == 1 ? c.y : lp == 2 ? c.z : c.w, lp == 0 ? c.y : c.x,
-                        lp <= 1 ? c.z : c.y, lp <= 2 ? c.w : c.z};
     double4 X[4], U[4], W[4];
+    if (lds) {
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      X[b] = p.Xi[ids[b]];
-      U[b] = p.ui[ids[b]];
-      W[b] = p.wi[ids[b]];
+      for (int b = 0; b < 4; ++b) {
+        const int l = int((loc >> (8 * b)) & 0xffu);
+        X[b] = make_double4(s_n[l], s_n[NLU + l], s_n[2 * NLU + l], 0.0);
+        U[b] = make_double4(s_n[3 * NLU + l], s_n[4 * NLU + l], s_n[5 * NLU + l], 0.0);
+        W[b] = make_double4(s_n[6 * NLU + l], s_n[7 * NLU + l], s_n[8 * NLU + l], 0.0);
+      }
+    } else {
+      const int4 c = *reinterpret_cast<const int4*>(p.conn + 4 * int64_t(e));
+      const int ids[4] = {lp == 0 ? c.x : lp == 1 ? c.y : lp == 2 ? c.z : c.w, lp == 0 ? c.y : c.x,
+                          lp <= 1 ? c.z : c.y, lp <= 2 ? c.w : c.z};
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        X[b] = p.Xi[ids[b]];
+        U[b] = p.ui[ids[b]];
+        W[b] = p.wi[ids[b]];
+      }
     }
     const double m = mu[e], l = lam[e];
     R out[3], dmu, dlam;
