@@ -80,12 +80,28 @@ def test_isolated_nodes_and_single_element(precision):
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 @pytest.mark.parametrize("n_elem", [32, 33, 64])
 def test_node_with_many_incidences(precision, n_elem):
-    # 32 is the largest run the 32-bit feed masks take; 33 and 64 use the
-    # 64-bit masks
+    """32 is the largest run the 32-bit feed masks take; 33 and 64 use the
+    64-bit masks.  The fan's tiles also cover both ways a tile reads its
+    nodes: the tile holding the centre of the 64-element fan has 193 own and
+    halo nodes, beyond the 128 a tile stages in LDS, so it gathers from the
+    device copy, while the other tiles (and the smaller fans) stage theirs.
+    f + K, the f-only kernel and the adjoint K_T w against the oracle."""
+    import torch
+
     wl = _fan(n_elem, seed=8)
     banner(f"edge: fan of {n_elem} elements {precision}")
-    plan, _, _, _ = _check(wl, precision)
+    plan, f, _, _ = _check(wl, precision)
     assert plan.info["tile"] == 256
+    dev = plan.device
+    u = torch.as_tensor(wl.u.reshape(-1), device=dev)
+    mu = torch.as_tensor(wl.mu, device=dev)
+    lam = torch.as_tensor(wl.lam, device=dev)
+    f_only, _ = plan.assemble(u, mu, lam, precision=precision, want_k=False)
+    assert rel_err(f_only.cpu().numpy(), f) < (1e-13 if precision == "fp64" else 1e-6)
+    _, K_ref = A.assemble(wl.mesh.nodes, wl.mesh.elements, wl.u, wl.mu, wl.lam)
+    w = np.random.default_rng(n_elem).standard_normal(plan.n_dofs)
+    gu, _, _ = plan.vjp(u, mu, lam, torch.as_tensor(w, device=dev), precision=precision, want_params=False)
+    assert rel_err(gu.cpu().numpy(), K_ref @ w) < (1e-12 if precision == "fp64" else 1e-5)
 
 
 @pytest.mark.parametrize("n_elem", [21, 64])
