@@ -219,11 +219,12 @@ __host__ __device__ constexpr int lds_source(int v) {
 
 // Sum of one value over the eight lanes of an aligned group (quads first,
 // then the two quads), the same bits in every lane of the group.
-__device__ __forceinline__ double oct_sum(double x, bool upper_quad) {
+// After the quad sums, lanes 0..3 of the group hold q0 and lanes 4..7 hold
+// q1; row_half_mirror gives each lane the other quad's sum, and q0 + q1 and
+// q1 + q0 are the same bits (IEEE addition commutes).
+__device__ __forceinline__ double oct_sum(double x) {
   x = quad_sum(x);
-  const double from_up = quad_dpp<0x104>(x);   // row_shl:4, lane i <- lane i + 4
-  const double from_lo = quad_dpp<0x114>(x);   // row_shr:4, lane i <- lane i - 4
-  return upper_quad ? from_lo + x : x + from_up;
+  return x + quad_dpp<0x141>(x);   // row_half_mirror: lane i <- lane 7 - i of its 8
 }
 
 // f_int + K_T, one tile of node rows per work-group (see the file comment).
@@ -447,9 +448,8 @@ __device__ __forceinline__ void rows_tile(const P& p, const MatParams& mp, const
   }
   if (diag) {
     // the eight partial sums of the node, combined in a fixed order
-    const bool upper = (t & 4) != 0;
 #pragma unroll
-    for (int c = 0; c < 9; ++c) acc[c] = oct_sum(acc[c], upper);
+    for (int c = 0; c < 9; ++c) acc[c] = oct_sum(acc[c]);
     const int part = int(unsigned(t) % ND);
     const int d = FBAR ? s_f2[NN + j] : s_uoff[j + 1] - s_uoff[j];
     if (part == 3) {
@@ -600,7 +600,7 @@ __global__ __launch_bounds__(TILE) void tet4_rows(DevPlan p, MatParams mp, doubl
       const int b = kb + (part * L) / ND, e = kb + ((part + 1) * L) / ND;
       sum_blocks(acc, b, e - b);
 #pragma unroll
-      for (int c = 0; c < 9; ++c) acc[c] = oct_sum(acc[c], (w & 4) != 0);
+      for (int c = 0; c < 9; ++c) acc[c] = oct_sum(acc[c]);
       if (part == 3) {
         const int64_t a = s_user[j];
 #pragma unroll

@@ -1,6 +1,8 @@
 // Tet4 element routines shared by the assembly and adjoint kernels.
 #pragma once
 
+#include <type_traits>
+
 #include "psx_nh.h"
 
 namespace psx {
@@ -230,7 +232,14 @@ __device__ __forceinline__ R tet4_row_core_m(const R h[4][3], R V, const R A[3][
         for (int i = 0; i < 3; ++i)
 #pragma unroll
           for (int k = (b == 0 ? i : 0); k < 3; ++k) {
-            R v = P[i] * g[b][k] + g[b][i] * Rv[k] + T[i] * q[b][k] + (i == k ? sb : R(0));
+            // FP32: the diagonal term first, so a diagonal value is three
+            // fused multiply-adds on sb (an off-diagonal one a product and
+            // two); FP64 keeps the order that holds it to 4 waves per SIMD
+            R v;
+            if constexpr (std::is_same<R, float>::value)
+              v = (i == k ? sb : R(0)) + P[i] * g[b][k] + g[b][i] * Rv[k] + T[i] * q[b][k];
+            else
+              v = P[i] * g[b][k] + g[b][i] * Rv[k] + T[i] * q[b][k] + (i == k ? sb : R(0));
             if constexpr (M == kMooneyRivlin)
               v += Qm[i] * q[b][k] + hh * x.b[i][k] + V * x.cb * q[b][i] * q[0][k] +
                    V * x.cp * g[0][i] * pb[k] + Pm[i] * g[b][k];
