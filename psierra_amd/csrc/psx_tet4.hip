@@ -69,24 +69,29 @@ struct IncInputs {
 // where it is used.
 // WITH_U = false (FP32 rows of a tile that stages its displacements in
 // LDS): no connectivity and no displacement loads; the caller fills in.U.
+// The element part of an FP32 incidence of a tile that stages its nodes
+// (in.pe already loaded): the material parameters and the four (h_b, .)
+// records in the row's order.
+template <int M, typename P>
+__device__ __forceinline__ void load_incidence_elem(const P& p, const MatParams& mp, IncInputs<float>& in) {
+  const int64_t e = int64_t(in.pe & kIncElemMask);
+  load_params<M>(mp, e, in.pr);
+  const int lp = int(in.pe >> kIncElemBits);
+  const int r1 = lp == 0 ? 1 : 0, r2 = lp <= 1 ? 2 : 1, r3 = lp <= 2 ? 3 : 2;
+  const float4* geo = M == kNeoHookeanBound ? p.geo_bound : p.geo;
+  in.G[0] = geo[4 * e + lp];
+  in.G[1] = geo[4 * e + r1];
+  in.G[2] = geo[4 * e + r2];
+  in.G[3] = geo[4 * e + r3];
+}
+
 template <typename R, int M, typename P, bool WITH_U = true>
 __device__ __forceinline__ void load_incidence(const P& p, const MatParams& mp, int k, IncInputs<R>& in) {
   in.pe = p.inc_elem[k];
   in.rank = p.inc_rank[k];
   const int64_t e = int64_t(in.pe & kIncElemMask);
   if constexpr (!WITH_U && std::is_same<R, float>::value && M != kFbarNeoHookean) {
-    load_params<M>(mp, e, in.pr);
-    const int lp = int(in.pe >> kIncElemBits);
-    const int r1 = lp == 0 ? 1 : 0, r2 = lp <= 1 ? 2 : 1, r3 = lp <= 2 ? 3 : 2;
-    const float4* geo = M == kNeoHookeanBound ? p.geo_bound : p.geo;
-    in.G[0] = geo[4 * e + lp];
-    in.G[1] = geo[4 * e + r1];
-    in.G[2] = geo[4 * e + r2];
-    in.G[3] = geo[4 * e + r3];
-    if constexpr (M == kNeoHookeanBound) {
-      in.pr[0] = in.G[1].w;
-      in.pr[1] = in.G[3].w;
-    }
+    load_incidence_elem<M>(p, mp, in);
     return;
   }
   const int4 c = *reinterpret_cast<const int4*>(p.conn + 4 * e);
@@ -109,11 +114,6 @@ __device__ __forceinline__ void load_incidence(const P& p, const MatParams& mp, 
     in.G[1] = geo[4 * e + r1];
     in.G[2] = geo[4 * e + r2];
     in.G[3] = geo[4 * e + r3];
-    if constexpr (M == kNeoHookeanBound) {
-      // local node r1 is 0 or 1 (V mu), r3 is 2 or 3 (V K) for every row
-      in.pr[0] = in.G[1].w;
-      in.pr[1] = in.G[3].w;
-    }
   } else {
     in.X[0] = p.Xi[i0];
     in.X[1] = p.Xi[i1];
@@ -130,7 +130,13 @@ __device__ __forceinline__ void load_incidence(const P& p, const MatParams& mp, 
 // nodes (row-major 3x3 each), out[36..38] = f_a.  Returns det F.
 template <typename R, int M, bool WANT_K>
 __device__ __forceinline__ R incidence_row(const IncInputs<R>& in, R* out) {
-  const R prm[4] = {R(in.pr[0]), R(in.pr[1]), R(in.pr[2]), R(in.pr[3])};
+  R prm[4] = {R(in.pr[0]), R(in.pr[1]), R(in.pr[2]), R(in.pr[3])};
+  if constexpr (std::is_same<R, float>::value && M == kNeoHookeanBound) {
+    // the bound material: a row's second node is local node 0 or 1 (V mu),
+    // its last node local node 2 or 3 (V K), whatever the row node
+    prm[0] = in.G[1].w;
+    prm[1] = in.G[3].w;
+  }
   if constexpr (std::is_same<R, float>::value)
     return tet4_row_grad_m<M, WANT_K>(in.G, in.U, prm, out);
   else
@@ -284,32 +290,63 @@ __device__ __forceinline__ void rows_tile(const P& p, const MatParams& mp, const
   int2 th = make_int2(0, -1);
   if constexpr (LDSU) th = load_uniform(p.tile_halo, tile);
   const bool lds_u = LDSU && th.y >= 0;
+  // the node records first: their loads join the first round below
+  int4 nr0 = make_int4(0, 0, 0, 0), nr1 = make_int4(0, 0, 0, 0);
+  if (t < nn) {
+    nr0 = p.node_rec[n0 + t];
+    nr1 = p.node_rec[n0 + t + 1];
+  }
 
   IncInputs<R> in;
   uint32_t loc = 0u;
-  if (t < nk) {
+  if constexpr (LDSU && TILE >= NLU) {
     if (lds_u) {
-      load_incidence<R, M, P, false>(p, mp, k0 + t, in);
-      loc = p.inc_loc[k0 + t];
-    } else {
+      // Two rounds of loads, each issued before anything waits on the
+      // other: the incidence records and this thread's staging node, then
+      // the element records and the node's displacement.
+      if (t < nk) {
+        in.pe = p.inc_elem[k0 + t];
+        in.rank = p.inc_rank[k0 + t];
+        loc = p.inc_loc[k0 + t];
+      }
+      int node = -1;
+      if (t < nn + th.y) node = t < nn ? n0 + t : p.halo[th.x + t - nn];
+      double4 u = make_double4(0.0, 0.0, 0.0, 0.0);
+      if (node >= 0) u = p.ui[node];
+      if (t < nk) load_incidence_elem<M>(p, mp, in);
+      if (node >= 0) {
+        s_u[t] = u.x;
+        s_u[NLU + t] = u.y;
+        s_u[2 * NLU + t] = u.z;
+      }
+    } else if (t < nk) {
       load_incidence<R, M>(p, mp, k0 + t, in);
     }
-  }
-  if constexpr (LDSU) {
-    if (lds_u) {
-      for (int i = t; i < nn + th.y; i += TILE) {
-        const int node = i < nn ? n0 + i : p.halo[th.x + i - nn];
-        const double4 u = p.ui[node];
-        s_u[i] = u.x;
-        s_u[NLU + i] = u.y;
-        s_u[2 * NLU + i] = u.z;
+  } else {
+    if (t < nk) {
+      if (lds_u) {
+        load_incidence<R, M, P, false>(p, mp, k0 + t, in);
+        loc = p.inc_loc[k0 + t];
+      } else {
+        load_incidence<R, M>(p, mp, k0 + t, in);
+      }
+    }
+    if constexpr (LDSU) {
+      if (lds_u) {
+        for (int i = t; i < nn + th.y; i += TILE) {
+          const int node = i < nn ? n0 + i : p.halo[th.x + i - nn];
+          const double4 u = p.ui[node];
+          s_u[i] = u.x;
+          s_u[NLU + i] = u.y;
+          s_u[2 * NLU + i] = u.z;
+        }
       }
     }
   }
   // node table
   int ub = 0, ue = 0;
   if (t < nn) {
-    const int4 r0 = p.node_rec[n0 + t], r1 = p.node_rec[n0 + t + 1];
+    const int4 r0 = nr0, r1 = nr1;
     const int kb = r0.x - k0, ke = r1.x - k0;
     s_incp[t] = kb;
     s_uoff[t] = r0.y;
