@@ -164,7 +164,27 @@ __device__ __forceinline__ void force_tile(const P& p, const MatParams& mp, doub
   const bool lds_u = LDSU && th.y >= 0;
   IncInputs<R> in;
   uint32_t loc = 0u;
-  if (t < nk) {
+  int incp = 0;
+  if (t <= nn) incp = p.inc_ptr[n0 + t];
+  if (lds_u && TILE >= NLU) {
+    // two rounds of loads, as in rows_tile
+    if (t < nk) {
+      in.pe = p.inc_elem[k0 + t];
+      loc = p.inc_loc[k0 + t];
+    }
+    int node = -1;
+    if (t < nn + th.y) node = t < nn ? n0 + t : p.halo[th.x + t - nn];
+    double4 u = make_double4(0.0, 0.0, 0.0, 0.0);
+    if (node >= 0) u = p.ui[node];
+    if constexpr (LDSU) {
+      if (t < nk) load_incidence_elem<M>(p, mp, in);
+      if (node >= 0) {
+        s_u[t] = u.x;
+        s_u[NLU + t] = u.y;
+        s_u[2 * NLU + t] = u.z;
+      }
+    }
+  } else if (t < nk) {
     if (lds_u) {
       load_incidence<R, M, P, false>(p, mp, k0 + t, in);
       loc = p.inc_loc[k0 + t];
@@ -172,15 +192,18 @@ __device__ __forceinline__ void force_tile(const P& p, const MatParams& mp, doub
       load_incidence<R, M>(p, mp, k0 + t, in);
     }
   }
-  for (int i = t; i <= nn; i += TILE) s_incp[i] = p.inc_ptr[n0 + i] - k0;
+  if (t <= nn) s_incp[t] = incp - k0;
+  for (int i = t + TILE; i <= nn; i += TILE) s_incp[i] = p.inc_ptr[n0 + i] - k0;
   if constexpr (LDSU) {
     if (lds_u) {
-      for (int i = t; i < nn + th.y; i += TILE) {
-        const int node = i < nn ? n0 + i : p.halo[th.x + i - nn];
-        const double4 u = p.ui[node];
-        s_u[i] = u.x;
-        s_u[NLU + i] = u.y;
-        s_u[2 * NLU + i] = u.z;
+      if (TILE < NLU) {
+        for (int i = t; i < nn + th.y; i += TILE) {
+          const int node = i < nn ? n0 + i : p.halo[th.x + i - nn];
+          const double4 u = p.ui[node];
+          s_u[i] = u.x;
+          s_u[NLU + i] = u.y;
+          s_u[2 * NLU + i] = u.z;
+        }
       }
       __syncthreads();
       if (t < nk) {
