@@ -210,11 +210,22 @@ __device__ __forceinline__ double quad_bcast(double x) {
 // lanes active): lane qp loads nodes qp, qp + 4, qp + 8 (besides node 0),
 // forms their differences, and the quad exchanges them.
 template <typename R, typename P>
+__device__ __forceinline__ void tet10_gather_quad_ids(const P& p, int4 c0, int4 c1, int2 c2, int qp, R dX[10][3],
+                                                      R dU[10][3]);
+
+template <typename R, typename P>
 __device__ __forceinline__ void tet10_gather_quad(const P& p, int e, int qp, R dX[10][3], R dU[10][3]) {
   const int32_t* c = p.conn + 12 * int64_t(e);
   const int4 c0 = *reinterpret_cast<const int4*>(c);
   const int4 c1 = *reinterpret_cast<const int4*>(c + 4);
   const int2 c2 = *reinterpret_cast<const int2*>(c + 8);
+  tet10_gather_quad_ids<R>(p, c0, c1, c2, qp, dX, dU);
+}
+
+// The same from the element's node ids (loaded by the caller).
+template <typename R, typename P>
+__device__ __forceinline__ void tet10_gather_quad_ids(const P& p, int4 c0, int4 c1, int2 c2, int qp, R dX[10][3],
+                                                      R dU[10][3]) {
   const int ids[10] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w, c2.x, c2.y};
   const double4 X0 = p.Xi[ids[0]];
   const double4 U0 = p.ui[ids[0]];
@@ -414,6 +425,28 @@ __device__ __forceinline__ void tet10_rows_tile(const P& p, const MatParams& mp,
   const int n0 = ti.x, nn = ti.y, k0 = ti.z, nk = ti.w;
   const int lane = t & 63;
 
+  // the incidence's records and its element's node ids, loaded before the
+  // node table so their latency overlaps it and the barrier
+  const int it = t >> 2, qp = t & 3;
+  uint32_t pe = 0u;
+  int4 c0 = make_int4(0, 0, 0, 0), c1 = make_int4(0, 0, 0, 0);
+  int2 c2 = make_int2(0, 0);
+  uint32_t w[3] = {0u, 0u, 0u};
+  double pr[4] = {0.0, 0.0, 0.0, 0.0};
+  if (it < nk) {
+    pe = p.inc_elem[k0 + it];
+    if (qp == 0) {
+#pragma unroll
+      for (int i = 0; i < 3; ++i) w[i] = p.inc_rank[3 * int64_t(k0 + it) + i];
+    }
+    const int e = int(pe & kIncElemMask);
+    const int32_t* c = p.conn + 12 * int64_t(e);
+    c0 = *reinterpret_cast<const int4*>(c);
+    c1 = *reinterpret_cast<const int4*>(c + 4);
+    c2 = *reinterpret_cast<const int2*>(c + 8);
+    load_params<M>(mp, e, pr);
+  }
+
   int ub = 0, ue = 0;
   if (t < nn) {
     const int4 r0 = p.node_rec[n0 + t], r1 = p.node_rec[n0 + t + 1];
@@ -442,15 +475,11 @@ __device__ __forceinline__ void tet10_rows_tile(const P& p, const MatParams& mp,
   // ---- phase 1: incidence it, quadrature point qp (a quad of lanes per incidence)
   const int u0 = s_uoff[0];
   if (t < nn && ue > ub) atomicOr(&s_ustart[(ub - u0) >> 6], 1ull << ((ub - u0) & 63));
-  const int it = t >> 2, qp = t & 3;
   if (it < nk) {   // the same for the four lanes of a quad
-    const uint32_t pe = p.inc_elem[k0 + it];
     const int e = int(pe & kIncElemMask);
     const int lp = int(pe >> kIncElemBits);
     R dX[10][3], dU[10][3];
-    tet10_gather_quad<R>(p, e, qp, dX, dU);
-    double pr[4];
-    load_params<M>(mp, e, pr);
+    tet10_gather_quad_ids<R>(p, c0, c1, c2, qp, dX, dU);
     const R prm[4] = {R(pr[0]), R(pr[1]), R(pr[2]), R(pr[3])};
     double L[4];
     tet10_qp_coords(qp, L);
@@ -463,9 +492,6 @@ __device__ __forceinline__ void tet10_rows_tile(const P& p, const MatParams& mp,
       status[0] = 1;
     }
     if (qp == 0) {
-      uint32_t w[3];
-#pragma unroll
-      for (int i = 0; i < 3; ++i) w[i] = p.inc_rank[3 * int64_t(k0 + it) + i];
 #pragma unroll
       for (int i = 0; i < 3; ++i) rk[i * TILE + it] = w[i];
       const int off = int((w[2] >> 16) & 0xffu);
