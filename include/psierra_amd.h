@@ -262,7 +262,9 @@ int32_t psx_fbar_csc_create(const psx_fbar* fbar, const int64_t* free_dofs, int6
  * (src/psierra/element/tet10_base.py:344-372): rho (n_elements, device) ->
  * kval (nnz, device, overwritten) in the CSR order of psx_plan_pattern; the
  * off-diagonal entries of every 3x3 block are explicit zeros.  V_e is the
- * volume of the element's corner tetrahedron (straight-edged Tet10). */
+ * volume of the element's corner tetrahedron (straight-edged Tet10).  The
+ * first call on a plan also allocates and fills the plan's mass tensor and
+ * element volumes (n_elements doubles), on `stream`. */
 int32_t psx_assemble_mass(psx_plan* plan, const double* rho, double* kval, void* stream);
 
 /* ---- Adjoint of the internal force (device, Tet4) ------------------------

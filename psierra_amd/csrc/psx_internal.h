@@ -201,6 +201,7 @@ struct psx_plan {
   int32_t max_deg = 0;
   int32_t max_run = 0;             // most incidences of any node
   double* mass_coef = nullptr;     // device npe x npe reference mass tensor (lazy)
+  double* elem_vol = nullptr;      // device per-element corner volume (lazy, first mass assembly)
   double* mu_bound = nullptr;      // device copies of the bound material (psx_plan_bind_material)
   double* lam_bound = nullptr;
   bool bound = false;
@@ -248,8 +249,9 @@ int32_t launch_tet4_vjp(const DevPlan& p, const double* u, const double* mu, con
 int32_t launch_ecsw_training(const DevPlan& p, const double* phi, int r, const double* snaps, int m,
                              const double* mu, const double* lam, int stiff, double* A, double* partial,
                              double* sumsq, hipStream_t s, int32_t* status);
-int32_t launch_mass(const DevPlan& p, const double* rho, const double* coef, double* kval, hipStream_t s,
-                    int64_t n_units);
+int32_t launch_element_volumes(const DevPlan& p, double* vol, hipStream_t s);
+int32_t launch_mass(const DevPlan& p, const double* rho, const double* vol, const double* coef, double* kval,
+                    hipStream_t s, int64_t n_units);
 int32_t launch_csc_extract(const DevCsc& c, const double* kval, int drop, int* colptr, int* rowind,
                            double* vals, hipStream_t s);
 int32_t launch_free_residual(const DevCsc& c, const double* f_int, const double* f_ext_free, double* r,

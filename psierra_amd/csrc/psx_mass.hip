@@ -8,8 +8,9 @@
 // (integral of N_a N_b over the element divided by its volume).
 //
 // Sixteen lanes per node a (four nodes per wave): a's incidences (at most
-// 64) leave rho_e V_e, a's local index and the rank bytes in LDS (the mass
-// tensor is in LDS too); then
+// 64) leave rho_e V_e (V_e from the plan's element volumes, computed once by
+// element_volume_kernel), a's local index and the rank bytes in LDS (the
+// mass tensor is in LDS too); then
 // one lane per CSR block (a, slot s) walks the staged run in order and adds
 // rho_e V_e c[p][q] for every element that contains the slot's node (p = a's
 // local index, q = the slot node's local index, found from the rank bytes);
@@ -37,8 +38,15 @@ __device__ __forceinline__ double corner_volume(const DevPlan& p, int64_t e) {
   return det3(D) * (1.0 / 6.0);
 }
 
+// V_e of every element (its corner tetrahedron), once per plan.
+__global__ __launch_bounds__(256) void element_volume_kernel(DevPlan p, double* __restrict__ vol) {
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e < p.E) vol[e] = corner_volume(p, e);
+}
+
 template <int NPE, int NPW>
 __global__ __launch_bounds__(256) void mass_node_kernel(DevPlan p, const double* __restrict__ rho,
+                                                        const double* __restrict__ vol,
                                                         const double* __restrict__ coef,
                                                         double* __restrict__ kval) {
   constexpr int NRW = NPE == 4 ? 1 : 3;   // rank words per incidence
@@ -51,21 +59,29 @@ __global__ __launch_bounds__(256) void mass_node_kernel(DevPlan p, const double*
   const int w = int(threadIdx.x >> 6);
   const int sub = int(threadIdx.x & 63) / W, lane = int(threadIdx.x & 63) % W;
   const int g = w * NPW + sub;
+  const int a = (xcd_tile() * 4 + w) * NPW + sub;
+  // the node's records are loaded before the mass tensor's barrier
+  int4 rec = make_int4(0, 0, 0, 0), rec1 = make_int4(0, 0, 0, 0);
+  if (a < p.N) {
+    rec = p.node_rec[a];
+    rec1 = p.node_rec[a + 1];
+  }
   for (int i = int(threadIdx.x); i < NPE * NPE; i += int(blockDim.x)) s_coef[i] = coef[i];
   __syncthreads();
-  const int a = (xcd_tile() * 4 + w) * NPW + sub;
   if (a >= p.N) return;
-  const int4 rec = p.node_rec[a];
-  const int4 rec1 = p.node_rec[a + 1];
   const int kb = rec.x, L = rec1.x - rec.x;
   const int d = rec1.y - rec.y;
-  for (int k = lane; k < L; k += W) {
-    const uint32_t pe = p.inc_elem[kb + k];
-    const int64_t e = int64_t(pe & kIncElemMask);
-    s_w[g][k] = rho[e] * corner_volume(p, e);
-    s_lp[g][k] = uint8_t(pe >> kIncElemBits);
 #pragma unroll
-    for (int i = 0; i < NRW; ++i) s_rank[g][k][i] = p.inc_rank[NRW * int64_t(kb + k) + i];
+  for (int kk = 0; kk < KMAX / W; ++kk) {
+    const int k = lane + kk * W;
+    if (k < L) {
+      const uint32_t pe = p.inc_elem[kb + k];
+      const int64_t e = int64_t(pe & kIncElemMask);
+      s_w[g][k] = rho[e] * vol[e];
+      s_lp[g][k] = uint8_t(pe >> kIncElemBits);
+#pragma unroll
+      for (int i = 0; i < NRW; ++i) s_rank[g][k][i] = p.inc_rank[NRW * int64_t(kb + k) + i];
+    }
   }
   // the run is this wave's own: its LDS writes land before its reads
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -100,16 +116,23 @@ __global__ __launch_bounds__(256) void mass_node_kernel(DevPlan p, const double*
   }
 }
 
-int32_t launch_mass(const DevPlan& p, const double* rho, const double* coef, double* kval, hipStream_t s,
-                    int64_t n_units) {
+int32_t launch_element_volumes(const DevPlan& p, double* vol, hipStream_t s) {
+  if (p.E <= 0) return PSX_OK;
+  hipLaunchKernelGGL(element_volume_kernel, dim3(unsigned((int64_t(p.E) + 255) / 256)), dim3(256), 0, s, p, vol);
+  PSX_HIP_TRY(hipGetLastError());
+  return PSX_OK;
+}
+
+int32_t launch_mass(const DevPlan& p, const double* rho, const double* vol, const double* coef, double* kval,
+                    hipStream_t s, int64_t n_units) {
   if (n_units <= 0 || p.N <= 0) return PSX_OK;
   // four nodes per wave (16 lanes each); groups in multiples of 8 for the
   // XCD-contiguous mapping
   const unsigned blocks = unsigned(8 * ((int64_t(p.N) + 127) / 128));
   if (p.npe == 4)
-    hipLaunchKernelGGL((mass_node_kernel<4, 4>), dim3(blocks), dim3(256), 0, s, p, rho, coef, kval);
+    hipLaunchKernelGGL((mass_node_kernel<4, 4>), dim3(blocks), dim3(256), 0, s, p, rho, vol, coef, kval);
   else
-    hipLaunchKernelGGL((mass_node_kernel<10, 4>), dim3(blocks), dim3(256), 0, s, p, rho, coef, kval);
+    hipLaunchKernelGGL((mass_node_kernel<10, 4>), dim3(blocks), dim3(256), 0, s, p, rho, vol, coef, kval);
   PSX_HIP_TRY(hipGetLastError());
   return PSX_OK;
 }

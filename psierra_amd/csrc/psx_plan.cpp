@@ -1038,7 +1038,15 @@ int32_t psx_assemble_mass(psx_plan* plan, const double* rho, double* kval, void*
     if (rc) return rc;
     plan->mass_coef = d;
   }
-  return psx::launch_mass(plan->d, rho, plan->mass_coef, kval, static_cast<hipStream_t>(stream),
+  if (!plan->elem_vol) {
+    double* v = nullptr;
+    int32_t rc = psx::upload<double>(plan, nullptr, size_t(plan->hp.E), &v);
+    if (rc) return rc;
+    rc = psx::launch_element_volumes(plan->d, v, static_cast<hipStream_t>(stream));
+    if (rc) return rc;
+    plan->elem_vol = v;
+  }
+  return psx::launch_mass(plan->d, rho, plan->elem_vol, plan->mass_coef, kval, static_cast<hipStream_t>(stream),
                           plan->hp.adj_ptr.back());
 }
 
