@@ -21,67 +21,84 @@
 // psx_free_residual forms r = f_ext_free - f_int[free]
 // (nonlinear_static.py:322-323) and sum(r^2) in FP64 with a fixed-order
 // two-level reduction.
+#include "psx_device.h"
 #include "psx_internal.h"
 
 namespace psx {
 
 constexpr int kCscBlock = 256;
 
-__device__ __forceinline__ int block_exclusive_scan(int v, int* sh) {
-  // Hillis-Steele over kCscBlock lanes; returns the exclusive prefix of v.
-  const int t = threadIdx.x;
-  sh[t] = v;
-  __syncthreads();
-  for (int off = 1; off < kCscBlock; off <<= 1) {
-    const int x = t >= off ? sh[t - off] : 0;
-    __syncthreads();
-    sh[t] += x;
-    __syncthreads();
+// Columns per thread in the scan of the kept counts.
+constexpr int kScanItems = 8;
+constexpr int kScanColumns = kCscBlock * kScanItems;  // per work-group of the local scan
+constexpr int kScanTopBlock = 1024;
+
+// Exclusive prefix of v over the work-group (blockDim.x a multiple of 64, at
+// most 1024): a shuffle scan per wave, then the wave totals through sh;
+// *total receives the work-group's sum.
+__device__ __forceinline__ int group_exclusive_scan(int v, int* sh, int* total) {
+  const int lane = int(threadIdx.x & 63), w = int(threadIdx.x >> 6), nw = int(blockDim.x >> 6);
+  int x = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int y = __shfl_up(x, off);
+    if (lane >= off) x += y;
   }
-  const int incl = sh[t];
+  if (lane == 63) sh[w] = x;
   __syncthreads();
-  return incl - v;
+  int before = 0, all = 0;
+  for (int i = 0; i < nw; ++i) {
+    const int t = sh[i];
+    before += i < w ? t : 0;
+    all += t;
+  }
+  *total = all;
+  return before + x - v;
 }
 
 // Column node b's free columns (one wave): kStructFill writes every
-// structural entry (free row) at its structural position (c.sstart: the
-// column starts with no entry dropped) and colptr[column] receives the count
+// structural entry (free row) at its structural position (the column starts
+// with no entry dropped, in b's record) and colptr[column] receives the count
 // of kept entries (nonzero, when exact zeros are dropped); when nothing is
 // dropped -- the scan of the counts then ends at the structural count -- this
 // is the whole output.  kCompactFill (only when something was dropped)
 // rewrites the kept rows and values from colptr[column], the final start.
+// The records carry the free indices as base + mask, so a wave's loads form
+// a chain of three: b's record, its unit records, the K values.
 enum CscPass { kStructFill = 0, kCompactFill = 1 };
 
+__device__ __forceinline__ int free_index(int base, int mask, int r) {
+  return (mask >> r) & 1 ? base + __popc(unsigned(mask) & ((1u << r) - 1u)) : -1;
+}
+
 template <int PASS>
-__device__ __forceinline__ void csc_node(const DevCsc& c, int b, int lane, const double* __restrict__ kval,
+__device__ __forceinline__ void csc_node(const DevCsc& c, int bi, int lane, const double* __restrict__ kval,
                                          int drop, int* __restrict__ colptr, int* __restrict__ rowind,
                                          double* __restrict__ vals) {
-  int jc[3];
+  const int4 h = load_uniform(c.node, 2 * bi), st = load_uniform(c.node, 2 * bi + 1);
+  const int u0 = h.x, items = 3 * h.y;
+  int jc[3], start[3];
 #pragma unroll
-  for (int k = 0; k < 3; ++k) jc[k] = c.newidx[3 * b + k];
-  if (jc[0] < 0 && jc[1] < 0 && jc[2] < 0) return;
-  const int u0 = c.adj_ptr[b], items = 3 * (c.adj_ptr[b + 1] - u0);
-  int start[3] = {0, 0, 0};
+  for (int k = 0; k < 3; ++k) jc[k] = free_index(h.z, h.w, k);
+  if (PASS == kStructFill) {
+    start[0] = st.x;
+    start[1] = st.y;
+    start[2] = st.z;
+  } else {
 #pragma unroll
-  for (int k = 0; k < 3; ++k)
-    if (jc[k] >= 0) start[k] = PASS == kStructFill ? c.sstart[jc[k]] : colptr[jc[k]];
+    for (int k = 0; k < 3; ++k) start[k] = jc[k] >= 0 ? colptr[jc[k]] : 0;
+  }
   int kept[3] = {0, 0, 0}, placed[3] = {0, 0, 0};
   const unsigned long long below = (1ull << lane) - 1ull;
   for (int base = 0; base < items; base += 64) {
     const int l = base + lane;
-    int ri = -1;
-    double v[3] = {0.0, 0.0, 0.0};
-    if (l < items) {
-      const int s = l / 3, r = l - 3 * s;
-      const int4 un = c.unit[u0 + s];
-      ri = c.newidx[3 * un.z + r];
-      if (ri >= 0) {
-        const double* src = kval + un.x + r * un.y;
-        v[0] = src[0];
-        v[1] = src[1];
-        v[2] = src[2];
-      }
-    }
+    // The loads are unconditional (lanes past the units read the last unit
+    // and its K row, in bounds): the compiler then issues them back to back.
+    const int s = l / 3, r = l - 3 * s;
+    const int4 un = c.unit[u0 + min(s, h.y - 1)];
+    const int ri = free_index(un.z, l < items ? un.w : 0, r);  // a select: the load stays whole
+    const double* src = kval + un.x + r * un.y;
+    const double v[3] = {src[0], src[1], src[2]};
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       const bool structural = ri >= 0 && jc[k] >= 0;
@@ -110,13 +127,75 @@ __device__ __forceinline__ void csc_node(const DevCsc& c, int b, int lane, const
   }
 }
 
-// One wave per column node, taken in locality order.
+// kCscStructNodes consecutive column nodes (of those with a free dof, in
+// locality order) per wave; each XCD takes a contiguous run of the order, so
+// the waves that read the rows of one node share that XCD's L2 (the grid is a
+// multiple of 8 work-groups).  When every node of the wave has at most 21
+// neighbours (one 64-lane chunk each), the nodes' unit records are loaded
+// together, then their K values, then the entries are placed: each wave keeps
+// kCscStructNodes times the loads of one node in flight.  Otherwise the nodes
+// go through csc_node one after the other.
+constexpr int kCscStructNodes = 4;
 __global__ __launch_bounds__(kCscBlock) void csc_struct_kernel(DevCsc c, const double* __restrict__ kval, int drop,
                                                                int* __restrict__ colptr, int* __restrict__ rowind,
                                                                double* __restrict__ vals) {
-  const int bi = blockIdx.x * (kCscBlock / 64) + int(threadIdx.x >> 6);
-  if (bi >= c.N) return;
-  csc_node<kStructFill>(c, c.order[bi], threadIdx.x & 63, kval, drop, colptr, rowind, vals);
+  constexpr int NPW = kCscStructNodes;
+  const int lane = int(threadIdx.x & 63);
+  const int b0 = (xcd_tile() * (kCscBlock / 64) + __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6))) * NPW;
+  if (b0 >= c.n_nodes) return;
+  int4 h[NPW], st[NPW];
+  bool one_chunk = true;
+#pragma unroll
+  for (int n = 0; n < NPW; ++n) {
+    if (b0 + n < c.n_nodes) {
+      h[n] = load_uniform(c.node, 2 * (b0 + n));
+      st[n] = load_uniform(c.node, 2 * (b0 + n) + 1);
+    } else {
+      h[n] = make_int4(0, 0, 0, 0);  // no units, no free column
+      st[n] = make_int4(0, 0, 0, 0);
+    }
+    one_chunk = one_chunk && 3 * h[n].y <= 64;
+  }
+  if (!one_chunk) {
+    for (int n = 0; n < NPW && b0 + n < c.n_nodes; ++n)
+      csc_node<kStructFill>(c, b0 + n, lane, kval, drop, colptr, rowind, vals);
+    return;
+  }
+  // Unconditional loads (lanes past a node's units read its last unit and
+  // that unit's K row; a node past the end reads unit 0), so that the
+  // compiler issues every node's loads back to back.
+  const int s = lane / 3, r = lane - 3 * s;
+  int4 un[NPW];
+#pragma unroll
+  for (int n = 0; n < NPW; ++n) un[n] = c.unit[h[n].x + max(0, min(s, h[n].y - 1))];
+  int ri[NPW];
+  double v[NPW][3];
+#pragma unroll
+  for (int n = 0; n < NPW; ++n) {
+    ri[n] = free_index(un[n].z, lane < 3 * h[n].y ? un[n].w : 0, r);  // a select: the load stays whole
+    const double* src = kval + un[n].x + r * un[n].y;
+    v[n][0] = src[0];
+    v[n][1] = src[1];
+    v[n][2] = src[2];
+  }
+  const unsigned long long below = (1ull << lane) - 1ull;
+#pragma unroll
+  for (int n = 0; n < NPW; ++n) {
+    const int start[3] = {st[n].x, st[n].y, st[n].z};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const int jc = free_index(h[n].z, h[n].w, k);
+      const bool structural = ri[n] >= 0 && jc >= 0;
+      const bool keep = structural && (!drop || v[n][k] != 0.0);
+      const unsigned long long ms = __ballot(structural), m = __ballot(keep);
+      if (structural) {
+        const int o = start[k] + __popcll(ms & below);
+        rowind[o] = ri[n];
+        vals[o] = v[n][k];
+      }
+      if (lane == 0 && jc >= 0) colptr[jc] = __popcll(m);
+    }
+  }
 }
 
 // kCscCompactNodes column nodes per wave; every wave returns at once when
@@ -126,68 +205,82 @@ __global__ __launch_bounds__(kCscBlock) void csc_compact_kernel(DevCsc c, const 
                                                                 int* __restrict__ colptr, int* __restrict__ rowind,
                                                                 double* __restrict__ vals) {
   if (colptr[c.n_free] == c.nnz_struct) return;
-  const int w = blockIdx.x * (kCscBlock / 64) + int(threadIdx.x >> 6);
-  const int b0 = w * kCscCompactNodes, b1 = min(c.N, b0 + kCscCompactNodes);
-  for (int bi = b0; bi < b1; ++bi) csc_node<kCompactFill>(c, c.order[bi], threadIdx.x & 63, kval, drop, colptr, rowind, vals);
+  const int w = blockIdx.x * (kCscBlock / 64) + __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
+  const int b0 = w * kCscCompactNodes, b1 = min(c.n_nodes, b0 + kCscCompactNodes);
+  for (int bi = b0; bi < b1; ++bi) csc_node<kCompactFill>(c, bi, threadIdx.x & 63, kval, drop, colptr, rowind, vals);
 }
 
-// colptr[j] += the offset of its 256-column block (the block-local prefix
-// becomes the column's start).
+// colptr[j] += the offset of its kScanColumns-column group (the group-local
+// prefix becomes the column's start).
 __global__ __launch_bounds__(kCscBlock) void csc_scan_add_kernel(int n_free, const int* __restrict__ blockoff,
                                                                  int* __restrict__ colptr) {
   const int j = blockIdx.x * kCscBlock + threadIdx.x;
-  if (j < n_free) colptr[j] += blockoff[j / kCscBlock];
+  if (j < n_free) colptr[j] += blockoff[j / kScanColumns];
 }
 
-// colptr[j] (the column counts) -> the exclusive prefix within each block of
-// 256 columns; blocksum[block] = the block's total.
+// colptr[j] (the column counts) -> the exclusive prefix within each group of
+// kScanColumns columns, kScanItems consecutive columns per thread;
+// blocksum[group] = the group's total.
 __global__ __launch_bounds__(kCscBlock) void csc_scan_local_kernel(int n_free, int* __restrict__ colptr,
                                                                    int* __restrict__ blocksum) {
-  __shared__ int sh[kCscBlock];
-  const int j = blockIdx.x * kCscBlock + threadIdx.x;
-  const int v = j < n_free ? colptr[j] : 0;
-  const int ex = block_exclusive_scan(v, sh);
-  if (j < n_free) colptr[j] = ex;
-  if (threadIdx.x == kCscBlock - 1) blocksum[blockIdx.x] = ex + v;
+  __shared__ int sh[kCscBlock / 64];
+  const int j0 = (blockIdx.x * kCscBlock + int(threadIdx.x)) * kScanItems;
+  int v[kScanItems], sum = 0;
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    v[i] = j0 + i < n_free ? colptr[j0 + i] : 0;
+    sum += v[i];
+  }
+  int total;
+  int ex = group_exclusive_scan(sum, sh, &total);
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    if (j0 + i < n_free) colptr[j0 + i] = ex;
+    ex += v[i];
+  }
+  if (threadIdx.x == 0) blocksum[blockIdx.x] = total;
 }
 
-__global__ __launch_bounds__(kCscBlock) void csc_scan_blocks_kernel(int nblocks, int* __restrict__ blocksum,
-                                                                    int* __restrict__ colptr, int n_free) {
-  __shared__ int sh[kCscBlock];
-  int carry = 0;
-  for (int base = 0; base < nblocks; base += kCscBlock) {
-    const int i = base + threadIdx.x;
-    const int v = i < nblocks ? blocksum[i] : 0;
-    const int ex = block_exclusive_scan(v, sh);
-    if (i < nblocks) blocksum[i] = carry + ex;
-    __shared__ int total;
-    if (threadIdx.x == kCscBlock - 1) total = ex + v;
-    __syncthreads();
-    carry += total;
-    __syncthreads();
+// One work-group of kScanTopBlock threads: the group totals -> their
+// exclusive prefix (each thread a consecutive run), and colptr[n_free] = the
+// kept total.
+__global__ __launch_bounds__(kScanTopBlock) void csc_scan_blocks_kernel(int nblocks, int* __restrict__ blocksum,
+                                                                        int* __restrict__ colptr, int n_free) {
+  __shared__ int sh[kScanTopBlock / 64];
+  const int per = (nblocks + kScanTopBlock - 1) / kScanTopBlock;
+  const int i0 = int(threadIdx.x) * per, i1 = min(nblocks, i0 + per);
+  int sum = 0;
+  for (int i = i0; i < i1; ++i) sum += blocksum[i];
+  int total;
+  int ex = group_exclusive_scan(sum, sh, &total);
+  for (int i = i0; i < i1; ++i) {
+    const int v = blocksum[i];
+    blocksum[i] = ex;
+    ex += v;
   }
-  if (threadIdx.x == 0) colptr[n_free] = carry;
+  if (threadIdx.x == 0) colptr[n_free] = total;
 }
 
 int32_t launch_csc_extract(const DevCsc& c, const double* kval, int drop, int* colptr, int* rowind,
                            double* vals, hipStream_t s) {
-  const int nb = (c.n_free + kCscBlock - 1) / kCscBlock;
+  const int nb = (c.n_free + kScanColumns - 1) / kScanColumns;
   if (nb == 0) {
-    // no free dof: only colptr[0] = 0 (the scan of zero blocks writes it)
-    hipLaunchKernelGGL(csc_scan_blocks_kernel, dim3(1), dim3(kCscBlock), 0, s, 0, c.scratch, colptr, 0);
+    // no free dof: only colptr[0] = 0 (the scan of zero groups writes it)
+    hipLaunchKernelGGL(csc_scan_blocks_kernel, dim3(1), dim3(kScanTopBlock), 0, s, 0, c.scratch, colptr, 0);
     PSX_HIP_TRY(hipGetLastError());
     return PSX_OK;
   }
   // structural fill and kept counts; scan of the counts; the compacting fill
   // only if the scanned total shows that entries were dropped
   constexpr int waves = kCscBlock / 64;
-  const int node_blocks = (c.N + waves - 1) / waves;
-  const int compact_blocks = (c.N + waves * kCscCompactNodes - 1) / (waves * kCscCompactNodes);
+  const int node_blocks = ((c.n_nodes + 8 * waves * kCscStructNodes - 1) / (8 * waves * kCscStructNodes)) * 8;
+  const int compact_blocks = (c.n_nodes + waves * kCscCompactNodes - 1) / (waves * kCscCompactNodes);
   hipLaunchKernelGGL(csc_struct_kernel, dim3(node_blocks), dim3(kCscBlock), 0, s, c, kval, drop, colptr, rowind,
                      vals);
   hipLaunchKernelGGL(csc_scan_local_kernel, dim3(nb), dim3(kCscBlock), 0, s, c.n_free, colptr, c.scratch);
-  hipLaunchKernelGGL(csc_scan_blocks_kernel, dim3(1), dim3(kCscBlock), 0, s, nb, c.scratch, colptr, c.n_free);
-  hipLaunchKernelGGL(csc_scan_add_kernel, dim3(nb), dim3(kCscBlock), 0, s, c.n_free, c.scratch, colptr);
+  hipLaunchKernelGGL(csc_scan_blocks_kernel, dim3(1), dim3(kScanTopBlock), 0, s, nb, c.scratch, colptr, c.n_free);
+  hipLaunchKernelGGL(csc_scan_add_kernel, dim3((c.n_free + kCscBlock - 1) / kCscBlock), dim3(kCscBlock), 0, s,
+                     c.n_free, c.scratch, colptr);
   hipLaunchKernelGGL(csc_compact_kernel, dim3(compact_blocks), dim3(kCscBlock), 0, s, c, kval, drop, colptr,
                      rowind, vals);
   PSX_HIP_TRY(hipGetLastError());

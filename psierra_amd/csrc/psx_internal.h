@@ -159,14 +159,14 @@ struct FbarDev {
 };
 
 // Structural CSC of K[free][:, free] (device view).
+// A node's free mask has bit r set when dof 3n + r is free; its free base is
+// the number of free dofs below 3n, so dof 3n + r (free) has free index
+// base + popcount(mask & ((1 << r) - 1)).
 struct DevCsc {
   int32_t n_free;
-  int32_t N;                // nodes
-  const int32_t* order;     // N: column nodes in locality order (waves close in time share cache lines)
-  const int32_t* adj_ptr;   // N + 1: node b's neighbours a (ascending) are units adj_ptr[b]..
-  const int4* unit;         // per unit (b, a): {K position of K[3a][3b], row stride 3 deg(a), a, 0}
-  const int32_t* newidx;    // 3N: free index of each dof, -1 if prescribed
-  const int32_t* sstart;    // n_free: structural start of each column (no entry dropped)
+  int32_t n_nodes;          // column nodes with a free dof, in locality order (waves close in time share cache lines)
+  const int4* node;         // 2 n_nodes: {first unit, units, free base, free mask}, {structural start of column 0, 1, 2, 0}
+  const int4* unit;         // per unit (b, a), a ascending: {K position of K[3a][3b], row stride 3 deg(a), a's free base, a's free mask}
   int32_t nnz_struct;       // structural entries of K[free][:, free]
   const int32_t* free;      // free dof ids (ascending)
   int32_t* scratch;         // per-block counts / offsets

@@ -795,40 +795,55 @@ int32_t create_csc_adj(const std::vector<int32_t>& adj_ptr, const std::vector<in
     const int32_t* nb = hp.adj.data() + hp.adj_ptr[a];
     return int32_t(std::lower_bound(nb, nb + deg(a), int32_t(b)) - nb);
   };
+  // Each node's free mask and free base (free dofs below 3n).
+  std::vector<int32_t> fmask(static_cast<size_t>(N)), fbase(static_cast<size_t>(N) + 1, 0);
+  for (int64_t a = 0; a < N; ++a) {
+    int32_t m = 0;
+    for (int r = 0; r < 3; ++r) m |= int32_t(newidx[3 * a + r] >= 0) << r;
+    fmask[a] = m;
+    fbase[a + 1] = fbase[a] + __builtin_popcount(unsigned(m));
+  }
   // Column node b's unit (b, a) for each neighbour a: where block K[3a.., 3b..]
-  // starts in the CSR values and a's row stride.  The structural count of
-  // K[free][:, free] is kept for the output sizes.
+  // starts in the CSR values, a's row stride, a's free base and mask.  The
+  // structural count of K[free][:, free] is kept for the output sizes.
   const int64_t n_units = int64_t(hp.adj.size());
   std::vector<int4> unit(size_t(std::max<int64_t>(n_units, 1)));
   int64_t nnz_struct = 0;
   for (int64_t b = 0; b < N; ++b) {
-    int64_t free_cols = 0;
-    for (int c = 0; c < 3; ++c) free_cols += newidx[3 * b + c] >= 0;
+    const int64_t free_cols = __builtin_popcount(unsigned(fmask[b]));
     for (int32_t t = hp.adj_ptr[b]; t < hp.adj_ptr[b + 1]; ++t) {
       const int64_t a = hp.adj[t];
-      unit[t] = make_int4(int32_t(9 * int64_t(hp.adj_ptr[a]) + 3 * rank(a, b)), int32_t(3 * deg(a)), int32_t(a), 0);
-      int64_t free_rows = 0;
-      for (int r = 0; r < 3; ++r) free_rows += newidx[3 * a + r] >= 0;
-      nnz_struct += free_cols * free_rows;
+      unit[t] = make_int4(int32_t(9 * int64_t(hp.adj_ptr[a]) + 3 * rank(a, b)), int32_t(3 * deg(a)), fbase[a],
+                          fmask[a]);
+      nnz_struct += free_cols * __builtin_popcount(unsigned(fmask[a]));
     }
   }
   if (nnz_struct >= (int64_t(1) << 31)) return fail(PSX_EINVAL, "free-dof matrix has 2^31 or more entries");
-  // structural column starts: free columns in dof order, each with the free
-  // rows of its node's neighbours
-  std::vector<int32_t> sstart(size_t(std::max<int64_t>(n_free, 1)));
+  // Structural column starts: free columns in dof order, each with the free
+  // rows of its node's neighbours.
+  std::vector<int32_t> sstart(static_cast<size_t>(n));
   {
     int64_t pos = 0;
     for (int64_t b = 0; b < N; ++b) {
       int64_t rows = 0;
-      for (int32_t t = hp.adj_ptr[b]; t < hp.adj_ptr[b + 1]; ++t)
-        for (int r = 0; r < 3; ++r) rows += newidx[3 * int64_t(hp.adj[t]) + r] >= 0;
+      for (int32_t t = hp.adj_ptr[b]; t < hp.adj_ptr[b + 1]; ++t) rows += __builtin_popcount(unsigned(fmask[hp.adj[t]]));
       for (int c = 0; c < 3; ++c) {
-        const int32_t k = newidx[3 * b + c];
-        if (k < 0) continue;
-        sstart[k] = int32_t(pos);
+        if (newidx[3 * b + c] < 0) continue;
+        sstart[3 * b + c] = int32_t(pos);
         pos += rows;
       }
     }
+  }
+  if (int64_t(order.size()) != N) return fail(PSX_EINVAL, "node order has the wrong length");
+  // Column node records in locality order, nodes with no free dof left out.
+  std::vector<int4> node;
+  node.reserve(2 * size_t(N));
+  for (int64_t i = 0; i < N; ++i) {
+    const int32_t b = order[i];
+    if (b < 0 || b >= N) return fail(PSX_EINVAL, "node order holds an id out of range");
+    if (!fmask[b]) continue;
+    node.push_back(make_int4(hp.adj_ptr[b], hp.adj_ptr[b + 1] - hp.adj_ptr[b], fbase[b], fmask[b]));
+    node.push_back(make_int4(sstart[3 * b], sstart[3 * b + 1], sstart[3 * b + 2], 0));
   }
   auto* csc = new psx_csc();
   auto bail = [&](int32_t rc) {
@@ -845,26 +860,21 @@ int32_t create_csc_adj(const std::vector<int32_t>& adj_ptr, const std::vector<in
     if (bytes && h) PSX_HIP_TRY(hipMemcpy(*d, h, bytes, hipMemcpyHostToDevice));
     return PSX_OK;
   };
-  if (int64_t(order.size()) != N) return bail(fail(PSX_EINVAL, "node order has the wrong length"));
-  void *dord, *dptr, *dunit, *dnew, *dfree, *dscr, *dpart, *dss;
+  void *dnode, *dunit, *dfree, *dscr, *dpart;
   const size_t nb = size_t((n_free + 255) / 256) + 1;
   int32_t rc;
-  if ((rc = up(order.data(), size_t(N) * 4, &dord)) || (rc = up(hp.adj_ptr.data(), size_t(N + 1) * 4, &dptr)) ||
+  if ((rc = up(node.data(), node.size() * sizeof(int4), &dnode)) ||
       (rc = up(unit.data(), size_t(n_units) * sizeof(int4), &dunit)) ||
-      (rc = up(newidx.data(), newidx.size() * 4, &dnew)) || (rc = up(freev.data(), freev.size() * 4, &dfree)) ||
-      (rc = up(nullptr, nb * 4, &dscr)) || (rc = up(nullptr, nb * 8, &dpart)) ||
-      (rc = up(sstart.data(), sstart.size() * 4, &dss)))
+      (rc = up(freev.data(), freev.size() * 4, &dfree)) || (rc = up(nullptr, nb * 4, &dscr)) ||
+      (rc = up(nullptr, nb * 8, &dpart)))
     return bail(rc);
   csc->d.n_free = int32_t(n_free);
-  csc->d.N = int32_t(N);
-  csc->d.order = static_cast<int32_t*>(dord);
-  csc->d.adj_ptr = static_cast<int32_t*>(dptr);
+  csc->d.n_nodes = int32_t(node.size() / 2);
+  csc->d.node = static_cast<int4*>(dnode);
   csc->d.unit = static_cast<int4*>(dunit);
-  csc->d.newidx = static_cast<int32_t*>(dnew);
   csc->d.free = static_cast<int32_t*>(dfree);
   csc->d.scratch = static_cast<int32_t*>(dscr);
   csc->d.partial = static_cast<double*>(dpart);
-  csc->d.sstart = static_cast<int32_t*>(dss);
   csc->d.nnz_struct = int32_t(nnz_struct);
   *out = csc;
   return PSX_OK;

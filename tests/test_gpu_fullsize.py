@@ -96,3 +96,66 @@ def test_fullsize_fp32_within_tolerance(big):
     ek, ef = rel_err(k32, k64), rel_err(f32, f64)
     print(f"fullsize fp32 vs fp64: K rel {ek:.3e}, f rel {ef:.3e}")
     assert ek < 1e-5 and ef < 1e-5
+
+
+def test_fullsize_free_csc(big):
+    """Free-DOF CSC of the full matrix clamped at x = 0 (3,060,300 columns, so
+    the column-count scan runs over many work-groups and its top level over
+    many groups per thread).  The pattern is symmetric, so column j of
+    K[free][:, free] holds the free columns of row j: every column's rows and
+    the column pointers are checked against that, the values of 300 sampled
+    columns against scipy's column slices of the host matrix, and the same
+    again with every entry of 64 free rows set to zero (some to -0.0): those
+    rows leave every column, which takes the compacting pass."""
+    import scipy.sparse as spsp
+    import torch
+
+    from psierra_amd.boundary import DirichletConstraint, DirichletManager, select_nodes_by_plane
+    from psierra_amd.solver import FreeDofSystem
+
+    wl, plan, f64, k64, f32, k32 = big
+    indptr, indices = plan.pattern()
+    n = plan.n_dofs
+    mgr = DirichletManager(n)
+    mgr.add_constraint(DirichletConstraint(select_nodes_by_plane(wl.mesh, "yz", "min"), (0, 1, 2), 0.0))
+    free = np.asarray(mgr.free_dofs)
+    assert free.size == 3_060_300
+    isfree = np.zeros(n, bool)
+    isfree[free] = True
+    newidx = np.full(n, -1, np.int64)
+    newidx[free] = np.arange(free.size)
+    row_of = np.repeat(np.arange(n), np.diff(indptr))
+    rng = np.random.default_rng(77)
+    sample = np.sort(rng.choice(free.size, 300, replace=False))
+    zero_rows = np.sort(rng.choice(free, 64, replace=False))
+    sysm = FreeDofSystem(plan, free)
+
+    def check(kv, dropped):
+        banner(f"fullsize free csc, {int(dropped.sum())} zeroed rows")
+        got = sysm.csc(torch.as_tensor(kv, device=plan.device))
+        sel = isfree[row_of] & isfree[indices] & ~dropped[indices]
+        counts = np.bincount(row_of[sel], minlength=n)[free]
+        assert np.array_equal(got.indptr, np.concatenate([[0], np.cumsum(counts)]))
+        assert np.array_equal(got.indices, newidx[indices[sel]])
+        K = spsp.csr_matrix((kv, indices, indptr), shape=(n, n))
+        cols = K[:, free[sample]].tocsc()
+        for t, j in enumerate(sample):
+            rows = cols.indices[cols.indptr[t] : cols.indptr[t + 1]]
+            vals = cols.data[cols.indptr[t] : cols.indptr[t + 1]]
+            order = np.argsort(rows)
+            rows, vals = rows[order], vals[order]
+            keep = isfree[rows] & (vals != 0.0)
+            lo, hi = got.indptr[j], got.indptr[j + 1]
+            assert np.array_equal(got.indices[lo:hi], newidx[rows[keep]])
+            assert np.array_equal(got.data[lo:hi], vals[keep])
+        return got.nnz
+
+    none = np.zeros(n, bool)
+    nnz_all = check(k64, none)
+    kz = k64.copy()
+    for i, r in enumerate(zero_rows):
+        kz[indptr[r] : indptr[r + 1]] = -0.0 if i % 4 == 0 else 0.0
+    dropped = np.zeros(n, bool)
+    dropped[zero_rows] = True
+    assert check(kz, dropped) < nnz_all
+    assert check(k64, none) == nnz_all
