@@ -156,6 +156,7 @@ __global__ __launch_bounds__(256) void fbar_node_kernel(DevPlan p, FbarDev fb, i
 // neighbour q of a) and add the 3x3 term into q's slot -- distinct q, distinct
 // slots, so each slot sums its terms in a-order.  The row is then written
 // with consecutive lanes on consecutive values.
+constexpr int kKvolBatch = 8;
 __global__ __launch_bounds__(256) void fbar_kvol_kernel(DevPlan p, FbarDev fb, int waves, double bulk, double* kval) {
   extern __shared__ double s_acc[];
   // two nodes per wave, 32 lanes each (a node's neighbours rarely exceed 32)
@@ -189,64 +190,57 @@ __global__ __launch_bounds__(256) void fbar_kvol_kernel(DevPlan p, FbarDev fb, i
       gl = fb.g[kl.z];
     }
     const int cnt = min(32, nmax - cb);
-    // unit i of this half, broadcast from lane i of the half
-    auto unit = [&](int i) {
-      const int l0 = i, l1 = 32 + i;
-      const int4 k0 = make_int4(__builtin_amdgcn_readlane(kl.x, l0), __builtin_amdgcn_readlane(kl.y, l0),
-                                __builtin_amdgcn_readlane(kl.z, l0), __builtin_amdgcn_readlane(kl.w, l0));
-      const int4 k1 = make_int4(__builtin_amdgcn_readlane(kl.x, l1), __builtin_amdgcn_readlane(kl.y, l1),
-                                __builtin_amdgcn_readlane(kl.z, l1), __builtin_amdgcn_readlane(kl.w, l1));
-      int4 k = half ? k1 : k0;
-      if (cb + i >= nu) k.y = 0;   // past this half's units: no feeds
-      return k;
-    };
-    auto bcast = [&](double x, int i) {
-      const long long v = __double_as_longlong(x);
-      const int lo = int(v), hi = int(v >> 32);
-      const int rlo = half ? __builtin_amdgcn_readlane(lo, 32 + i) : __builtin_amdgcn_readlane(lo, i);
-      const int rhi = half ? __builtin_amdgcn_readlane(hi, 32 + i) : __builtin_amdgcn_readlane(hi, i);
-      return __longlong_as_double((static_cast<long long>(rhi) << 32) | static_cast<unsigned>(rlo));
-    };
-    // feeds of unit i (lane j < deg a: g_a[q_j] and q_j's slot in p's row),
-    // loaded one unit ahead of their use
-    int4 ku = unit(0);
-    double4 gq = make_double4(0.0, 0.0, 0.0, 0.0);
-    int sl = 0;
-    if (lane < ku.y) {
-      gq = fb.g[ku.x + lane];
-      sl = fb.kv_slot[ku.w + lane];
-    }
-    for (int i = 0; i < cnt; ++i) {
-      const double gw = bcast(gl.w, i);
-      const double wp[3] = {gw * bcast(gl.x, i), gw * bcast(gl.y, i), gw * bcast(gl.z, i)};
-      int4 kn = make_int4(0, 0, 0, 0);
-      double4 gqn = make_double4(0.0, 0.0, 0.0, 0.0);
-      int sln = 0;
-      if (i + 1 < cnt) {
-        kn = unit(i + 1);
-        if (lane < kn.y) {
-          gqn = fb.g[kn.x + lane];
-          sln = fb.kv_slot[kn.w + lane];
-        }
-      }
-      for (int j = lane; j < ku.y; j += 32) {
-        double4 g = gq;
-        int s = sl;
-        if (j != lane) {   // a neighbour with more than 32 element neighbours
-          g = fb.g[ku.x + j];
-          s = fb.kv_slot[ku.w + j];
-        }
-        double* a9 = acc + 9 * s;
-        const double q3[3] = {g.x, g.y, g.z};
+    // V0_a g_a[p] formed by the unit's lane; unit i's fields and this product
+    // reach the lanes of its half by a 32-wide shuffle from lane i (a select
+    // between readlanes of lanes i and 32 + i compiles to a divergent-index
+    // readlane, i.e. branches).  Past the half's units kl is zero: no feeds.
+    const double wl0 = gl.w * gl.x, wl1 = gl.w * gl.y, wl2 = gl.w * gl.z;
+    // feeds of units i0 .. i0 + kKvolBatch - 1 (lane j < deg a: g_a[q_j] and
+    // q_j's slot in p's row), all loaded before the batch's first sum: the
+    // loads are unconditional (lanes past deg a, and units past the half's
+    // last, read the unit's first feed or feed 0, in bounds), so the
+    // compiler issues them back to back and a node waits on the memory once
+    // per batch instead of once per unit
+    for (int i0 = 0; i0 < cnt; i0 += kKvolBatch) {
+      int deg_a[kKvolBatch], sl[kKvolBatch];
+      double gx[kKvolBatch], gy[kKvolBatch], gz[kKvolBatch];
 #pragma unroll
-        for (int r = 0; r < 3; ++r)
-#pragma unroll
-          for (int c = 0; c < 3; ++c) a9[3 * r + c] += wp[r] * q3[c];
+      for (int t = 0; t < kKvolBatch; ++t) {
+        const int i = min(i0 + t, 31);
+        const int kx = __shfl(kl.x, i, 32), kw = __shfl(kl.w, i, 32);
+        const int ky = i0 + t < cnt ? __shfl(kl.y, i, 32) : 0;
+        deg_a[t] = ky;
+        const int j = min(lane, max(ky - 1, 0));
+        const double4 g = fb.g[kx + j];
+        gx[t] = g.x;
+        gy[t] = g.y;
+        gz[t] = g.z;
+        sl[t] = fb.kv_slot[kw + j];
       }
-      __builtin_amdgcn_wave_barrier();
-      ku = kn;
-      gq = gqn;
-      sl = sln;
+#pragma unroll
+      for (int t = 0; t < kKvolBatch; ++t) {
+        if (i0 + t >= cnt) break;
+        const int i = i0 + t;
+        const double wp[3] = {__shfl(wl0, i, 32), __shfl(wl1, i, 32), __shfl(wl2, i, 32)};
+        for (int j = lane; j < deg_a[t]; j += 32) {
+          double q3[3] = {gx[t], gy[t], gz[t]};
+          int s = sl[t];
+          if (j != lane) {   // a neighbour with more than 32 element neighbours
+            const int kx = __shfl(kl.x, i, 32), kw = __shfl(kl.w, i, 32);
+            const double4 g = fb.g[kx + j];
+            q3[0] = g.x;
+            q3[1] = g.y;
+            q3[2] = g.z;
+            s = fb.kv_slot[kw + j];
+          }
+          double* a9 = acc + 9 * s;
+#pragma unroll
+          for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) a9[3 * r + c] += wp[r] * q3[c];
+        }
+        __builtin_amdgcn_wave_barrier();
+      }
     }
   }
   // CSR order of p's rows: value r * 3 deg2 + 3 s + c

@@ -300,7 +300,10 @@ __device__ __forceinline__ void rows_tile(const P& p, const MatParams& mp, const
   __shared__ uint8_t s_nz[NN];
   __shared__ uint8_t s_first[TILE];
   __shared__ int32_t s_f2[FBAR ? 2 * NN : 1];   // F-bar: diagonal block offset, row length
-  __shared__ int32_t s_user[NN];                 // user id of each node (where its f goes)
+  // user id of each node (where its f goes); the F-bar rows read it at the
+  // store instead (loaded with the node records it costs their FP64
+  // instantiation 36 VGPRs)
+  __shared__ int32_t s_user[FBAR ? 1 : NN];
   // FP32 rows stage the displacements of the tile's nodes and its halo in
   // LDS (one gather per node instead of four per incidence), unless the
   // tile has more than kTileLdsNodes of them
@@ -320,7 +323,7 @@ __device__ __forceinline__ void rows_tile(const P& p, const MatParams& mp, const
   if (t < nn) {
     nr0 = p.node_rec[n0 + t];
     nr1 = p.node_rec[n0 + t + 1];
-    user = p.perm[n0 + t];
+    if constexpr (!FBAR) user = p.perm[n0 + t];
   }
 
   IncInputs<R> in;
@@ -378,7 +381,7 @@ __device__ __forceinline__ void rows_tile(const P& p, const MatParams& mp, const
     s_uoff[t] = r0.y;
     s_kofs[t] = r0.z;
     s_self[t] = uint8_t(r0.w);
-    s_user[t] = user;
+    if constexpr (!FBAR) s_user[t] = user;
     if constexpr (FBAR) {
       const int4 r2 = fv.node_rec2[n0 + t];
       s_kofs[t] = r2.x;
@@ -517,7 +520,7 @@ __device__ __forceinline__ void rows_tile(const P& p, const MatParams& mp, const
     const int part = int(unsigned(t) % ND);
     const int d = FBAR ? s_f2[NN + j] : s_uoff[j + 1] - s_uoff[j];
     if (part == 3) {
-      const int64_t a = s_user[j];
+      const int64_t a = FBAR ? p.perm[n0 + j] : s_user[j];
 #pragma unroll
       for (int c = 0; c < 3; ++c) f[3 * a + c] = acc[6 + c];
     } else if (part < 3 && d > 0) {
