@@ -176,7 +176,60 @@ int32_t build(psx_plan* plan, psx_fbar* fb) {
     }
   });
 
+  // K_vol by slot: node p's terms (a, q) sorted by q's slot in p's row, a
+  // ascending within a slot (a stable bucket pass over the (a, q) order), so
+  // each slot sums its terms in the order the unit-step kernel uses; per
+  // term the unit index of (p, a) and the rank of q among a's neighbours
+  // (g_a[q] is g[first unit of a + rank]); per node the first term of each
+  // slot.
+  std::vector<int4> kv_node(N);
+  std::vector<uint16_t> kv_term(n_map);
+  std::vector<uint16_t> kv_soff(static_cast<size_t>(fb->adj2_ptr[N]) + static_cast<size_t>(N));
+  parallel_nodes(N, [&](int64_t b, int64_t e) {
+    std::vector<int32_t> cur;
+    for (int64_t i = b; i < e; ++i) {
+      const int32_t a0 = perm[i];
+      const int32_t nu = uoff[i + 1] - uoff[i], d2 = deg2[a0];
+      const int64_t P = moff[uoff[i]], F = moff[uoff[i + 1]] - P;
+      const int32_t sbase = fb->adj2_ptr[a0] + a0;
+      kv_node[i] = make_int4(int32_t(P), sbase, int32_t(F), nu);
+      if (nu > kKvolSlotMaxUnits || F > kKvolSlotMaxTerms) continue;   // the plan keeps the unit-step kernel
+      cur.assign(size_t(d2) + 1, 0);
+      for (int64_t m = P; m < P + F; ++m) ++cur[size_t(kv_slot[m]) + 1];
+      for (int32_t t = 0; t < d2; ++t) cur[size_t(t) + 1] += cur[size_t(t)];
+      for (int32_t t = 0; t <= d2; ++t) kv_soff[size_t(sbase) + size_t(t)] = uint16_t(cur[size_t(t)]);
+      for (int32_t su = 0; su < nu; ++su) {
+        const int4 ku = kv_unit[uoff[i] + su];
+        for (int32_t q = 0; q < ku.y; ++q) {
+          const int64_t pos = P + cur[kv_slot[size_t(ku.w) + size_t(q)]]++;
+          kv_term[size_t(pos)] = uint16_t(su | (q << 8));   // q < deg a <= kMaxRankDegree
+        }
+      }
+    }
+  });
+  int32_t max_terms = 0, max_units = 0;
+  for (int64_t i = 0; i < N; ++i) {
+    max_terms = std::max(max_terms, kv_node[i].z);
+    max_units = std::max(max_units, kv_node[i].w);
+  }
+  const bool slots = max_terms <= kKvolSlotMaxTerms && max_units <= kKvolSlotMaxUnits && max_deg2 <= kKvolSlotMaxSlots;
+
   FbarDev& d = fb->d;
+  if (slots) {
+    int4* pkn = nullptr;
+    uint16_t* pkt = nullptr;
+    uint16_t* pso = nullptr;
+    int32_t rc1;
+    if ((rc1 = to_device(fb, kv_node.data(), kv_node.size(), &pkn))) return rc1;
+    if ((rc1 = to_device(fb, kv_term.data(), kv_term.size(), &pkt))) return rc1;
+    if ((rc1 = to_device(fb, kv_soff.data(), kv_soff.size(), &pso))) return rc1;
+    d.kv_slots = 1;
+    d.kv_max_terms = max_terms;
+    d.kv_max_units = max_units;
+    d.kv_node = pkn;
+    d.kv_term = pkt;
+    d.kv_soff = pso;
+  }
   int32_t* ps2 = nullptr;
   int4* pr2 = nullptr;
   int4* pku = nullptr;

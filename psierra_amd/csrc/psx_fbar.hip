@@ -253,6 +253,114 @@ __global__ __launch_bounds__(256) void fbar_kvol_kernel(DevPlan p, FbarDev fb, i
     }
 }
 
+// One wave per node p, K_vol by slot: the node's terms (a, q) are staged in
+// LDS in slot order (g_a[q] and the unit index of (p, a)), with V0_a g_a[p]
+// per unit; then lane s sums the terms of slot s, a ascending -- the fused
+// multiply-adds of fbar_kvol_kernel in the same order from 0.0, so the same
+// bits -- in registers and writes the slot's nine values.  No slot is
+// updated by more than one lane, so the sums need no read-modify-write of
+// LDS.  Loads: the node's records, then its unit, term and slot records,
+// then the g values they index.  The row's 9 deg2 values go through LDS
+// (over the terms, once every lane is done with them) so that consecutive
+// lanes store consecutive values.
+__global__ __launch_bounds__(256) void fbar_kvol_slots_kernel(DevPlan p, FbarDev fb, int waves, int wave_bytes,
+                                                              double bulk, double* kval) {
+  extern __shared__ double s_dyn[];
+  const int w = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
+  const int lane = int(threadIdx.x & 63);
+  const int pn = xcd_tile() * waves + w;
+  if (pn >= p.N) return;
+  char* base = reinterpret_cast<char*>(s_dyn) + size_t(w) * size_t(wave_bytes);
+  double* s_g = reinterpret_cast<double*>(base);                                  // 3 per term, then the row
+  double* s_u = s_g + kvol_slot_area(fb.kv_max_terms, fb.max_deg2);               // 3 per unit
+  uint8_t* s_unit = reinterpret_cast<uint8_t*>(s_u + 3 * fb.kv_max_units);        // 1 per term
+  uint16_t* s_soff = reinterpret_cast<uint16_t*>(s_unit + ((fb.kv_max_terms + 7) & ~7));   // deg2 + 1
+  const int4 nd = load_uniform(fb.kv_node, pn);   // {first term, first slot offset, terms, units}
+  const int4 rec2 = load_uniform(fb.node_rec2, pn);
+  const int u1b = p.uoff[pn];
+  const int P = nd.x, F = nd.z, nu = nd.w, deg2 = rec2.y;
+  constexpr int R = kKvolSlotMaxTerms / 64;
+  // unit lane's unit record (first unit of a, unit (a, p)); the terms'
+  // unit indices and ranks; the loads are unconditional from clamped indices
+  // (every node has a term: it is its own element neighbour), so the
+  // compiler issues them back to back
+  const int4 ku = fb.kv_unit[u1b + min(lane, nu - 1)];
+  int tm[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    if (64 * r >= F) break;
+    tm[r] = fb.kv_term[P + min(lane + 64 * r, F - 1)];
+  }
+  for (int t = lane; t <= deg2; t += 64) s_soff[t] = fb.kv_soff[nd.y + t];
+  const double4 gl = fb.g[ku.z];
+  double4 gt[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    if (64 * r >= F) break;
+    gt[r] = fb.g[__shfl(ku.x, tm[r] & 0xff) + (tm[r] >> 8)];   // g_a[q]: a's first unit from a's unit lane
+  }
+  if (lane < nu) {
+    s_u[3 * lane] = gl.w * gl.x;
+    s_u[3 * lane + 1] = gl.w * gl.y;
+    s_u[3 * lane + 2] = gl.w * gl.z;
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    if (64 * r >= F) break;
+    const int k = lane + 64 * r;
+    if (k < F) {
+      s_g[3 * k] = gt[r].x;
+      s_g[3 * k + 1] = gt[r].y;
+      s_g[3 * k + 2] = gt[r].z;
+      s_unit[k] = uint8_t(tm[r] & 0xff);
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  // CSR order of p's rows: value r * 3 deg2 + 3 s + c.  The sums go to LDS
+  // in that order (over the terms, once every lane is done with them), then
+  // consecutive lanes store consecutive values.
+  const int row = 3 * deg2;
+  double acc[2][9];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int s = lane + 64 * h;
+#pragma unroll
+    for (int v = 0; v < 9; ++v) acc[h][v] = 0.0;
+    if (s < deg2) {
+      const int k0 = s_soff[s], k1 = s_soff[s + 1];
+      for (int k = k0; k < k1; ++k) {
+        const int j = s_unit[k];
+        const double g3[3] = {s_g[3 * k], s_g[3 * k + 1], s_g[3 * k + 2]};
+        const double u3[3] = {s_u[3 * j], s_u[3 * j + 1], s_u[3 * j + 2]};
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) acc[h][3 * r + c] = fma(u3[r], g3[c], acc[h][3 * r + c]);
+      }
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  double* s_row = s_g;   // 9 deg2 values: the term area, no longer read
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int s = lane + 64 * h;
+    if (s < deg2) {
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) s_row[r * row + 3 * s + c] = acc[h][3 * r + c];
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  for (int v = lane; v < 3 * row; v += 64) kval[rec2.x + v] = bulk * s_row[v];
+}
+
 int32_t launch_tet4_fbar(const DevPlan& p, const FbarDev& fb, const double* u, const double* mu,
                          const double* lam, double bulk, double* f, double* kval, int precision, hipStream_t s,
                          int32_t* status) {
@@ -265,7 +373,14 @@ int32_t launch_tet4_fbar(const DevPlan& p, const FbarDev& fb, const double* u, c
     hipLaunchKernelGGL(fbar_node_kernel<2>, grid(p.N, 8), dim3(256), 0, s, p, fb, kval ? 1 : 0);
   else
     hipLaunchKernelGGL(fbar_node_kernel<1>, grid(p.N, 4), dim3(256), 0, s, p, fb, kval ? 1 : 0);
-  if (kval && fb.max_deg2 > 0) {
+  if (kval && fb.max_deg2 > 0 && fb.kv_slots) {
+    // one node per wave: its terms, units and slot offsets in LDS
+    const int wave_bytes = 8 * kvol_slot_area(fb.kv_max_terms, fb.max_deg2) + 24 * fb.kv_max_units +
+                           ((fb.kv_max_terms + 7) & ~7) + ((2 * (fb.max_deg2 + 1) + 7) & ~7);
+    const int waves = std::max(1, std::min(4, (64 << 10) / wave_bytes));
+    hipLaunchKernelGGL(fbar_kvol_slots_kernel, grid(p.N, waves), dim3(64 * waves), size_t(wave_bytes) * waves, s,
+                       p, fb, waves, wave_bytes, bulk, kval);
+  } else if (kval && fb.max_deg2 > 0) {
     // up to four waves (two nodes each) per work-group within 64 KiB of LDS
     const size_t per_wave = size_t(2 * 9) * fb.max_deg2 * sizeof(double);
     const int waves = int(std::max<size_t>(1, std::min<size_t>(4, (size_t(64) << 10) / per_wave)));

@@ -118,6 +118,16 @@ struct DevPlan {
 
 // Own + halo nodes whose displacements a Tet4 row tile stages in LDS.
 constexpr int kTileLdsNodes = 128;
+// F-bar K_vol by slot: one wave per node stages its terms in LDS (24 B per
+// term, 24 B per unit, a byte per term and two per slot), each lane sums at
+// most two slots, and the row goes back through the term area.
+constexpr int kKvolSlotMaxUnits = 64;
+constexpr int kKvolSlotMaxTerms = 512;
+constexpr int kKvolSlotMaxSlots = 128;
+// doubles of the term area: the terms' g_a[q], then the row's values
+__host__ __device__ inline int kvol_slot_area(int max_terms, int max_deg2) {
+  return 3 * max_terms > 9 * max_deg2 ? 3 * max_terms : 9 * max_deg2;
+}
 
 // Device view of a batch of Tet4 plans assembled by one persistent launch:
 // the tiles of all plans form one global list (plan i owns tiles
@@ -151,6 +161,15 @@ struct FbarDev {
                                        //   unit (a, p), first entry of the slot map}
   const uint16_t* kv_slot = nullptr;   // per (p, a, q in N1(a)): slot of q in p's distance-2 row
   int32_t max_deg2 = 0;                // longest distance-2 row (blocks)
+  // K_vol by slot (when every node has at most kKvolSlotMaxUnits units and
+  // kKvolSlotMaxTerms terms; else kv_unit / kv_slot): node p's terms (a, q)
+  // sorted by q's slot in p's row, a ascending within a slot
+  int32_t kv_slots = 0;                // 1: the slot layout below is built
+  int32_t kv_max_terms = 0;            // most terms of any node
+  int32_t kv_max_units = 0;            // most units of any node
+  const int4* kv_node = nullptr;       // N: {first term, first slot offset, terms, units}
+  const uint16_t* kv_term = nullptr;   // per term: index of (p, a) among p's units | rank of q in N1(a) << 8
+  const uint16_t* kv_soff = nullptr;   // per node deg2 + 1: first term of each slot (node-relative)
   double* qv = nullptr;                // per element: V / 4
   double* vjm1 = nullptr;              // per element: V (J - 1) / 4
   double* wq = nullptr;                // per element: 12 values V J / 4 F h_b
