@@ -86,7 +86,11 @@ __global__ __launch_bounds__(256) void mass_node_kernel(DevPlan p, const double*
   // the run is this wave's own: its LDS writes land before its reads
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
-  for (int s = lane; s < d; s += W) {
+  // a node of at most W slots: one slot per lane, then each row stored with
+  // consecutive lanes on consecutive values (the slot sums shuffled within
+  // the node's lanes); otherwise each lane stores its slots' blocks
+  const bool narrow = d <= W;
+  for (int s = lane; s < (narrow ? W : d); s += W) {
     const bool self = s == rec.w;
     double sum = 0.0;
 #pragma unroll 4
@@ -108,11 +112,22 @@ __global__ __launch_bounds__(256) void mass_node_kernel(DevPlan p, const double*
       }
       if (q >= 0) sum += s_w[g][k] * s_coef[lp * NPE + q];
     }
-    double* dst = kval + rec.z + 3 * s;
+    if (narrow) {
+      double* dst = kval + rec.z;
 #pragma unroll
-    for (int r = 0; r < 3; ++r)
+      for (int r = 0; r < 3; ++r)
+        for (int v = lane; v < 3 * W; v += W) {
+          const int sv = v / 3, c = v - 3 * sv;
+          const double x = __shfl(sum, sv, W);   // every lane of the node takes part
+          if (v < 3 * d) dst[r * 3 * d + v] = c == r ? x : 0.0;
+        }
+    } else {
+      double* dst = kval + rec.z + 3 * s;
 #pragma unroll
-      for (int c = 0; c < 3; ++c) dst[r * 3 * d + c] = r == c ? sum : 0.0;
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) dst[r * 3 * d + c] = r == c ? sum : 0.0;
+    }
   }
 }
 
