@@ -170,39 +170,34 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
             for (int r = 0; r < 12; ++r) fr += s_f[12 * (lb + e) + r] * s_phi[e * kPhi + r * kRank + t];
         }
         // A^T B on the matrix cores, wave wv owning columns 16 wv .. 16 wv + 15.
-        // Per element and K-step ks (dof rows 4 ks .. 4 ks + 3), lane (kq, x)
-        // supplies B[4 ks + kq][16 wv + x] = sum_c (w k_e)[4 ks + kq][c] Phi_e[c][16 wv + x]
-        // (its Phi column in registers) and, for M-tile mt, A = Phi_e[4 ks + kq][4 x + mt]
-        // (one 16-byte LDS read for the four tiles): tile mt accumulates
+        // Per element, B's 12 x 16 block (w k_e) Phi_e[:, 16 wv ..] comes from
+        // the matrix cores too: A1 = (w k_e) with its rows ordered so that
+        // output row 4 q + s is k_e row 4 s + q (rows 12..15 zero), hence
+        // register s of lane (kq, x) holds B[4 s + kq][16 wv + x] -- the B
+        // operand of K-step s below.  Then per K-step ks (dof rows 4 ks ..
+        // 4 ks + 3) and M-tile mt, A = Phi_e[4 ks + kq][4 x + mt] (one 16-byte
+        // LDS read for the four tiles): tile mt accumulates
         // K_r[4 m + mt][16 wv + n].
+        const int sx = x & 3, qx = x >> 2;   // A1 row x is k_e row 4 sx + qx
         for (int e = 0; e < nb; ++e) {
           const float* ph = s_phi + e * kPhi;
           const float* ke = s_k + 144 * (lb + e);
-          float pc[12];
+          floatx4 bq;
 #pragma unroll
-          for (int c = 0; c < 12; ++c) pc[c] = ph[c * kRank + 16 * wv + x];
+          for (int r = 0; r < 4; ++r) bq[r] = 0.0f;
+#pragma unroll
+          for (int j = 0; j < 3; ++j) {
+            const float a1 = sx < 3 ? ke[12 * (4 * sx + qx) + 4 * j + kq] : 0.0f;
+            const float b1 = ph[(4 * j + kq) * kRank + 16 * wv + x];
+            bq = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, bq, 0, 0, 0);
+          }
 #pragma unroll
           for (int ks = 0; ks < 3; ++ks) {
-            const int r = 4 * ks + kq;
-            const float4* kr = reinterpret_cast<const float4*>(ke + 12 * r);
-            const float4 k0 = kr[0], k1 = kr[1], k2 = kr[2];
-            float bv = k0.x * pc[0];
-            bv += k0.y * pc[1];
-            bv += k0.z * pc[2];
-            bv += k0.w * pc[3];
-            bv += k1.x * pc[4];
-            bv += k1.y * pc[5];
-            bv += k1.z * pc[6];
-            bv += k1.w * pc[7];
-            bv += k2.x * pc[8];
-            bv += k2.y * pc[9];
-            bv += k2.z * pc[10];
-            bv += k2.w * pc[11];
-            const float4 av = reinterpret_cast<const float4*>(ph + r * kRank)[x];
-            acc[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.x, bv, acc[0], 0, 0, 0);
-            acc[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.y, bv, acc[1], 0, 0, 0);
-            acc[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.z, bv, acc[2], 0, 0, 0);
-            acc[3] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.w, bv, acc[3], 0, 0, 0);
+            const float4 av = reinterpret_cast<const float4*>(ph + (4 * ks + kq) * kRank)[x];
+            acc[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.x, bq[ks], acc[0], 0, 0, 0);
+            acc[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.y, bq[ks], acc[1], 0, 0, 0);
+            acc[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.z, bq[ks], acc[2], 0, 0, 0);
+            acc[3] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.w, bq[ks], acc[3], 0, 0, 0);
           }
         }
       }
