@@ -203,9 +203,30 @@ def test_fbar_isolated_nodes_and_single_element(precision):
     assert np.all(np.diff(indptr)[12:] == 0)
 
 
+@pytest.mark.parametrize("n_elem", [12, 20])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_fbar_kvol_by_slot_with_many_units(precision, n_elem):
+    """Fans of 12 and 20 elements: node 0 has 37 / 61 element neighbours
+    (units), 181 / 301 K_vol terms and 37 / 61 slots, inside the limits of
+    K_vol by slot (64 units, 512 terms, 128 slots), so the unit records of
+    lanes beyond 32 feed the terms' g indices; the 33-element fan below
+    (100 units) takes the unit-step kernel."""
+    from tests.test_gpu_edge_cases import _fan
+
+    wl = _fan(n_elem, seed=42)
+    banner(f"fbar: fan of {n_elem} elements {precision}")
+    fb, f, k = _gpu(wl, precision)
+    X, conn, u = wl.mesh.nodes, wl.mesh.elements, wl.u.reshape(-1)
+    f_ref = FB.internal_forces(X, conn, u, wl.mu, wl.lam)
+    K_ref = FB.tangent_stiffness(X, conn, u, wl.mu, wl.lam, BULK)
+    tf, tk = (1e-11, 1e-12) if precision == "fp64" else (1e-5, 1e-5)
+    assert rel_err(f, f_ref) < tf and _k_err(fb, k, K_ref) < tk
+
+
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_fbar_node_with_33_incidences(precision):
-    # a fan of 33 elements around node 0: the 64-bit feed masks of the row kernel
+    # a fan of 33 elements around node 0: the 64-bit feed masks of the row
+    # kernel, and K_vol by unit steps (100 units: beyond the slot layout)
     from tests.test_gpu_edge_cases import _fan
 
     wl = _fan(33, seed=42)
