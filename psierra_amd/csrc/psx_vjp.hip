@@ -23,11 +23,22 @@
 
 namespace psx {
 
-__global__ void permute_w_kernel(DevPlan p, const double* __restrict__ w) {
+// u and w to internal order in one pass (one user id load per node); the
+// status words are reset here, as the displacement permutation of the other
+// paths does.
+__global__ void permute_uw_kernel(DevPlan p, const double* __restrict__ u, const double* __restrict__ w,
+                                  int32_t* status) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) {
+    status[0] = 0;
+    status[1] = INT_MAX;
+  }
   if (i < p.N) {
     const int64_t a = p.perm[i];
-    p.wi[i] = make_double4(w[3 * a], w[3 * a + 1], w[3 * a + 2], 0.0);
+    const double4 ua = make_double4(u[3 * a], u[3 * a + 1], u[3 * a + 2], 0.0);
+    const double4 wa = make_double4(w[3 * a], w[3 * a + 1], w[3 * a + 2], 0.0);
+    p.ui[i] = ua;
+    p.wi[i] = wa;
   }
 }
 
@@ -210,9 +221,7 @@ static void launch_vjp_t(const DevPlan& p, const double* mu, const double* lam, 
 int32_t launch_tet4_vjp(const DevPlan& p, const double* u, const double* mu, const double* lam,
                         const double* w, double* gu, double* gmu, double* glam, int precision,
                         hipStream_t s, int32_t* status) {
-  int32_t rc = launch_permute_u(p, u, s, status);
-  if (rc) return rc;
-  hipLaunchKernelGGL(permute_w_kernel, dim3((p.N + 255) / 256), dim3(256), 0, s, p, w);
+  hipLaunchKernelGGL(permute_uw_kernel, dim3((p.N + 255) / 256), dim3(256), 0, s, p, u, w, status);
   PSX_HIP_TRY(hipGetLastError());
   if (p.n_tiles <= 0) return PSX_OK;
   const bool fp32 = precision == PSX_FP32;
