@@ -48,6 +48,13 @@ constexpr int kTrainGroup = 16;       // elements per work-group
 constexpr int kTrainThreads = 256;
 constexpr int kTrainSnapPass = 16;    // snapshots whose forces are staged together
 
+// Position of strain component q of mode i of element el in LDS: two modes
+// share a row of 32 doubles (mode 2m at 0..15, mode 2m + 1 at 16..31), so the
+// quarter-waves of one wave, reading consecutive modes, use different banks.
+__device__ __forceinline__ int strain_pos(int i, int q, int el) {
+  return ((i >> 1) * 6 + q) * (2 * kTrainGroup) + (i & 1) * kTrainGroup + el;
+}
+
 __device__ __forceinline__ void train_flag(int32_t* status, int e) {
   atomicMin(&status[1], e);
   status[0] = 1;
@@ -65,7 +72,9 @@ __global__ __launch_bounds__(kTrainThreads) void ecsw_train_kernel(DevPlan p, co
                                                                    int32_t* status) {
   // forces of a snapshot pass [element][snapshot][dof]; afterwards the
   // strain vectors of every mode [mode][component][element]
-  constexpr int kForce = kTrainGroup * kTrainSnapPass * 13;
+  // forces element fastest: [snapshot][component][element], so the 16
+  // elements of a quarter-wave read 16 consecutive doubles
+  constexpr int kForce = kTrainGroup * kTrainSnapPass * 12;
   constexpr int kStrain = kTrainMaxRank * 6 * kTrainGroup;
   __shared__ double s_buf[kForce > kStrain ? kForce : kStrain];
   __shared__ double s_red[2][kTrainThreads];
@@ -111,7 +120,7 @@ __global__ __launch_bounds__(kTrainThreads) void ecsw_train_kernel(DevPlan p, co
       const double J = tet4_element_force<double>(X, U, mue, Ke, fe);
       if (J <= 0.0) train_flag(status, int(e));
 #pragma unroll
-      for (int d = 0; d < 12; ++d) s_buf[(el * kTrainSnapPass + kk) * 13 + d] = fe[d];
+      for (int d = 0; d < 12; ++d) s_buf[(kk * 12 + d) * kTrainGroup + el] = fe[d];
     }
     __syncthreads();
     if (valid) {
@@ -121,10 +130,10 @@ __global__ __launch_bounds__(kTrainThreads) void ecsw_train_kernel(DevPlan p, co
         for (int d = 0; d < 12; ++d) ph[d] = phi[(3 * un[d / 3] + d % 3) * r + i];
         double* dst = A + (int64_t(k0) * r + i) * E + e;
         for (int kk = 0; kk < nk; ++kk, dst += int64_t(r) * E) {
-          const double* fk = s_buf + (el * kTrainSnapPass + kk) * 13;
+          const double* fk = s_buf + kk * 12 * kTrainGroup + el;
           double cv = 0.0;
 #pragma unroll
-          for (int d = 0; d < 12; ++d) cv += ph[d] * fk[d];
+          for (int d = 0; d < 12; ++d) cv += ph[d] * fk[d * kTrainGroup];
           *dst = cv;
           sq_f += cv * cv;
         }
@@ -154,7 +163,7 @@ __global__ __launch_bounds__(kTrainThreads) void ecsw_train_kernel(DevPlan p, co
         const double v[6] = {G[0][0], G[1][1], G[2][2], 0.5 * (G[0][1] + G[1][0]), 0.5 * (G[1][2] + G[2][1]),
                              0.5 * (G[2][0] + G[0][2])};
 #pragma unroll
-        for (int q = 0; q < 6; ++q) s_buf[(i * 6 + q) * kTrainGroup + el] = v[q];
+        for (int q = 0; q < 6; ++q) s_buf[strain_pos(i, q, el)] = v[q];
       }
     }
     __syncthreads();
@@ -174,16 +183,14 @@ __global__ __launch_bounds__(kTrainThreads) void ecsw_train_kernel(DevPlan p, co
       double a[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, ta = 0.0;
       for (int tt = sub; tt < n_tri; tt += kStep, dst += int64_t(kStep) * E) {
         if (i != ci) {
-          const double* pi = s_buf + i * 6 * kTrainGroup + el;
 #pragma unroll
-          for (int q = 0; q < 6; ++q) a[q] = pi[q * kTrainGroup];
+          for (int q = 0; q < 6; ++q) a[q] = s_buf[strain_pos(i, q, el)];
           ta = (a[0] + a[1]) + a[2];
           ci = i;
         }
-        const double* pj = s_buf + j * 6 * kTrainGroup + el;
         double b[6];
 #pragma unroll
-        for (int q = 0; q < 6; ++q) b[q] = pj[q * kTrainGroup];
+        for (int q = 0; q < 6; ++q) b[q] = s_buf[strain_pos(j, q, el)];
         double ee = a[0] * b[0];
         ee += a[1] * b[1];
         ee += a[2] * b[2];
