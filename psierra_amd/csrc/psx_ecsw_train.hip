@@ -63,7 +63,7 @@ __device__ __forceinline__ void train_flag(int32_t* status, int e) {
 // Element groups of the training matrix (and partial-sum pairs per call).
 __host__ __device__ constexpr int64_t train_groups(int64_t E) { return (E + kTrainGroup - 1) / kTrainGroup; }
 
-__global__ __launch_bounds__(kTrainThreads) void ecsw_train_kernel(DevPlan p, const double* __restrict__ phi, int r,
+__global__ __launch_bounds__(kTrainThreads) __attribute__((amdgpu_waves_per_eu(3))) void ecsw_train_kernel(DevPlan p, const double* __restrict__ phi, int r,
                                                                    const double* __restrict__ snaps, int m,
                                                                    const double* __restrict__ mu,
                                                                    const double* __restrict__ lam, int stiff,
