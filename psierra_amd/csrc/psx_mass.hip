@@ -56,6 +56,8 @@ __global__ __launch_bounds__(256) void mass_node_kernel(DevPlan p, const double*
   __shared__ uint32_t s_rank[4 * NPW][KMAX][NRW];
   __shared__ uint8_t s_lp[4 * NPW][KMAX];
   __shared__ double s_coef[NPE * NPE];
+  // Tet4 nodes of at most W slots: per slot, the incidences that feed it
+  __shared__ unsigned long long s_fm[NPE == 4 ? 4 * NPW : 1][NPE == 4 ? W : 1];
   const int w = int(threadIdx.x >> 6);
   const int sub = int(threadIdx.x & 63) / W, lane = int(threadIdx.x & 63) % W;
   const int g = w * NPW + sub;
@@ -71,6 +73,12 @@ __global__ __launch_bounds__(256) void mass_node_kernel(DevPlan p, const double*
   if (a >= p.N) return;
   const int kb = rec.x, L = rec1.x - rec.x;
   const int d = rec1.y - rec.y;
+  const bool narrow = d <= W;
+  constexpr bool MASKS = NPE == 4;
+  if (MASKS) s_fm[MASKS ? g : 0][MASKS ? lane : 0] = 0ull;
+  // the node's lanes are this wave's own: its LDS writes land in order
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
 #pragma unroll
   for (int kk = 0; kk < KMAX / W; ++kk) {
     const int k = lane + kk * W;
@@ -79,17 +87,69 @@ __global__ __launch_bounds__(256) void mass_node_kernel(DevPlan p, const double*
       const int64_t e = int64_t(pe & kIncElemMask);
       s_w[g][k] = rho[e] * vol[e];
       s_lp[g][k] = uint8_t(pe >> kIncElemBits);
+      uint32_t rk[NRW];
 #pragma unroll
-      for (int i = 0; i < NRW; ++i) s_rank[g][k][i] = p.inc_rank[NRW * int64_t(kb + k) + i];
+      for (int i = 0; i < NRW; ++i) {
+        rk[i] = p.inc_rank[NRW * int64_t(kb + k) + i];
+        s_rank[g][k][i] = rk[i];
+      }
+      if (MASKS && narrow) {
+        // bytes 1..3: the slots of the element's other three nodes
+#pragma unroll
+        for (int j = 1; j < 4; ++j)
+          atomicOr(&s_fm[MASKS ? g : 0][MASKS ? int((rk[0] >> (8 * j)) & 0xffu) : 0], 1ull << k);
+      }
     }
   }
   // the run is this wave's own: its LDS writes land before its reads
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
+  if (MASKS && narrow) {
+    // diagonal slot: the node's lanes sum every W-th incidence in order, then
+    // combine in a fixed butterfly; off-diagonal slots walk their feeds in
+    // increasing incidence order
+    double part = 0.0;
+#pragma unroll
+    for (int kk = 0; kk < KMAX / W; ++kk) {
+      const int k = lane + kk * W;
+      if (k < L) {
+        const int lp = s_lp[g][k];
+        part += s_w[g][k] * s_coef[lp * NPE + lp];
+      }
+    }
+#pragma unroll
+    for (int off = W / 2; off > 0; off >>= 1) part += __shfl_xor(part, off, W);
+    const int s = lane;
+    double sum = 0.0;
+    if (s == rec.w) {
+      sum = part;
+    } else if (s < d) {
+      unsigned long long m = s_fm[MASKS ? g : 0][MASKS ? s : 0];
+      while (m) {
+        const int k = low_bit(m);
+        m &= m - 1;
+        const int lp = s_lp[g][k];
+        const uint32_t r = s_rank[g][k][0];
+        int q = 0;
+#pragma unroll
+        for (int j = 1; j < 4; ++j)
+          if (((r >> (8 * j)) & 0xffu) == uint32_t(s)) q = (j - 1 < lp) ? j - 1 : j;
+        sum += s_w[g][k] * s_coef[lp * NPE + q];
+      }
+    }
+    double* dst = kval + rec.z;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+      for (int v = lane; v < 3 * W; v += W) {
+        const int sv = v / 3, c = v - 3 * sv;
+        const double x = __shfl(sum, sv, W);   // every lane of the node takes part
+        if (v < 3 * d) dst[r * 3 * d + v] = c == r ? x : 0.0;
+      }
+    return;
+  }
   // a node of at most W slots: one slot per lane, then each row stored with
   // consecutive lanes on consecutive values (the slot sums shuffled within
   // the node's lanes); otherwise each lane stores its slots' blocks
-  const bool narrow = d <= W;
   for (int s = lane; s < (narrow ? W : d); s += W) {
     const bool self = s == rec.w;
     double sum = 0.0;
