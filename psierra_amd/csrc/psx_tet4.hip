@@ -90,8 +90,8 @@ __device__ __forceinline__ void load_incidence(const P& p, const MatParams& mp, 
   in.pe = p.inc_elem[k];
   in.rank = p.inc_rank[k];
   const int64_t e = int64_t(in.pe & kIncElemMask);
-  if constexpr (!WITH_U && std::is_same<R, float>::value && M != kFbarNeoHookean) {
-    load_incidence_elem<M>(p, mp, in);
+  if constexpr (!WITH_U && std::is_same<R, float>::value) {
+    load_incidence_elem<M>(p, mp, in);   // F-bar: J_bar - 1 from the staged nodal values
     return;
   }
   const int4 c = *reinterpret_cast<const int4*>(p.conn + 4 * e);
@@ -307,9 +307,10 @@ __device__ __forceinline__ void rows_tile(const P& p, const MatParams& mp, const
   // FP32 rows stage the displacements of the tile's nodes and its halo in
   // LDS (one gather per node instead of four per incidence), unless the
   // tile has more than kTileLdsNodes of them
-  constexpr bool LDSU = std::is_same<R, float>::value && !FBAR;
+  constexpr bool LDSU = std::is_same<R, float>::value;
   constexpr int NLU = kTileLdsNodes;
   __shared__ double s_u[LDSU ? 3 * NLU : 1];
+  __shared__ double s_j[LDSU && FBAR ? NLU : 1];   // F-bar: J_node - 1 of the staged nodes
 
   const int4 ti = load_uniform(p.tile_info, tile);
   const int n0 = ti.x, nn = ti.y, k0 = ti.z, nk = ti.w;
@@ -318,12 +319,13 @@ __device__ __forceinline__ void rows_tile(const P& p, const MatParams& mp, const
   if constexpr (LDSU) th = load_uniform(p.tile_halo, tile);
   const bool lds_u = LDSU && th.y >= 0;
   // the node records first: their loads join the first round below
-  int4 nr0 = make_int4(0, 0, 0, 0), nr1 = make_int4(0, 0, 0, 0);
+  int4 nr0 = make_int4(0, 0, 0, 0), nr1 = make_int4(0, 0, 0, 0), nr2 = make_int4(0, 0, 0, 0);
   int user = 0;
   if (t < nn) {
     nr0 = p.node_rec[n0 + t];
     nr1 = p.node_rec[n0 + t + 1];
     if constexpr (!FBAR) user = p.perm[n0 + t];
+    if constexpr (FBAR) nr2 = fv.node_rec2[n0 + t];
   }
 
   IncInputs<R> in;
@@ -341,12 +343,17 @@ __device__ __forceinline__ void rows_tile(const P& p, const MatParams& mp, const
       int node = -1;
       if (t < nn + th.y) node = t < nn ? n0 + t : p.halo[th.x + t - nn];
       double4 u = make_double4(0.0, 0.0, 0.0, 0.0);
-      if (node >= 0) u = p.ui[node];
+      double jm = 0.0;
+      if (node >= 0) {
+        u = p.ui[node];
+        if constexpr (FBAR) jm = mp.v[2][node];
+      }
       if (t < nk) load_incidence_elem<M>(p, mp, in);
       if (node >= 0) {
         s_u[t] = u.x;
         s_u[NLU + t] = u.y;
         s_u[2 * NLU + t] = u.z;
+        if constexpr (FBAR) s_j[t] = jm;
       }
     } else if (t < nk) {
       load_incidence<R, M>(p, mp, k0 + t, in);
@@ -368,6 +375,7 @@ __device__ __forceinline__ void rows_tile(const P& p, const MatParams& mp, const
           s_u[i] = u.x;
           s_u[NLU + i] = u.y;
           s_u[2 * NLU + i] = u.z;
+          if constexpr (FBAR) s_j[i] = mp.v[2][node];
         }
       }
     }
@@ -383,7 +391,7 @@ __device__ __forceinline__ void rows_tile(const P& p, const MatParams& mp, const
     s_self[t] = uint8_t(r0.w);
     if constexpr (!FBAR) s_user[t] = user;
     if constexpr (FBAR) {
-      const int4 r2 = fv.node_rec2[n0 + t];
+      const int4 r2 = nr2;
       s_kofs[t] = r2.x;
       s_f2[t] = r2.x + 3 * r2.z;
       s_f2[NN + t] = r2.y;
@@ -416,6 +424,19 @@ __device__ __forceinline__ void rows_tile(const P& p, const MatParams& mp, const
       for (int q = 0; q < 4; ++q) {
         const int l = int((loc >> (8 * q)) & 0xffu);
         in.U[q] = make_double4(s_u[l], s_u[NLU + l], s_u[2 * NLU + l], 0.0);
+      }
+      if constexpr (FBAR) {
+        // J_bar - 1: mean of the nodal J - 1 in element order (local node L
+        // sits at row position 0 when L is the row node, else L + 1 below it
+        // and L above), as the gathering path forms it
+        const int lp = int(in.pe >> kIncElemBits);
+        double jv[4];
+#pragma unroll
+        for (int L = 0; L < 4; ++L) {
+          const int q = L == lp ? 0 : (L < lp ? L + 1 : L);
+          jv[L] = s_j[int((loc >> (8 * q)) & 0xffu)];
+        }
+        in.pr[2] = (((jv[0] + jv[1]) + jv[2]) + jv[3]) * 0.25;
       }
     }
   }
