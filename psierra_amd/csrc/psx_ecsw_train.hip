@@ -87,7 +87,6 @@ __global__ __launch_bounds__(kTrainThreads) __attribute__((amdgpu_waves_per_eu(3
   const int64_t e0 = g * kTrainGroup;
   const int64_t e = e0 + el;
   const bool valid = g < groups && e < E;
-  const int n_tri = r * (r + 1) / 2;
   int64_t un[4] = {0, 0, 0, 0};
   double4 X[4];
   double mue = 0.0, Ke = 0.0, lame = 0.0;
@@ -169,41 +168,38 @@ __global__ __launch_bounds__(kTrainThreads) __attribute__((amdgpu_waves_per_eu(3
     __syncthreads();
     const double wl = V * lame, wm = 2.0 * V * mue;
     if (valid) {
+      // whole triangle rows per thread: rows p and r - 1 - p (together r + 1
+      // entries) for the pairs p = sub, sub + 16, ..., so every thread has
+      // the same number of entries and each row is a plain loop over j
+      // (row i of the row-major upper triangle starts at entry
+      // i r - i (i - 1) / 2)
       constexpr int kStep = kTrainThreads / kTrainGroup;
-      double* dst = A + (int64_t(m) * r + sub) * E + e;
-      // (i, j) of triangle row tt (row-major upper triangle): row i holds
-      // j = i .. r - 1, and the entry after (i, r - 1) is (i + 1, i + 1)
-      int i = 0, j = sub;
-      while (j >= r && i < r) {
-        j = j - r + i + 1;
-        ++i;
-      }
-      // eps of mode i, kept while the walk stays on triangle row i
-      int ci = -1;
-      double a[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, ta = 0.0;
-      for (int tt = sub; tt < n_tri; tt += kStep, dst += int64_t(kStep) * E) {
-        if (i != ci) {
+      for (int pidx = sub; pidx < (r + 1) / 2; pidx += kStep) {
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+          const int i = side == 0 ? pidx : r - 1 - pidx;
+          if (side == 1 && i == pidx) break;   // odd rank: the middle row once
+          double a[6];
 #pragma unroll
           for (int q = 0; q < 6; ++q) a[q] = s_buf[strain_pos(i, q, el)];
-          ta = (a[0] + a[1]) + a[2];
-          ci = i;
-        }
-        double b[6];
+          const double ta = (a[0] + a[1]) + a[2];
+          const int64_t t0 = int64_t(i) * r - int64_t(i) * (i - 1) / 2;
+          double* dst = A + (int64_t(m) * r + t0) * E + e;
+#pragma unroll 4
+          for (int j = i; j < r; ++j, dst += E) {
+            double b[6];
 #pragma unroll
-        for (int q = 0; q < 6; ++q) b[q] = s_buf[strain_pos(j, q, el)];
-        double ee = a[0] * b[0];
-        ee += a[1] * b[1];
-        ee += a[2] * b[2];
-        double oo = a[3] * b[3];
-        oo += a[4] * b[4];
-        oo += a[5] * b[5];
-        const double sv = wl * ta * ((b[0] + b[1]) + b[2]) + wm * (ee + 2.0 * oo);
-        *dst = sv;
-        sq_s += sv * sv;
-        j += kStep;
-        while (j >= r && i < r) {
-          j = j - r + i + 1;
-          ++i;
+            for (int q = 0; q < 6; ++q) b[q] = s_buf[strain_pos(j, q, el)];
+            double ee = a[0] * b[0];
+            ee += a[1] * b[1];
+            ee += a[2] * b[2];
+            double oo = a[3] * b[3];
+            oo += a[4] * b[4];
+            oo += a[5] * b[5];
+            const double sv = wl * ta * ((b[0] + b[1]) + b[2]) + wm * (ee + 2.0 * oo);
+            *dst = sv;
+            sq_s += sv * sv;
+          }
         }
       }
     }
