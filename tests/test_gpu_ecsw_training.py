@@ -4,8 +4,8 @@ psx_ecsw_training / ECSWTrainingMatrix.build against
 oracle.ecsw.training_matrix, the restatement of ECSWTrainingMatrix.build
 (src/psierra/hyperreduction/ecsw.py:171-365), which test_oracle_ecsw_training
 pins to the assembled force and stiffness.  Tolerance 1e-12 relative
-(Frobenius) for A and b; rank 8 and 64 (the kernel's lane-per-mode limit),
-70 snapshots to cross the 16-snapshot passes, the stiffness block with and
+(Frobenius) for A and b; rank 8, 13 (odd: the stiffness triangle's middle
+row) and 64 (the kernel's lane-per-mode limit), 70 snapshots to cross the 16-snapshot passes, the stiffness block with and
 without a weight, a ragged last group of elements, the sums of squares the
 norm balancing uses, and the reference's exception for an inverting
 snapshot.
@@ -39,7 +39,8 @@ def _inputs(wl, r, m, seed):
     return basis, snaps
 
 
-@pytest.mark.parametrize("r,m,stiff,weight", [(8, 3, False, None), (64, 4, True, None), (8, 70, True, 0.5)])
+@pytest.mark.parametrize("r,m,stiff,weight", [(8, 3, False, None), (64, 4, True, None), (8, 70, True, 0.5),
+                                               (13, 5, True, None)])
 def test_training_matrix_matches_oracle(r, m, stiff, weight):
     from psierra_amd.ecsw import ECSWTrainingMatrix
 
