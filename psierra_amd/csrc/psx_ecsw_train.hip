@@ -22,14 +22,15 @@
 // h_b^T (the gradient of mode i over the element) and eps_i = sym(G_i),
 //   (Phi_e^T K0_e Phi_e)_ij = V (lambda tr(G_i) tr(G_j) + 2 mu eps_i : eps_j),
 // a short inner product: each thread forms eps of its four modes (6
-// components) into LDS, then the work-group writes the upper triangle row by
-// row, element fastest, each thread keeping eps_i while its walk stays on
-// triangle row i.  Measured on the 48^3 box (rank 64, 8 snapshots): 9.4 ms
-// for A's 13.8 GB, against 11.5 ms for an element-major work block and a
-// transpose; the writes, 128-byte runs 5.3 MB apart, set the pace.  Each thread keeps the sums of squares of its force and stiffness
-// values; the work-group adds them in a fixed tree into two partials per
-// group, and a second kernel adds the partials in a fixed order, so the
-// norms are deterministic.
+// components) into LDS, then writes whole rows of the upper triangle (rows p
+// and r - 1 - p for p = sub, sub + 16), element fastest, keeping eps_i in
+// registers along row i.  Measured on the 48^3 box (rank 64, 8 snapshots +
+// stiffness): 4.2 ms for A's 13.8 GB (3.3 TB/s), against 11.5 ms for an
+// element-major work block and a transpose; a write-only kernel of the same
+// layout takes 2.1 ms.  Each thread keeps the sums of squares of its force
+// and stiffness values; the work-group adds them in a fixed tree into two
+// partials per group, and a second kernel adds the partials in a fixed
+// order, so the norms are deterministic.
 #include <algorithm>
 #include <climits>
 #include <mutex>
@@ -70,10 +71,9 @@ __global__ __launch_bounds__(kTrainThreads) __attribute__((amdgpu_waves_per_eu(3
                                                                    double* __restrict__ A, int64_t groups,
                                                                    double* __restrict__ partial,
                                                                    int32_t* status) {
-  // forces of a snapshot pass [element][snapshot][dof]; afterwards the
-  // strain vectors of every mode [mode][component][element]
-  // forces element fastest: [snapshot][component][element], so the 16
-  // elements of a quarter-wave read 16 consecutive doubles
+  // forces of a snapshot pass, element fastest: [snapshot][dof][element], so
+  // the 16 elements of a quarter-wave read 16 consecutive doubles; afterwards
+  // the strain vectors of every mode at strain_pos(mode, component, element)
   constexpr int kForce = kTrainGroup * kTrainSnapPass * 12;
   constexpr int kStrain = kTrainMaxRank * 6 * kTrainGroup;
   __shared__ double s_buf[kForce > kStrain ? kForce : kStrain];
